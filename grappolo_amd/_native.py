@@ -1,0 +1,34 @@
+"""Locating and loading the native libraries (ctypes).
+
+Loading fails loudly when a library is missing: there is no Python fallback
+for any native entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+HOST_LIB = os.path.join(LIB_DIR, "libglv_host.so")
+HIP_LIB = os.path.join(LIB_DIR, "libgrappolo_hip.so")
+
+_cache = {}
+
+
+def _load(path, what):
+    if path in _cache:
+        return _cache[path]
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{what} not built: {path} is missing (run __graft_entry__.build() or "
+            f"`make -C grappolo_amd/csrc`)")
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    _cache[path] = lib
+    return lib
+
+
+def host_lib():
+    return _load(HOST_LIB, "libglv_host.so")
+
+
+def hip_lib():
+    return _load(HIP_LIB, "libgrappolo_hip.so")

@@ -1,0 +1,873 @@
+/*
+ * grappolo_oracle.c -- CPU restatement of Grappolo's colored multi-phase
+ * Louvain (the reference's OpenMP code run at one thread).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Floating-point expressions are written in the reference's evaluation order
+ * and this file is compiled with -ffp-contract=off, so that at one thread the
+ * oracle reproduces the reference bit for bit (checked against the compiled
+ * reference in tests/test_oracle_vs_reference.py).
+ */
+#include "grappolo_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t i64;
+
+#define ORC_MAX_COLORS 4096 /* Coloring/coloringUtils.h:37 (MaxDegree) */
+
+/* ------------------------------------------------------------------------ */
+/* Primitives: utilityClusteringFunctions.cpp                               */
+/* ------------------------------------------------------------------------ */
+
+/* sumVertexDegree, utilityClusteringFunctions.cpp:70-87: row-order sum. */
+void orc_degrees(i64 nv, const i64 *off, const double *w, double *k) {
+    for (i64 i = 0; i < nv; i++) {
+        double s = 0;
+        for (i64 j = off[i]; j < off[i + 1]; j++) s += w[j];
+        k[i] = s;
+    }
+}
+
+/* calConstantForSecondTerm, utilityClusteringFunctions.cpp:89-96. */
+double orc_constant(i64 nv, const double *k) {
+    double s = 0;
+    for (i64 i = 0; i < nv; i++) s += k[i];
+    return (double)1 / s;
+}
+
+/* Neighbour-community aggregation: buildLocalMapCounter,
+ * utilityClusteringFunctions.cpp:172-194.  The vertex's own community is
+ * entered first with weight 0; every entry adds its weight in adjacency order,
+ * so Counter[y] is the left-to-right sum the reference forms. */
+typedef struct {
+    double *acc;
+    i64 *list;
+    unsigned char *mark;
+    i64 n;
+} Agg;
+
+static int agg_init(Agg *A, i64 nv) {
+    A->acc = (double *)calloc((size_t)(nv > 0 ? nv : 1), sizeof(double));
+    A->list = (i64 *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(i64));
+    A->mark = (unsigned char *)calloc((size_t)(nv > 0 ? nv : 1), 1);
+    A->n = 0;
+    return A->acc && A->list && A->mark;
+}
+static void agg_free(Agg *A) {
+    free(A->acc);
+    free(A->list);
+    free(A->mark);
+}
+static inline void agg_push(Agg *A, i64 c) {
+    if (!A->mark[c]) {
+        A->mark[c] = 1;
+        A->acc[c] = 0;
+        A->list[A->n++] = c;
+    }
+}
+static void agg_clear(Agg *A) {
+    for (i64 q = 0; q < A->n; q++) A->mark[A->list[q]] = 0;
+    A->n = 0;
+}
+/* returns selfLoop */
+static double agg_build(Agg *A, const i64 *off, const i64 *tails, const double *w, const i64 *C,
+                        i64 v, i64 s) {
+    double selfLoop = 0;
+    agg_clear(A);
+    agg_push(A, s);
+    for (i64 j = off[v]; j < off[v + 1]; j++) {
+        i64 t = tails[j];
+        if (t == v) selfLoop += w[j];
+        i64 c = C[t];
+        agg_push(A, c);
+        A->acc[c] += w[j];
+    }
+    return selfLoop;
+}
+
+/* max(), utilityClusteringFunctions.cpp:226-259: positive gain required,
+ * ties to the smaller community id, then swap protection. */
+static i64 agg_argmax(const Agg *A, double selfLoop, const double *a, const i64 *n, double degree,
+                      i64 sc, double constant) {
+    i64 maxIndex = sc;
+    double curGain = 0, maxGain = 0;
+    double eix = A->acc[sc] - selfLoop;
+    double ax = a[sc] - degree;
+    for (i64 q = 0; q < A->n; q++) {
+        i64 y = A->list[q];
+        if (y == sc) continue;
+        double ay = a[y];
+        double eiy = A->acc[y];
+        curGain = 2 * (eiy - eix) - 2 * degree * (ay - ax) * constant;
+        if ((curGain > maxGain) || ((curGain == maxGain) && (curGain != 0) && (y < maxIndex))) {
+            maxGain = curGain;
+            maxIndex = y;
+        }
+    }
+    if (n[maxIndex] == 1 && n[sc] == 1 && maxIndex > sc) maxIndex = sc;
+    return maxIndex;
+}
+
+/* Sparse community-update accumulator (cUpdate, parallelLouvainWithColoring.cpp
+ * :160-165 and :213-217).  Only touched entries are reset/applied; adding a
+ * zero update leaves a value unchanged, so this equals the dense passes. */
+typedef struct {
+    double *deg;
+    i64 *size;
+    i64 *list;
+    unsigned char *mark;
+    i64 n;
+} Upd;
+static int upd_init(Upd *U, i64 nv) {
+    size_t m = (size_t)(nv > 0 ? nv : 1);
+    U->deg = (double *)calloc(m, sizeof(double));
+    U->size = (i64 *)calloc(m, sizeof(i64));
+    U->list = (i64 *)malloc(m * sizeof(i64));
+    U->mark = (unsigned char *)calloc(m, 1);
+    U->n = 0;
+    return U->deg && U->size && U->list && U->mark;
+}
+static void upd_free(Upd *U) {
+    free(U->deg);
+    free(U->size);
+    free(U->list);
+    free(U->mark);
+}
+static inline void upd_touch(Upd *U, i64 c) {
+    if (!U->mark[c]) {
+        U->mark[c] = 1;
+        U->deg[c] = 0;
+        U->size[c] = 0;
+        U->list[U->n++] = c;
+    }
+}
+static inline void upd_move(Upd *U, i64 target, i64 src, double kv) {
+    upd_touch(U, target);
+    U->deg[target] += kv;
+    U->size[target] += 1;
+    upd_touch(U, src);
+    U->deg[src] -= kv;
+    U->size[src] -= 1;
+}
+static void upd_apply(Upd *U, double *a, i64 *n) {
+    for (i64 q = 0; q < U->n; q++) {
+        i64 c = U->list[q];
+        n[c] += U->size[c];
+        a[c] += U->deg[c];
+        U->mark[c] = 0;
+    }
+    U->n = 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Coloring                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* algoDistanceOneVertexColoringOpt (Coloring/coloringDistanceOne.cpp:50-233)
+ * at one thread: the tentative pass visits the queue 0..NV-1 in order and
+ * sees every earlier choice, so it is greedy first-fit (distanceOneMarkArray,
+ * coloringUtils.cpp:44-67, ignoring self-loops and uncoloured neighbours) and
+ * the conflict pass (coloringUtils.cpp:70-96) finds nothing to undo. */
+int orc_color_greedy(i64 nv, const i64 *off, const i64 *tails, int32_t *colors) {
+    unsigned char mark[ORC_MAX_COLORS];
+    int maxc = -1;
+    for (i64 v = 0; v < nv; v++) colors[v] = -1;
+    for (i64 v = 0; v < nv; v++) {
+        memset(mark, 0, sizeof(mark));
+        int maxColor = -1;
+        for (i64 j = off[v]; j < off[v + 1]; j++) {
+            i64 t = tails[j];
+            if (t == v) continue;
+            int c = colors[t];
+            if (c >= 0) {
+                if (c >= ORC_MAX_COLORS) return -2;
+                mark[c] = 1;
+                if (c > maxColor) maxColor = c;
+            }
+        }
+        int my;
+        for (my = 0; my <= maxColor; my++)
+            if (!mark[my]) break;
+        colors[v] = my;
+        if (my > maxc) maxc = my;
+    }
+    return maxc;
+}
+
+/* Jones-Plassmann priority used by the engine: the splitmix64 finaliser of
+ * the vertex id, a bijection on 64-bit words, so priorities never tie. */
+uint64_t orc_jp_priority(i64 v) {
+    uint64_t z = (uint64_t)v + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* Restatement of the engine's Jones-Plassmann rounds (grappolo_amd/csrc/
+ * k_color.hip): in each round every uncoloured vertex whose priority beats all
+ * uncoloured neighbours (self-loops ignored) takes the smallest color absent
+ * from its coloured neighbours.  Winners of one round are pairwise
+ * non-adjacent, so the result is independent of processing order. */
+int orc_color_jp(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int32_t *rounds_out) {
+    unsigned char mark[ORC_MAX_COLORS];
+    i64 *work = (i64 *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(i64));
+    i64 *win = (i64 *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(i64));
+    if (!work || !win) {
+        free(work);
+        free(win);
+        return -3;
+    }
+    i64 nwork = nv, rounds = 0;
+    int maxc = -1;
+    for (i64 v = 0; v < nv; v++) {
+        colors[v] = -1;
+        work[v] = v;
+    }
+    while (nwork > 0) {
+        i64 nwin = 0, nrest = 0;
+        for (i64 q = 0; q < nwork; q++) {
+            i64 v = work[q];
+            uint64_t pv = orc_jp_priority(v);
+            int ok = 1;
+            for (i64 j = off[v]; j < off[v + 1]; j++) {
+                i64 u = tails[j];
+                if (u == v || colors[u] >= 0) continue;
+                if (orc_jp_priority(u) > pv) {
+                    ok = 0;
+                    break;
+                }
+            }
+            if (ok)
+                win[nwin++] = v;
+            else
+                work[nrest++] = v;
+        }
+        for (i64 q = 0; q < nwin; q++) {
+            i64 v = win[q];
+            memset(mark, 0, sizeof(mark));
+            for (i64 j = off[v]; j < off[v + 1]; j++) {
+                i64 u = tails[j];
+                if (u == v) continue;
+                int c = colors[u];
+                if (c >= 0 && c < ORC_MAX_COLORS) mark[c] = 1;
+            }
+            int my = 0;
+            while (my < ORC_MAX_COLORS && mark[my]) my++;
+            if (my >= ORC_MAX_COLORS) {
+                free(work);
+                free(win);
+                return -2;
+            }
+            colors[v] = my;
+            if (my > maxc) maxc = my;
+        }
+        nwork = nrest;
+        rounds++;
+    }
+    free(work);
+    free(win);
+    if (rounds_out) *rounds_out = (int32_t)rounds;
+    return maxc;
+}
+
+/* The verification pass of coloringDistanceOne.cpp:207-226 (conflicting
+ * non-self-loop edges, each counted from both ends, halved). */
+i64 orc_color_conflicts(i64 nv, const i64 *off, const i64 *tails, const int32_t *colors) {
+    i64 c = 0;
+    for (i64 v = 0; v < nv; v++)
+        for (i64 j = off[v]; j < off[v + 1]; j++) {
+            i64 t = tails[j];
+            if (t == v) continue;
+            if (colors[v] == colors[t]) c++;
+        }
+    return c / 2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Colored sweep: Coloring/parallelLouvainWithColoring.cpp:47-286            */
+/* ------------------------------------------------------------------------ */
+
+/* In-sweep modularity, parallelLouvainWithColoring.cpp:226-249. */
+static double colored_modularity(i64 nv, const i64 *off, const i64 *tails, const double *w,
+                                 const i64 *C, const double *a, double constant) {
+    double e_xx = 0, a2_x = 0;
+    for (i64 i = 0; i < nv; i++) {
+        double cwi = 0;
+        for (i64 j = off[i]; j < off[i + 1]; j++)
+            if (C[tails[j]] == C[i]) cwi += w[j];
+        e_xx += cwi;
+    }
+    for (i64 i = 0; i < nv; i++) a2_x += a[i] * a[i];
+    return e_xx * (double)constant - a2_x * (double)constant * (double)constant;
+}
+
+double orc_louvain_colored(i64 nv, const i64 *off, const i64 *tails, const double *w, i64 *C,
+                           const int32_t *color, int numColor, double Lower, double thresh,
+                           int frozen, int32_t *numItr, double *trace, int32_t trace_cap) {
+    size_t m = (size_t)(nv > 0 ? nv : 1);
+    double *k = (double *)malloc(m * sizeof(double));
+    double *a = (double *)malloc(m * sizeof(double));
+    i64 *n = (i64 *)malloc(m * sizeof(i64));
+    i64 *tgt = (i64 *)malloc(m * sizeof(i64));
+    i64 *colorPtr = (i64 *)calloc((size_t)numColor + 1, sizeof(i64));
+    i64 *colorIndex = (i64 *)malloc(m * sizeof(i64));
+    Agg A;
+    Upd U;
+    agg_init(&A, nv);
+    upd_init(&U, nv);
+
+    /* sumVertexDegree + calConstantForSecondTerm + initCommAss (:99-108) */
+    orc_degrees(nv, off, w, k);
+    double constant = orc_constant(nv, k);
+    for (i64 i = 0; i < nv; i++) {
+        a[i] = k[i];
+        n[i] = 1;
+        C[i] = i;
+    }
+    /* color CSR (:112-141); at one thread vertices of a color are in id order */
+    for (i64 i = 0; i < nv; i++) colorPtr[(i64)color[i] + 1]++;
+    for (int c = 0; c < numColor; c++) colorPtr[c + 1] += colorPtr[c];
+    {
+        i64 *added = (i64 *)calloc((size_t)numColor + 1, sizeof(i64));
+        for (i64 i = 0; i < nv; i++) {
+            i64 tc = color[i];
+            colorIndex[colorPtr[tc] + added[tc]++] = i;
+        }
+        free(added);
+    }
+
+    double prevMod = Lower, currMod = -1;
+    int numItrs = 0;
+    while (1) {
+        numItrs++;
+        for (int ci = 0; ci < numColor; ci++) {
+            for (i64 K = colorPtr[ci]; K < colorPtr[ci + 1]; K++) {
+                i64 i = colorIndex[K];
+                i64 localTarget;
+                if (off[i] != off[i + 1]) {
+                    double selfLoop = agg_build(&A, off, tails, w, C, i, C[i]);
+                    localTarget = agg_argmax(&A, selfLoop, a, n, k[i], C[i], constant);
+                } else {
+                    localTarget = -1;
+                }
+                if (localTarget != C[i] && localTarget != -1) upd_move(&U, localTarget, C[i], k[i]);
+                if (frozen)
+                    tgt[i] = localTarget;
+                else
+                    C[i] = localTarget; /* :208 */
+            }
+            if (frozen)
+                for (i64 K = colorPtr[ci]; K < colorPtr[ci + 1]; K++) C[colorIndex[K]] = tgt[colorIndex[K]];
+            upd_apply(&U, a, n); /* :213-217 */
+        }
+        currMod = colored_modularity(nv, off, tails, w, C, a, constant);
+        if (trace && numItrs <= trace_cap) trace[numItrs - 1] = currMod;
+        if ((currMod - prevMod) < thresh) break; /* :260 */
+        prevMod = currMod;
+    }
+    if (numItr) *numItr = numItrs;
+    free(k);
+    free(a);
+    free(n);
+    free(tgt);
+    free(colorPtr);
+    free(colorIndex);
+    agg_free(&A);
+    upd_free(&U);
+    return prevMod; /* :284 */
+}
+
+int64_t orc_color_step(i64 nv, const i64 *off, const i64 *tails, const double *w, i64 *C,
+                       const int32_t *color, int32_t which_color) {
+    size_t m = (size_t)(nv > 0 ? nv : 1);
+    double *k = (double *)malloc(m * sizeof(double));
+    double *a = (double *)calloc(m, sizeof(double));
+    i64 *n = (i64 *)calloc(m, sizeof(i64));
+    i64 *tgt = (i64 *)malloc(m * sizeof(i64));
+    Agg A;
+    agg_init(&A, nv);
+    orc_degrees(nv, off, w, k);
+    double constant = orc_constant(nv, k);
+    for (i64 i = 0; i < nv; i++)
+        if (C[i] >= 0) {
+            a[C[i]] += k[i];
+            n[C[i]] += 1;
+        }
+    i64 moves = 0;
+    for (i64 i = 0; i < nv; i++) {
+        tgt[i] = C[i];
+        if (color[i] != which_color) continue;
+        i64 lt = -1;
+        if (off[i] != off[i + 1]) {
+            double selfLoop = agg_build(&A, off, tails, w, C, i, C[i]);
+            lt = agg_argmax(&A, selfLoop, a, n, k[i], C[i], constant);
+        }
+        if (lt != C[i]) moves++;
+        tgt[i] = lt;
+    }
+    for (i64 i = 0; i < nv; i++) C[i] = tgt[i];
+    free(k);
+    free(a);
+    free(n);
+    free(tgt);
+    agg_free(&A);
+    return moves;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plain sweep: BasicCommunitiesDetection/parallelLouvainMethod.cpp:65-326  */
+/* and parallelLouvainMethodNoMap.cpp:48-268                                 */
+/* ------------------------------------------------------------------------ */
+
+/* initCommAssOpt + updateAxForOpt, utilityClusteringFunctions.cpp:46-67 and
+ * :108-169: one greedy move assuming singleton communities.  Kept quirks: the
+ * self-loop weight is truncated to an integer in selfLoop but kept whole in
+ * the own-cluster counter, and duplicate neighbours are not merged. */
+static void init_comm_ass_opt(i64 nv, const i64 *off, const i64 *tails, const double *w,
+                              i64 *past, i64 *curr, double *a, i64 *n, const double *k,
+                              double constant) {
+    for (i64 v = 0; v < nv; v++) {
+        past[v] = v;
+        double selfLoop = 0, cself = 0;
+        for (i64 j = off[v]; j < off[v + 1]; j++)
+            if (tails[j] == v) {
+                selfLoop += (double)(long)w[j];
+                cself = w[j];
+            }
+        i64 maxIndex = v;
+        double curGain = 0, maxGain = 0;
+        double eix = cself - selfLoop;
+        double ax = a[v] - k[v];
+        for (i64 j = off[v]; j < off[v + 1]; j++) {
+            i64 cid = tails[j];
+            if (cid == v) continue;
+            double ay = a[cid];
+            double eiy = w[j];
+            curGain = 2 * (eiy - eix) - 2 * k[v] * (ay - ax) * constant;
+            if ((curGain > maxGain) || ((curGain == maxGain) && (curGain != 0) && (cid < maxIndex))) {
+                maxGain = curGain;
+                maxIndex = cid;
+            }
+        }
+        if (n[maxIndex] == 1 && n[v] == 1 && maxIndex > v) maxIndex = v;
+        curr[v] = maxIndex;
+    }
+    for (i64 i = 0; i < nv; i++) {
+        if (curr[i] != i) {
+            a[i] -= k[i];
+            n[i] -= 1;
+            a[curr[i]] += k[i];
+            n[curr[i]] += 1;
+        }
+    }
+}
+
+double orc_louvain_plain(i64 nv, const i64 *off, const i64 *tails, const double *w, i64 *C,
+                         double Lower, double thresh, int nomap, int32_t *numItr, double *trace,
+                         int32_t trace_cap) {
+    size_t m = (size_t)(nv > 0 ? nv : 1);
+    double *k = (double *)malloc(m * sizeof(double));
+    double *a = (double *)malloc(m * sizeof(double));
+    i64 *n = (i64 *)malloc(m * sizeof(i64));
+    i64 *past = (i64 *)malloc(m * sizeof(i64));
+    i64 *curr = (i64 *)malloc(m * sizeof(i64));
+    i64 *target = (i64 *)malloc(m * sizeof(i64));
+    Agg A;
+    Upd U;
+    agg_init(&A, nv);
+    upd_init(&U, nv);
+
+    orc_degrees(nv, off, w, k);
+    for (i64 i = 0; i < nv; i++) {
+        a[i] = k[i];
+        n[i] = 1;
+    }
+    double constant = orc_constant(nv, k);
+    if (nomap) {
+        init_comm_ass_opt(nv, off, tails, w, past, curr, a, n, k, constant);
+    } else {
+        for (i64 i = 0; i < nv; i++) past[i] = curr[i] = i;
+    }
+    for (i64 i = 0; i < nv; i++) target[i] = -1;
+
+    double prevMod = -1, currMod = -1;
+    int numItrs = 0;
+    while (1) {
+        numItrs++;
+        double e_xx = 0, a2_x = 0;
+        for (i64 i = 0; i < nv; i++) {
+            double cwi = 0;
+            if (off[i] != off[i + 1]) {
+                double selfLoop = agg_build(&A, off, tails, w, curr, i, curr[i]);
+                cwi += A.acc[curr[i]];
+                target[i] = agg_argmax(&A, selfLoop, a, n, k[i], curr[i], constant);
+            } else {
+                target[i] = -1;
+            }
+            if (target[i] != curr[i] && target[i] != -1) upd_move(&U, target[i], curr[i], k[i]);
+            e_xx += cwi;
+        }
+        for (i64 i = 0; i < nv; i++) a2_x += a[i] * a[i];
+        currMod = (e_xx * (double)constant) - (a2_x * (double)constant * (double)constant);
+        if (trace && numItrs <= trace_cap) trace[numItrs - 1] = currMod;
+        if ((currMod - prevMod) < thresh) break;
+        prevMod = currMod;
+        if (prevMod < Lower) prevMod = Lower;
+        upd_apply(&U, a, n);
+        i64 *tmp = past;
+        past = curr;
+        curr = target;
+        target = tmp;
+    }
+    if (numItr) *numItr = numItrs;
+    for (i64 i = 0; i < nv; i++) C[i] = past[i];
+    free(k);
+    free(a);
+    free(n);
+    free(past);
+    free(curr);
+    free(target);
+    agg_free(&A);
+    upd_free(&U);
+    return prevMod;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Modularity of a labelling                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Q = sum_i e_{i,C(i)} / 2m - sum_c a_c^2 / (2m)^2 with a_c the summed degree
+ * of the members of c, the quantity of parallelLouvainWithColoring.cpp
+ * :226-249, accumulated in long double. */
+double orc_modularity(i64 nv, const i64 *off, const i64 *tails, const double *w, const i64 *C) {
+    long double *a = (long double *)calloc((size_t)(nv > 0 ? nv : 1), sizeof(long double));
+    long double two_m = 0, exx = 0, a2 = 0;
+    for (i64 i = 0; i < nv; i++) {
+        long double ki = 0;
+        for (i64 j = off[i]; j < off[i + 1]; j++) {
+            ki += w[j];
+            if (C[i] >= 0 && C[tails[j]] == C[i]) exx += w[j];
+        }
+        two_m += ki;
+        if (C[i] >= 0) a[C[i]] += ki;
+    }
+    for (i64 c = 0; c < nv; c++) a2 += a[c] * a[c];
+    free(a);
+    return (double)(exx / two_m - a2 / (two_m * two_m));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Renumber and coarsen: Utility/buildNextPhase.cpp                          */
+/* ------------------------------------------------------------------------ */
+
+/* renumberClustersContiguously, buildNextPhase.cpp:48-79: new id = order of
+ * first appearance in a serial scan; negative ids are left alone. */
+i64 orc_renumber(i64 *C, i64 size) {
+    i64 *map = (i64 *)malloc((size_t)(size > 0 ? size : 1) * sizeof(i64));
+    for (i64 i = 0; i < size; i++) map[i] = -1;
+    i64 k = 0;
+    for (i64 i = 0; i < size; i++) {
+        if (C[i] >= 0) {
+            if (map[C[i]] < 0) map[C[i]] = k++;
+            C[i] = map[C[i]];
+        }
+    }
+    free(map);
+    return k;
+}
+
+/* buildNextLevelGraphOpt, buildNextPhase.cpp:83-255.  Entries i->j with
+ * C(i) >= C(j) are summed into (C(i), C(j)) in CSR order (the one-thread lock
+ * order, :145-174); every cluster gets a self-loop entry even when its weight
+ * is 0 (:122); a cut pair is emitted in both rows with the same weight
+ * (:205-223).  Rows are returned sorted by tail (the canonical form; the
+ * reference's row order is thread-dependent). */
+i64 orc_coarsen(i64 nv, const i64 *off, const i64 *tails, const double *w, const i64 *C, i64 k,
+                i64 *out_off, i64 *out_tails, double *out_w, i64 *out_num_edges) {
+    /* pass 1: number of aggregated (a,b) entries, via a two-key counting sort */
+    i64 m = 0;
+    for (i64 i = 0; i < nv; i++)
+        for (i64 j = off[i]; j < off[i + 1]; j++)
+            if (C[i] >= 0 && C[i] >= C[tails[j]]) m++;
+    i64 tot = m + k; /* plus one zero self-loop seed per cluster */
+    i64 *ka = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
+    i64 *kb = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
+    double *kw = (double *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(double));
+    i64 p = 0;
+    for (i64 c = 0; c < k; c++) {
+        ka[p] = c;
+        kb[p] = c;
+        kw[p] = 0;
+        p++;
+    }
+    for (i64 i = 0; i < nv; i++)
+        for (i64 j = off[i]; j < off[i + 1]; j++)
+            if (C[i] >= 0 && C[i] >= C[tails[j]]) {
+                ka[p] = C[i];
+                kb[p] = C[tails[j]];
+                kw[p] = w[j];
+                p++;
+            }
+    /* stable LSD counting sort by b then by a */
+    i64 *cnt = (i64 *)calloc((size_t)k + 1, sizeof(i64));
+    i64 *perm1 = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
+    i64 *perm2 = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
+    for (i64 q = 0; q < tot; q++) cnt[kb[q] + 1]++;
+    for (i64 c = 0; c < k; c++) cnt[c + 1] += cnt[c];
+    for (i64 q = 0; q < tot; q++) perm1[cnt[kb[q]]++] = q;
+    memset(cnt, 0, ((size_t)k + 1) * sizeof(i64));
+    for (i64 q = 0; q < tot; q++) cnt[ka[q] + 1]++;
+    for (i64 c = 0; c < k; c++) cnt[c + 1] += cnt[c];
+    for (i64 q = 0; q < tot; q++) {
+        i64 e = perm1[q];
+        perm2[cnt[ka[e]]++] = e;
+    }
+    /* segmented sums in stable order */
+    i64 nu = 0;
+    i64 *ua = perm1; /* reuse buffers */
+    i64 *ub = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
+    double *uw = (double *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(double));
+    for (i64 q = 0; q < tot; q++) {
+        i64 e = perm2[q];
+        if (nu > 0 && ua[nu - 1] == ka[e] && ub[nu - 1] == kb[e]) {
+            uw[nu - 1] += kw[e];
+        } else {
+            ua[nu] = ka[e];
+            ub[nu] = kb[e];
+            uw[nu] = kw[e];
+            nu++;
+        }
+    }
+    i64 ne_out = 0;
+    for (i64 q = 0; q < nu; q++)
+        if (ua[q] != ub[q]) ne_out++;
+    i64 total_entries = 2 * ne_out + k;
+    if (out_num_edges) *out_num_edges = ne_out + k;
+    if (out_off) {
+        /* row a: (a,b) for b<=a in increasing b, then mirrors (b',a) for b'>a */
+        i64 *deg = (i64 *)calloc((size_t)k + 1, sizeof(i64));
+        for (i64 q = 0; q < nu; q++) {
+            deg[ua[q]]++;
+            if (ua[q] != ub[q]) deg[ub[q]]++;
+        }
+        out_off[0] = 0;
+        for (i64 c = 0; c < k; c++) out_off[c + 1] = out_off[c] + deg[c];
+        i64 *fill = deg;
+        for (i64 c = 0; c < k; c++) fill[c] = out_off[c];
+        for (i64 q = 0; q < nu; q++) { /* lower part, increasing b within a */
+            i64 r = ua[q];
+            out_tails[fill[r]] = ub[q];
+            out_w[fill[r]] = uw[q];
+            fill[r]++;
+        }
+        for (i64 q = 0; q < nu; q++) { /* mirrors in increasing a for fixed b */
+            if (ua[q] == ub[q]) continue;
+            i64 r = ub[q];
+            out_tails[fill[r]] = ua[q];
+            out_w[fill[r]] = uw[q];
+            fill[r]++;
+        }
+        free(deg);
+    }
+    free(ka);
+    free(kb);
+    free(kw);
+    free(cnt);
+    free(perm1);
+    free(perm2);
+    free(ub);
+    free(uw);
+    return total_entries;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-phase drivers                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    i64 nv;
+    i64 *off;
+    i64 *tails;
+    double *w;
+    int owned;
+} OGraph;
+
+static void og_free(OGraph *g) {
+    if (g->owned) {
+        free(g->off);
+        free(g->tails);
+        free(g->w);
+    }
+    g->owned = 0;
+}
+
+static int og_coarsen(const OGraph *g, const i64 *C, i64 k, OGraph *out) {
+    i64 ne;
+    i64 tot = orc_coarsen(g->nv, g->off, g->tails, g->w, C, k, NULL, NULL, NULL, &ne);
+    out->nv = k;
+    out->off = (i64 *)malloc(((size_t)k + 1) * sizeof(i64));
+    out->tails = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
+    out->w = (double *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(double));
+    out->owned = 1;
+    if (!out->off || !out->tails || !out->w) return -3;
+    orc_coarsen(g->nv, g->off, g->tails, g->w, C, k, out->off, out->tails, out->w, &ne);
+    return 0;
+}
+
+static void st_phase(orc_stats *st, i64 nv, i64 nnz, int iters, int ncol, int colored, double mod,
+                     i64 clusters) {
+    if (!st) return;
+    int p = st->n_phase;
+    if (p < st->phase_cap) {
+        if (st->phase_nv) st->phase_nv[p] = nv;
+        if (st->phase_nnz) st->phase_nnz[p] = nnz;
+        if (st->phase_iters) st->phase_iters[p] = iters;
+        if (st->phase_colors) st->phase_colors[p] = ncol;
+        if (st->phase_colored) st->phase_colored[p] = colored;
+        if (st->phase_mod) st->phase_mod[p] = mod;
+        if (st->phase_clusters) st->phase_clusters[p] = clusters;
+    }
+    st->n_phase = p + 1;
+}
+
+/* runMultiPhaseColoring, Coloring/runMultiPhaseColoring.cpp:51-238, for
+ * coloring == 1. */
+int orc_run_colored(i64 nv, const i64 *off, const i64 *tails, const double *w, i64 *C_orig,
+                    int coloring_mode, int frozen, int replaceMap, i64 minGraphSize,
+                    double threshold, double C_threshold, orc_stats *st) {
+    i64 NV = nv;
+    OGraph G = {nv, (i64 *)off, (i64 *)tails, (double *)w, 0};
+    int32_t *colors = (int32_t *)malloc((size_t)(NV > 0 ? NV : 1) * sizeof(int32_t));
+    i64 *C = (i64 *)malloc((size_t)(NV > 0 ? NV : 1) * sizeof(i64));
+    if (!colors || !C) return -3;
+    for (i64 i = 0; i < NV; i++) colors[i] = -1;
+    int mc = coloring_mode ? orc_color_jp(G.nv, G.off, G.tails, colors, NULL)
+                           : orc_color_greedy(G.nv, G.off, G.tails, colors);
+    if (mc < -1) {
+        free(colors);
+        free(C);
+        return -2;
+    }
+    int nColors = mc + 1;
+    if (st) st->colors_first = nColors;
+    double prevMod = -1, currMod = -1;
+    i64 phase = 1, numClusters = 0;
+    int totItr = 0, tmpItr = 0;
+    i64 ev = 0;
+    for (i64 i = 0; i < NV; i++) C[i] = -1;
+    int nonColor = 0;
+    while (1) {
+        prevMod = currMod;
+        int colored = !nonColor;
+        if (!nonColor) {
+            currMod = orc_louvain_colored(G.nv, G.off, G.tails, G.w, C, colors, nColors, currMod,
+                                          C_threshold, frozen, &tmpItr, NULL, 0);
+        } else {
+            currMod = orc_louvain_plain(G.nv, G.off, G.tails, G.w, C, currMod, threshold,
+                                        replaceMap == 1, &tmpItr, NULL, 0);
+        }
+        totItr += tmpItr;
+        ev += (i64)tmpItr * G.off[G.nv];
+        numClusters = orc_renumber(C, G.nv);
+        st_phase(st, G.nv, G.off[G.nv], tmpItr, colored ? nColors : 0, colored, currMod,
+                 numClusters);
+        if (phase == 1) {
+            for (i64 i = 0; i < NV; i++) C_orig[i] = C[i];
+        } else {
+            for (i64 i = 0; i < NV; i++)
+                if (C_orig[i] >= 0) C_orig[i] = C[C_orig[i]];
+        }
+        if ((phase > 200) || (totItr > 10000)) break;
+        if ((currMod - prevMod) > threshold) {
+            OGraph Gn;
+            if (og_coarsen(&G, C, numClusters, &Gn)) return -3;
+            og_free(&G);
+            G = Gn;
+            free(C);
+            C = (i64 *)malloc((size_t)(numClusters > 0 ? numClusters : 1) * sizeof(i64));
+            for (i64 i = 0; i < numClusters; i++) C[i] = -1;
+            phase++;
+            if ((G.nv > minGraphSize) && (nonColor == 0)) {
+                for (i64 i = 0; i < G.nv; i++) colors[i] = -1;
+                mc = coloring_mode ? orc_color_jp(G.nv, G.off, G.tails, colors, NULL)
+                                   : orc_color_greedy(G.nv, G.off, G.tails, colors);
+                if (mc < -1) return -2;
+                nColors = mc + 1;
+            }
+        } else {
+            if (nonColor == 0)
+                nonColor = 1;
+            else
+                break;
+        }
+    }
+    if (st) {
+        st->phases = (int32_t)phase;
+        st->total_iterations = totItr;
+        st->final_clusters = numClusters;
+        st->final_modularity = prevMod;
+        st->last_modularity = currMod;
+        st->edges_visited = ev;
+    }
+    og_free(&G);
+    free(C);
+    free(colors);
+    return 0;
+}
+
+/* runMultiPhaseBasic, BasicCommunitiesDetection/runMultiPhaseBasic.cpp:52-170
+ * (threadsOpt == 1, the value the driver forces). */
+int orc_run_basic(i64 nv, const i64 *off, const i64 *tails, const double *w, i64 *C_orig,
+                  int basicOpt, double threshold, orc_stats *st) {
+    i64 NV = nv;
+    OGraph G = {nv, (i64 *)off, (i64 *)tails, (double *)w, 0};
+    i64 *C = (i64 *)malloc((size_t)(NV > 0 ? NV : 1) * sizeof(i64));
+    double prevMod = -1, currMod = -1;
+    i64 phase = 1, numClusters = 0, ev = 0;
+    int totItr = 0, tmpItr = 0;
+    for (i64 i = 0; i < NV; i++) C[i] = -1;
+    while (1) {
+        prevMod = currMod;
+        currMod = orc_louvain_plain(G.nv, G.off, G.tails, G.w, C, currMod, threshold,
+                                    basicOpt == 1, &tmpItr, NULL, 0);
+        totItr += tmpItr;
+        ev += (i64)tmpItr * G.off[G.nv];
+        numClusters = orc_renumber(C, G.nv);
+        st_phase(st, G.nv, G.off[G.nv], tmpItr, 0, 0, currMod, numClusters);
+        if (phase == 1) {
+            for (i64 i = 0; i < NV; i++) C_orig[i] = C[i];
+        } else {
+            for (i64 i = 0; i < NV; i++)
+                if (C_orig[i] >= 0) C_orig[i] = C[C_orig[i]];
+        }
+        if ((phase > 200) || (totItr > 100000)) break;
+        if ((currMod - prevMod) > threshold) {
+            OGraph Gn;
+            if (og_coarsen(&G, C, numClusters, &Gn)) return -3;
+            og_free(&G);
+            G = Gn;
+            free(C);
+            C = (i64 *)malloc((size_t)(numClusters > 0 ? numClusters : 1) * sizeof(i64));
+            for (i64 i = 0; i < numClusters; i++) C[i] = -1;
+            phase++;
+        } else {
+            break;
+        }
+    }
+    if (st) {
+        st->phases = (int32_t)phase;
+        st->total_iterations = totItr;
+        st->final_clusters = numClusters;
+        st->final_modularity = prevMod;
+        st->last_modularity = currMod;
+        st->edges_visited = ev;
+        st->colors_first = 0;
+    }
+    og_free(&G);
+    free(C);
+    return 0;
+}

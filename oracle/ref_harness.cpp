@@ -1,0 +1,184 @@
+// ref_harness.cpp -- a thin C-ABI around the compiled reference (Grappolo's
+// own sources, built by oracle/Makefile from the reference tree into
+// oracle/_ref/).  TEST INFRASTRUCTURE ONLY: used to pin the oracle against
+// the real reference in this container and as the CPU baseline in bench.py.
+//
+// The reference functions are called unchanged; this file only converts the
+// structure-of-arrays CSR used by the tests into the reference's `graph`
+// (DefineStructure/defs.h:85-92) and collects results.
+#include <fcntl.h>
+#include <stdint.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "defs.h"  // must precede the other reference headers
+#include "basic_comm.h"
+#include "basic_util.h"
+#include "color_comm.h"
+#include "coloring.h"
+
+namespace {
+
+graph *make_graph(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                  const double *w) {
+    graph *G = (graph *)malloc(sizeof(graph));
+    int64_t nnz = off[nv];
+    G->numVertices = nv;
+    G->sVertices = nv;
+    G->numEdges = num_edges;
+    G->edgeListPtrs = (long *)malloc((nv + 1) * sizeof(long));
+    G->edgeList = (edge *)malloc((nnz > 0 ? nnz : 1) * sizeof(edge));
+    for (int64_t i = 0; i <= nv; i++) G->edgeListPtrs[i] = off[i];
+    for (int64_t i = 0; i < nv; i++)
+        for (int64_t j = off[i]; j < off[i + 1]; j++) {
+            G->edgeList[j].head = i;
+            G->edgeList[j].tail = tails[j];
+            G->edgeList[j].weight = w[j];
+        }
+    return G;
+}
+
+void free_graph(graph *G) {
+    if (!G) return;
+    free(G->edgeListPtrs);
+    free(G->edgeList);
+    free(G);
+}
+
+// Redirect the reference's stdout chatter into a file while `fn` runs.
+struct Quiet {
+    int saved = -1;
+    explicit Quiet(const char *path) {
+        fflush(stdout);
+        saved = dup(1);
+        int fd = open(path ? path : "/dev/null", O_WRONLY | O_CREAT | O_TRUNC, 0644);
+        if (fd >= 0) {
+            dup2(fd, 1);
+            close(fd);
+        }
+    }
+    ~Quiet() {
+        fflush(stdout);
+        if (saved >= 0) {
+            dup2(saved, 1);
+            close(saved);
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ref_color(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+              const double *w, int32_t *colors, int nthreads) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    for (int64_t i = 0; i < nv; i++) colors[i] = -1;
+    double t = 0;
+    int mc;
+    {
+        Quiet q(nullptr);
+        mc = algoDistanceOneVertexColoringOpt(G, colors, nthreads, &t);
+    }
+    free_graph(G);
+    return mc;
+}
+
+double ref_colored_phase(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                         const double *w, int64_t *C, int32_t *colors, int ncolors, double Lower,
+                         double thresh, int nomap, int nthreads, int32_t *numItr) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    double t = 0, mod;
+    int it = 0;
+    {
+        Quiet q(nullptr);
+        if (nomap)
+            mod = algoLouvainWithDistOneColoringNoMap(G, (long *)C, nthreads, colors, ncolors, Lower,
+                                                      thresh, &t, &it);
+        else
+            mod = algoLouvainWithDistOneColoring(G, (long *)C, nthreads, colors, ncolors, Lower,
+                                                 thresh, &t, &it);
+    }
+    *numItr = it;
+    free_graph(G);
+    return mod;
+}
+
+double ref_plain_phase(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                       const double *w, int64_t *C, double Lower, double thresh, int nomap,
+                       int nthreads, int32_t *numItr) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    double t = 0, mod;
+    int it = 0;
+    {
+        Quiet q(nullptr);
+        if (nomap)
+            mod = parallelLouvianMethodNoMap(G, (long *)C, nthreads, Lower, thresh, &t, &it);
+        else
+            mod = parallelLouvianMethod(G, (long *)C, nthreads, Lower, thresh, &t, &it);
+    }
+    *numItr = it;
+    free_graph(G);
+    return mod;
+}
+
+int64_t ref_renumber(int64_t *C, int64_t n) {
+    Quiet q(nullptr);
+    return renumberClustersContiguously((long *)C, n);
+}
+
+// Coarsen; the output arrays are malloc()ed and returned through the out
+// pointers (free with ref_free).  Rows come back in the reference's order.
+int64_t ref_coarsen(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                    const double *w, int64_t *C, int64_t k, int nthreads, int64_t **out_off,
+                    int64_t **out_tails, double **out_w, int64_t *out_num_edges) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    graph Gout;
+    {
+        Quiet q(nullptr);
+        buildNextLevelGraphOpt(G, &Gout, (long *)C, k, nthreads);
+    }
+    int64_t tot = Gout.edgeListPtrs[Gout.numVertices];
+    *out_off = (int64_t *)malloc((k + 1) * sizeof(int64_t));
+    *out_tails = (int64_t *)malloc((tot > 0 ? tot : 1) * sizeof(int64_t));
+    *out_w = (double *)malloc((tot > 0 ? tot : 1) * sizeof(double));
+    for (int64_t i = 0; i <= k; i++) (*out_off)[i] = Gout.edgeListPtrs[i];
+    for (int64_t j = 0; j < tot; j++) {
+        (*out_tails)[j] = Gout.edgeList[j].tail;
+        (*out_w)[j] = Gout.edgeList[j].weight;
+    }
+    *out_num_edges = Gout.numEdges;
+    free(Gout.edgeListPtrs);
+    free(Gout.edgeList);
+    free_graph(G);
+    return tot;
+}
+
+void ref_free(void *p) { free(p); }
+
+// The reference's own drivers, unchanged.  They consume the graph and print
+// their summary; the summary goes to `log_path` so the caller can read the
+// printed "Final modularity" line.  C_orig receives the partition.
+void ref_run_colored(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                     const double *w, int64_t *C_orig, int replaceMap, int64_t minGraphSize,
+                     double threshold, double C_threshold, int nthreads, const char *log_path) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    for (int64_t i = 0; i < nv; i++) C_orig[i] = -1;
+    Quiet q(log_path);
+    runMultiPhaseColoring(G, (long *)C_orig, 1, 0, replaceMap, minGraphSize, threshold, C_threshold,
+                          nthreads, 1);
+}
+
+void ref_run_basic(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                   const double *w, int64_t *C_orig, int basicOpt, double threshold, int nthreads,
+                   const char *log_path) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    for (int64_t i = 0; i < nv; i++) C_orig[i] = -1;
+    Quiet q(log_path);
+    runMultiPhaseBasic(G, (long *)C_orig, basicOpt, 100000, threshold, 0.01, nthreads, 1);
+}
+
+}  // extern "C"
