@@ -1,0 +1,742 @@
+// capi.cpp -- the C ABI (include/grappolo_hip.h) and Grappolo's C++ entry
+// points (include/grappolo/*.h) on top of the engine.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "grappolo/basic_comm.h"
+#include "grappolo/basic_util.h"
+#include "grappolo/color_comm.h"
+#include "grappolo/coloring.h"
+#include "grappolo_hip.h"
+#include "kernels.hpp"
+#include "louvain.hpp"
+
+using glv::DBuf;
+using glv::DevGraph;
+using glv::Engine;
+using glv::i64;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceCtx {
+    hipStream_t stream = nullptr;
+    bool ready = false;
+};
+std::mutex g_mu;
+DeviceCtx g_ctx[64];
+
+int current_device() {
+    int d = 0;
+    GLV_HIP(hipGetDevice(&d));
+    return d;
+}
+
+// The engine's stream on the current device; the device's default memory
+// pool keeps freed blocks (release threshold = max) so that repeated calls
+// reuse memory instead of going back to the driver.
+hipStream_t engine_stream(void *user) {
+    if (user) return (hipStream_t)user;
+    int d = current_device();
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceCtx &c = g_ctx[d & 63];
+    if (!c.ready) {
+        GLV_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, d) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+        c.ready = true;
+    }
+    return c.stream;
+}
+
+template <class F>
+int guarded(F fn) {
+    try {
+        fn();
+        return GLV_OK;
+    } catch (const glv::Error &e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return GLV_E_OOM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return GLV_E_DEVICE;
+    }
+}
+
+void check_host_csr(i64 nv, const int64_t *ptrs, const glv_edge *edges) {
+    if (nv < 0) glv::fail(GLV_E_INVALID_ARG, "negative vertex count");
+    if (nv > 0 && !ptrs) glv::fail(GLV_E_INVALID_ARG, "null edgeListPtrs");
+    if (nv > 0 && ptrs[0] != 0) glv::fail(GLV_E_INVALID_GRAPH, "edgeListPtrs[0] != 0");
+    if (nv > 0 && ptrs[nv] > 0 && !edges) glv::fail(GLV_E_INVALID_ARG, "null edgeList");
+    if (nv > 0 && ptrs[nv] < 0) glv::fail(GLV_E_INVALID_GRAPH, "negative entry count");
+}
+
+// Validate a device CSR (offsets monotone and in range, tails in [0, nv)).
+void check_dev_csr(i64 nv, i64 nnz, const i64 *off, const i64 *tails, hipStream_t s) {
+    if (nv < 0 || nnz < 0) glv::fail(GLV_E_INVALID_ARG, "negative size");
+    if (nv == 0) return;
+    if (!off || (nnz > 0 && !tails)) glv::fail(GLV_E_INVALID_ARG, "null device pointer");
+    i64 ends[2];
+    GLV_HIP(hipMemcpyAsync(&ends[0], off, sizeof(i64), hipMemcpyDeviceToHost, s));
+    GLV_HIP(hipMemcpyAsync(&ends[1], off + nv, sizeof(i64), hipMemcpyDeviceToHost, s));
+    DBuf<int32_t> err(1, s);
+    glv::launch_fill_i32(err.p, 1, 0, s);
+    glv::launch_check_graph(off, tails, nv, nnz, err.p, s);
+    int32_t e = 0;
+    GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s));
+    GLV_HIP(hipStreamSynchronize(s));
+    if (ends[0] != 0 || ends[1] != nnz) glv::fail(GLV_E_INVALID_GRAPH, "offsets do not span [0, nnz]");
+    if (e & 1) glv::fail(GLV_E_INVALID_GRAPH, "offsets not monotone");
+    if (e & 2) glv::fail(GLV_E_INVALID_GRAPH, "tail id out of range");
+}
+
+// Caller-supplied labels: every label in [-1, limit), -1 only on vertices
+// without entries.
+void check_labels(const DevGraph &g, const i64 *C, i64 limit, hipStream_t s) {
+    if (g.nv == 0) return;
+    if (!C) glv::fail(GLV_E_INVALID_ARG, "null labels");
+    DBuf<int32_t> err(1, s);
+    glv::launch_fill_i32(err.p, 1, 0, s);
+    glv::launch_check_labels(g.off, C, g.nv, limit, err.p, s);
+    int32_t e = 0;
+    GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s));
+    GLV_HIP(hipStreamSynchronize(s));
+    if (e & 1) glv::fail(GLV_E_INVALID_ARG, "cluster id out of range");
+    if (e & 2) glv::fail(GLV_E_INVALID_ARG, "vertex with edges has cluster id -1");
+}
+
+DevGraph view(i64 nv, i64 nnz, const int64_t *off, const int64_t *tails, const double *w) {
+    DevGraph g;
+    g.nv = nv;
+    g.nnz = nnz;
+    g.num_edges = nnz / 2;
+    g.off = off;
+    g.tails = tails;
+    g.w = w;
+    return g;
+}
+
+// Upload the reference's host arrays; the AoS records are split on the device.
+void upload(i64 nv, const int64_t *ptrs, const glv_edge *edges, DevGraph &g, hipStream_t s) {
+    check_host_csr(nv, ptrs, edges);
+    const i64 nnz = nv > 0 ? ptrs[nv] : 0;
+    g.nv = nv;
+    g.nnz = nnz;
+    g.num_edges = nnz / 2;
+    g.off_buf.alloc(nv + 1, s);
+    g.tails_buf.alloc(nnz, s);
+    g.w_buf.alloc(nnz, s);
+    g.adopt();
+    if (nv + 1 > 0 && ptrs)
+        GLV_HIP(hipMemcpyAsync(g.off_buf.p, ptrs, (nv + 1) * sizeof(i64), hipMemcpyHostToDevice, s));
+    if (nnz > 0) {
+        DBuf<glv_edge> aos(nnz, s);
+        GLV_HIP(hipMemcpyAsync(aos.p, edges, nnz * sizeof(glv_edge), hipMemcpyHostToDevice, s));
+        glv::launch_split_edges(aos.p, nnz, g.tails_buf.p, g.w_buf.p, s);
+    }
+    check_dev_csr(nv, nnz, g.off, g.tails, s);
+}
+
+void fill_stats(const glv::RunResult &r, glv_stats *st) {
+    if (!st) return;
+    st->phases = r.phases;
+    st->total_iterations = r.total_iterations;
+    st->final_clusters = r.final_clusters;
+    st->final_modularity = r.final_modularity;
+    st->last_modularity = r.last_modularity;
+    st->time_clustering = r.t_clustering;
+    st->time_building = r.t_building;
+    st->time_coloring = r.t_coloring;
+    st->time_renumber = r.t_renumber;
+    st->time_total = r.t_total;
+    st->edges_visited = r.edges_visited;
+    st->edges_visited_mod = r.edges_visited_mod;
+    st->colors_first = r.colors_first;
+    st->n_phase = (int32_t)r.phase.size();
+    for (int i = 0; i < (int)r.phase.size() && i < st->phase_cap; i++) {
+        const auto &p = r.phase[i];
+        if (st->phase_nv) st->phase_nv[i] = p.nv;
+        if (st->phase_nnz) st->phase_nnz[i] = p.nnz;
+        if (st->phase_iters) st->phase_iters[i] = p.iters;
+        if (st->phase_colors) st->phase_colors[i] = p.ncolors;
+        if (st->phase_colored) st->phase_colored[i] = p.colored;
+        if (st->phase_mod) st->phase_mod[i] = p.mod;
+        if (st->phase_clusters) st->phase_clusters[i] = p.clusters;
+    }
+    st->n_trace = (int32_t)r.trace.size();
+    for (int i = 0; i < (int)r.trace.size() && i < st->trace_cap; i++)
+        if (st->trace_q) st->trace_q[i] = r.trace[i];
+}
+
+// A graph without any entry has no defined modularity (1/(2m) is infinite);
+// the reference loops on NaN there.  Every vertex is isolated: label -1.
+bool edgeless(const DevGraph &g, int64_t *d_C_orig, hipStream_t s, glv::RunResult &r) {
+    if (g.nnz > 0) return false;
+    glv::launch_fill_i64(d_C_orig, g.nv, -1, s);
+    GLV_HIP(hipStreamSynchronize(s));
+    r = glv::RunResult();
+    r.phases = g.nv > 0 ? 1 : 0;
+    r.final_modularity = 0;
+    r.last_modularity = 0;
+    return true;
+}
+
+void run_pipeline(bool colored, const DevGraph &g, int64_t *d_C_orig, const glv::RunParams &p,
+                  hipStream_t s, glv::RunResult &r) {
+    if (edgeless(g, d_C_orig, s, r)) return;
+    Engine E(s);
+    if (colored)
+        E.run_colored(g, d_C_orig, p, r);
+    else
+        E.run_basic(g, d_C_orig, p, r);
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+const char *glv_last_error(void) { return g_last_error.c_str(); }
+
+const char *glv_version(void) { return "grappolo_amd 0.1 (gfx950)"; }
+
+int glv_set_device(int device) {
+    return guarded([&] { GLV_HIP(hipSetDevice(device)); });
+}
+
+int glv_get_device_count(int *count) {
+    return guarded([&] {
+        if (!count) glv::fail(GLV_E_INVALID_ARG, "null count");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+void glv_free(void *p) { free(p); }
+
+int glv_runMultiPhaseColoring(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                              int64_t *C_orig, int coloring, int numColors, int replaceMap,
+                              int64_t minGraphSize, double threshold, double C_threshold,
+                              int numThreads, int threadsOpt, glv_stats *stats) {
+    (void)numColors;
+    (void)numThreads;
+    (void)threadsOpt;
+    return guarded([&] {
+        if (coloring != 1) glv::fail(GLV_E_UNSUPPORTED, "only coloring == 1 is implemented");
+        if (nv > 0 && !C_orig) glv::fail(GLV_E_INVALID_ARG, "null C_orig");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        DBuf<i64> dC(nv, s);
+        glv::RunParams p;
+        p.coloring = coloring;
+        p.replaceMap = replaceMap;
+        p.minGraphSize = minGraphSize;
+        p.threshold = threshold;
+        p.C_threshold = C_threshold;
+        glv::RunResult r;
+        run_pipeline(true, g, dC.p, p, s, r);
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(C_orig, dC.p, nv * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        fill_stats(r, stats);
+    });
+}
+
+int glv_runMultiPhaseBasic(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                           int64_t *C_orig, int basicOpt, int64_t minGraphSize, double threshold,
+                           double C_threshold, int numThreads, int threadsOpt, glv_stats *stats) {
+    (void)minGraphSize;
+    (void)C_threshold;
+    (void)numThreads;
+    (void)threadsOpt;
+    return guarded([&] {
+        if (nv > 0 && !C_orig) glv::fail(GLV_E_INVALID_ARG, "null C_orig");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        DBuf<i64> dC(nv, s);
+        glv::RunParams p;
+        p.replaceMap = basicOpt == 1;
+        p.threshold = threshold;
+        p.max_total_iters = 100000;  // runMultiPhaseBasic.cpp:116
+        glv::RunResult r;
+        run_pipeline(false, g, dC.p, p, s, r);
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(C_orig, dC.p, nv * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        fill_stats(r, stats);
+    });
+}
+
+int glv_algoDistanceOneVertexColoringOpt(int64_t nv, const int64_t *edgeListPtrs,
+                                         const glv_edge *edgeList, int32_t *vtxColor,
+                                         int32_t *maxColor, double *totTime) {
+    return guarded([&] {
+        if (nv > 0 && (!vtxColor || !maxColor)) glv::fail(GLV_E_INVALID_ARG, "null output");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        DBuf<int32_t> dc(nv, s);
+        Engine E(s);
+        auto t0 = std::chrono::steady_clock::now();
+        int mc = E.color(g, dc.p);
+        auto t1 = std::chrono::steady_clock::now();
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(vtxColor, dc.p, nv * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        if (maxColor) *maxColor = mc;
+        if (totTime) *totTime = std::chrono::duration<double>(t1 - t0).count();
+    });
+}
+
+int glv_algoLouvainWithDistOneColoring(int64_t nv, const int64_t *edgeListPtrs,
+                                       const glv_edge *edgeList, int64_t *C, const int32_t *color,
+                                       int numColor, double Lower, double thresh,
+                                       double *modularity, double *totTime, int32_t *numItr) {
+    return guarded([&] {
+        if (nv > 0 && (!C || !color)) glv::fail(GLV_E_INVALID_ARG, "null C or color");
+        if (numColor < 1 && nv > 0) glv::fail(GLV_E_INVALID_ARG, "numColor < 1");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        if (g.nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        DBuf<i64> dC(nv, s);
+        DBuf<int32_t> dcol(nv, s);
+        GLV_HIP(hipMemcpyAsync(dcol.p, color, nv * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        Engine E(s);
+        int it = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        double q = E.colored_phase(g, dC.p, dcol.p, numColor, Lower, thresh, &it);
+        auto t1 = std::chrono::steady_clock::now();
+        GLV_HIP(hipMemcpyAsync(C, dC.p, nv * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        if (modularity) *modularity = q;
+        if (numItr) *numItr = it;
+        if (totTime) *totTime = std::chrono::duration<double>(t1 - t0).count();
+    });
+}
+
+int glv_parallelLouvianMethod(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                              int64_t *C, double Lower, double thresh, int noMap, double *modularity,
+                              double *totTime, int32_t *numItr) {
+    return guarded([&] {
+        if (nv > 0 && !C) glv::fail(GLV_E_INVALID_ARG, "null C");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        if (g.nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        DBuf<i64> dC(nv, s);
+        Engine E(s);
+        int it = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        double q = E.plain_phase(g, dC.p, Lower, thresh, noMap != 0, &it);
+        auto t1 = std::chrono::steady_clock::now();
+        GLV_HIP(hipMemcpyAsync(C, dC.p, nv * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        if (modularity) *modularity = q;
+        if (numItr) *numItr = it;
+        if (totTime) *totTime = std::chrono::duration<double>(t1 - t0).count();
+    });
+}
+
+int glv_renumberClustersContiguously(int64_t *C, int64_t size, int64_t *numClusters) {
+    return guarded([&] {
+        if (size < 0 || (size > 0 && !C)) glv::fail(GLV_E_INVALID_ARG, "bad C/size");
+        hipStream_t s = engine_stream(nullptr);
+        DBuf<i64> dC(size, s);
+        if (size > 0) GLV_HIP(hipMemcpyAsync(dC.p, C, size * sizeof(i64), hipMemcpyHostToDevice, s));
+        Engine E(s);
+        i64 k = E.renumber(dC.p, size);
+        if (size > 0) GLV_HIP(hipMemcpyAsync(C, dC.p, size * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        if (numClusters) *numClusters = k;
+    });
+}
+
+int glv_buildNextLevelGraphOpt(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                               const int64_t *C, int64_t numUniqueClusters,
+                               int64_t **outEdgeListPtrs, glv_edge **outEdgeList,
+                               int64_t *outNumEdges) {
+    return guarded([&] {
+        if (!outEdgeListPtrs || !outEdgeList || (nv > 0 && !C))
+            glv::fail(GLV_E_INVALID_ARG, "null argument");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g, out;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        DBuf<i64> dC(nv, s);
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(dC.p, C, nv * sizeof(i64), hipMemcpyHostToDevice, s));
+        check_labels(g, dC.p, numUniqueClusters, s);
+        Engine E(s);
+        E.coarsen(g, dC.p, numUniqueClusters, out);
+        const i64 k = out.nv, tot = out.nnz;
+        int64_t *hp = (int64_t *)malloc((k + 1) * sizeof(int64_t));
+        glv_edge *he = (glv_edge *)malloc((tot > 0 ? tot : 1) * sizeof(glv_edge));
+        std::vector<i64> tails(tot);
+        std::vector<double> w(tot);
+        if (!hp || !he) {
+            free(hp);
+            free(he);
+            glv::fail(GLV_E_OOM, "host allocation failed");
+        }
+        GLV_HIP(hipMemcpyAsync(hp, out.off, (k + 1) * sizeof(i64), hipMemcpyDeviceToHost, s));
+        if (tot > 0) {
+            GLV_HIP(hipMemcpyAsync(tails.data(), out.tails, tot * sizeof(i64), hipMemcpyDeviceToHost, s));
+            GLV_HIP(hipMemcpyAsync(w.data(), out.w, tot * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        GLV_HIP(hipStreamSynchronize(s));
+        for (i64 i = 0; i < k; i++)
+            for (i64 j = hp[i]; j < hp[i + 1]; j++) he[j] = glv_edge{i, tails[j], w[j]};
+        *outEdgeListPtrs = hp;
+        *outEdgeList = he;
+        if (outNumEdges) *outNumEdges = out.num_edges;
+    });
+}
+
+int glv_modularity(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                   const int64_t *C, double *modularity) {
+    return guarded([&] {
+        if (!modularity || (nv > 0 && !C)) glv::fail(GLV_E_INVALID_ARG, "null argument");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        if (g.nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        DBuf<i64> dC(nv, s);
+        GLV_HIP(hipMemcpyAsync(dC.p, C, nv * sizeof(i64), hipMemcpyHostToDevice, s));
+        check_labels(g, dC.p, nv, s);
+        Engine E(s);
+        *modularity = E.modularity(g, dC.p);
+    });
+}
+
+// ---------------------------------------------------------------- device forms
+int glv_runMultiPhaseColoring_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                                  const int64_t *d_tails, const double *d_weights, int64_t *d_C_orig,
+                                  int coloring, int replaceMap, int64_t minGraphSize,
+                                  double threshold, double C_threshold, void *stream,
+                                  glv_stats *stats) {
+    return guarded([&] {
+        if (coloring != 1) glv::fail(GLV_E_UNSUPPORTED, "only coloring == 1 is implemented");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        glv::RunParams p;
+        p.replaceMap = replaceMap;
+        p.minGraphSize = minGraphSize;
+        p.threshold = threshold;
+        p.C_threshold = C_threshold;
+        glv::RunResult r;
+        run_pipeline(true, g, d_C_orig, p, s, r);
+        fill_stats(r, stats);
+    });
+}
+
+int glv_runMultiPhaseBasic_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                               const int64_t *d_tails, const double *d_weights, int64_t *d_C_orig,
+                               int basicOpt, double threshold, void *stream, glv_stats *stats) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        glv::RunParams p;
+        p.replaceMap = basicOpt == 1;
+        p.threshold = threshold;
+        p.max_total_iters = 100000;
+        glv::RunResult r;
+        run_pipeline(false, g, d_C_orig, p, s, r);
+        fill_stats(r, stats);
+    });
+}
+
+int glv_color_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                  int32_t *d_colors, int32_t *maxColor, int32_t *rounds, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, nullptr);
+        Engine E(s);
+        int r = 0;
+        int mc = E.color(g, d_colors, &r);
+        if (maxColor) *maxColor = mc;
+        if (rounds) *rounds = r;
+    });
+}
+
+int glv_colored_phase_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                          const double *d_weights, int64_t *d_C, const int32_t *d_colors,
+                          int numColor, double Lower, double thresh, double *modularity,
+                          int32_t *numItr, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        if (nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        if (numColor < 1) glv::fail(GLV_E_INVALID_ARG, "numColor < 1");
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        Engine E(s);
+        int it = 0;
+        double q = E.colored_phase(g, d_C, d_colors, numColor, Lower, thresh, &it);
+        E.sync();
+        if (modularity) *modularity = q;
+        if (numItr) *numItr = it;
+    });
+}
+
+int glv_plain_phase_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                        const double *d_weights, int64_t *d_C, double Lower, double thresh,
+                        int noMap, double *modularity, int32_t *numItr, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        if (nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        Engine E(s);
+        int it = 0;
+        double q = E.plain_phase(g, d_C, Lower, thresh, noMap != 0, &it);
+        E.sync();
+        if (modularity) *modularity = q;
+        if (numItr) *numItr = it;
+    });
+}
+
+int glv_color_step_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                       const double *d_weights, int64_t *d_C, const int32_t *d_colors, int numColor,
+                       int which, int64_t *moves, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        if (nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        if (numColor < 1 || which < 0 || which >= numColor) glv::fail(GLV_E_INVALID_ARG, "bad color");
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        check_labels(g, d_C, nv, s);
+        Engine E(s);
+        i64 m = E.color_step(g, d_C, d_colors, numColor, which);
+        if (moves) *moves = m;
+    });
+}
+
+int glv_modularity_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                       const double *d_weights, const int64_t *d_C, double *modularity,
+                       void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        if (nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        check_labels(g, d_C, nv, s);
+        Engine E(s);
+        double q = E.modularity(g, d_C);
+        if (modularity) *modularity = q;
+    });
+}
+
+int glv_renumber_dev(int64_t n, int64_t *d_C, int64_t *numClusters, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        Engine E(s);
+        i64 k = E.renumber(d_C, n);
+        if (numClusters) *numClusters = k;
+    });
+}
+
+int glv_coarsen_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                    const double *d_weights, const int64_t *d_C, int64_t k, int64_t **d_out_offsets,
+                    int64_t **d_out_tails, double **d_out_weights, int64_t *out_nnz,
+                    int64_t *out_num_edges, void *stream) {
+    return guarded([&] {
+        if (!d_out_offsets || !d_out_tails || !d_out_weights) glv::fail(GLV_E_INVALID_ARG, "null output");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights), out;
+        check_labels(g, d_C, k, s);
+        Engine E(s);
+        E.coarsen(g, d_C, k, out);
+        E.sync();
+        // hand the buffers to the caller (released with glv_dev_free)
+        *d_out_offsets = out.off_buf.p;
+        *d_out_tails = out.tails_buf.p;
+        *d_out_weights = out.w_buf.p;
+        out.off_buf.p = nullptr;
+        out.tails_buf.p = nullptr;
+        out.w_buf.p = nullptr;
+        if (out_nnz) *out_nnz = out.nnz;
+        if (out_num_edges) *out_num_edges = out.num_edges;
+    });
+}
+
+int glv_dev_free(void *d_ptr) {
+    return guarded([&] {
+        if (!d_ptr) return;
+        hipStream_t s = engine_stream(nullptr);
+        GLV_HIP(hipFreeAsync(d_ptr, s));
+        GLV_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int glv_count_conflicts_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                            const int64_t *d_tails, const int32_t *d_colors, int64_t *conflicts,
+                            void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, nullptr);
+        Engine E(s);
+        i64 c = E.count_conflicts(g, d_colors);
+        if (conflicts) *conflicts = c;
+    });
+}
+
+}  // extern "C"
+
+// ===================================================== C++ drop-in (reference API)
+namespace {
+
+[[noreturn]] void die(const char *fn, int rc) {
+    fprintf(stderr, "%s failed (status %d): %s\n", fn, rc, glv_last_error());
+    exit(1);
+}
+
+void free_graph(graph *G) {
+    if (!G) return;
+    free(G->edgeListPtrs);
+    free(G->edgeList);
+    free(G);
+}
+
+void print_summary(const glv_stats &st, int coloring, int numThreads) {
+    printf("********************************************\n");
+    printf("*********    Compact Summary   *************\n");
+    printf("********************************************\n");
+    printf("Number of threads              : %d\n", numThreads);
+    printf("Total number of phases         : %d\n", st.phases);
+    printf("Total number of iterations     : %d\n", st.total_iterations);
+    printf("Final number of clusters       : %ld\n", (long)st.final_clusters);
+    printf("Final modularity               : %lf\n", st.final_modularity);
+    printf("Total time for clustering      : %lf\n", st.time_clustering);
+    printf("Total time for building phases : %lf\n", st.time_building);
+    if (coloring == 1) printf("Total time for coloring        : %lf\n", st.time_coloring);
+    printf("********************************************\n");
+    printf("TOTAL TIME                     : %lf\n",
+           st.time_clustering + st.time_building + (coloring == 1 ? st.time_coloring : 0.0));
+    printf("********************************************\n");
+}
+
+}  // namespace
+
+void runMultiPhaseColoring(graph *G, long *C_orig, int coloring, int numColors, int replaceMap,
+                           long minGraphSize, double threshold, double C_threshold, int numThreads,
+                           int threadsOpt) {
+    glv_stats st;
+    memset(&st, 0, sizeof(st));
+    int rc = glv_runMultiPhaseColoring(G->numVertices, G->edgeListPtrs, (const glv_edge *)G->edgeList,
+                                       (int64_t *)C_orig, coloring, numColors, replaceMap, minGraphSize,
+                                       threshold, C_threshold, numThreads, threadsOpt, &st);
+    if (rc) die("runMultiPhaseColoring", rc);
+    print_summary(st, coloring, numThreads);
+    free_graph(G);  // runMultiPhaseColoring.cpp:227-232: the routine owns G
+}
+
+void runMultiPhaseBasic(graph *G, long *C_orig, int basicOpt, long minGraphSize, double threshold,
+                        double C_threshold, int numThreads, int threadsOpt) {
+    glv_stats st;
+    memset(&st, 0, sizeof(st));
+    int rc = glv_runMultiPhaseBasic(G->numVertices, G->edgeListPtrs, (const glv_edge *)G->edgeList,
+                                    (int64_t *)C_orig, basicOpt, minGraphSize, threshold, C_threshold,
+                                    numThreads, threadsOpt, &st);
+    if (rc) die("runMultiPhaseBasic", rc);
+    print_summary(st, 0, numThreads);
+    free_graph(G);  // runMultiPhaseBasic.cpp:164-169
+}
+
+double algoLouvainWithDistOneColoring(graph *G, long *C, int nThreads, int *color, int numColor,
+                                      double Lower, double thresh, double *totTime, int *numItr) {
+    (void)nThreads;
+    double q = 0, t = 0;
+    int32_t it = 0;
+    int rc = glv_algoLouvainWithDistOneColoring(G->numVertices, G->edgeListPtrs,
+                                                (const glv_edge *)G->edgeList, (int64_t *)C,
+                                                (const int32_t *)color, numColor, Lower, thresh, &q,
+                                                &t, &it);
+    if (rc) die("algoLouvainWithDistOneColoring", rc);
+    if (totTime) *totTime = t;
+    if (numItr) *numItr = it;
+    return q;
+}
+
+double algoLouvainWithDistOneColoringNoMap(graph *G, long *C, int nThreads, int *color, int numColor,
+                                           double Lower, double thresh, double *totTime, int *numItr) {
+    // the NoMap variant computes the same targets (parallelLouvainWithColoringNoMap.cpp)
+    return algoLouvainWithDistOneColoring(G, C, nThreads, color, numColor, Lower, thresh, totTime,
+                                          numItr);
+}
+
+int algoDistanceOneVertexColoringOpt(graph *G, int *vtxColor, int nThreads, double *totTime) {
+    (void)nThreads;
+    int32_t mc = -1;
+    double t = 0;
+    int rc = glv_algoDistanceOneVertexColoringOpt(G->numVertices, G->edgeListPtrs,
+                                                  (const glv_edge *)G->edgeList, (int32_t *)vtxColor,
+                                                  &mc, &t);
+    if (rc) die("algoDistanceOneVertexColoringOpt", rc);
+    if (totTime) *totTime = t;
+    return mc;
+}
+
+double parallelLouvianMethod(graph *G, long *C, int nThreads, double Lower, double thresh,
+                             double *totTime, int *numItr) {
+    (void)nThreads;
+    double q = 0, t = 0;
+    int32_t it = 0;
+    int rc = glv_parallelLouvianMethod(G->numVertices, G->edgeListPtrs, (const glv_edge *)G->edgeList,
+                                       (int64_t *)C, Lower, thresh, 0, &q, &t, &it);
+    if (rc) die("parallelLouvianMethod", rc);
+    if (totTime) *totTime = t;
+    if (numItr) *numItr = it;
+    return q;
+}
+
+double parallelLouvianMethodNoMap(graph *G, long *C, int nThreads, double Lower, double thresh,
+                                  double *totTime, int *numItr) {
+    (void)nThreads;
+    double q = 0, t = 0;
+    int32_t it = 0;
+    int rc = glv_parallelLouvianMethod(G->numVertices, G->edgeListPtrs, (const glv_edge *)G->edgeList,
+                                       (int64_t *)C, Lower, thresh, 1, &q, &t, &it);
+    if (rc) die("parallelLouvianMethodNoMap", rc);
+    if (totTime) *totTime = t;
+    if (numItr) *numItr = it;
+    return q;
+}
+
+long renumberClustersContiguously(long *C, long size) {
+    int64_t k = 0;
+    int rc = glv_renumberClustersContiguously((int64_t *)C, size, &k);
+    if (rc) die("renumberClustersContiguously", rc);
+    return k;
+}
+
+double buildNextLevelGraphOpt(graph *Gin, graph *Gout, long *C, long numUniqueClusters, int nThreads) {
+    (void)nThreads;
+    int64_t *ptrs = nullptr, ne = 0;
+    glv_edge *edges = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = glv_buildNextLevelGraphOpt(Gin->numVertices, Gin->edgeListPtrs,
+                                        (const glv_edge *)Gin->edgeList, (const int64_t *)C,
+                                        numUniqueClusters, &ptrs, &edges, &ne);
+    auto t1 = std::chrono::steady_clock::now();
+    if (rc) die("buildNextLevelGraphOpt", rc);
+    Gout->numVertices = numUniqueClusters;
+    Gout->sVertices = numUniqueClusters;
+    Gout->numEdges = ne;
+    Gout->edgeListPtrs = (long *)ptrs;
+    Gout->edgeList = (edge *)edges;
+    return std::chrono::duration<double>(t1 - t0).count();
+}

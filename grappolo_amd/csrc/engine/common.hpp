@@ -1,0 +1,124 @@
+// common.hpp -- shared types and helpers of the HIP engine (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "glv_status.h"
+
+namespace glv {
+
+using i64 = int64_t;
+using u64 = uint64_t;
+
+constexpr int kWave = 64;         // CDNA wavefront width
+constexpr int kBlock = 256;       // default workgroup size (4 waves)
+constexpr int kMaxColors = 4096;  // Coloring/coloringUtils.h:37 (MaxDegree)
+constexpr i64 kEmpty = -1;        // empty hash slot key
+
+// Degree bins of the sweep (see sweep kernels in kernels.hip).
+//  0: isolated        -> no sweep
+//  1..5: deg <= 4/8/16/32/64 -> sub-wave groups of G lanes, one vertex each
+//  6: deg <= 256      -> one wavefront per vertex, LDS hash of 512 slots
+//  7: deg <= 2048     -> one workgroup per vertex, LDS hash of 4096 slots
+//  8: larger          -> one workgroup per vertex, hash table in HBM
+constexpr int kNumBins = 9;
+constexpr int kBinGroup[6] = {0, 4, 8, 16, 32, 64};
+constexpr i64 kBinMaxDeg[kNumBins] = {0, 4, 8, 16, 32, 64, 256, 2048, INT64_MAX};
+constexpr int kWaveHashSlots = 512;
+constexpr int kBlockHashSlots = 4096;
+
+__host__ __device__ inline int degree_bin(i64 d) {
+    if (d == 0) return 0;
+    if (d <= 4) return 1;
+    if (d <= 8) return 2;
+    if (d <= 16) return 3;
+    if (d <= 32) return 4;
+    if (d <= 64) return 5;
+    if (d <= 256) return 6;
+    if (d <= 2048) return 7;
+    return 8;
+}
+
+// Error carried from the HIP layer up to the C ABI.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define GLV_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            ::glv::fail(e_ == hipErrorOutOfMemory ? GLV_E_OOM : GLV_E_DEVICE,                  \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// Stream-ordered device buffer (memory comes from the device's default pool,
+// which keeps freed blocks cached, so per-phase scratch does not hit the
+// driver allocator after warm-up).
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipStream_t s = nullptr;
+    DBuf() = default;
+    DBuf(size_t count, hipStream_t stream) { alloc(count, stream); }
+    void alloc(size_t count, hipStream_t stream) {
+        release();
+        s = stream;
+        n = count;
+        if (count) GLV_HIP(hipMallocAsync((void **)&p, count * sizeof(T), stream));
+    }
+    void release() {
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            s = o.s;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    T *get() const { return p; }
+};
+
+// Device-resident CSR graph (structure of arrays; the reference's `head`
+// field is not stored).
+struct DevGraph {
+    i64 nv = 0, nnz = 0, num_edges = 0;
+    const i64 *off = nullptr;
+    const i64 *tails = nullptr;
+    const double *w = nullptr;
+    // storage when the engine owns the graph (coarsened levels)
+    DBuf<i64> off_buf, tails_buf;
+    DBuf<double> w_buf;
+    void adopt() {
+        off = off_buf.p;
+        tails = tails_buf.p;
+        w = w_buf.p;
+    }
+};
+
+inline unsigned grid_for(i64 n, int block = kBlock, i64 cap = 1 << 20) {
+    i64 g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace glv

@@ -1,0 +1,1169 @@
+// kernels.hip -- the HIP kernels of the engine, written for gfx950 (CDNA4,
+// 64-lane wavefronts, 160 KB LDS per CU, FP64 atomics in LDS and HBM).
+//
+// Reference correspondence (file:line of the CPU routine each one replaces):
+//   k_degree*           sumVertexDegree       Utility/utilityClusteringFunctions.cpp:70-87
+//   k_sweep_*           buildLocalMapCounter + max (:172-194, :226-259) inside the per-color
+//                       loop of Coloring/parallelLouvainWithColoring.cpp:169-210 and the Jacobi
+//                       loop of BasicCommunitiesDetection/parallelLouvainMethod.cpp:199-240
+//   k_apply_*           cInfo += cUpdate      parallelLouvainWithColoring.cpp:213-217
+//   k_mod_*             in-sweep modularity   parallelLouvainWithColoring.cpp:226-249
+//   k_jp_*              distance-1 coloring   Coloring/coloringDistanceOne.cpp:50-233 (JP rounds)
+//   k_first/flags/relab renumberClustersContiguously Utility/buildNextPhase.cpp:48-79
+//   k_coarsen_*         buildNextLevelGraphOpt Utility/buildNextPhase.cpp:83-255
+#include <algorithm>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "kernels.hpp"
+
+namespace glv {
+
+namespace {
+
+constexpr u64 kFib = 0x9E3779B97F4A7C15ULL;
+
+__device__ inline u64 jp_priority(i64 v) {
+    // splitmix64 finaliser: a bijection, so priorities of distinct vertices differ
+    u64 z = (u64)v + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ inline i64 ld_relaxed(const i64 *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
+__device__ inline int32_t ld_relaxed(const int32_t *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
+
+// Gain of moving a vertex of degree kv from community s (internal weight
+// eix excluding its self-loop, degree ax excluding the vertex) to community y
+// holding weight eiy towards it and degree ay: max() at
+// utilityClusteringFunctions.cpp:243, same operation order, no contraction.
+__device__ inline double gain_of(double eiy, double eix, double kv, double ay, double ax,
+                                 double constant) {
+    return 2 * (eiy - eix) - 2 * kv * (ay - ax) * constant;
+}
+
+// "better" of two candidates: larger gain, then smaller community id.
+__device__ inline bool better(bool oc, double og, i64 oid, bool c, double g, i64 id) {
+    return oc && (!c || og > g || (og == g && oid < id));
+}
+
+template <int W>
+__device__ inline void group_argmax(bool &cand, double &gain, i64 &id) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        double og = __shfl_xor(gain, o, W);
+        long long oid = __shfl_xor((long long)id, o, W);
+        int ocd = __shfl_xor((int)cand, o, W);
+        if (better(ocd != 0, og, oid, cand, gain, id)) {
+            cand = true;
+            gain = og;
+            id = oid;
+        }
+    }
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block-wide sum, result valid in every thread.  Deterministic order.
+__device__ double block_sum(double v) {
+    __shared__ double sh[kBlock / kWave];
+    __shared__ double res;
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int q = 0; q < kBlock / kWave; q++) r += sh[q];
+        res = r;
+    }
+    __syncthreads();
+    double r = res;
+    __syncthreads();
+    return r;
+}
+
+// Block-wide argmax, result valid in every thread.
+__device__ void block_argmax(bool &cand, double &gain, i64 &id) {
+    __shared__ int sc[kBlock / kWave];
+    __shared__ double sg[kBlock / kWave];
+    __shared__ i64 si[kBlock / kWave];
+    group_argmax<kWave>(cand, gain, id);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) {
+        sc[w] = cand;
+        sg[w] = gain;
+        si[w] = id;
+    }
+    __syncthreads();
+    bool c = sc[0] != 0;
+    double g = sg[0];
+    i64 d = si[0];
+    for (int q = 1; q < kBlock / kWave; q++)
+        if (better(sc[q] != 0, sg[q], si[q], c, g, d)) {
+            c = true;
+            g = sg[q];
+            d = si[q];
+        }
+    cand = c;
+    gain = g;
+    id = d;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_fill_i64(i64 *p, i64 n, i64 v) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_f64(double *p, i64 n, double v) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_i32(int32_t *p, i64 n, int32_t v) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_iota(i64 *p, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        p[i] = i;
+}
+// initCommAss (utilityClusteringFunctions.cpp:98-104); vertices without
+// entries are labelled -1 at once, which is what their first color step
+// writes (parallelLouvainWithColoring.cpp:189-191,208).
+__global__ void k_init_labels(const i64 *off, i64 nv, i64 *C) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x)
+        C[i] = (off[i + 1] != off[i]) ? i : -1;
+}
+
+__global__ void k_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
+                           i64 *vid, int32_t *hist) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
+        int b = degree_bin(off[i + 1] - off[i]);
+        int c = colors ? colors[i] : 0;
+        vid[i] = i;
+        if (c < 0 || c >= ncol) {  // counted nowhere: the caller sees sum(hist) != nv
+            keys[i] = 0;
+            continue;
+        }
+        uint32_t key = (uint32_t)(b * ncol + c);
+        keys[i] = key;
+        atomicAdd(&hist[key], 1);
+    }
+}
+
+// sumVertexDegree: one thread per row sums in row order (the reference's
+// order) for rows up to 2048 entries; longer rows use a workgroup each.
+__global__ void k_degree_rows(const i64 *off, const double *w, i64 nv, double *k) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
+        i64 b = off[i], e = off[i + 1];
+        if (e - b > kBinMaxDeg[7]) continue;
+        double s = 0;
+        for (i64 j = b; j < e; j++) s += w[j];
+        k[i] = s;
+    }
+}
+__global__ void k_degree_big(const i64 *off, const double *w, const i64 *verts, i64 count, double *k) {
+    for (i64 q = blockIdx.x; q < count; q += gridDim.x) {
+        i64 v = verts[q];
+        double s = 0;
+        for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += blockDim.x) s += w[j];
+        s = block_sum(s);
+        if (threadIdx.x == 0) k[v] = s;
+    }
+}
+__global__ void k_init_comm(const double *k, i64 nv, double *a, i64 *n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
+        a[i] = k[i];
+        n[i] = 1;
+    }
+}
+__global__ void k_comm_from_labels(const i64 *C, const double *k, i64 nv, double *a, i64 *n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
+        i64 c = C[i];
+        if (c >= 0) {
+            atomicAdd(&a[c], k[i]);
+            atomicAdd((unsigned long long *)&n[c], 1ULL);
+        }
+    }
+}
+
+// ----------------------------------------------------------------- reductions
+template <bool SQ>
+__global__ __launch_bounds__(kBlock) void k_red(const double *x, i64 n, double *part) {
+    double s = 0;
+    for (i64 i = blockIdx.x * (i64)kBlock + threadIdx.x; i < n; i += (i64)gridDim.x * kBlock) {
+        double v = x[i];
+        s += SQ ? v * v : v;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *part, i64 np, double *out) {
+    double s = 0;
+    for (i64 i = threadIdx.x; i < np; i += kBlock) s += part[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// --------------------------------------------------------------------- sweeps
+// Sub-wave groups of G lanes, one vertex (degree <= G) per group.  Each lane
+// loads one entry; aggregation walks the group's entries in adjacency order
+// from LDS, so every per-community weight is the same left-to-right sum that
+// buildLocalMapCounter forms.  The lane holding the first occurrence of a
+// community scores it; a butterfly over the group picks the best.
+template <int G, bool PLAIN>
+__global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
+    constexpr int VPB = kBlock / G;
+    __shared__ i64 s_c[kBlock];
+    __shared__ double s_w[kBlock];
+    __shared__ int s_self[kBlock];
+    __shared__ double s_part[VPB];
+    const int tid = threadIdx.x, lane = tid % G, grp = tid / G;
+    const i64 idx = (i64)blockIdx.x * VPB + grp;
+    const bool valid = idx < A.count;
+    i64 v = 0, beg = 0;
+    int deg = 0;
+    if (valid) {
+        v = A.verts[idx];
+        beg = A.off[v];
+        deg = (int)(A.off[v + 1] - beg);
+    }
+    i64 c = -2;
+    double ww = 0;
+    int self = 0;
+    if (lane < deg) {
+        i64 t = A.tails[beg + lane];
+        ww = A.w[beg + lane];
+        c = A.C[t];
+        self = (t == v);
+    }
+    s_c[tid] = c;
+    s_w[tid] = ww;
+    s_self[tid] = self;
+    __syncthreads();
+    const i64 s = valid ? A.C[v] : -3;
+    double e_mine = 0, e_s = 0, sl = 0;
+    bool first = true;
+    const int gb = grp * G;
+    for (int m = 0; m < deg; m++) {
+        i64 cm = s_c[gb + m];
+        double wm = s_w[gb + m];
+        if (cm == c) {
+            e_mine += wm;
+            if (m < lane) first = false;
+        }
+        if (cm == s) e_s += wm;
+        if (s_self[gb + m]) sl += wm;
+    }
+    bool cand = lane < deg && first && c != s;
+    double gain = 0;
+    if (cand) {
+        const double kv = A.k[v];
+        gain = gain_of(e_mine, e_s - sl, kv, A.a[c], A.a[s] - kv, A.constant);
+        cand = gain > 0;  // max() starts from maxGain = 0 and needs a strict increase
+    }
+    group_argmax<G>(cand, gain, c);
+    if (valid && lane == 0) {
+        i64 target = s;
+        if (cand) {
+            target = c;
+            if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;  // swap protection
+        }
+        A.tgt[v] = target;
+    }
+    if (PLAIN) {
+        if (lane == 0) s_part[grp] = valid ? e_s : 0.0;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0;
+            for (int q = 0; q < VPB; q++) t += s_part[q];
+            A.eii_part[blockIdx.x] = t;
+        }
+    }
+}
+
+// Open-addressing hash of (community -> weight), linear probing, Fibonacci
+// hashing of the id.  `keys` holds kEmpty in free slots.
+__device__ inline void hash_insert(i64 *keys, double *vals, u64 mask, int shift, i64 c, double w) {
+    u64 h = ((u64)c * kFib) >> shift;
+    while (true) {
+        i64 k = ld_relaxed(&keys[h]);
+        if (k == kEmpty) {
+            k = (i64)atomicCAS((unsigned long long *)&keys[h], (unsigned long long)kEmpty,
+                               (unsigned long long)c);
+            if (k == kEmpty) k = c;
+        }
+        if (k == c) {
+            atomicAdd(&vals[h], w);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+__device__ inline double hash_lookup(const i64 *keys, const double *vals, u64 mask, int shift, i64 c) {
+    u64 h = ((u64)c * kFib) >> shift;
+    while (true) {
+        i64 k = keys[h];
+        if (k == c) return vals[h];
+        if (k == kEmpty) return 0.0;
+        h = (h + 1) & mask;
+    }
+}
+
+// Score every occupied slot of a table and reduce over the calling group.
+__device__ inline void score_slots(const i64 *keys, const double *vals, i64 slots, int first, int step,
+                                   const SweepArgs &A, i64 s, double kv, double eix, double ax,
+                                   bool &cand, double &gain, i64 &id) {
+    for (i64 h = first; h < slots; h += step) {
+        i64 y = keys[h];
+        if (y == kEmpty || y == s) continue;
+        double g = gain_of(vals[h], eix, kv, A.a[y], ax, A.constant);
+        bool c = g > 0;
+        if (better(c, g, y, cand, gain, id)) {
+            cand = true;
+            gain = g;
+            id = y;
+        }
+    }
+}
+
+// One wavefront per vertex (degree <= 256), a private 512-slot LDS table.
+template <bool PLAIN>
+__global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
+    constexpr int WPB = kBlock / kWave;
+    constexpr int SLOTS = kWaveHashSlots;
+    constexpr int SHIFT = 64 - 9;
+    __shared__ i64 s_keys[WPB][SLOTS];
+    __shared__ double s_vals[WPB][SLOTS];
+    __shared__ double s_part[WPB];
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const i64 idx = (i64)blockIdx.x * WPB + wid;
+    const bool valid = idx < A.count;
+    i64 *keys = s_keys[wid];
+    double *vals = s_vals[wid];
+    for (int h = lane; h < SLOTS; h += kWave) {
+        keys[h] = kEmpty;
+        vals[h] = 0;
+    }
+    __syncthreads();
+    i64 v = valid ? A.verts[idx] : 0;
+    i64 beg = valid ? A.off[v] : 0, end = valid ? A.off[v + 1] : 0;
+    double sl = 0;
+    for (i64 j = beg + lane; j < end; j += kWave) {
+        i64 t = A.tails[j];
+        double ww = A.w[j];
+        if (t == v) sl += ww;
+        hash_insert(keys, vals, SLOTS - 1, SHIFT, A.C[t], ww);
+    }
+    __syncthreads();
+    sl = wave_sum(sl);
+    const i64 s = valid ? A.C[v] : -3;
+    const double e_s = valid ? hash_lookup(keys, vals, SLOTS - 1, SHIFT, s) : 0.0;
+    bool cand = false;
+    double gain = 0;
+    i64 id = INT64_MAX;
+    if (valid) {
+        const double kv = A.k[v];
+        score_slots(keys, vals, SLOTS, lane, kWave, A, s, kv, e_s - sl, A.a[s] - kv, cand, gain, id);
+    }
+    group_argmax<kWave>(cand, gain, id);
+    if (valid && lane == 0) {
+        i64 target = s;
+        if (cand) {
+            target = id;
+            if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
+        }
+        A.tgt[v] = target;
+    }
+    if (PLAIN) {
+        if (lane == 0) s_part[wid] = valid ? e_s : 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (int q = 0; q < WPB; q++) t += s_part[q];
+            A.eii_part[blockIdx.x] = t;
+        }
+    }
+}
+
+// One workgroup per vertex.  GLOBAL=false: degree <= 2048, a 4096-slot LDS
+// table.  GLOBAL=true: any degree, a table of the smallest power of two >= 2x
+// the degree in the block's private HBM region; blocks stride over vertices.
+template <bool PLAIN, bool GLOBAL>
+__global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
+    constexpr int LSLOTS = GLOBAL ? 1 : kBlockHashSlots;
+    __shared__ i64 l_keys[LSLOTS];
+    __shared__ double l_vals[LSLOTS];
+    double part = 0;
+    for (i64 q = blockIdx.x; q < A.count; q += gridDim.x) {
+        const i64 v = A.verts[q];
+        const i64 beg = A.off[v], end = A.off[v + 1];
+        i64 *keys;
+        double *vals;
+        i64 slots;
+        if (GLOBAL) {
+            keys = A.gkeys + (i64)blockIdx.x * A.gslots;
+            vals = A.gvals + (i64)blockIdx.x * A.gslots;
+            slots = 1;
+            while (slots < 2 * (end - beg)) slots <<= 1;
+        } else {
+            keys = l_keys;
+            vals = l_vals;
+            slots = kBlockHashSlots;
+        }
+        int lg = 0;
+        while (((i64)1 << lg) < slots) lg++;
+        const int shift = 64 - lg;
+        const u64 mask = (u64)slots - 1;
+        for (i64 h = threadIdx.x; h < slots; h += kBlock) {
+            keys[h] = kEmpty;
+            vals[h] = 0;
+        }
+        if (GLOBAL) __threadfence_block();
+        __syncthreads();
+        double sl = 0;
+        for (i64 j = beg + threadIdx.x; j < end; j += kBlock) {
+            i64 t = A.tails[j];
+            double ww = A.w[j];
+            if (t == v) sl += ww;
+            hash_insert(keys, vals, mask, shift, A.C[t], ww);
+        }
+        if (GLOBAL) __threadfence_block();
+        __syncthreads();
+        sl = block_sum(sl);
+        const i64 s = A.C[v];
+        const double e_s = hash_lookup(keys, vals, mask, shift, s);
+        const double kv = A.k[v];
+        bool cand = false;
+        double gain = 0;
+        i64 id = INT64_MAX;
+        score_slots(keys, vals, slots, threadIdx.x, kBlock, A, s, kv, e_s - sl, A.a[s] - kv, cand,
+                    gain, id);
+        block_argmax(cand, gain, id);
+        if (threadIdx.x == 0) {
+            i64 target = s;
+            if (cand) {
+                target = id;
+                if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
+            }
+            A.tgt[v] = target;
+        }
+        part += e_s;
+        __syncthreads();
+    }
+    if (PLAIN && threadIdx.x == 0) A.eii_part[blockIdx.x] = part;
+}
+
+__global__ void k_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out) {
+    i64 m = 0;
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < count; q += (i64)gridDim.x * blockDim.x) {
+        const i64 v = verts[q];
+        const i64 d = off[v + 1] - off[v];
+        m = d > m ? d : m;
+    }
+    if (m) atomicMax((unsigned long long *)out, (unsigned long long)m);
+}
+
+__global__ void k_isolated_targets(const i64 *verts, i64 count, i64 *tgt) {
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < count; q += (i64)gridDim.x * blockDim.x)
+        tgt[verts[q]] = -1;
+}
+
+__device__ inline void move_vertex(i64 t, i64 s, double kv, double *a, i64 *n) {
+    atomicAdd(&a[t], kv);
+    atomicAdd(&a[s], -kv);
+    atomicAdd((unsigned long long *)&n[t], 1ULL);
+    atomicAdd((unsigned long long *)&n[s], ~0ULL);  // -1
+}
+
+// Commit of one color step: community updates of the movers, then labels.
+__global__ void k_apply_colored(const i64 *perm, Slices S, const i64 *tgt, i64 *C, const double *k,
+                                double *a, i64 *n) {
+    i64 total = 0;
+    for (int b = 0; b < kNumBins; b++) total += S.hi[b] - S.lo[b];
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < total; q += (i64)gridDim.x * blockDim.x) {
+        i64 r = q;
+        int b = 0;
+        while (r >= S.hi[b] - S.lo[b]) {
+            r -= S.hi[b] - S.lo[b];
+            b++;
+        }
+        const i64 v = perm[S.lo[b] + r];
+        const i64 t = tgt[v], s = C[v];
+        if (t != s) {
+            move_vertex(t, s, k[v], a, n);
+            C[v] = t;
+        }
+    }
+}
+// cInfo += cUpdate of a Jacobi iteration (parallelLouvainMethod.cpp:223-236, :275-279).
+__global__ void k_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a,
+                              i64 *n) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
+        const i64 t = tgt[v], s = curr[v];
+        if (t != s && t != -1) move_vertex(t, s, k[v], a, n);
+    }
+}
+
+// initCommAssOpt (utilityClusteringFunctions.cpp:108-169): one thread per
+// vertex, quirks kept (integer-truncated self-loop in selfLoop, last
+// self-loop weight as the own-cluster counter, duplicates not merged).
+__global__ void k_init_comm_ass_opt(const i64 *off, const i64 *tails, const double *w, i64 nv,
+                                    const double *k, const double *a, const i64 *n, double constant,
+                                    i64 *curr) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
+        double selfLoop = 0, cself = 0;
+        for (i64 j = off[v]; j < off[v + 1]; j++)
+            if (tails[j] == v) {
+                selfLoop += (double)(long)w[j];
+                cself = w[j];
+            }
+        i64 maxIndex = v;
+        double maxGain = 0;
+        const double eix = cself - selfLoop, ax = a[v] - k[v];
+        for (i64 j = off[v]; j < off[v + 1]; j++) {
+            i64 cid = tails[j];
+            if (cid == v) continue;
+            double g = gain_of(w[j], eix, k[v], a[cid], ax, constant);
+            if ((g > maxGain) || ((g == maxGain) && (g != 0) && (cid < maxIndex))) {
+                maxGain = g;
+                maxIndex = cid;
+            }
+        }
+        if (n[maxIndex] == 1 && n[v] == 1 && maxIndex > v) maxIndex = v;
+        curr[v] = maxIndex;
+    }
+}
+// updateAxForOpt (utilityClusteringFunctions.cpp:46-67).
+__global__ void k_update_ax_opt(i64 nv, const i64 *curr, const double *k, double *a, i64 *n) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        if (curr[v] != v) move_vertex(curr[v], v, k[v], a, n);
+}
+
+// ----------------------------------------------------------------- modularity
+// sum over vertices of the weight to their own community, G lanes per vertex.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_mod_pass(const i64 *off, const i64 *tails, const double *w,
+                                                  const i64 *C, const i64 *verts, i64 count,
+                                                  double *part) {
+    constexpr int VPB = kBlock / G;
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    double acc = 0;
+    for (i64 q = (i64)blockIdx.x * VPB + grp; q < count; q += (i64)gridDim.x * VPB) {
+        const i64 v = verts[q];
+        const i64 s = C[v];
+        if (s < 0) continue;
+        for (i64 j = off[v] + lane; j < off[v + 1]; j += G)
+            if (C[tails[j]] == s) acc += w[j];
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// ------------------------------------------------------------------- coloring
+// Packed coloring state: -1 uncoloured, else (round << 12) | color.
+constexpr int kColorBits = 12;
+
+__global__ void k_jp_partition(const i64 *off, i64 nv, i64 *small, i64 *large, i64 *counts) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
+        if (off[v + 1] - off[v] > kBinMaxDeg[6]) {
+            i64 p = (i64)atomicAdd((unsigned long long *)&counts[1], 1ULL);
+            large[p] = v;
+        } else {
+            i64 p = (i64)atomicAdd((unsigned long long *)&counts[0], 1ULL);
+            small[p] = v;
+        }
+    }
+}
+
+// A neighbour blocks v in round r when it is uncoloured at the start of the
+// round (state < 0 or coloured in round r itself) and has a higher priority.
+__device__ inline bool blocks_round(int32_t su, int round) {
+    return su < 0 || (su >> kColorBits) == round;
+}
+
+// One thread per vertex of degree <= 256.
+__global__ void k_jp_round_thread(const i64 *off, const i64 *tails, int32_t *state, const i64 *work,
+                                  i64 nwork, i64 *next, i64 *counts, int round, int32_t *err) {
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < nwork; q += (i64)gridDim.x * blockDim.x) {
+        const i64 v = work[q];
+        const u64 pv = jp_priority(v);
+        const i64 b = off[v], e = off[v + 1];
+        bool win = true;
+        for (i64 j = b; j < e; j++) {
+            const i64 u = tails[j];
+            if (u == v) continue;
+            if (blocks_round(ld_relaxed(&state[u]), round) && jp_priority(u) > pv) {
+                win = false;
+                break;
+            }
+        }
+        if (!win) {
+            i64 p = (i64)atomicAdd((unsigned long long *)&counts[0], 1ULL);
+            next[p] = v;
+            continue;
+        }
+        int color = -1;
+        for (int base = 0; base < kMaxColors && color < 0; base += 64) {
+            u64 mask = 0;
+            for (i64 j = b; j < e; j++) {
+                const i64 u = tails[j];
+                if (u == v) continue;
+                const int32_t su = ld_relaxed(&state[u]);
+                if (su < 0 || (su >> kColorBits) >= round) continue;
+                const int c = su & (kMaxColors - 1);
+                if (c >= base && c < base + 64) mask |= 1ULL << (c - base);
+            }
+            if (~mask) color = base + __ffsll((long long)~mask) - 1;
+        }
+        if (color < 0) {
+            atomicOr(err, 1);
+            color = 0;
+        }
+        __atomic_store_n(&state[v], (int32_t)((round << kColorBits) | color), __ATOMIC_RELAXED);
+    }
+}
+
+// One workgroup per vertex of degree > 256.
+__global__ __launch_bounds__(kBlock) void k_jp_round_block(const i64 *off, const i64 *tails,
+                                                           int32_t *state, const i64 *work, i64 nwork,
+                                                           i64 *next, i64 *counts, int round,
+                                                           int32_t *err) {
+    __shared__ int s_lose;
+    __shared__ uint32_t s_mark[kMaxColors / 32];
+    __shared__ int s_color;
+    for (i64 q = blockIdx.x; q < nwork; q += gridDim.x) {
+        const i64 v = work[q];
+        const u64 pv = jp_priority(v);
+        const i64 b = off[v], e = off[v + 1];
+        if (threadIdx.x == 0) {
+            s_lose = 0;
+            s_color = kMaxColors;
+        }
+        for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) s_mark[i] = 0;
+        __syncthreads();
+        for (i64 j = b + threadIdx.x; j < e; j += kBlock) {
+            if (__atomic_load_n(&s_lose, __ATOMIC_RELAXED)) break;
+            const i64 u = tails[j];
+            if (u == v) continue;
+            if (blocks_round(ld_relaxed(&state[u]), round) && jp_priority(u) > pv) {
+                __atomic_store_n(&s_lose, 1, __ATOMIC_RELAXED);
+                break;
+            }
+        }
+        __syncthreads();
+        const bool lose = s_lose != 0;
+        if (lose) {
+            if (threadIdx.x == 0) {
+                i64 p = (i64)atomicAdd((unsigned long long *)&counts[1], 1ULL);
+                next[p] = v;
+            }
+        } else {
+            for (i64 j = b + threadIdx.x; j < e; j += kBlock) {
+                const i64 u = tails[j];
+                if (u == v) continue;
+                const int32_t su = ld_relaxed(&state[u]);
+                if (su < 0 || (su >> kColorBits) >= round) continue;
+                const int c = su & (kMaxColors - 1);
+                atomicOr(&s_mark[c >> 5], 1u << (c & 31));
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
+                uint32_t m = s_mark[i];
+                if (m != 0xFFFFFFFFu) atomicMin(&s_color, i * 32 + __ffs((int)~m) - 1);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int color = s_color;
+                if (color >= kMaxColors) {
+                    atomicOr(err, 1);
+                    color = 0;
+                }
+                __atomic_store_n(&state[v], (int32_t)((round << kColorBits) | color), __ATOMIC_RELAXED);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_jp_finish(int32_t *state, i64 nv, int32_t *maxc) {
+    int m = -1;
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
+        int c = state[v] & (kMaxColors - 1);
+        state[v] = c;
+        m = c > m ? c : m;
+    }
+    atomicMax(maxc, m);
+}
+
+__global__ void k_color_conflicts(const i64 *off, const i64 *tails, i64 nv, const int32_t *colors,
+                                  i64 *count) {
+    i64 local = 0;
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        for (i64 j = off[v]; j < off[v + 1]; j++) {
+            i64 u = tails[j];
+            if (u != v && colors[u] == colors[v]) local++;
+        }
+    if (local) atomicAdd((unsigned long long *)count, (unsigned long long)local);
+}
+
+// ------------------------------------------------------------------- renumber
+__global__ void k_first_index(const i64 *C, i64 n, i64 *first, int32_t *err) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        i64 c = C[i];
+        if (c >= n) {
+            atomicOr(err, 1);
+            continue;
+        }
+        if (c >= 0) atomicMin((unsigned long long *)&first[c], (unsigned long long)i);
+    }
+}
+__global__ void k_head_flags(const i64 *C, i64 n, const i64 *first, i64 *flags) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        i64 c = C[i];
+        flags[i] = (c >= 0 && c < n && first[c] == i) ? 1 : 0;
+    }
+}
+__global__ void k_relabel(i64 *C, i64 n, const i64 *first, const i64 *pos) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        i64 c = C[i];
+        if (c >= 0 && c < n) C[i] = pos[first[c]];
+    }
+}
+// C_orig update, runMultiPhaseColoring.cpp:132-144.
+__global__ void k_compose(i64 *C_orig, i64 n0, const i64 *C, bool first_phase) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n0; i += (i64)gridDim.x * blockDim.x) {
+        if (first_phase)
+            C_orig[i] = C[i];
+        else if (C_orig[i] >= 0)
+            C_orig[i] = C[C_orig[i]];
+    }
+}
+
+// -------------------------------------------------------------------- coarsen
+// One wavefront per row: count / emit the entries i->j with C(i) >= C(j)
+// (buildNextPhase.cpp:157), keeping CSR order.
+__global__ __launch_bounds__(kBlock) void k_coarsen_count(const i64 *off, const i64 *tails, const i64 *C,
+                                                       i64 nv, i64 *cnt) {
+    const int lane = threadIdx.x % kWave;
+    for (i64 v = (i64)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < nv;
+         v += (i64)gridDim.x * (kBlock / kWave)) {
+        const i64 cv = C[v];
+        i64 c = 0;
+        if (cv >= 0)
+            for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) c += (C[tails[j]] <= cv);
+        for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor((long long)c, o);
+        if (lane == 0) cnt[v] = c;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_coarsen_emit(const i64 *off, const i64 *tails,
+                                                      const double *w, const i64 *C, i64 nv,
+                                                      const i64 *pos, i64 k, int kb, u64 *keys,
+                                                      double *vals) {
+    const int lane = threadIdx.x % kWave;
+    // seeds: one zero self-loop per cluster (buildNextPhase.cpp:122)
+    for (i64 c = blockIdx.x * (i64)kBlock + threadIdx.x; c < k; c += (i64)gridDim.x * kBlock) {
+        keys[c] = ((u64)c << kb) | (u64)c;
+        vals[c] = 0.0;
+    }
+    const u64 lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    for (i64 v = (i64)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < nv;
+         v += (i64)gridDim.x * (kBlock / kWave)) {
+        const i64 cv = C[v];
+        if (cv < 0) continue;
+        i64 base = k + pos[v];
+        for (i64 j0 = off[v]; j0 < off[v + 1]; j0 += kWave) {
+            const i64 j = j0 + lane;
+            i64 ct = -1;
+            bool p = false;
+            if (j < off[v + 1]) {
+                ct = C[tails[j]];
+                p = ct <= cv;
+            }
+            const u64 m = __ballot(p);
+            if (p) {
+                const i64 at = base + __popcll(m & lt);
+                keys[at] = ((u64)cv << kb) | (u64)ct;
+                vals[at] = w[j];
+            }
+            base += __popcll(m);
+        }
+    }
+}
+// Each unique (a,b) becomes the entry of row a, and of row b when a != b;
+// the second slot of a self-loop gets an all-ones key that sorts last.
+__global__ void k_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, int kb, u64 *keys,
+                                 double *vals) {
+    const u64 mask = ((u64)1 << kb) - 1;
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < nu; q += (i64)gridDim.x * blockDim.x) {
+        const u64 key = ukeys[q];
+        const u64 a = key >> kb, b = key & mask;
+        keys[2 * q] = key;
+        vals[2 * q] = uvals[q];
+        if (a != b) {
+            keys[2 * q + 1] = (b << kb) | a;
+            vals[2 * q + 1] = uvals[q];
+        } else {
+            keys[2 * q + 1] = ~0ULL;
+            vals[2 * q + 1] = 0.0;
+        }
+    }
+}
+__global__ void k_coarsen_rows(const u64 *keys, const double *vals, i64 total, i64 k, int kb, i64 *off,
+                               i64 *tails, double *w) {
+    const u64 mask = ((u64)1 << kb) - 1;
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < total; i += (i64)gridDim.x * blockDim.x) {
+        const u64 key = keys[i];
+        const i64 r = (i64)(key >> kb);
+        tails[i] = (i64)(key & mask);
+        w[i] = vals[i];
+        if (i == 0 || (i64)(keys[i - 1] >> kb) != r) off[r] = i;
+        if (i == 0) off[k] = total;
+    }
+}
+
+// ----------------------------------------------------------------- AoS/checks
+// Labels supplied by a caller: bit 1 = label outside [-1, limit), bit 2 = a
+// vertex with entries labelled -1 (only vertices without entries may be).
+__global__ void k_check_labels(const i64 *off, const i64 *C, i64 nv, i64 limit, int32_t *err) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
+        const i64 c = C[i];
+        if (c < -1 || c >= limit) atomicOr(err, 1);
+        else if (c == -1 && off[i + 1] > off[i]) atomicOr(err, 2);
+    }
+}
+__global__ void k_split_edges(const i64 *aos, i64 nnz, i64 *tails, double *w) {
+    for (i64 j = blockIdx.x * (i64)blockDim.x + threadIdx.x; j < nnz; j += (i64)gridDim.x * blockDim.x) {
+        tails[j] = aos[3 * j + 1];
+        w[j] = __longlong_as_double(aos[3 * j + 2]);
+    }
+}
+__global__ void k_check_graph(const i64 *off, const i64 *tails, i64 nv, i64 nnz, int32_t *err) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x)
+        if (off[i] > off[i + 1] || off[i] < 0 || off[i + 1] > nnz) atomicOr(err, 1);
+    for (i64 j = blockIdx.x * (i64)blockDim.x + threadIdx.x; j < nnz; j += (i64)gridDim.x * blockDim.x)
+        if (tails[j] < 0 || tails[j] >= nv) atomicOr(err, 2);
+}
+
+}  // namespace
+
+// =================================================================== launchers
+#define GLV_LAUNCH_CHECK() GLV_HIP(hipGetLastError())
+
+void launch_fill_i64(i64 *p, i64 n, i64 v, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill_i64<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(p, n, v);
+    GLV_LAUNCH_CHECK();
+}
+void launch_fill_f64(double *p, i64 n, double v, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill_f64<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(p, n, v);
+    GLV_LAUNCH_CHECK();
+}
+void launch_fill_i32(int32_t *p, i64 n, int32_t v, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill_i32<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(p, n, v);
+    GLV_LAUNCH_CHECK();
+}
+void launch_iota(i64 *p, i64 n, hipStream_t s) {
+    if (n <= 0) return;
+    k_iota<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(p, n);
+    GLV_LAUNCH_CHECK();
+}
+void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s) {
+    if (nv <= 0) return;
+    k_init_labels<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, C);
+    GLV_LAUNCH_CHECK();
+}
+void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys, i64 *vid,
+                     int32_t *hist, hipStream_t s) {
+    if (nv <= 0) return;
+    k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, hist);
+    GLV_LAUNCH_CHECK();
+}
+void launch_degree(const DevGraph &g, const i64 *big, i64 nbig, double *k, hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_degree_rows<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.w, g.nv, k);
+    GLV_LAUNCH_CHECK();
+    if (nbig > 0) {
+        k_degree_big<<<(unsigned)std::min<i64>(nbig, 4096), kBlock, 0, s>>>(g.off, g.w, big, nbig, k);
+        GLV_LAUNCH_CHECK();
+    }
+}
+void launch_init_comm(const double *k, i64 nv, double *a, i64 *n, hipStream_t s) {
+    if (nv <= 0) return;
+    k_init_comm<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(k, nv, a, n);
+    GLV_LAUNCH_CHECK();
+}
+void launch_comm_from_labels(const i64 *C, const double *k, i64 nv, double *a, i64 *n, hipStream_t s) {
+    launch_fill_f64(a, nv, 0.0, s);
+    launch_fill_i64(n, nv, 0, s);
+    if (nv <= 0) return;
+    k_comm_from_labels<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(C, k, nv, a, n);
+    GLV_LAUNCH_CHECK();
+}
+
+void launch_sum(const double *x, i64 n, double *part, double *out, hipStream_t s) {
+    k_red<false><<<kRedBlocks, kBlock, 0, s>>>(x, n, part);
+    GLV_LAUNCH_CHECK();
+    launch_sum_partials(part, kRedBlocks, out, s);
+}
+void launch_sumsq(const double *x, i64 n, double *part, double *out, hipStream_t s) {
+    k_red<true><<<kRedBlocks, kBlock, 0, s>>>(x, n, part);
+    GLV_LAUNCH_CHECK();
+    launch_sum_partials(part, kRedBlocks, out, s);
+}
+void launch_sum_partials(const double *part, i64 np, double *out, hipStream_t s) {
+    k_sum_partials<<<1, kBlock, 0, s>>>(part, np, out);
+    GLV_LAUNCH_CHECK();
+}
+
+i64 sweep_blocks(int bin, i64 count) {
+    if (count <= 0) return 0;
+    switch (bin) {
+        case 1: return (count + kBlock / 4 - 1) / (kBlock / 4);
+        case 2: return (count + kBlock / 8 - 1) / (kBlock / 8);
+        case 3: return (count + kBlock / 16 - 1) / (kBlock / 16);
+        case 4: return (count + kBlock / 32 - 1) / (kBlock / 32);
+        case 5: return (count + kBlock / 64 - 1) / (kBlock / 64);
+        case 6: return (count + kBlock / kWave - 1) / (kBlock / kWave);
+        case 7: return count;
+        case 8: return std::min<i64>(count, kGlobalHashBlocks);
+        default: return 0;
+    }
+}
+
+template <bool PLAIN>
+static void sweep_dispatch(int bin, const SweepArgs &A, hipStream_t s) {
+    const unsigned g = (unsigned)sweep_blocks(bin, A.count);
+    if (g == 0) return;
+    switch (bin) {
+        case 1: k_sweep_group<4, PLAIN><<<g, kBlock, 0, s>>>(A); break;
+        case 2: k_sweep_group<8, PLAIN><<<g, kBlock, 0, s>>>(A); break;
+        case 3: k_sweep_group<16, PLAIN><<<g, kBlock, 0, s>>>(A); break;
+        case 4: k_sweep_group<32, PLAIN><<<g, kBlock, 0, s>>>(A); break;
+        case 5: k_sweep_group<64, PLAIN><<<g, kBlock, 0, s>>>(A); break;
+        case 6: k_sweep_wavehash<PLAIN><<<g, kBlock, 0, s>>>(A); break;
+        case 7: k_sweep_blockhash<PLAIN, false><<<g, kBlock, 0, s>>>(A); break;
+        case 8: k_sweep_blockhash<PLAIN, true><<<g, kBlock, 0, s>>>(A); break;
+        default: return;
+    }
+    GLV_LAUNCH_CHECK();
+}
+void launch_sweep(int bin, bool plain, const SweepArgs &A, hipStream_t s) {
+    if (plain)
+        sweep_dispatch<true>(bin, A, s);
+    else
+        sweep_dispatch<false>(bin, A, s);
+}
+void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hipStream_t s) {
+    if (count <= 0) return;
+    k_max_degree<<<grid_for(count, kBlock, 4096), kBlock, 0, s>>>(off, verts, count, out);
+    GLV_LAUNCH_CHECK();
+}
+void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s) {
+    if (count <= 0) return;
+    k_isolated_targets<<<grid_for(count, kBlock, 8192), kBlock, 0, s>>>(verts, count, tgt);
+    GLV_LAUNCH_CHECK();
+}
+void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 *C, const double *k,
+                          double *a, i64 *n, hipStream_t s) {
+    i64 total = 0;
+    for (int b = 0; b < kNumBins; b++) total += S.hi[b] - S.lo[b];
+    if (total <= 0) return;
+    k_apply_colored<<<grid_for(total, kBlock, 8192), kBlock, 0, s>>>(perm, S, tgt, C, k, a, n);
+    GLV_LAUNCH_CHECK();
+}
+void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a, i64 *n,
+                        hipStream_t s) {
+    if (nv <= 0) return;
+    k_apply_plain<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(nv, tgt, curr, k, a, n);
+    GLV_LAUNCH_CHECK();
+}
+void launch_init_comm_ass_opt(const DevGraph &g, const double *k, const double *a, const i64 *n,
+                              double constant, i64 *curr, hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_init_comm_ass_opt<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.w, g.nv, k, a,
+                                                                        n, constant, curr);
+    GLV_LAUNCH_CHECK();
+}
+void launch_update_ax_opt(i64 nv, const i64 *curr, const double *k, double *a, i64 *n, hipStream_t s) {
+    if (nv <= 0) return;
+    k_update_ax_opt<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(nv, curr, k, a, n);
+    GLV_LAUNCH_CHECK();
+}
+
+i64 mod_blocks(int bin, i64 count) {
+    if (count <= 0 || bin == 0) return 0;
+    const int G = bin <= 5 ? kBinGroup[bin] : (bin == 6 ? kWave : kBlock);
+    const i64 vpb = kBlock / G;
+    return std::min<i64>((count + vpb - 1) / vpb, 8192);
+}
+void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts, i64 count, double *part,
+                     hipStream_t s) {
+    const unsigned gr = (unsigned)mod_blocks(bin, count);
+    if (gr == 0) return;
+    switch (bin) {
+        case 1: k_mod_pass<4><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
+        case 2: k_mod_pass<8><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
+        case 3: k_mod_pass<16><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
+        case 4: k_mod_pass<32><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
+        case 5:
+        case 6: k_mod_pass<64><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
+        default: k_mod_pass<256><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
+    }
+    GLV_LAUNCH_CHECK();
+}
+
+void launch_jp_partition(const i64 *off, i64 nv, i64 *small, i64 *large, i64 *counts, hipStream_t s) {
+    if (nv <= 0) return;
+    k_jp_partition<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, small, large, counts);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_round(const DevGraph &g, int32_t *state, const i64 *small, i64 nsmall, const i64 *large,
+                     i64 nlarge, i64 *nsmall_out, i64 *nlarge_out, i64 *counts, int round, int32_t *err,
+                     hipStream_t s) {
+    if (nsmall > 0) {
+        k_jp_round_thread<<<grid_for(nsmall, kBlock, 8192), kBlock, 0, s>>>(
+            g.off, g.tails, state, small, nsmall, nsmall_out, counts, round, err);
+        GLV_LAUNCH_CHECK();
+    }
+    if (nlarge > 0) {
+        k_jp_round_block<<<(unsigned)std::min<i64>(nlarge, 8192), kBlock, 0, s>>>(
+            g.off, g.tails, state, large, nlarge, nlarge_out, counts, round, err);
+        GLV_LAUNCH_CHECK();
+    }
+}
+void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s) {
+    if (nv <= 0) return;
+    k_jp_finish<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(state, nv, maxc);
+    GLV_LAUNCH_CHECK();
+}
+void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_color_conflicts<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.nv, colors, count);
+    GLV_LAUNCH_CHECK();
+}
+
+void launch_first_index(const i64 *C, i64 n, i64 *first, int32_t *err, hipStream_t s) {
+    if (n <= 0) return;
+    k_first_index<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(C, n, first, err);
+    GLV_LAUNCH_CHECK();
+}
+void launch_head_flags(const i64 *C, i64 n, const i64 *first, i64 *flags, hipStream_t s) {
+    if (n <= 0) return;
+    k_head_flags<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(C, n, first, flags);
+    GLV_LAUNCH_CHECK();
+}
+void launch_relabel(i64 *C, i64 n, const i64 *first, const i64 *pos, hipStream_t s) {
+    if (n <= 0) return;
+    k_relabel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(C, n, first, pos);
+    GLV_LAUNCH_CHECK();
+}
+void launch_compose(i64 *C_orig, i64 n0, const i64 *C, bool first_phase, hipStream_t s) {
+    if (n0 <= 0) return;
+    k_compose<<<grid_for(n0, kBlock, 8192), kBlock, 0, s>>>(C_orig, n0, C, first_phase);
+    GLV_LAUNCH_CHECK();
+}
+
+void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *cnt, hipStream_t s) {
+    if (g.nv <= 0) return;
+    const i64 vpb = kBlock / kWave;
+    k_coarsen_count<<<(unsigned)std::min<i64>((g.nv + vpb - 1) / vpb, 16384), kBlock, 0, s>>>(
+        g.off, g.tails, C, g.nv, cnt);
+    GLV_LAUNCH_CHECK();
+}
+void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *pos, i64 k, int kb, u64 *keys,
+                         double *vals, hipStream_t s) {
+    const i64 vpb = kBlock / kWave;
+    i64 blocks = std::max<i64>((g.nv + vpb - 1) / vpb, (k + kBlock - 1) / kBlock);
+    blocks = std::max<i64>(1, std::min<i64>(blocks, 16384));
+    k_coarsen_emit<<<(unsigned)blocks, kBlock, 0, s>>>(g.off, g.tails, g.w, C, g.nv, pos, k, kb, keys,
+                                                       vals);
+    GLV_LAUNCH_CHECK();
+}
+void launch_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, int kb, u64 *keys, double *vals,
+                           hipStream_t s) {
+    if (nu <= 0) return;
+    k_coarsen_mirror<<<grid_for(nu, kBlock, 8192), kBlock, 0, s>>>(ukeys, uvals, nu, kb, keys, vals);
+    GLV_LAUNCH_CHECK();
+}
+void launch_coarsen_rows(const u64 *keys, const double *vals, i64 total, i64 k, int kb, i64 *off,
+                         i64 *tails, double *w, hipStream_t s) {
+    if (total <= 0) return;
+    k_coarsen_rows<<<grid_for(total, kBlock, 8192), kBlock, 0, s>>>(keys, vals, total, k, kb, off, tails,
+                                                                    w);
+    GLV_LAUNCH_CHECK();
+}
+
+void launch_split_edges(const void *aos, i64 nnz, i64 *tails, double *w, hipStream_t s) {
+    if (nnz <= 0) return;
+    k_split_edges<<<grid_for(nnz, kBlock, 8192), kBlock, 0, s>>>((const i64 *)aos, nnz, tails, w);
+    GLV_LAUNCH_CHECK();
+}
+void launch_check_labels(const i64 *off, const i64 *C, i64 nv, i64 limit, int32_t *err, hipStream_t s) {
+    if (nv <= 0) return;
+    k_check_labels<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, C, nv, limit, err);
+    GLV_LAUNCH_CHECK();
+}
+void launch_check_graph(const i64 *off, const i64 *tails, i64 nv, i64 nnz, int32_t *err, hipStream_t s) {
+    i64 n = std::max(nv, nnz);
+    if (n <= 0) return;
+    k_check_graph<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(off, tails, nv, nnz, err);
+    GLV_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------- rocPRIM
+void scan_exclusive_i64(const i64 *in, i64 *out, i64 n, hipStream_t s) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    GLV_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (i64)0, (size_t)n, rocprim::plus<i64>(), s));
+    DBuf<char> tmp(bytes, s);
+    GLV_HIP(rocprim::exclusive_scan(tmp.p, bytes, in, out, (i64)0, (size_t)n, rocprim::plus<i64>(), s));
+}
+void sort_pairs_u32_i64(uint32_t *keys, i64 *vals, i64 n, int end_bit, hipStream_t s) {
+    if (n <= 1) return;
+    DBuf<uint32_t> k2(n, s);
+    DBuf<i64> v2(n, s);
+    size_t bytes = 0;
+    GLV_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys, k2.p, vals, v2.p, (size_t)n, 0, end_bit, s));
+    DBuf<char> tmp(bytes, s);
+    GLV_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, keys, k2.p, vals, v2.p, (size_t)n, 0, end_bit, s));
+    GLV_HIP(hipMemcpyAsync(keys, k2.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    GLV_HIP(hipMemcpyAsync(vals, v2.p, n * sizeof(i64), hipMemcpyDeviceToDevice, s));
+}
+void sort_pairs_u64_f64(u64 *keys, double *vals, i64 n, int end_bit, hipStream_t s) {
+    if (n <= 1) return;
+    DBuf<u64> k2(n, s);
+    DBuf<double> v2(n, s);
+    size_t bytes = 0;
+    GLV_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys, k2.p, vals, v2.p, (size_t)n, 0, end_bit, s));
+    DBuf<char> tmp(bytes, s);
+    GLV_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, keys, k2.p, vals, v2.p, (size_t)n, 0, end_bit, s));
+    GLV_HIP(hipMemcpyAsync(keys, k2.p, n * sizeof(u64), hipMemcpyDeviceToDevice, s));
+    GLV_HIP(hipMemcpyAsync(vals, v2.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+i64 reduce_by_key_u64_f64(const u64 *keys, const double *vals, i64 n, u64 *ukeys, double *usums,
+                          hipStream_t s) {
+    if (n <= 0) return 0;
+    DBuf<size_t> cnt(1, s);
+    size_t bytes = 0;
+    GLV_HIP(rocprim::reduce_by_key(nullptr, bytes, keys, vals, (size_t)n, ukeys, usums, cnt.p,
+                                   rocprim::plus<double>(), rocprim::equal_to<u64>(), s));
+    DBuf<char> tmp(bytes, s);
+    GLV_HIP(rocprim::reduce_by_key(tmp.p, bytes, keys, vals, (size_t)n, ukeys, usums, cnt.p,
+                                   rocprim::plus<double>(), rocprim::equal_to<u64>(), s));
+    size_t h = 0;
+    GLV_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(size_t), hipMemcpyDeviceToHost, s));
+    GLV_HIP(hipStreamSynchronize(s));
+    return (i64)h;
+}
+
+}  // namespace glv

@@ -1,0 +1,117 @@
+// kernels.hpp -- host-side launchers of the HIP kernels in kernels.hip.
+// Every launcher enqueues on `s` and returns immediately.
+#pragma once
+
+#include "common.hpp"
+
+namespace glv {
+
+// Arguments shared by the sweep kernels.  `C`, `a`, `n` are the frozen state
+// of the current color step (or Jacobi iteration); `tgt[v]` receives the
+// chosen community of every vertex in `verts[0..count)`.
+struct SweepArgs {
+    const i64 *off;
+    const i64 *tails;
+    const double *w;
+    const i64 *C;
+    const double *k;
+    const double *a;
+    const i64 *n;
+    double constant;
+    const i64 *verts;
+    i64 count;
+    i64 *tgt;
+    double *eii_part;  // plain sweeps: per-block sums of e_{i,C(i)}, else null
+    i64 *gkeys;        // global hash (bin 8): per-block regions of gslots slots
+    double *gvals;
+    i64 gslots;
+};
+
+// Up to one slice per degree bin of a color class (perm[lo[b] .. hi[b])).
+struct Slices {
+    i64 lo[kNumBins];
+    i64 hi[kNumBins];
+};
+
+// --- elementwise / setup -------------------------------------------------
+void launch_fill_i64(i64 *p, i64 n, i64 v, hipStream_t s);
+void launch_fill_f64(double *p, i64 n, double v, hipStream_t s);
+void launch_fill_i32(int32_t *p, i64 n, int32_t v, hipStream_t s);
+void launch_iota(i64 *p, i64 n, hipStream_t s);
+void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s);  // C[i] = deg? i : -1
+void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
+                     i64 *vid, int32_t *hist, hipStream_t s);
+void launch_degree(const DevGraph &g, const i64 *big, i64 nbig, double *k, hipStream_t s);
+void launch_init_comm(const double *k, i64 nv, double *a, i64 *n, hipStream_t s);
+void launch_comm_from_labels(const i64 *C, const double *k, i64 nv, double *a, i64 *n,
+                             hipStream_t s);
+
+// --- reductions ----------------------------------------------------------
+constexpr int kRedBlocks = 1024;
+// out[0] = sum(x[0..n)), two-level: kRedBlocks block partials into part, then one block
+void launch_sum(const double *x, i64 n, double *part, double *out, hipStream_t s);
+void launch_sumsq(const double *x, i64 n, double *part, double *out, hipStream_t s);
+void launch_sum_partials(const double *part, i64 np, double *out, hipStream_t s);
+
+// --- sweeps --------------------------------------------------------------
+// number of block partials the sweep of `bin` writes for `count` vertices
+i64 sweep_blocks(int bin, i64 count);
+void launch_sweep(int bin, bool plain, const SweepArgs &A, hipStream_t s);
+void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hipStream_t s);
+constexpr int kGlobalHashBlocks = 256;
+void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s);
+void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 *C, const double *k,
+                          double *a, i64 *n, hipStream_t s);
+void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a, i64 *n,
+                        hipStream_t s);
+void launch_init_comm_ass_opt(const DevGraph &g, const double *k, const double *a, const i64 *n,
+                              double constant, i64 *curr, hipStream_t s);
+void launch_update_ax_opt(i64 nv, const i64 *curr, const double *k, double *a, i64 *n,
+                          hipStream_t s);
+
+// --- modularity ----------------------------------------------------------
+i64 mod_blocks(int bin, i64 count);
+void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts, i64 count,
+                     double *part, hipStream_t s);
+
+// --- coloring ------------------------------------------------------------
+void launch_jp_partition(const i64 *off, i64 nv, i64 *small, i64 *large, i64 *counts,
+                         hipStream_t s);
+void launch_jp_round(const DevGraph &g, int32_t *state, const i64 *small, i64 nsmall,
+                     const i64 *large, i64 nlarge, i64 *nsmall_out, i64 *nlarge_out, i64 *counts,
+                     int round, int32_t *err, hipStream_t s);
+void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
+void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);
+
+// --- renumber / compose --------------------------------------------------
+void launch_first_index(const i64 *C, i64 n, i64 *first, int32_t *err, hipStream_t s);
+void launch_head_flags(const i64 *C, i64 n, const i64 *first, i64 *flags, hipStream_t s);
+void launch_relabel(i64 *C, i64 n, const i64 *first, const i64 *pos, hipStream_t s);
+void launch_compose(i64 *C_orig, i64 n0, const i64 *C, bool first_phase, hipStream_t s);
+
+// --- coarsen -------------------------------------------------------------
+void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *cnt, hipStream_t s);
+void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *pos, i64 k, int kb,
+                         u64 *keys, double *vals, hipStream_t s);
+void launch_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, int kb, u64 *keys,
+                           double *vals, hipStream_t s);
+void launch_coarsen_rows(const u64 *keys, const double *vals, i64 total, i64 k, int kb, i64 *off,
+                         i64 *tails, double *w, hipStream_t s);
+
+// --- AoS <-> SoA ---------------------------------------------------------
+void launch_split_edges(const void *aos, i64 nnz, i64 *tails, double *w, hipStream_t s);
+void launch_check_labels(const i64 *off, const i64 *C, i64 nv, i64 limit, int32_t *err,
+                         hipStream_t s);
+void launch_check_graph(const i64 *off, const i64 *tails, i64 nv, i64 nnz, int32_t *err,
+                        hipStream_t s);
+
+// --- scans / sorts (rocPRIM) ---------------------------------------------
+// exclusive prefix sum of in[0..n) into out[0..n); returns nothing, total = out[n-1]+in[n-1]
+void scan_exclusive_i64(const i64 *in, i64 *out, i64 n, hipStream_t s);
+void sort_pairs_u32_i64(uint32_t *keys, i64 *vals, i64 n, int end_bit, hipStream_t s);
+void sort_pairs_u64_f64(u64 *keys, double *vals, i64 n, int end_bit, hipStream_t s);
+// returns number of unique keys (synchronises)
+i64 reduce_by_key_u64_f64(const u64 *keys, const double *vals, i64 n, u64 *ukeys, double *usums,
+                          hipStream_t s);
+
+}  // namespace glv

@@ -1,0 +1,585 @@
+// louvain.cpp -- host orchestration of the engine (see louvain.hpp).
+#include "louvain.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace glv {
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+namespace {
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int bits_for(i64 x) {  // smallest b with 2^b > x
+    int b = 0;
+    while (b < 63 && ((i64)1 << b) <= x) b++;
+    return b;
+}
+
+// Iterations of one phase never exceed this; the reference has no bound and
+// a phase whose modularity keeps rising by at least `thresh` per iteration
+// could only run longer than this with thresh below ~1e-5.
+constexpr int kMaxPhaseIters = 100000;
+
+}  // namespace
+
+template <class T>
+T Engine::fetch(const T *dptr) {
+    T h;
+    GLV_HIP(hipMemcpyAsync(&h, dptr, sizeof(T), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipStreamSynchronize(s_));
+    return h;
+}
+
+void Engine::sync() { GLV_HIP(hipStreamSynchronize(s_)); }
+
+void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetup &P) {
+    const i64 nv = g.nv;
+    P.ncol = ncol;
+    const i64 nkeys = (i64)kNumBins * ncol;
+    P.k.alloc(nv, s_);
+    P.a.alloc(nv, s_);
+    P.n.alloc(nv, s_);
+    P.perm.alloc(nv, s_);
+    DBuf<uint32_t> keys(nv, s_);
+    DBuf<int32_t> hist(nkeys, s_);
+    launch_fill_i32(hist.p, nkeys, 0, s_);
+    launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, hist.p, s_);
+    sort_pairs_u32_i64(keys.p, P.perm.p, nv, bits_for(nkeys), s_);
+    std::vector<int32_t> h(nkeys);
+    GLV_HIP(hipMemcpyAsync(h.data(), hist.p, nkeys * sizeof(int32_t), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipStreamSynchronize(s_));
+    P.ptr.assign(nkeys + 1, 0);
+    for (i64 q = 0; q < nkeys; q++) P.ptr[q + 1] = P.ptr[q] + h[q];
+    if (P.ptr[nkeys] != nv) fail(GLV_E_INVALID_ARG, "color values out of range [0, nColors)");
+    // degrees and 1/(2m)
+    const i64 big_lo = P.bin_lo(8), big_hi = P.bin_hi(8);
+    launch_degree(g, P.perm.p + big_lo, big_hi - big_lo, P.k.p, s_);
+    DBuf<double> part(kRedBlocks, s_), res(1, s_);
+    launch_sum(P.k.p, nv, part.p, res.p, s_);
+    P.total_weight = fetch(res.p);
+    P.constant = (double)1 / P.total_weight;
+    P.maxdeg_global = 0;
+    if (big_hi > big_lo) {
+        DBuf<i64> md(1, s_);
+        launch_fill_i64(md.p, 1, 0, s_);
+        launch_max_degree(g.off, P.perm.p + big_lo, big_hi - big_lo, md.p, s_);
+        P.maxdeg_global = fetch(md.p);
+    }
+}
+
+// ----------------------------------------------------------------- coloring
+int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
+    const i64 nv = g.nv;
+    if (nv == 0) return -1;
+    launch_fill_i32(colors, nv, -1, s_);
+    DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), counts(2, s_);
+    DBuf<int32_t> err(1, s_), maxc(1, s_);
+    launch_fill_i64(counts.p, 2, 0, s_);
+    launch_fill_i32(err.p, 1, 0, s_);
+    launch_fill_i32(maxc.p, 1, -1, s_);
+    launch_jp_partition(g.off, nv, sA.p, lA.p, counts.p, s_);
+    i64 hc[2];
+    GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipStreamSynchronize(s_));
+    i64 ns = hc[0], nl = hc[1];
+    i64 *small = sA.p, *large = lA.p, *small_next = sB.p, *large_next = lB.p;
+    int round = 0;
+    while (ns + nl > 0) {
+        if (round >= (1 << 18)) fail(GLV_E_DEVICE, "coloring did not converge");
+        launch_fill_i64(counts.p, 2, 0, s_);
+        launch_jp_round(g, colors, small, ns, large, nl, small_next, large_next, counts.p, round, err.p, s_);
+        GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipStreamSynchronize(s_));
+        ns = hc[0];
+        nl = hc[1];
+        std::swap(small, small_next);
+        std::swap(large, large_next);
+        round++;
+    }
+    launch_jp_finish(colors, nv, maxc.p, s_);
+    if (fetch(err.p)) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
+    if (rounds_out) *rounds_out = round;
+    return fetch(maxc.p);
+}
+
+i64 Engine::count_conflicts(const DevGraph &g, const int32_t *colors) {
+    DBuf<i64> c(1, s_);
+    launch_fill_i64(c.p, 1, 0, s_);
+    launch_color_conflicts(g, colors, c.p, s_);
+    return fetch(c.p) / 2;
+}
+
+// --------------------------------------------------------------- modularity
+// e_xx over all vertices (per-bin passes into block partials), sum of a_c^2.
+double Engine::colored_modularity(const DevGraph &g, const PhaseSetup &P, const i64 *C, double *part,
+                                  double *res) {
+    i64 off = 0;
+    for (int b = 1; b < kNumBins; b++) {
+        const i64 lo = P.bin_lo(b), cnt = P.bin_hi(b) - lo;
+        if (cnt <= 0) continue;
+        launch_mod_pass(b, g, C, P.perm.p + lo, cnt, part + off, s_);
+        off += mod_blocks(b, cnt);
+    }
+    if (off == 0) launch_fill_f64(res, 1, 0.0, s_);
+    else launch_sum_partials(part, off, res, s_);
+    launch_sumsq(P.a.p, g.nv, part + off, res + 1, s_);
+    double h[2];
+    GLV_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipStreamSynchronize(s_));
+    const double c = P.constant;
+    return h[0] * (double)c - h[1] * (double)c * (double)c;
+}
+
+static i64 mod_part_count(const PhaseSetup &P) {
+    i64 n = 0;
+    for (int b = 1; b < kNumBins; b++) n += mod_blocks(b, P.bin_hi(b) - P.bin_lo(b));
+    return n + kRedBlocks;
+}
+
+// ----------------------------------------------------------- colored phase
+// algoLouvainWithDistOneColoring, Coloring/parallelLouvainWithColoring.cpp:47-286.
+double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, double Lower,
+                             double thresh, int *iters, std::vector<double> *trace) {
+    const i64 nv = g.nv;
+    PhaseSetup P;
+    setup(g, colors, ncol, P);
+    launch_init_labels(g.off, nv, C, s_);
+    launch_init_comm(P.k.p, nv, P.a.p, P.n.p, s_);
+    DBuf<i64> tgt(nv, s_);
+    DBuf<double> part(mod_part_count(P), s_), res(2, s_);
+    DBuf<i64> gkeys;
+    DBuf<double> gvals;
+    i64 gslots = 0;
+    if (P.maxdeg_global > 0) {
+        gslots = 1;
+        while (gslots < 2 * P.maxdeg_global) gslots <<= 1;
+        const i64 nb = std::min<i64>(kGlobalHashBlocks, P.bin_hi(8) - P.bin_lo(8));
+        gkeys.alloc(nb * gslots, s_);
+        gvals.alloc(nb * gslots, s_);
+    }
+    SweepArgs A;
+    A.off = g.off;
+    A.tails = g.tails;
+    A.w = g.w;
+    A.C = C;
+    A.k = P.k.p;
+    A.a = P.a.p;
+    A.n = P.n.p;
+    A.constant = P.constant;
+    A.tgt = tgt.p;
+    A.eii_part = nullptr;
+    A.gkeys = gkeys.p;
+    A.gvals = gvals.p;
+    A.gslots = gslots;
+
+    double prevMod = Lower, currMod = -1;
+    int numItrs = 0;
+    while (true) {
+        numItrs++;
+        for (int c = 0; c < ncol; c++) {
+            Slices S;
+            S.lo[0] = S.hi[0] = 0;  // isolated vertices were labelled -1 at init
+            for (int b = 1; b < kNumBins; b++) {
+                S.lo[b] = P.lo(b, c);
+                S.hi[b] = P.hi(b, c);
+                if (S.hi[b] <= S.lo[b]) continue;
+                A.verts = P.perm.p + S.lo[b];
+                A.count = S.hi[b] - S.lo[b];
+                launch_sweep(b, false, A, s_);
+            }
+            launch_apply_colored(P.perm.p, S, tgt.p, C, P.k.p, P.a.p, P.n.p, s_);
+        }
+        currMod = colored_modularity(g, P, C, part.p, res.p);
+        if (trace) trace->push_back(currMod);
+        if ((currMod - prevMod) < thresh) break;  // :260
+        if (numItrs >= kMaxPhaseIters) break;
+        prevMod = currMod;
+    }
+    if (iters) *iters = numItrs;
+    return prevMod;  // :284
+}
+
+// ------------------------------------------------------------- plain phase
+// parallelLouvianMethod / parallelLouvianMethodNoMap,
+// BasicCommunitiesDetection/parallelLouvainMethod.cpp:65-326 and
+// parallelLouvainMethodNoMap.cpp:48-268.
+double Engine::plain_phase(const DevGraph &g, i64 *C, double Lower, double thresh, bool nomap,
+                           int *iters, std::vector<double> *trace) {
+    const i64 nv = g.nv;
+    PhaseSetup P;
+    setup(g, nullptr, 1, P);
+    launch_init_comm(P.k.p, nv, P.a.p, P.n.p, s_);
+    DBuf<i64> b0(nv, s_), b1(nv, s_), b2(nv, s_);
+    i64 *past = b0.p, *curr = b1.p, *tgt = b2.p;
+    launch_iota(past, nv, s_);
+    if (nomap) {
+        launch_init_comm_ass_opt(g, P.k.p, P.a.p, P.n.p, P.constant, curr, s_);
+        launch_update_ax_opt(nv, curr, P.k.p, P.a.p, P.n.p, s_);
+    } else {
+        launch_iota(curr, nv, s_);
+    }
+    i64 nparts = kRedBlocks;
+    for (int b = 1; b < kNumBins; b++) nparts += sweep_blocks(b, P.bin_hi(b) - P.bin_lo(b));
+    DBuf<double> part(nparts, s_), res(2, s_);
+    DBuf<i64> gkeys;
+    DBuf<double> gvals;
+    i64 gslots = 0;
+    if (P.maxdeg_global > 0) {
+        gslots = 1;
+        while (gslots < 2 * P.maxdeg_global) gslots <<= 1;
+        const i64 nb = std::min<i64>(kGlobalHashBlocks, P.bin_hi(8) - P.bin_lo(8));
+        gkeys.alloc(nb * gslots, s_);
+        gvals.alloc(nb * gslots, s_);
+    }
+    double prevMod = -1, currMod = -1;
+    int numItrs = 0;
+    while (true) {
+        numItrs++;
+        SweepArgs A;
+        A.off = g.off;
+        A.tails = g.tails;
+        A.w = g.w;
+        A.C = curr;
+        A.k = P.k.p;
+        A.a = P.a.p;
+        A.n = P.n.p;
+        A.constant = P.constant;
+        A.tgt = tgt;
+        A.gkeys = gkeys.p;
+        A.gvals = gvals.p;
+        A.gslots = gslots;
+        i64 off = 0;
+        for (int b = 1; b < kNumBins; b++) {
+            const i64 lo = P.bin_lo(b), cnt = P.bin_hi(b) - lo;
+            if (cnt <= 0) continue;
+            A.verts = P.perm.p + lo;
+            A.count = cnt;
+            A.eii_part = part.p + off;
+            launch_sweep(b, true, A, s_);
+            off += sweep_blocks(b, cnt);
+        }
+        launch_isolated_targets(P.perm.p + P.bin_lo(0), P.bin_hi(0) - P.bin_lo(0), tgt, s_);
+        if (off == 0) launch_fill_f64(res.p, 1, 0.0, s_);
+        else launch_sum_partials(part.p, off, res.p, s_);
+        launch_sumsq(P.a.p, nv, part.p + off, res.p + 1, s_);
+        double h[2];
+        GLV_HIP(hipMemcpyAsync(h, res.p, sizeof(h), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipStreamSynchronize(s_));
+        const double c = P.constant;
+        currMod = (h[0] * (double)c) - (h[1] * (double)c * (double)c);
+        if (trace) trace->push_back(currMod);
+        if ((currMod - prevMod) < thresh) break;  // :266
+        if (numItrs >= kMaxPhaseIters) break;
+        prevMod = currMod;
+        if (prevMod < Lower) prevMod = Lower;  // :272-273
+        launch_apply_plain(nv, tgt, curr, P.k.p, P.a.p, P.n.p, s_);
+        i64 *t = past;
+        past = curr;
+        curr = tgt;
+        tgt = t;
+    }
+    GLV_HIP(hipMemcpyAsync(C, past, nv * sizeof(i64), hipMemcpyDeviceToDevice, s_));  // :304-307
+    if (iters) *iters = numItrs;
+    return prevMod;
+}
+
+i64 Engine::color_step(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, int which) {
+    const i64 nv = g.nv;
+    PhaseSetup P;
+    setup(g, colors, ncol, P);
+    launch_comm_from_labels(C, P.k.p, nv, P.a.p, P.n.p, s_);
+    DBuf<i64> tgt(nv, s_);
+    DBuf<i64> gkeys;
+    DBuf<double> gvals;
+    i64 gslots = 0;
+    if (P.maxdeg_global > 0) {
+        gslots = 1;
+        while (gslots < 2 * P.maxdeg_global) gslots <<= 1;
+        const i64 nb = std::min<i64>(kGlobalHashBlocks, P.bin_hi(8) - P.bin_lo(8));
+        gkeys.alloc(nb * gslots, s_);
+        gvals.alloc(nb * gslots, s_);
+    }
+    SweepArgs A;
+    A.off = g.off;
+    A.tails = g.tails;
+    A.w = g.w;
+    A.C = C;
+    A.k = P.k.p;
+    A.a = P.a.p;
+    A.n = P.n.p;
+    A.constant = P.constant;
+    A.tgt = tgt.p;
+    A.eii_part = nullptr;
+    A.gkeys = gkeys.p;
+    A.gvals = gvals.p;
+    A.gslots = gslots;
+    Slices S;
+    S.lo[0] = S.hi[0] = 0;
+    for (int b = 1; b < kNumBins; b++) {
+        S.lo[b] = P.lo(b, which);
+        S.hi[b] = P.hi(b, which);
+        if (S.hi[b] <= S.lo[b]) continue;
+        A.verts = P.perm.p + S.lo[b];
+        A.count = S.hi[b] - S.lo[b];
+        launch_sweep(b, false, A, s_);
+    }
+    // count movers before committing: compare on the host after the commit
+    DBuf<i64> before(nv, s_);
+    GLV_HIP(hipMemcpyAsync(before.p, C, nv * sizeof(i64), hipMemcpyDeviceToDevice, s_));
+    launch_apply_colored(P.perm.p, S, tgt.p, C, P.k.p, P.a.p, P.n.p, s_);
+    // isolated vertices of this color become -1 (parallelLouvainWithColoring.cpp:189-191)
+    launch_isolated_targets(P.perm.p + P.lo(0, which), P.hi(0, which) - P.lo(0, which), C, s_);
+    std::vector<i64> hb(nv), ha(nv);
+    GLV_HIP(hipMemcpyAsync(hb.data(), before.p, nv * sizeof(i64), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipMemcpyAsync(ha.data(), C, nv * sizeof(i64), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipStreamSynchronize(s_));
+    i64 moves = 0;
+    for (i64 i = 0; i < nv; i++) moves += hb[i] != ha[i];
+    return moves;
+}
+
+// ----------------------------------------------------------------- renumber
+// renumberClustersContiguously, Utility/buildNextPhase.cpp:48-79: new id =
+// rank of the community's first index; -1 stays -1.
+i64 Engine::renumber(i64 *C, i64 n) {
+    if (n <= 0) return 0;
+    DBuf<i64> first(n, s_), flags(n, s_), pos(n, s_);
+    DBuf<int32_t> err(1, s_);
+    launch_fill_i32(err.p, 1, 0, s_);
+    launch_fill_i64(first.p, n, n, s_);
+    launch_first_index(C, n, first.p, err.p, s_);
+    launch_head_flags(C, n, first.p, flags.p, s_);
+    scan_exclusive_i64(flags.p, pos.p, n, s_);
+    launch_relabel(C, n, first.p, pos.p, s_);
+    i64 last[2];
+    GLV_HIP(hipMemcpyAsync(&last[0], pos.p + n - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipMemcpyAsync(&last[1], flags.p + n - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
+    int32_t e = 0;
+    GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipStreamSynchronize(s_));
+    if (e) fail(GLV_E_INVALID_ARG, "cluster id >= number of vertices");
+    return last[0] + last[1];
+}
+
+// ------------------------------------------------------------------ coarsen
+// buildNextLevelGraphOpt, Utility/buildNextPhase.cpp:83-255, as sort plus
+// segmented reduce.  Output rows are sorted by tail.
+void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out) {
+    if (k <= 0) fail(GLV_E_INVALID_ARG, "coarsening needs at least one cluster");
+    const int kb = bits_for(k);
+    if (2 * kb > 64) fail(GLV_E_UNSUPPORTED, "more than 2^32 clusters");
+    const i64 nv = g.nv;
+    DBuf<i64> cnt(std::max<i64>(nv, 1), s_), pos(std::max<i64>(nv, 1), s_);
+    launch_coarsen_count(g, C, cnt.p, s_);
+    scan_exclusive_i64(cnt.p, pos.p, nv, s_);
+    i64 m = 0;
+    if (nv > 0) {
+        i64 h[2];
+        GLV_HIP(hipMemcpyAsync(&h[0], pos.p + nv - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipMemcpyAsync(&h[1], cnt.p + nv - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipStreamSynchronize(s_));
+        m = h[0] + h[1];
+    }
+    const i64 tot = m + k;
+    DBuf<u64> keys(tot, s_);
+    DBuf<double> vals(tot, s_);
+    launch_coarsen_emit(g, C, pos.p, k, kb, keys.p, vals.p, s_);
+    cnt.release();
+    pos.release();
+    sort_pairs_u64_f64(keys.p, vals.p, tot, 2 * kb, s_);
+    DBuf<u64> ukeys(tot, s_);
+    DBuf<double> usums(tot, s_);
+    const i64 nu = reduce_by_key_u64_f64(keys.p, vals.p, tot, ukeys.p, usums.p, s_);
+    keys.release();
+    vals.release();
+    DBuf<u64> k2(2 * nu, s_);
+    DBuf<double> v2(2 * nu, s_);
+    launch_coarsen_mirror(ukeys.p, usums.p, nu, kb, k2.p, v2.p, s_);
+    ukeys.release();
+    usums.release();
+    sort_pairs_u64_f64(k2.p, v2.p, 2 * nu, 2 * kb, s_);
+    const i64 total = 2 * nu - k;
+    out.nv = k;
+    out.nnz = total;
+    out.num_edges = nu;  // NE_out + k (buildNextPhase.cpp:237)
+    out.off_buf.alloc(k + 1, s_);
+    out.tails_buf.alloc(total, s_);
+    out.w_buf.alloc(total, s_);
+    out.adopt();
+    launch_coarsen_rows(k2.p, v2.p, total, k, kb, out.off_buf.p, out.tails_buf.p, out.w_buf.p, s_);
+}
+
+double Engine::modularity(const DevGraph &g, const i64 *C) {
+    PhaseSetup P;
+    setup(g, nullptr, 1, P);
+    launch_comm_from_labels(C, P.k.p, g.nv, P.a.p, P.n.p, s_);
+    DBuf<double> part(mod_part_count(P), s_), res(2, s_);
+    return colored_modularity(g, P, C, part.p, res.p);
+}
+
+// ------------------------------------------------------------ multi-phase
+// runMultiPhaseColoring, Coloring/runMultiPhaseColoring.cpp:51-238 (coloring == 1).
+void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, RunResult &r) {
+    if (p.coloring != 1) fail(GLV_E_UNSUPPORTED, "only coloring == 1 (distance-1) is implemented");
+    const double t_start = now();
+    const i64 NV = g0.nv;
+    r = RunResult();
+    if (NV == 0) return;
+    DBuf<int32_t> colors(NV, s_);
+    DBuf<i64> Cbuf(NV, s_);
+    double t0 = now();
+    int nColors = color(g0, colors.p) + 1;
+    r.t_coloring += now() - t0;
+    r.colors_first = nColors;
+
+    const DevGraph *G = &g0;
+    DevGraph levels[2];
+    int cur = -1;  // which `levels` entry holds G (-1: the input)
+    double prevMod = -1, currMod = -1;
+    int phase = 1, totItr = 0;
+    i64 numClusters = 0;
+    bool nonColor = false;
+    while (true) {
+        prevMod = currMod;
+        PhaseRecord rec;
+        rec.nv = G->nv;
+        rec.nnz = G->nnz;
+        int tmpItr = 0;
+        t0 = now();
+        if (!nonColor) {
+            currMod = colored_phase(*G, Cbuf.p, colors.p, nColors, currMod, p.C_threshold, &tmpItr,
+                                    &r.trace);
+            rec.colored = 1;
+            rec.ncolors = nColors;
+            r.edges_visited_mod += (i64)tmpItr * G->nnz;
+        } else {
+            currMod = plain_phase(*G, Cbuf.p, currMod, p.threshold, p.replaceMap == 1, &tmpItr,
+                                  &r.trace);
+        }
+        rec.t_sweep = now() - t0;
+        r.t_clustering += rec.t_sweep;
+        totItr += tmpItr;
+        rec.iters = tmpItr;
+        rec.mod = currMod;
+        r.edges_visited += (i64)tmpItr * G->nnz;
+        t0 = now();
+        numClusters = renumber(Cbuf.p, G->nv);
+        launch_compose(C_orig, NV, Cbuf.p, phase == 1, s_);
+        rec.t_renumber = now() - t0;
+        r.t_renumber += rec.t_renumber;
+        rec.clusters = numClusters;
+        bool stop = (phase > p.max_phases) || (totItr > p.max_total_iters);
+        if (!stop && (currMod - prevMod) > p.threshold && numClusters > 0) {
+            t0 = now();
+            const int nxt = cur == 0 ? 1 : 0;
+            coarsen(*G, Cbuf.p, numClusters, levels[nxt]);
+            sync();
+            if (cur >= 0) {
+                levels[cur].off_buf.release();
+                levels[cur].tails_buf.release();
+                levels[cur].w_buf.release();
+            }
+            cur = nxt;
+            G = &levels[cur];
+            rec.t_coarsen = now() - t0;
+            r.t_building += rec.t_coarsen;
+            launch_fill_i64(Cbuf.p, G->nv, -1, s_);
+            phase++;
+            if (G->nv > p.minGraphSize && !nonColor) {
+                t0 = now();
+                nColors = color(*G, colors.p) + 1;
+                rec.t_color = now() - t0;
+                r.t_coloring += rec.t_color;
+            }
+            r.phase.push_back(rec);
+            continue;
+        }
+        r.phase.push_back(rec);
+        if (stop) break;
+        if (!nonColor && numClusters > 0)
+            nonColor = true;
+        else
+            break;
+    }
+    sync();
+    r.phases = phase;
+    r.total_iterations = totItr;
+    r.final_clusters = numClusters;
+    r.final_modularity = prevMod;
+    r.last_modularity = currMod;
+    r.t_total = now() - t_start;
+}
+
+// runMultiPhaseBasic, BasicCommunitiesDetection/runMultiPhaseBasic.cpp:52-170.
+void Engine::run_basic(const DevGraph &g0, i64 *C_orig, const RunParams &p, RunResult &r) {
+    const double t_start = now();
+    const i64 NV = g0.nv;
+    r = RunResult();
+    if (NV == 0) return;
+    DBuf<i64> Cbuf(NV, s_);
+    const DevGraph *G = &g0;
+    DevGraph levels[2];
+    int cur = -1;
+    double prevMod = -1, currMod = -1;
+    int phase = 1, totItr = 0;
+    i64 numClusters = 0;
+    while (true) {
+        prevMod = currMod;
+        PhaseRecord rec;
+        rec.nv = G->nv;
+        rec.nnz = G->nnz;
+        int tmpItr = 0;
+        double t0 = now();
+        currMod = plain_phase(*G, Cbuf.p, currMod, p.threshold, p.replaceMap == 1, &tmpItr, &r.trace);
+        rec.t_sweep = now() - t0;
+        r.t_clustering += rec.t_sweep;
+        totItr += tmpItr;
+        rec.iters = tmpItr;
+        rec.mod = currMod;
+        r.edges_visited += (i64)tmpItr * G->nnz;
+        t0 = now();
+        numClusters = renumber(Cbuf.p, G->nv);
+        launch_compose(C_orig, NV, Cbuf.p, phase == 1, s_);
+        rec.t_renumber = now() - t0;
+        r.t_renumber += rec.t_renumber;
+        rec.clusters = numClusters;
+        r.phase.push_back(rec);
+        if ((phase > p.max_phases) || (totItr > p.max_total_iters)) break;
+        if ((currMod - prevMod) > p.threshold && numClusters > 0) {
+            t0 = now();
+            const int nxt = cur == 0 ? 1 : 0;
+            coarsen(*G, Cbuf.p, numClusters, levels[nxt]);
+            sync();
+            if (cur >= 0) {
+                levels[cur].off_buf.release();
+                levels[cur].tails_buf.release();
+                levels[cur].w_buf.release();
+            }
+            cur = nxt;
+            G = &levels[cur];
+            r.t_building += now() - t0;
+            r.phase.back().t_coarsen = now() - t0;
+            launch_fill_i64(Cbuf.p, G->nv, -1, s_);
+            phase++;
+        } else {
+            break;
+        }
+    }
+    sync();
+    r.phases = phase;
+    r.total_iterations = totItr;
+    r.final_clusters = numClusters;
+    r.final_modularity = prevMod;
+    r.last_modularity = currMod;
+    r.t_total = now() - t_start;
+}
+
+}  // namespace glv

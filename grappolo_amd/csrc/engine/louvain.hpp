@@ -1,0 +1,102 @@
+// louvain.hpp -- host orchestration of the engine: one phase of the colored
+// sweep, one phase of the plain sweep, renumbering, coarsening, coloring and
+// the multi-phase drivers.  All device work is enqueued on one stream.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace glv {
+
+struct PhaseRecord {
+    i64 nv = 0, nnz = 0;
+    int iters = 0;
+    int ncolors = 0;  // colors used by the phase's sweep (0: plain sweep)
+    int colored = 0;
+    double mod = 0;   // value returned by the phase's sweep
+    i64 clusters = 0;
+    double t_sweep = 0, t_color = 0, t_coarsen = 0, t_renumber = 0;
+};
+
+struct RunResult {
+    int phases = 0;
+    int total_iterations = 0;
+    i64 final_clusters = 0;
+    double final_modularity = -1;  // prevMod at exit (what the reference prints)
+    double last_modularity = -1;   // currMod at exit
+    double t_clustering = 0, t_building = 0, t_coloring = 0, t_renumber = 0, t_total = 0;
+    i64 edges_visited = 0;      // sum over phases of iterations * nnz(phase graph)
+    i64 edges_visited_mod = 0;  // separate modularity passes (colored phases)
+    int colors_first = 0;
+    std::vector<PhaseRecord> phase;
+    std::vector<double> trace;   // modularity of every iteration, all phases in order
+};
+
+struct RunParams {
+    int coloring = 1;
+    int replaceMap = 0;  // 1: NoMap plain sweeps (initCommAssOpt start)
+    i64 minGraphSize = 1000;
+    double threshold = 1e-9;
+    double C_threshold = 1e-6;
+    int max_phases = 200;       // runMultiPhaseColoring.cpp:147 (phase > 200)
+    int max_total_iters = 10000;  // :147 (totItr > 10000); 100000 for runMultiPhaseBasic
+};
+
+// Per-phase device state shared by the sweeps: vertex degrees, community
+// degrees/sizes and the vertices grouped by (degree bin, color).
+struct PhaseSetup {
+    int ncol = 1;
+    DBuf<double> k, a;
+    DBuf<i64> n;
+    DBuf<i64> perm;
+    std::vector<i64> ptr;  // (kNumBins*ncol + 1): slice starts in perm, key = bin*ncol + color
+    double constant = 0;   // 1 / sum of degrees
+    double total_weight = 0;
+    i64 maxdeg_global = 0;  // largest degree in the HBM-hash bin
+    i64 lo(int bin, int color) const { return ptr[(size_t)bin * ncol + color]; }
+    i64 hi(int bin, int color) const { return ptr[(size_t)bin * ncol + color + 1]; }
+    i64 bin_lo(int bin) const { return ptr[(size_t)bin * ncol]; }
+    i64 bin_hi(int bin) const { return ptr[(size_t)(bin + 1) * ncol]; }
+};
+
+class Engine {
+public:
+    explicit Engine(hipStream_t s) : s_(s) {}
+    hipStream_t stream() const { return s_; }
+
+    void setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetup &P);
+
+    // algoDistanceOneVertexColoringOpt: returns the largest color
+    int color(const DevGraph &g, int32_t *colors, int *rounds = nullptr);
+    i64 count_conflicts(const DevGraph &g, const int32_t *colors);
+
+    // algoLouvainWithDistOneColoring: C (device, nv) receives the labels
+    double colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, double Lower,
+                         double thresh, int *iters, std::vector<double> *trace = nullptr);
+    // parallelLouvianMethod(NoMap): C receives pastCommAss
+    double plain_phase(const DevGraph &g, i64 *C, double Lower, double thresh, bool nomap,
+                       int *iters, std::vector<double> *trace = nullptr);
+    // one color step from an arbitrary labelling (community info derived from C)
+    i64 color_step(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, int which);
+
+    i64 renumber(i64 *C, i64 n);
+    void coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out);
+    // modularity of a labelling with community degrees summed from C
+    double modularity(const DevGraph &g, const i64 *C);
+
+    void run_colored(const DevGraph &g, i64 *C_orig, const RunParams &p, RunResult &r);
+    void run_basic(const DevGraph &g, i64 *C_orig, const RunParams &p, RunResult &r);
+
+    void sync();
+
+private:
+    hipStream_t s_;
+    // small pinned-free scratch for scalars copied back to the host
+    template <class T>
+    T fetch(const T *dptr);
+    double colored_modularity(const DevGraph &g, const PhaseSetup &P, const i64 *C,
+                              double *part, double *res);
+};
+
+}  // namespace glv

@@ -1,0 +1,367 @@
+"""ctypes bindings of libgrappolo_hip.so (include/grappolo_hip.h).
+
+Names and argument meanings follow the reference's C++ API
+(DefineStructure/color_comm.h, basic_comm.h, basic_util.h, coloring.h); the
+reference returns through out-pointers, these return tuples.  Every call runs
+on the GPU; a missing library or device raises, there is no CPU fallback.
+
+Device-resident forms (``*_dev``) take a :class:`DeviceGraph` whose arrays are
+torch CUDA tensors (torch is used only to hold device memory and streams).
+"""
+import ctypes
+
+import numpy as np
+
+from ._native import hip_lib
+from .graph import EDGE_DTYPE, Graph
+
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_F64P = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.c_void_p
+
+STATUS = {0: "GLV_OK", 1: "GLV_E_INVALID_GRAPH", 2: "GLV_E_OOM", 3: "GLV_E_DEVICE",
+          4: "GLV_E_TOO_MANY_COLORS", 5: "GLV_E_UNSUPPORTED", 6: "GLV_E_INVALID_ARG", 7: "GLV_E_IO"}
+
+
+class GlvError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn}: {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("phases", ctypes.c_int32), ("total_iterations", ctypes.c_int32),
+                ("final_clusters", ctypes.c_int64), ("final_modularity", ctypes.c_double),
+                ("last_modularity", ctypes.c_double), ("time_clustering", ctypes.c_double),
+                ("time_building", ctypes.c_double), ("time_coloring", ctypes.c_double),
+                ("time_renumber", ctypes.c_double), ("time_total", ctypes.c_double),
+                ("edges_visited", ctypes.c_int64), ("edges_visited_mod", ctypes.c_int64),
+                ("colors_first", ctypes.c_int32), ("n_phase", ctypes.c_int32),
+                ("phase_cap", ctypes.c_int32), ("trace_cap", ctypes.c_int32),
+                ("n_trace", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("phase_nv", _I64P), ("phase_nnz", _I64P), ("phase_iters", _I32P),
+                ("phase_colors", _I32P), ("phase_colored", _I32P), ("phase_mod", _F64P),
+                ("phase_clusters", _I64P), ("trace_q", _F64P)]
+
+
+_SIG = {
+    "glv_set_device": [ctypes.c_int],
+    "glv_get_device_count": [_I32P],
+    "glv_runMultiPhaseColoring": [ctypes.c_int64, _I64P, _VP, _I64P, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)],
+    "glv_runMultiPhaseBasic": [ctypes.c_int64, _I64P, _VP, _I64P, ctypes.c_int, ctypes.c_int64,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(Stats)],
+    "glv_algoDistanceOneVertexColoringOpt": [ctypes.c_int64, _I64P, _VP, _I32P, _I32P, _F64P],
+    "glv_algoLouvainWithDistOneColoring": [ctypes.c_int64, _I64P, _VP, _I64P, _I32P, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_double, _F64P, _F64P, _I32P],
+    "glv_parallelLouvianMethod": [ctypes.c_int64, _I64P, _VP, _I64P, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, _F64P, _F64P, _I32P],
+    "glv_renumberClustersContiguously": [_I64P, ctypes.c_int64, _I64P],
+    "glv_buildNextLevelGraphOpt": [ctypes.c_int64, _I64P, _VP, _I64P, ctypes.c_int64,
+                                   ctypes.POINTER(_I64P), ctypes.POINTER(_VP), _I64P],
+    "glv_modularity": [ctypes.c_int64, _I64P, _VP, _I64P, _F64P],
+    "glv_runMultiPhaseColoring_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                                      ctypes.c_double, _VP, ctypes.POINTER(Stats)],
+    "glv_runMultiPhaseBasic_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP,
+                                   ctypes.c_int, ctypes.c_double, _VP, ctypes.POINTER(Stats)],
+    "glv_color_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _I32P, _I32P, _VP],
+    "glv_colored_phase_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, _VP,
+                              ctypes.c_int, ctypes.c_double, ctypes.c_double, _F64P, _I32P, _VP],
+    "glv_plain_phase_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_int, _F64P, _I32P, _VP],
+    "glv_color_step_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, _VP, ctypes.c_int,
+                           ctypes.c_int, _I64P, _VP],
+    "glv_modularity_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, _F64P, _VP],
+    "glv_renumber_dev": [ctypes.c_int64, _VP, _I64P, _VP],
+    "glv_coarsen_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, ctypes.c_int64,
+                        ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I64P,
+                        _I64P, _VP],
+    "glv_dev_free": [_VP],
+    "glv_count_conflicts_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _I64P, _VP],
+}
+
+_fns = {}
+
+
+def _f(name):
+    if name not in _fns:
+        lib = hip_lib()
+        f = getattr(lib, name)
+        f.argtypes = _SIG[name]
+        f.restype = ctypes.c_int
+        _fns[name] = f
+        lib.glv_last_error.restype = ctypes.c_char_p
+        lib.glv_last_error.argtypes = []
+    return _fns[name]
+
+
+def _call(name, *args):
+    rc = _f(name)(*args)
+    if rc != 0:
+        msg = hip_lib().glv_last_error()
+        raise GlvError(name, rc, msg.decode() if msg else "")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _host_graph(G):
+    edges = G.edge_list()
+    return [G.numVertices, _p(G.edgeListPtrs, _I64P), edges.ctypes.data_as(_VP)], edges
+
+
+def new_stats(phase_cap=256, trace_cap=4096):
+    st = Stats()
+    arrs = dict(phase_nv=np.zeros(phase_cap, np.int64), phase_nnz=np.zeros(phase_cap, np.int64),
+                phase_iters=np.zeros(phase_cap, np.int32),
+                phase_colors=np.zeros(phase_cap, np.int32),
+                phase_colored=np.zeros(phase_cap, np.int32), phase_mod=np.zeros(phase_cap),
+                phase_clusters=np.zeros(phase_cap, np.int64), trace_q=np.zeros(trace_cap))
+    st.phase_cap = phase_cap
+    st.trace_cap = trace_cap
+    types = dict(Stats._fields_)
+    for k, v in arrs.items():
+        setattr(st, k, v.ctypes.data_as(types[k]))
+    st._arrays = arrs  # keep alive
+    return st
+
+
+def stats_dict(st):
+    d = {name: getattr(st, name) for name, _ in Stats._fields_[:18]}
+    n = min(st.n_phase, st.phase_cap)
+    d["per_phase"] = {k: v[:n].copy() for k, v in st._arrays.items() if k != "trace_q"}
+    d["trace"] = st._arrays["trace_q"][:min(st.n_trace, st.trace_cap)].copy()
+    return d
+
+
+def device_count():
+    c = ctypes.c_int32(0)
+    _call("glv_get_device_count", ctypes.byref(c))
+    return c.value
+
+
+def set_device(d):
+    _call("glv_set_device", int(d))
+
+
+# ------------------------------------------------------------------ host API
+def runMultiPhaseColoring(G, coloring=1, numColors=16, replaceMap=0, minGraphSize=1000,
+                          threshold=1e-9, C_threshold=1e-6, numThreads=1, threadsOpt=1):
+    """runMultiPhaseColoring (color_comm.h:7).  Returns (C_orig, stats dict)."""
+    args, keep = _host_graph(G)
+    C_orig = np.full(G.numVertices, -1, dtype=np.int64)
+    st = new_stats()
+    _call("glv_runMultiPhaseColoring", *args, _p(C_orig, _I64P), coloring, numColors, replaceMap,
+          minGraphSize, threshold, C_threshold, numThreads, threadsOpt, ctypes.byref(st))
+    return C_orig, stats_dict(st)
+
+
+def runMultiPhaseBasic(G, basicOpt=0, minGraphSize=100000, threshold=1e-9, C_threshold=0.01,
+                       numThreads=1, threadsOpt=1):
+    """runMultiPhaseBasic (basic_comm.h:5).  Returns (C_orig, stats dict)."""
+    args, keep = _host_graph(G)
+    C_orig = np.full(G.numVertices, -1, dtype=np.int64)
+    st = new_stats()
+    _call("glv_runMultiPhaseBasic", *args, _p(C_orig, _I64P), basicOpt, minGraphSize, threshold,
+          C_threshold, numThreads, threadsOpt, ctypes.byref(st))
+    return C_orig, stats_dict(st)
+
+
+def algoDistanceOneVertexColoringOpt(G):
+    """coloring.h:8.  Returns (colors int32[NV], largest color)."""
+    args, keep = _host_graph(G)
+    colors = np.empty(G.numVertices, dtype=np.int32)
+    mc = ctypes.c_int32(-1)
+    t = ctypes.c_double(0)
+    _call("glv_algoDistanceOneVertexColoringOpt", *args, _p(colors, _I32P), ctypes.byref(mc),
+          ctypes.byref(t))
+    return colors, mc.value
+
+
+def algoLouvainWithDistOneColoring(G, color, numColor, Lower=-1.0, thresh=1e-6):
+    """color_comm.h:10.  Returns (modularity, C, numItr)."""
+    args, keep = _host_graph(G)
+    color = np.ascontiguousarray(color, dtype=np.int32)
+    C = np.empty(G.numVertices, dtype=np.int64)
+    q, t, it = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int32(0)
+    _call("glv_algoLouvainWithDistOneColoring", *args, _p(C, _I64P), _p(color, _I32P),
+          int(numColor), Lower, thresh, ctypes.byref(q), ctypes.byref(t), ctypes.byref(it))
+    return q.value, C, it.value
+
+
+def parallelLouvianMethod(G, Lower=-1.0, thresh=1e-6, noMap=False):
+    """basic_comm.h:21 (noMap: basic_comm.h:28).  Returns (modularity, C, numItr)."""
+    args, keep = _host_graph(G)
+    C = np.empty(G.numVertices, dtype=np.int64)
+    q, t, it = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int32(0)
+    _call("glv_parallelLouvianMethod", *args, _p(C, _I64P), Lower, thresh, int(bool(noMap)),
+          ctypes.byref(q), ctypes.byref(t), ctypes.byref(it))
+    return q.value, C, it.value
+
+
+def renumberClustersContiguously(C):
+    """basic_util.h:5.  Returns (renumbered copy of C, number of clusters)."""
+    C = np.array(C, dtype=np.int64)
+    k = ctypes.c_int64(0)
+    _call("glv_renumberClustersContiguously", _p(C, _I64P), C.shape[0], ctypes.byref(k))
+    return C, k.value
+
+
+def buildNextLevelGraphOpt(G, C, numUniqueClusters):
+    """basic_util.h:6.  Returns the coarsened Graph (rows sorted by tail)."""
+    args, keep = _host_graph(G)
+    C = np.ascontiguousarray(C, dtype=np.int64)
+    po, pe, ne = _I64P(), _VP(), ctypes.c_int64(0)
+    _call("glv_buildNextLevelGraphOpt", *args, _p(C, _I64P), int(numUniqueClusters),
+          ctypes.byref(po), ctypes.byref(pe), ctypes.byref(ne))
+    k = int(numUniqueClusters)
+    ptrs = np.ctypeslib.as_array(po, shape=(k + 1,)).copy()
+    nnz = int(ptrs[-1])
+    buf = (ctypes.c_char * max(nnz * 24, 1)).from_address(pe.value)
+    edges = np.frombuffer(buf, dtype=EDGE_DTYPE, count=nnz).copy()
+    free = hip_lib().glv_free
+    free.argtypes = [_VP]
+    free.restype = None
+    free(ctypes.cast(po, _VP))
+    free(pe)
+    return Graph.from_edge_list(ptrs, edges, numEdges=ne.value)
+
+
+def modularity(G, C):
+    """The in-sweep modularity (parallelLouvainWithColoring.cpp:226-249) of labels C."""
+    args, keep = _host_graph(G)
+    C = np.ascontiguousarray(C, dtype=np.int64)
+    q = ctypes.c_double(0)
+    _call("glv_modularity", *args, _p(C, _I64P), ctypes.byref(q))
+    return q.value
+
+
+# ---------------------------------------------------------------- device API
+class DeviceGraph:
+    """A Graph resident in device memory (torch tensors on cuda)."""
+
+    def __init__(self, G, device="cuda"):
+        import torch
+        self.numVertices = G.numVertices
+        self.nnz = G.nnz
+        self.numEdges = G.numEdges
+        self.off = torch.from_numpy(G.edgeListPtrs).to(device)
+        self.tails = torch.from_numpy(G.tails).to(device)
+        self.w = torch.from_numpy(G.weights).to(device)
+
+    def ptrs(self):
+        return [self.numVertices, self.nnz, _VP(self.off.data_ptr()), _VP(self.tails.data_ptr()),
+                _VP(self.w.data_ptr())]
+
+
+def _stream():
+    import torch
+    return _VP(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return _VP(t.data_ptr())
+
+
+def run_colored_dev(dg, C_orig, coloring=1, replaceMap=0, minGraphSize=1000, threshold=1e-9,
+                    C_threshold=1e-6, stats=None):
+    st = stats if stats is not None else new_stats()
+    _call("glv_runMultiPhaseColoring_dev", *dg.ptrs(), _ptr(C_orig), coloring, replaceMap,
+          minGraphSize, threshold, C_threshold, _stream(), ctypes.byref(st))
+    return stats_dict(st)
+
+
+def run_basic_dev(dg, C_orig, basicOpt=0, threshold=1e-9):
+    st = new_stats()
+    _call("glv_runMultiPhaseBasic_dev", *dg.ptrs(), _ptr(C_orig), basicOpt, threshold,
+          _stream(), ctypes.byref(st))
+    return stats_dict(st)
+
+
+def color_dev(dg, colors):
+    mc, rounds = ctypes.c_int32(0), ctypes.c_int32(0)
+    _call("glv_color_dev", dg.numVertices, dg.nnz, _ptr(dg.off), _ptr(dg.tails), _ptr(colors),
+          ctypes.byref(mc), ctypes.byref(rounds), _stream())
+    return mc.value, rounds.value
+
+
+def count_conflicts_dev(dg, colors):
+    c = ctypes.c_int64(0)
+    _call("glv_count_conflicts_dev", dg.numVertices, dg.nnz, _ptr(dg.off), _ptr(dg.tails),
+          _ptr(colors), ctypes.byref(c), _stream())
+    return c.value
+
+
+def colored_phase_dev(dg, C, colors, numColor, Lower=-1.0, thresh=1e-6):
+    q, it = ctypes.c_double(0), ctypes.c_int32(0)
+    _call("glv_colored_phase_dev", *dg.ptrs(), _ptr(C), _ptr(colors), int(numColor), Lower,
+          thresh, ctypes.byref(q), ctypes.byref(it), _stream())
+    return q.value, it.value
+
+
+def plain_phase_dev(dg, C, Lower=-1.0, thresh=1e-6, noMap=False):
+    q, it = ctypes.c_double(0), ctypes.c_int32(0)
+    _call("glv_plain_phase_dev", *dg.ptrs(), _ptr(C), Lower, thresh, int(bool(noMap)),
+          ctypes.byref(q), ctypes.byref(it), _stream())
+    return q.value, it.value
+
+
+def color_step_dev(dg, C, colors, numColor, which):
+    m = ctypes.c_int64(0)
+    _call("glv_color_step_dev", *dg.ptrs(), _ptr(C), _ptr(colors), int(numColor), int(which),
+          ctypes.byref(m), _stream())
+    return m.value
+
+
+def modularity_dev(dg, C):
+    q = ctypes.c_double(0)
+    _call("glv_modularity_dev", *dg.ptrs(), _ptr(C), ctypes.byref(q), _stream())
+    return q.value
+
+
+def renumber_dev(C):
+    k = ctypes.c_int64(0)
+    _call("glv_renumber_dev", C.shape[0], _ptr(C), ctypes.byref(k), _stream())
+    return k.value
+
+
+def coarsen_dev(dg, C, k):
+    """Returns a host Graph copied from the engine's coarsened device graph."""
+    import torch
+    po, pt, pw = _VP(), _VP(), _VP()
+    nnz, ne = ctypes.c_int64(0), ctypes.c_int64(0)
+    _call("glv_coarsen_dev", *dg.ptrs(), _ptr(C), int(k), ctypes.byref(po), ctypes.byref(pt),
+          ctypes.byref(pw), ctypes.byref(nnz), ctypes.byref(ne), _stream())
+    torch.cuda.synchronize()
+    off = np.empty(int(k) + 1, np.int64)
+    tails = np.empty(max(nnz.value, 1), np.int64)
+    w = np.empty(max(nnz.value, 1))
+    hip_copy_d2h(off.ctypes.data, po.value, off.nbytes)
+    if nnz.value:
+        hip_copy_d2h(tails.ctypes.data, pt.value, nnz.value * 8)
+        hip_copy_d2h(w.ctypes.data, pw.value, nnz.value * 8)
+    for p in (po, pt, pw):
+        _call("glv_dev_free", p)
+    return Graph(off, tails[:nnz.value], w[:nnz.value], numEdges=ne.value)
+
+
+_hip_rt = None
+
+
+def hip_copy_d2h(dst, src, nbytes):
+    """Blocking device-to-host copy of engine-owned memory (HIP runtime)."""
+    global _hip_rt
+    if nbytes == 0:
+        return
+    if _hip_rt is None:
+        _hip_rt = ctypes.CDLL("libamdhip64.so")
+        _hip_rt.hipMemcpy.argtypes = [_VP, _VP, ctypes.c_size_t, ctypes.c_int]
+        _hip_rt.hipMemcpy.restype = ctypes.c_int
+    rc = _hip_rt.hipMemcpy(_VP(dst), _VP(src), nbytes, 2)  # hipMemcpyDeviceToHost
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")

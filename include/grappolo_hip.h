@@ -1,0 +1,184 @@
+/*
+ * grappolo_hip.h -- C ABI of libgrappolo_hip.so, the MI355X (gfx950) engine
+ * for Grappolo's colored multi-phase Louvain.
+ *
+ * Two families of entry points:
+ *   glv_<ReferenceName>        graph given as the reference's host arrays
+ *                              (edgeListPtrs[NV+1] + edge{head,tail,weight}[nnz]);
+ *                              the engine uploads, runs on the GPU, downloads.
+ *   glv_<stage>_dev            graph already resident in device memory as
+ *                              structure-of-arrays CSR (offsets, tails, weights;
+ *                              int64/int64/double), results left in device memory.
+ * Each entry point names the reference function it replaces (file:line of
+ * the reference tree).  Plain pointers and sizes only.  Every function returns
+ * GLV_OK or a GLV_E_* status (glv_status.h); glv_last_error() gives the
+ * message of the last failure on the calling thread.  Calls are not re-entrant
+ * on one device: concurrent calls from several host threads must use distinct
+ * streams (the _dev forms) and are serialised by the caller.
+ *
+ * Graph contract (the reference's, DefineStructure/defs.h:85-92): symmetric
+ * CSR, every undirected edge in both rows, self-loops once, non-negative
+ * weights, tails in [0, NV).  Violations return GLV_E_INVALID_GRAPH.
+ */
+#ifndef GRAPPOLO_HIP_H
+#define GRAPPOLO_HIP_H
+
+#include <stdint.h>
+
+#include "glv_status.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* edge record, layout of DefineStructure/defs.h:78-83 (24 bytes) */
+typedef struct glv_edge {
+    int64_t head;
+    int64_t tail;
+    double weight;
+} glv_edge;
+
+/* Outcome of a multi-phase run.  The optional arrays are caller-owned; set
+ * their pointer and capacity, the engine fills up to the capacity. */
+typedef struct glv_stats {
+    int32_t phases;           /* "Total number of phases" (runMultiPhaseColoring.cpp:213) */
+    int32_t total_iterations; /* "Total number of iterations" */
+    int64_t final_clusters;   /* "Final number of clusters" */
+    double final_modularity;  /* "Final modularity" (prevMod at exit, :216) */
+    double last_modularity;   /* currMod at exit */
+    double time_clustering;   /* seconds: sweeps incl. their modularity passes */
+    double time_building;     /* coarsening */
+    double time_coloring;
+    double time_renumber;     /* renumbering and C_orig updates */
+    double time_total;        /* wall time of the whole call on the device side */
+    int64_t edges_visited;    /* sum over phases of iterations * nnz(phase graph) */
+    int64_t edges_visited_mod;/* modularity passes of colored phases */
+    int32_t colors_first;     /* colors of the first coloring */
+    int32_t n_phase;          /* phases recorded (may exceed phase_cap) */
+    int32_t phase_cap;
+    int32_t trace_cap;
+    int32_t n_trace;          /* iterations recorded in trace_q (may exceed trace_cap) */
+    int32_t reserved;
+    int64_t *phase_nv;
+    int64_t *phase_nnz;
+    int32_t *phase_iters;
+    int32_t *phase_colors;
+    int32_t *phase_colored;
+    double *phase_mod;
+    int64_t *phase_clusters;
+    double *trace_q;          /* modularity after every iteration of every phase */
+} glv_stats;
+
+/* ---- device and errors ------------------------------------------------ */
+int glv_set_device(int device);
+int glv_get_device_count(int *count);
+const char *glv_last_error(void);
+/* The engine's build: "gfx950", plus version text. */
+const char *glv_version(void);
+
+/* ---- whole-pipeline entry points (host arrays) --------------------------- */
+
+/* runMultiPhaseColoring, DefineStructure/color_comm.h:7 and
+ * Coloring/runMultiPhaseColoring.cpp:51-238.  coloring must be 1.
+ * C_orig[NV] receives the partition (-1 for vertices without edges).
+ * The graph is not modified (the C++ drop-in frees it, as the reference does).
+ * numThreads/threadsOpt are accepted and ignored (no host threads are used). */
+int glv_runMultiPhaseColoring(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                              int64_t *C_orig, int coloring, int numColors, int replaceMap,
+                              int64_t minGraphSize, double threshold, double C_threshold,
+                              int numThreads, int threadsOpt, glv_stats *stats);
+
+/* runMultiPhaseBasic, DefineStructure/basic_comm.h:5 and
+ * BasicCommunitiesDetection/runMultiPhaseBasic.cpp:52-170.  basicOpt 1 selects
+ * the NoMap sweep (initCommAssOpt start). */
+int glv_runMultiPhaseBasic(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                           int64_t *C_orig, int basicOpt, int64_t minGraphSize, double threshold,
+                           double C_threshold, int numThreads, int threadsOpt, glv_stats *stats);
+
+/* ---- single stages (host arrays) ------------------------------------------ */
+
+/* algoDistanceOneVertexColoringOpt, DefineStructure/coloring.h:8 and
+ * Coloring/coloringDistanceOne.cpp:50-233.  *maxColor = largest color used.
+ * Jones-Plassmann rounds with hash priorities: a valid coloring, not the
+ * reference's colors. */
+int glv_algoDistanceOneVertexColoringOpt(int64_t nv, const int64_t *edgeListPtrs,
+                                         const glv_edge *edgeList, int32_t *vtxColor,
+                                         int32_t *maxColor, double *totTime);
+
+/* algoLouvainWithDistOneColoring(NoMap), DefineStructure/color_comm.h:10,13 and
+ * Coloring/parallelLouvainWithColoring.cpp:47-286.  C[NV] receives the labels;
+ * *modularity the value the reference returns (prevMod). */
+int glv_algoLouvainWithDistOneColoring(int64_t nv, const int64_t *edgeListPtrs,
+                                       const glv_edge *edgeList, int64_t *C, const int32_t *color,
+                                       int numColor, double Lower, double thresh,
+                                       double *modularity, double *totTime, int32_t *numItr);
+
+/* parallelLouvianMethod / parallelLouvianMethodNoMap, DefineStructure/
+ * basic_comm.h:21,28 and BasicCommunitiesDetection/parallelLouvainMethod.cpp:65-326. */
+int glv_parallelLouvianMethod(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                              int64_t *C, double Lower, double thresh, int noMap,
+                              double *modularity, double *totTime, int32_t *numItr);
+
+/* renumberClustersContiguously, DefineStructure/basic_util.h:5 and
+ * Utility/buildNextPhase.cpp:48-79. */
+int glv_renumberClustersContiguously(int64_t *C, int64_t size, int64_t *numClusters);
+
+/* buildNextLevelGraphOpt, DefineStructure/basic_util.h:6 and
+ * Utility/buildNextPhase.cpp:83-255.  Outputs are malloc()ed (free with free()
+ * or glv_free); rows are sorted by tail; *outNumEdges = NE_out + k. */
+int glv_buildNextLevelGraphOpt(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                               const int64_t *C, int64_t numUniqueClusters,
+                               int64_t **outEdgeListPtrs, glv_edge **outEdgeList,
+                               int64_t *outNumEdges);
+
+/* The in-sweep modularity of Coloring/parallelLouvainWithColoring.cpp:226-249
+ * for a given labelling (community degrees summed from C; -1 labels ignored). */
+int glv_modularity(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                   const int64_t *C, double *modularity);
+
+void glv_free(void *p);
+
+/* ---- device-resident forms (stream may be NULL: the engine's own stream) -- */
+
+int glv_runMultiPhaseColoring_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                                  const int64_t *d_tails, const double *d_weights, int64_t *d_C_orig,
+                                  int coloring, int replaceMap, int64_t minGraphSize,
+                                  double threshold, double C_threshold, void *stream,
+                                  glv_stats *stats);
+int glv_runMultiPhaseBasic_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                               const int64_t *d_tails, const double *d_weights, int64_t *d_C_orig,
+                               int basicOpt, double threshold, void *stream, glv_stats *stats);
+int glv_color_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                  int32_t *d_colors, int32_t *maxColor, int32_t *rounds, void *stream);
+int glv_colored_phase_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                          const double *d_weights, int64_t *d_C, const int32_t *d_colors,
+                          int numColor, double Lower, double thresh, double *modularity,
+                          int32_t *numItr, void *stream);
+int glv_plain_phase_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                        const double *d_weights, int64_t *d_C, double Lower, double thresh,
+                        int noMap, double *modularity, int32_t *numItr, void *stream);
+/* One color step of the colored sweep from an arbitrary labelling d_C (the
+ * community degrees and sizes are summed from it); d_C is updated in place.
+ * *moves = vertices whose label changed. */
+int glv_color_step_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                       const double *d_weights, int64_t *d_C, const int32_t *d_colors,
+                       int numColor, int which, int64_t *moves, void *stream);
+int glv_modularity_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                       const double *d_weights, const int64_t *d_C, double *modularity,
+                       void *stream);
+int glv_renumber_dev(int64_t n, int64_t *d_C, int64_t *numClusters, void *stream);
+/* Coarsened graph in engine-allocated device memory; release with
+ * glv_dev_free on each of the three pointers. */
+int glv_coarsen_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                    const double *d_weights, const int64_t *d_C, int64_t k, int64_t **d_out_offsets,
+                    int64_t **d_out_tails, double **d_out_weights, int64_t *out_nnz,
+                    int64_t *out_num_edges, void *stream);
+int glv_dev_free(void *d_ptr);
+int glv_count_conflicts_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                            const int64_t *d_tails, const int32_t *d_colors, int64_t *conflicts,
+                            void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAPPOLO_HIP_H */

@@ -1,0 +1,57 @@
+"""Small seeded graphs shared by the tests (sizes the oracle finishes in seconds)."""
+import os
+
+import numpy as np
+
+from grappolo_amd.graph import Graph, gen_lfr, gen_planted, gen_rmat, load_metis
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def karate():
+    return load_metis(os.path.join(GOLDEN, "karate.graph"))
+
+
+def hubs(seed=7, nv=12000, hub_degrees=(300, 1500, 2500, 6000)):
+    """Planted background plus a few hub vertices: every degree bin is used,
+    including the HBM-hash one (> 2048 entries)."""
+    g = gen_planted(nv, 32, 12, 0.3, seed=seed)
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(nv), g.degrees())
+    keep = rows < g.tails
+    pairs = [np.stack([rows[keep], g.tails[keep]], 1)]
+    for h, d in enumerate(hub_degrees):
+        others = rng.choice(np.arange(len(hub_degrees), nv), size=d, replace=False)
+        pairs.append(np.stack([np.full(d, h), others], 1))
+    p = np.concatenate(pairs)
+    p = np.unique(np.sort(p, 1), axis=0)
+    return Graph.from_pairs(nv, p)
+
+
+def with_loops_and_isolated(seed=11, nv=3000):
+    """Self-loops (weight 2), isolated vertices and a weighted planted core."""
+    g = gen_planted(nv - 100, 10, 10, 0.25, seed=seed, wlo=0.5, whi=2.0)
+    rows = np.repeat(np.arange(g.numVertices), g.degrees())
+    keep = rows < g.tails
+    pairs = np.stack([rows[keep], g.tails[keep]], 1)
+    w = g.weights[keep]
+    loops = np.arange(0, g.numVertices, 7)
+    pairs = np.concatenate([pairs, np.stack([loops, loops], 1)])
+    w = np.concatenate([w, np.full(len(loops), 2.0)])
+    return Graph.from_pairs(nv, pairs, w)  # vertices nv-100.. have no edges
+
+
+def catalog():
+    return {
+        "karate": karate(),
+        "planted_unit": gen_planted(4096, 16, 16, 0.3, seed=5),
+        "planted_w": gen_planted(4096, 16, 16, 0.3, seed=6, wlo=0.5, whi=2.0),
+        "lfr_w": gen_lfr(20000, seed=3, dmax=600, cmax=1500),
+        "rmat12": gen_rmat(12, seed=3),
+        "hubs": hubs(),
+        "loops_isolated": with_loops_and_isolated(),
+    }
+
+
+def unit_weights(g):
+    return bool(np.all(g.weights == np.round(g.weights)))
