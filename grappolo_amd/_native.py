@@ -21,7 +21,9 @@ def _load(path, what):
         raise RuntimeError(
             f"{what} not built: {path} is missing (run __graft_entry__.build() or "
             f"`make -C grappolo_amd/csrc`)")
-    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    # local scope: the engine exports the reference's C++ names (drop-in), which
+    # must not interpose on another library that defines them too
+    lib = ctypes.CDLL(path)
     _cache[path] = lib
     return lib
 

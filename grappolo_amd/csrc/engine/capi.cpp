@@ -14,6 +14,7 @@
 #include "grappolo_hip.h"
 #include "kernels.hpp"
 #include "louvain.hpp"
+#include "profiler.hpp"
 
 using glv::DBuf;
 using glv::DevGraph;
@@ -578,6 +579,25 @@ int glv_dev_free(void *d_ptr) {
         hipStream_t s = engine_stream(nullptr);
         GLV_HIP(hipFreeAsync(d_ptr, s));
         GLV_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int glv_profile_enable(int on) {
+    return guarded([&] { glv::Profiler::get().enable(on != 0); });
+}
+
+int glv_profile_reset(void) {
+    return guarded([&] { glv::Profiler::get().reset(); });
+}
+
+int glv_profile_read(int kernel_class, double *ms, double *bytes, int64_t *launches) {
+    return guarded([&] {
+        if (kernel_class < 0 || kernel_class >= glv::K_NUM) glv::fail(GLV_E_INVALID_ARG, "bad class");
+        glv::Profiler::get().collect();
+        glv::KStat st = glv::Profiler::get().stat(kernel_class);
+        if (ms) *ms = st.ms;
+        if (bytes) *bytes = st.bytes;
+        if (launches) *launches = st.launches;
     });
 }
 

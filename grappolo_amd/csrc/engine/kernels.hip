@@ -143,7 +143,7 @@ __global__ void k_init_labels(const i64 *off, i64 nv, i64 *C) {
 }
 
 __global__ void k_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
-                           i64 *vid, int32_t *hist) {
+                           i64 *vid, int32_t *hist, i64 *hist_nnz) {
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
         int b = degree_bin(off[i + 1] - off[i]);
         int c = colors ? colors[i] : 0;
@@ -155,6 +155,8 @@ __global__ void k_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int nc
         uint32_t key = (uint32_t)(b * ncol + c);
         keys[i] = key;
         atomicAdd(&hist[key], 1);
+        const i64 d = off[i + 1] - off[i];
+        if (d) atomicAdd((unsigned long long *)&hist_nnz[key], (unsigned long long)d);
     }
 }
 
@@ -883,9 +885,10 @@ void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s) {
     GLV_LAUNCH_CHECK();
 }
 void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys, i64 *vid,
-                     int32_t *hist, hipStream_t s) {
+                     int32_t *hist, i64 *hist_nnz, hipStream_t s) {
     if (nv <= 0) return;
-    k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, hist);
+    k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, hist,
+                                                            hist_nnz);
     GLV_LAUNCH_CHECK();
 }
 void launch_degree(const DevGraph &g, const i64 *big, i64 nbig, double *k, hipStream_t s) {

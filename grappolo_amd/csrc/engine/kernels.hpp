@@ -40,7 +40,7 @@ void launch_fill_i32(int32_t *p, i64 n, int32_t v, hipStream_t s);
 void launch_iota(i64 *p, i64 n, hipStream_t s);
 void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s);  // C[i] = deg? i : -1
 void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
-                     i64 *vid, int32_t *hist, hipStream_t s);
+                     i64 *vid, int32_t *hist, i64 *hist_nnz, hipStream_t s);
 void launch_degree(const DevGraph &g, const i64 *big, i64 nbig, double *k, hipStream_t s);
 void launch_init_comm(const double *k, i64 nv, double *a, i64 *n, hipStream_t s);
 void launch_comm_from_labels(const i64 *C, const double *k, i64 nv, double *a, i64 *n,

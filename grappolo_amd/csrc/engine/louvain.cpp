@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "kernels.hpp"
+#include "profiler.hpp"
 
 namespace glv {
 
@@ -50,11 +51,15 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
     P.perm.alloc(nv, s_);
     DBuf<uint32_t> keys(nv, s_);
     DBuf<int32_t> hist(nkeys, s_);
+    DBuf<i64> hist_nnz(nkeys, s_);
     launch_fill_i32(hist.p, nkeys, 0, s_);
-    launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, hist.p, s_);
+    launch_fill_i64(hist_nnz.p, nkeys, 0, s_);
+    launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, hist.p, hist_nnz.p, s_);
     sort_pairs_u32_i64(keys.p, P.perm.p, nv, bits_for(nkeys), s_);
     std::vector<int32_t> h(nkeys);
+    P.pnnz.assign(nkeys, 0);
     GLV_HIP(hipMemcpyAsync(h.data(), hist.p, nkeys * sizeof(int32_t), hipMemcpyDeviceToHost, s_));
+    GLV_HIP(hipMemcpyAsync(P.pnnz.data(), hist_nnz.p, nkeys * sizeof(i64), hipMemcpyDeviceToHost, s_));
     GLV_HIP(hipStreamSynchronize(s_));
     P.ptr.assign(nkeys + 1, 0);
     for (i64 q = 0; q < nkeys; q++) P.ptr[q + 1] = P.ptr[q] + h[q];
@@ -95,7 +100,9 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     while (ns + nl > 0) {
         if (round >= (1 << 18)) fail(GLV_E_DEVICE, "coloring did not converge");
         launch_fill_i64(counts.p, 2, 0, s_);
+        Profiler::get().begin(K_COLOR, s_);
         launch_jp_round(g, colors, small, ns, large, nl, small_next, large_next, counts.p, round, err.p, s_);
+        Profiler::get().end(K_COLOR, 0.0, s_);
         GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
         GLV_HIP(hipStreamSynchronize(s_));
         ns = hc[0];
@@ -121,6 +128,8 @@ i64 Engine::count_conflicts(const DevGraph &g, const int32_t *colors) {
 // e_xx over all vertices (per-bin passes into block partials), sum of a_c^2.
 double Engine::colored_modularity(const DevGraph &g, const PhaseSetup &P, const i64 *C, double *part,
                                   double *res) {
+    // algorithmic bytes: tail, weight and C[tail] per entry; C[v] per vertex; a_c per community
+    Profiler::get().begin(K_MOD, s_);
     i64 off = 0;
     for (int b = 1; b < kNumBins; b++) {
         const i64 lo = P.bin_lo(b), cnt = P.bin_hi(b) - lo;
@@ -131,6 +140,7 @@ double Engine::colored_modularity(const DevGraph &g, const PhaseSetup &P, const 
     if (off == 0) launch_fill_f64(res, 1, 0.0, s_);
     else launch_sum_partials(part, off, res, s_);
     launch_sumsq(P.a.p, g.nv, part + off, res + 1, s_);
+    Profiler::get().end(K_MOD, 24.0 * g.nnz + 16.0 * g.nv, s_);
     double h[2];
     GLV_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s_));
     GLV_HIP(hipStreamSynchronize(s_));
@@ -193,9 +203,15 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 if (S.hi[b] <= S.lo[b]) continue;
                 A.verts = P.perm.p + S.lo[b];
                 A.count = S.hi[b] - S.lo[b];
+                Profiler::get().begin(K_SWEEP_B1 + b - 1, s_);
                 launch_sweep(b, false, A, s_);
+                Profiler::get().end(K_SWEEP_B1 + b - 1, 24.0 * P.nnz_of(b, c) + 32.0 * A.count, s_);
             }
+            i64 ncnt = 0;
+            for (int b = 1; b < kNumBins; b++) ncnt += S.hi[b] - S.lo[b];
+            Profiler::get().begin(K_APPLY, s_);
             launch_apply_colored(P.perm.p, S, tgt.p, C, P.k.p, P.a.p, P.n.p, s_);
+            Profiler::get().end(K_APPLY, 24.0 * ncnt, s_);
         }
         currMod = colored_modularity(g, P, C, part.p, res.p);
         if (trace) trace->push_back(currMod);
@@ -263,7 +279,9 @@ double Engine::plain_phase(const DevGraph &g, i64 *C, double Lower, double thres
             A.verts = P.perm.p + lo;
             A.count = cnt;
             A.eii_part = part.p + off;
+            Profiler::get().begin(K_SWEEP_B1 + b - 1, s_);
             launch_sweep(b, true, A, s_);
+            Profiler::get().end(K_SWEEP_B1 + b - 1, 24.0 * P.bin_nnz(b) + 32.0 * cnt, s_);
             off += sweep_blocks(b, cnt);
         }
         launch_isolated_targets(P.perm.p + P.bin_lo(0), P.bin_hi(0) - P.bin_lo(0), tgt, s_);
@@ -280,7 +298,9 @@ double Engine::plain_phase(const DevGraph &g, i64 *C, double Lower, double thres
         if (numItrs >= kMaxPhaseIters) break;
         prevMod = currMod;
         if (prevMod < Lower) prevMod = Lower;  // :272-273
+        Profiler::get().begin(K_APPLY, s_);
         launch_apply_plain(nv, tgt, curr, P.k.p, P.a.p, P.n.p, s_);
+        Profiler::get().end(K_APPLY, 16.0 * nv, s_);
         i64 *t = past;
         past = curr;
         curr = tgt;
@@ -354,11 +374,13 @@ i64 Engine::renumber(i64 *C, i64 n) {
     DBuf<i64> first(n, s_), flags(n, s_), pos(n, s_);
     DBuf<int32_t> err(1, s_);
     launch_fill_i32(err.p, 1, 0, s_);
+    Profiler::get().begin(K_RENUMBER, s_);
     launch_fill_i64(first.p, n, n, s_);
     launch_first_index(C, n, first.p, err.p, s_);
     launch_head_flags(C, n, first.p, flags.p, s_);
     scan_exclusive_i64(flags.p, pos.p, n, s_);
     launch_relabel(C, n, first.p, pos.p, s_);
+    Profiler::get().end(K_RENUMBER, 16.0 * n, s_);
     i64 last[2];
     GLV_HIP(hipMemcpyAsync(&last[0], pos.p + n - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
     GLV_HIP(hipMemcpyAsync(&last[1], flags.p + n - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
@@ -377,6 +399,7 @@ void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out) {
     const int kb = bits_for(k);
     if (2 * kb > 64) fail(GLV_E_UNSUPPORTED, "more than 2^32 clusters");
     const i64 nv = g.nv;
+    Profiler::get().begin(K_COARSEN, s_);
     DBuf<i64> cnt(std::max<i64>(nv, 1), s_), pos(std::max<i64>(nv, 1), s_);
     launch_coarsen_count(g, C, cnt.p, s_);
     scan_exclusive_i64(cnt.p, pos.p, nv, s_);
@@ -415,6 +438,8 @@ void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out) {
     out.w_buf.alloc(total, s_);
     out.adopt();
     launch_coarsen_rows(k2.p, v2.p, total, k, kb, out.off_buf.p, out.tails_buf.p, out.w_buf.p, s_);
+    // B_coarsen = 24 nnz_in + 8 NV_in + 24 nnz_out + 8 (k+1)
+    Profiler::get().end(K_COARSEN, 24.0 * g.nnz + 8.0 * nv + 24.0 * total + 8.0 * (k + 1), s_);
 }
 
 double Engine::modularity(const DevGraph &g, const i64 *C) {

@@ -51,6 +51,7 @@ struct PhaseSetup {
     DBuf<i64> n;
     DBuf<i64> perm;
     std::vector<i64> ptr;  // (kNumBins*ncol + 1): slice starts in perm, key = bin*ncol + color
+    std::vector<i64> pnnz; // entries of the vertices of each slice
     double constant = 0;   // 1 / sum of degrees
     double total_weight = 0;
     i64 maxdeg_global = 0;  // largest degree in the HBM-hash bin
@@ -58,6 +59,12 @@ struct PhaseSetup {
     i64 hi(int bin, int color) const { return ptr[(size_t)bin * ncol + color + 1]; }
     i64 bin_lo(int bin) const { return ptr[(size_t)bin * ncol]; }
     i64 bin_hi(int bin) const { return ptr[(size_t)(bin + 1) * ncol]; }
+    i64 nnz_of(int bin, int color) const { return pnnz[(size_t)bin * ncol + color]; }
+    i64 bin_nnz(int bin) const {
+        i64 t = 0;
+        for (int c = 0; c < ncol; c++) t += nnz_of(bin, c);
+        return t;
+    }
 };
 
 class Engine {

@@ -82,7 +82,15 @@ _SIG = {
                         _I64P, _VP],
     "glv_dev_free": [_VP],
     "glv_count_conflicts_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _I64P, _VP],
+    "glv_profile_enable": [ctypes.c_int],
+    "glv_profile_reset": [],
+    "glv_profile_read": [ctypes.c_int, _F64P, _F64P, _I64P],
 }
+
+KERNEL_CLASSES = ["sweep_bin1_deg4", "sweep_bin2_deg8", "sweep_bin3_deg16", "sweep_bin4_deg32",
+                  "sweep_bin5_deg64", "sweep_bin6_wavehash", "sweep_bin7_blockhash",
+                  "sweep_bin8_hbmhash", "modularity", "commit", "coloring_round", "coarsen",
+                  "renumber"]
 
 _fns = {}
 
@@ -137,6 +145,24 @@ def stats_dict(st):
     d["per_phase"] = {k: v[:n].copy() for k, v in st._arrays.items() if k != "trace_q"}
     d["trace"] = st._arrays["trace_q"][:min(st.n_trace, st.trace_cap)].copy()
     return d
+
+
+def profile_enable(on=True):
+    _call("glv_profile_enable", int(bool(on)))
+
+
+def profile_reset():
+    _call("glv_profile_reset")
+
+
+def profile_read():
+    """{class name: (ms, algorithmic bytes, launches)} accumulated since reset."""
+    out = {}
+    for i, name in enumerate(KERNEL_CLASSES):
+        ms, by, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int64(0)
+        _call("glv_profile_read", i, ctypes.byref(ms), ctypes.byref(by), ctypes.byref(n))
+        out[name] = (ms.value, by.value, n.value)
+    return out
 
 
 def device_count():

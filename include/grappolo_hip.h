@@ -178,6 +178,17 @@ int glv_count_conflicts_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
                             const int64_t *d_tails, const int32_t *d_colors, int64_t *conflicts,
                             void *stream);
 
+/* ---- kernel timing (no counterpart in the reference) ----------------------
+ * When enabled, every launch of the instrumented kernel classes is bracketed
+ * by HIP events on the engine's stream; glv_profile_read returns the summed
+ * device time (ms), the summed algorithmic bytes and the launch count of one
+ * class: 0..7 sweep kernels of degree bins 1..8 (algorithmic bytes
+ * 24*entries + 32*vertices per launch), 8 modularity pass, 9 commits,
+ * 10 coloring rounds, 11 coarsening, 12 renumbering. */
+int glv_profile_enable(int on);
+int glv_profile_reset(void);
+int glv_profile_read(int kernel_class, double *ms, double *bytes, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -130,6 +130,13 @@ def _declare_ref(R):
     R.ref_run_basic.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                     ctypes.c_char_p]
     R.ref_run_basic.restype = None
+    for name in ("ref_load_binary", "ref_load_metis"):
+        f = getattr(R, name)
+        f.argtypes = [ctypes.c_char_p, ctypes.POINTER(_I64P), ctypes.POINTER(_I64P),
+                      ctypes.POINTER(_F64P), _I64P]
+        f.restype = ctypes.c_int64
+    R.ref_write_binary.argtypes = [ctypes.c_char_p] + G
+    R.ref_write_binary.restype = None
 
 
 def _p(a, t):
@@ -327,6 +334,36 @@ def ref_coarsen(g, C, k, nthreads=1):
     for p in (po, pt, pw):
         ref().ref_free(ctypes.cast(p, ctypes.c_void_p))
     return Graph(off, tails, w, numEdges=ne.value)
+
+
+def _ref_load(fn, path):
+    from grappolo_amd.graph import Graph
+    po, pt, pw = _I64P(), _I64P(), _F64P()
+    ne = ctypes.c_int64(0)
+    nv = getattr(ref(), fn)(str(path).encode(), ctypes.byref(po), ctypes.byref(pt),
+                            ctypes.byref(pw), ctypes.byref(ne))
+    off = np.ctypeslib.as_array(po, shape=(nv + 1,)).copy()
+    nnz = int(off[-1])
+    tails = np.ctypeslib.as_array(pt, shape=(max(nnz, 1),))[:nnz].copy()
+    w = np.ctypeslib.as_array(pw, shape=(max(nnz, 1),))[:nnz].copy()
+    for p in (po, pt, pw):
+        ref().ref_free(ctypes.cast(p, ctypes.c_void_p))
+    return Graph(off, tails, w, numEdges=ne.value)
+
+
+def ref_load_binary(path):
+    """The reference's parse_EdgeListBinaryNew (InputsOutput/loadBinary.cpp:6-48)."""
+    return _ref_load("ref_load_binary", path)
+
+
+def ref_load_metis(path):
+    """The reference's loadMetisFileFormat (InputsOutput/loadMetis.cpp:19-232)."""
+    return _ref_load("ref_load_metis", path)
+
+
+def ref_write_binary(path, g):
+    """The reference's writeGraphBinaryFormatNew (InputsOutput/writeBinary.cpp:2-29)."""
+    ref().ref_write_binary(str(path).encode(), *_rg(g))
 
 
 def _parse_final(log):

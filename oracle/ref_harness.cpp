@@ -19,6 +19,7 @@
 #include "basic_util.h"
 #include "color_comm.h"
 #include "coloring.h"
+#include "input_output.h"
 
 namespace {
 
@@ -158,6 +159,52 @@ int64_t ref_coarsen(int64_t nv, int64_t num_edges, const int64_t *off, const int
 }
 
 void ref_free(void *p) { free(p); }
+
+// The reference's own file readers: parse_EdgeListBinaryNew (-f 9) and
+// loadMetisFileFormat (-f 5).  Returns the graph as malloc()ed SoA arrays.
+static int64_t take_graph(graph *G, int64_t **off, int64_t **tails, double **w, int64_t *ne) {
+    int64_t nv = G->numVertices, nnz = G->edgeListPtrs[nv];
+    *off = (int64_t *)malloc((nv + 1) * sizeof(int64_t));
+    *tails = (int64_t *)malloc((nnz > 0 ? nnz : 1) * sizeof(int64_t));
+    *w = (double *)malloc((nnz > 0 ? nnz : 1) * sizeof(double));
+    for (int64_t i = 0; i <= nv; i++) (*off)[i] = G->edgeListPtrs[i];
+    for (int64_t j = 0; j < nnz; j++) {
+        (*tails)[j] = G->edgeList[j].tail;
+        (*w)[j] = G->edgeList[j].weight;
+    }
+    *ne = G->numEdges;
+    free_graph(G);
+    return nv;
+}
+
+int64_t ref_load_binary(const char *path, int64_t **off, int64_t **tails, double **w, int64_t *ne) {
+    graph *G = (graph *)calloc(1, sizeof(graph));
+    {
+        Quiet q(nullptr);
+        parse_EdgeListBinaryNew(G, (char *)path);
+    }
+    return take_graph(G, off, tails, w, ne);
+}
+
+int64_t ref_load_metis(const char *path, int64_t **off, int64_t **tails, double **w, int64_t *ne) {
+    graph *G = (graph *)calloc(1, sizeof(graph));
+    {
+        Quiet q(nullptr);
+        loadMetisFileFormat(G, path);
+    }
+    return take_graph(G, off, tails, w, ne);
+}
+
+// writeGraphBinaryFormatNew (-f 9 writer)
+void ref_write_binary(const char *path, int64_t nv, int64_t num_edges, const int64_t *off,
+                      const int64_t *tails, const double *w) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    {
+        Quiet q(nullptr);
+        writeGraphBinaryFormatNew(G, (char *)path, 1);
+    }
+    free_graph(G);
+}
 
 // The reference's own drivers, unchanged.  They consume the graph and print
 // their summary; the summary goes to `log_path` so the caller can read the

@@ -1,0 +1,283 @@
+"""Pins the oracle (oracle/grappolo_oracle.c) before it is trusted as the
+checker of the HIP engine:
+  * against the golden fixtures written by the compiled reference
+    (tests/golden/make_golden.py) -- runs everywhere;
+  * against the compiled reference itself on freshly generated graphs
+    (skipped where oracle/_ref was not built);
+  * against exact rational brute force on tiny graphs (best move, tie and
+    swap rules, modularity closed forms) and the invariants of coarsening.
+"""
+import glob
+import itertools
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from grappolo_amd.graph import Graph, gen_lfr, gen_planted
+
+from graphs import GOLDEN, catalog, karate
+
+FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "ref_*.npz")))
+need_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference checker not built")
+
+
+def _fixture(path):
+    z = np.load(path, allow_pickle=False)
+    g = Graph(z["off"], z["tails"], z["w"], numEdges=int(z["num_edges"]))
+    return g, z
+
+
+# ------------------------------------------------------------ golden vectors
+@pytest.mark.parametrize("path", FIXTURES, ids=lambda p: os.path.basename(p))
+def test_oracle_matches_golden(path):
+    g, z = _fixture(path)
+    colors, mc = oracle.color_greedy(g)
+    np.testing.assert_array_equal(colors, z["colors"])
+    assert mc == int(z["max_color"])
+    for nomap in (0, 1):
+        # both colored variants compute the same targets
+        q, C, it, _ = oracle.louvain_colored(g, colors, mc + 1, -1.0, 1e-6, frozen=False)
+        assert q == float(z[f"colored{nomap}_mod"]) and it == int(z[f"colored{nomap}_it"])
+        np.testing.assert_array_equal(C, z[f"colored{nomap}_C"])
+        q, C, it, _ = oracle.louvain_plain(g, -1.0, 1e-6, nomap=bool(nomap))
+        assert q == float(z[f"plain{nomap}_mod"]) and it == int(z[f"plain{nomap}_it"])
+        np.testing.assert_array_equal(C, z[f"plain{nomap}_C"])
+    Cr, k = oracle.renumber(z["colored0_C"])
+    np.testing.assert_array_equal(Cr, z["renum_C"])
+    assert k == int(z["renum_k"])
+    cg = oracle.coarsen(g, Cr, k)
+    np.testing.assert_array_equal(cg.edgeListPtrs, z["coarse_off"])
+    np.testing.assert_array_equal(cg.tails, z["coarse_tails"])
+    np.testing.assert_allclose(cg.weights, z["coarse_w"], rtol=1e-15, atol=0)
+    assert cg.numEdges == int(z["coarse_num_edges"])
+    C_orig, st = oracle.run_colored(g, coloring="greedy", frozen=False,
+                                    minGraphSize=int(z["run_colored_mgs"]))
+    np.testing.assert_array_equal(C_orig, z["run_colored_C"])
+    assert round(st["final_modularity"], 6) == pytest.approx(float(z["run_colored_final"]),
+                                                             abs=1e-6)
+    for b in (0, 1):
+        C_orig, st = oracle.run_basic(g, basicOpt=b)
+        np.testing.assert_array_equal(C_orig, z[f"run_basic{b}_C"])
+        assert st["final_modularity"] == pytest.approx(float(z[f"run_basic{b}_final"]), abs=1e-6)
+
+
+# --------------------------------------------------- the compiled reference
+REF_GRAPHS = {
+    "planted4k": lambda: gen_planted(4096, 16, 16, 0.3, seed=5),
+    "planted4k_w": lambda: gen_planted(4096, 16, 16, 0.3, seed=6, wlo=0.5, whi=2.0),
+    "lfr10k_w": lambda: gen_lfr(10000, seed=3, dmax=300, cmax=800),
+    "hubs": lambda: catalog()["hubs"],
+    "loops": lambda: catalog()["loops_isolated"],
+}
+
+
+@need_ref
+@pytest.mark.parametrize("name", list(REF_GRAPHS))
+def test_oracle_matches_compiled_reference(name):
+    g = REF_GRAPHS[name]()
+    colors, mc = oracle.ref_color(g, 1)
+    oc, omc = oracle.color_greedy(g)
+    np.testing.assert_array_equal(colors, oc)
+    for nomap in (False, True):
+        q, C, it = oracle.ref_colored_phase(g, colors, mc + 1, -1.0, 1e-6, nomap=nomap)
+        qo, Co, ito, _ = oracle.louvain_colored(g, colors, mc + 1, -1.0, 1e-6, frozen=False)
+        assert (q, it) == (qo, ito)
+        np.testing.assert_array_equal(C, Co)
+        q, C, it = oracle.ref_plain_phase(g, -1.0, 1e-6, nomap=nomap)
+        qo, Co, ito, _ = oracle.louvain_plain(g, -1.0, 1e-6, nomap=nomap)
+        assert (q, it) == (qo, ito)
+        np.testing.assert_array_equal(C, Co)
+    Cr, k = oracle.ref_renumber(C)
+    Co, ko = oracle.renumber(C)
+    assert k == ko
+    np.testing.assert_array_equal(Cr, Co)
+    rc = oracle.canonical(oracle.ref_coarsen(g, Cr, k, nthreads=1))
+    oc = oracle.coarsen(g, Co, ko)
+    np.testing.assert_array_equal(rc.edgeListPtrs, oc.edgeListPtrs)
+    np.testing.assert_array_equal(rc.tails, oc.tails)
+    np.testing.assert_array_equal(rc.weights, oc.weights)  # same one-thread sum order
+    assert rc.numEdges == oc.numEdges
+    C_orig, final = oracle.ref_run_colored(g, minGraphSize=100)
+    Co, st = oracle.run_colored(g, coloring="greedy", frozen=False, minGraphSize=100)
+    np.testing.assert_array_equal(C_orig, Co)
+    assert float(f"{st['final_modularity']:.6f}") == final
+
+
+@need_ref
+def test_coarsen_matches_reference_with_threads():
+    """With 8 threads the reference's rows come out in thread order and its
+    sums in lock order; the canonical form is the same for integer weights."""
+    g = gen_planted(4096, 16, 16, 0.3, seed=9)
+    C = np.random.default_rng(0).integers(0, 300, g.numVertices)
+    Cr, k = oracle.renumber(C)
+    rc = oracle.canonical(oracle.ref_coarsen(g, Cr, k, nthreads=8))
+    oc = oracle.coarsen(g, Cr, k)
+    for a, b in ((rc.edgeListPtrs, oc.edgeListPtrs), (rc.tails, oc.tails),
+                 (rc.weights, oc.weights)):
+        np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------------------- brute force
+def _exact_q(g, C):
+    """Modularity with rational arithmetic."""
+    two_m = Fraction(0)
+    a = {}
+    e = Fraction(0)
+    for i in range(g.numVertices):
+        for j in range(g.edgeListPtrs[i], g.edgeListPtrs[i + 1]):
+            w = Fraction(float(g.weights[j]))
+            two_m += w
+            if C[i] >= 0:
+                a[C[i]] = a.get(C[i], Fraction(0)) + w
+                if C[g.tails[j]] == C[i]:
+                    e += w
+    return e / two_m - sum(x * x for x in a.values()) / (two_m * two_m)
+
+
+def _tiny_graphs():
+    rng = np.random.default_rng(42)
+    out = []
+    for t in range(25):
+        n = int(rng.integers(4, 11))
+        pairs = [(u, v) for u, v in itertools.combinations(range(n), 2) if rng.random() < 0.45]
+        if t % 3 == 0:
+            pairs += [(u, u) for u in range(n) if rng.random() < 0.3]
+        if not pairs:
+            pairs = [(0, 1)]
+        w = rng.integers(1, 4, len(pairs)).astype(float) if t % 2 else None
+        out.append(Graph.from_pairs(n, pairs, w))
+    return out
+
+
+@pytest.mark.parametrize("gi", range(25))
+def test_best_move_is_exact_argmax_of_modularity_gain(gi):
+    """max() of utilityClusteringFunctions.cpp:226-259: the target maximises the
+    exact modularity gain among the neighbours' communities, needs a strictly
+    positive gain, prefers the smaller id on exact ties, and a singleton never
+    moves to a larger singleton."""
+    g = _tiny_graphs()[gi]
+    rng = np.random.default_rng(gi)
+    n = g.numVertices
+    for trial in range(4):
+        C = rng.integers(0, max(2, n // 2), n).astype(np.int64)
+        deg = g.degrees()
+        C[deg == 0] = -1
+        q0 = _exact_q(g, C)
+        for v in range(n):
+            if deg[v] == 0:
+                continue
+            colors = np.ones(n, dtype=np.int32)
+            colors[v] = 0
+            got, _ = oracle.color_step(g, C, colors, 0)
+            s = C[v]
+            cands = {int(C[t]) for t in g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]]} - {s}
+            gains = {}
+            for y in cands:
+                C2 = C.copy()
+                C2[v] = y
+                gains[y] = _exact_q(g, C2) - q0
+            best = s
+            pos = {y: d for y, d in gains.items() if d > 0}
+            if pos:
+                top = max(pos.values())
+                best = min(y for y, d in pos.items() if d == top)
+                size = np.bincount(C[C >= 0], minlength=n)
+                if size[best] == 1 and size[s] == 1 and best > s:
+                    best = s
+            assert got[v] == best, (gi, trial, v, gains)
+
+
+def test_modularity_closed_forms():
+    # two disjoint triangles in two clusters: 1 - 2 * (1/2)^2
+    t = Graph.from_pairs(6, [(0, 1), (1, 2), (0, 2), (3, 4), (4, 5), (3, 5)])
+    assert oracle.modularity(t, np.array([0, 0, 0, 1, 1, 1])) == pytest.approx(0.5, abs=1e-15)
+    # everything in one cluster: 0
+    assert oracle.modularity(t, np.zeros(6, np.int64)) == pytest.approx(0.0, abs=1e-15)
+    # ring of 4 cliques K5 joined by single edges, one cluster per clique
+    pairs = []
+    for c in range(4):
+        pairs += [(5 * c + i, 5 * c + j) for i, j in itertools.combinations(range(5), 2)]
+        pairs.append((5 * c + 4, (5 * c + 5) % 20))
+    g = Graph.from_pairs(20, pairs)
+    m = 4 * 10 + 4
+    exp = 4 * (10 / m - (22 / (2 * m)) ** 2)  # a_c = 2*10 internal + 2 ring endpoints
+    assert oracle.modularity(g, np.repeat(np.arange(4), 5)) == pytest.approx(exp, abs=1e-15)
+    # against exact arithmetic on random labellings
+    for gi, g in enumerate(_tiny_graphs()[:10]):
+        C = np.random.default_rng(gi).integers(0, 3, g.numVertices).astype(np.int64)
+        C[g.degrees() == 0] = -1
+        assert oracle.modularity(g, C) == pytest.approx(float(_exact_q(g, C)), abs=1e-15)
+
+
+def test_renumber_is_first_appearance_order():
+    C = np.array([5, 5, -1, 2, 9, 2, 5, -1, 0, 9], np.int64)
+    got, k = oracle.renumber(C)
+    np.testing.assert_array_equal(got, [0, 0, -1, 1, 2, 1, 0, -1, 3, 2])
+    assert k == 4
+
+
+@pytest.mark.parametrize("name", ["karate", "planted_w", "loops_isolated", "hubs"])
+def test_coarsen_invariants(name):
+    g = catalog()[name]
+    C = np.random.default_rng(3).integers(0, max(2, g.numVertices // 10), g.numVertices)
+    C[g.degrees() == 0] = -1
+    Cr, k = oracle.renumber(C)
+    cg = oracle.coarsen(g, Cr, k)
+    assert cg.numVertices == k
+    rows = np.repeat(np.arange(k), cg.degrees())
+    # every cluster has exactly one self-loop entry; rows sorted, no repeats
+    loops = rows == cg.tails
+    assert np.bincount(rows[loops], minlength=k).tolist() == [1] * k
+    key = rows * k + cg.tails
+    assert np.all(np.diff(key) > 0)
+    # symmetric with equal weights
+    order = np.lexsort((rows, cg.tails))
+    np.testing.assert_array_equal(cg.tails[order], rows)
+    np.testing.assert_array_equal(cg.weights[order], cg.weights)
+    # total weight kept; numEdges = NE_out + k (buildNextPhase.cpp:237)
+    assert cg.weights.sum() == pytest.approx(g.weights.sum(), rel=1e-12)
+    assert cg.numEdges == (cg.nnz - k) // 2 + k
+    # contraction keeps modularity: Q(G, C) == Q(G', singletons)
+    assert oracle.modularity(cg, np.arange(k, dtype=np.int64)) == pytest.approx(
+        oracle.modularity(g, Cr), abs=1e-12)
+
+
+@pytest.mark.parametrize("name", ["karate", "planted_unit", "lfr_w", "rmat12", "hubs"])
+def test_colorings_valid(name):
+    g = catalog()[name]
+    for colors, mc in (oracle.color_greedy(g), oracle.color_jp(g)[:2]):
+        assert colors.min() >= 0 and colors.max() == mc
+        assert oracle.color_conflicts(g, colors) == 0
+
+
+def test_jp_winners_are_local_maxima():
+    """Jones-Plassmann: every vertex's color is the smallest one absent from
+    its neighbours of higher priority (those are colored in earlier rounds)."""
+    g = catalog()["planted_unit"]
+    colors, mc, rounds = oracle.color_jp(g)
+    pr = np.array([oracle.jp_priority(v) for v in range(g.numVertices)], dtype=np.uint64)
+    for v in range(0, g.numVertices, 7):
+        nb = g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]]
+        nb = nb[nb != v]
+        higher = set(colors[nb[pr[nb] > pr[v]]].tolist())
+        first_free = next(c for c in range(mc + 2) if c not in higher)
+        assert colors[v] == first_free
+
+
+def test_frozen_equals_sequential_with_valid_coloring():
+    g = catalog()["lfr_w"]
+    colors, mc = oracle.color_greedy(g)
+    a = oracle.louvain_colored(g, colors, mc + 1, -1.0, 1e-6, frozen=True)
+    b = oracle.louvain_colored(g, colors, mc + 1, -1.0, 1e-6, frozen=False)
+    assert a[0] == b[0] and a[2] == b[2]
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_karate_modularity_near_known_optimum():
+    """Zachary's karate club: the best partition has Q ~ 0.4198; Louvain runs
+    land just below it."""
+    C, st = oracle.run_colored(karate(), coloring="greedy", minGraphSize=10)
+    assert 0.40 <= st["final_modularity"] <= 0.4198 + 1e-4
