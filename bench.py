@@ -164,11 +164,12 @@ def main():
     elapsed = t1 - t0
     ev = sum(r["edges_visited"] for r in runs)
     last = runs[-1]
+    pp = last["per_phase"]
     value = ev / elapsed
 
     roofline = None
     if kern:
-        sweeps = {k: v for k, v in kern.items() if v[2] > 0}
+        sweeps = {k: v for k, v in kern.items() if v[2] > 0 and k.startswith("sweep")}
         dom = max(sweeps, key=lambda k: sweeps[k][0])
         ms, by, n = sweeps[dom]
         achieved = by / (ms * 1e-3) / 1e9
@@ -196,6 +197,11 @@ def main():
         "final_modularity": last["final_modularity"], "phases": last["phases"],
         "iterations": last["total_iterations"], "colors": last["colors_first"],
         "clusters": last["final_clusters"], "edges_visited_per_step": ev / args.steps,
+        "per_phase": [{"nv": int(pp["phase_nv"][i]), "nnz": int(pp["phase_nnz"][i]),
+                       "colors": int(pp["phase_colors"][i]), "iterations": int(pp["phase_iters"][i]),
+                       "clusters": int(pp["phase_clusters"][i]),
+                       "ms": round(float(pp["phase_seconds"][i]) * 1e3, 2)}
+                      for i in range(len(pp["phase_nv"]))],
         "roofline": roofline, "cpu_baseline": cpu,
         "kernels": {k: {"ms": round(v[0], 3), "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1)
                         if v[0] > 0 else None, "launches": v[2]} for k, v in kern.items() if v[2]},

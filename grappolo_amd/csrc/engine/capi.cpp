@@ -174,6 +174,7 @@ void fill_stats(const glv::RunResult &r, glv_stats *st) {
         if (st->phase_colored) st->phase_colored[i] = p.colored;
         if (st->phase_mod) st->phase_mod[i] = p.mod;
         if (st->phase_clusters) st->phase_clusters[i] = p.clusters;
+        if (st->phase_seconds) st->phase_seconds[i] = p.t_sweep + p.t_renumber + p.t_coarsen + p.t_color;
     }
     st->n_trace = (int32_t)r.trace.size();
     for (int i = 0; i < (int)r.trace.size() && i < st->trace_cap; i++)

@@ -29,6 +29,10 @@ constexpr int kBinGroup[6] = {0, 4, 8, 16, 32, 64};
 constexpr i64 kBinMaxDeg[kNumBins] = {0, 4, 8, 16, 32, 64, 256, 2048, INT64_MAX};
 constexpr int kWaveHashSlots = 512;
 constexpr int kBlockHashSlots = 4096;
+// Graphs of at most this many vertices sweep their high-degree bins with a
+// dense LDS accumulator indexed by community id (16384 doubles = 128 KB).
+constexpr i64 kDenseMaxNV = 16384;
+constexpr int kDenseBlocks = 1024;
 
 __host__ __device__ inline int degree_bin(i64 d) {
     if (d == 0) return 0;

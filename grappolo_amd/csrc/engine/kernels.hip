@@ -143,20 +143,41 @@ __global__ void k_init_labels(const i64 *off, i64 nv, i64 *C) {
 }
 
 __global__ void k_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
-                           i64 *vid, int32_t *hist, i64 *hist_nnz) {
+                           i64 *vid, int32_t *err) {
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
-        int b = degree_bin(off[i + 1] - off[i]);
-        int c = colors ? colors[i] : 0;
+        const int b = degree_bin(off[i + 1] - off[i]);
+        const int c = colors ? colors[i] : 0;
         vid[i] = i;
-        if (c < 0 || c >= ncol) {  // counted nowhere: the caller sees sum(hist) != nv
+        if (c < 0 || c >= ncol) {
+            atomicOr(err, 1);
             keys[i] = 0;
             continue;
         }
-        uint32_t key = (uint32_t)(b * ncol + c);
-        keys[i] = key;
-        atomicAdd(&hist[key], 1);
-        const i64 d = off[i + 1] - off[i];
-        if (d) atomicAdd((unsigned long long *)&hist_nnz[key], (unsigned long long)d);
+        keys[i] = (uint32_t)(b * ncol + c);
+    }
+}
+// start[k] = first position of key k in the sorted keys (n when absent past the end)
+__global__ void k_slice_starts(const uint32_t *keys, i64 n, i64 nkeys, i64 *start) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const i64 k = keys[i];
+        const i64 prev = i == 0 ? -1 : (i64)keys[i - 1];
+        for (i64 kk = prev + 1; kk <= k; kk++) start[kk] = i;
+        if (i == n - 1)
+            for (i64 kk = k + 1; kk <= nkeys; kk++) start[kk] = n;
+    }
+}
+__global__ void k_perm_degree(const i64 *perm, const i64 *off, i64 n, i64 *degp) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const i64 v = perm[i];
+        degp[i] = off[v + 1] - off[v];
+    }
+}
+__global__ void k_slice_nnz(const i64 *start, const i64 *dscan, const i64 *degp, i64 n, i64 nkeys,
+                            i64 *pnnz) {
+    const i64 total = dscan[n - 1] + degp[n - 1];
+    for (i64 k = blockIdx.x * (i64)blockDim.x + threadIdx.x; k < nkeys; k += (i64)gridDim.x * blockDim.x) {
+        const i64 a = start[k], b = start[k + 1];
+        pnnz[k] = (b < n ? dscan[b] : total) - (a < n ? dscan[a] : total);
     }
 }
 
@@ -418,7 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
         } else {
             keys = l_keys;
             vals = l_vals;
-            slots = kBlockHashSlots;
+            slots = 64;
+            while (slots < 2 * (end - beg)) slots <<= 1;
         }
         int lg = 0;
         while (((i64)1 << lg) < slots) lg++;
@@ -458,6 +480,81 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
             A.tgt[v] = target;
         }
         part += e_s;
+        __syncthreads();
+    }
+    if (PLAIN && threadIdx.x == 0) A.eii_part[blockIdx.x] = part;
+}
+
+// One workgroup per vertex with a dense LDS accumulator indexed by community
+// id (graphs of at most kDenseMaxNV vertices: the coarse phases).  A bitmap
+// records which communities were touched, so clearing and scoring cost the
+// vertex's own entries plus nv/32 bitmap words, not nv slots.
+template <bool PLAIN>
+__global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
+    __shared__ double acc[kDenseMaxNV];
+    __shared__ uint32_t touched[kDenseMaxNV / 32];
+    const int nv = (int)A.nv;
+    const int nwords = (nv + 31) / 32;
+    // acc[] and touched[] are all zero between vertices
+    for (int i = threadIdx.x; i < nv; i += kBlock) acc[i] = 0.0;
+    for (int i = threadIdx.x; i < nwords; i += kBlock) touched[i] = 0;
+    double part = 0;
+    __syncthreads();
+    for (i64 q = blockIdx.x; q < A.count; q += gridDim.x) {
+        const i64 v = A.verts[q];
+        const i64 beg = A.off[v], end = A.off[v + 1];
+        double sl = 0;
+        for (i64 j = beg + threadIdx.x; j < end; j += kBlock) {
+            const i64 t = A.tails[j];
+            const double ww = A.w[j];
+            if (t == v) sl += ww;
+            const i64 c = A.C[t];
+            atomicOr(&touched[c >> 5], 1u << (c & 31));
+            atomicAdd(&acc[c], ww);
+        }
+        __syncthreads();
+        sl = block_sum(sl);
+        const i64 s = A.C[v];
+        const double e_s = acc[s];
+        const double kv = A.k[v];
+        const double eix = e_s - sl, ax = A.a[s] - kv;
+        bool cand = false;
+        double gain = 0;
+        i64 id = INT64_MAX;
+        for (int wi = threadIdx.x; wi < nwords; wi += kBlock) {
+            uint32_t m = touched[wi];
+            while (m) {
+                const int b = __ffs((int)m) - 1;
+                m &= m - 1;
+                const i64 y = (i64)wi * 32 + b;
+                if (y == s) continue;
+                const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
+                if (better(g > 0, g, y, cand, gain, id)) {
+                    cand = true;
+                    gain = g;
+                    id = y;
+                }
+            }
+        }
+        block_argmax(cand, gain, id);  // ends with a barrier: every read of acc is done
+        if (threadIdx.x == 0) {
+            i64 target = s;
+            if (cand) {
+                target = id;
+                if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
+            }
+            A.tgt[v] = target;
+        }
+        part += e_s;
+        for (int wi = threadIdx.x; wi < nwords; wi += kBlock) {
+            uint32_t m = touched[wi];
+            while (m) {
+                const int b = __ffs((int)m) - 1;
+                m &= m - 1;
+                acc[wi * 32 + b] = 0.0;
+            }
+            touched[wi] = 0;
+        }
         __syncthreads();
     }
     if (PLAIN && threadIdx.x == 0) A.eii_part[blockIdx.x] = part;
@@ -573,15 +670,79 @@ __global__ __launch_bounds__(kBlock) void k_mod_pass(const i64 *off, const i64 *
 // Packed coloring state: -1 uncoloured, else (round << 12) | color.
 constexpr int kColorBits = 12;
 
+// Appends `v` to list[] when `pred`, one atomic per wavefront.
+__device__ inline void wave_append(bool pred, i64 v, i64 *list, i64 *count) {
+    const u64 m = __ballot(pred);
+    if (!m) return;
+    const int lane = threadIdx.x % kWave;
+    const int leader = __ffsll((long long)m) - 1;
+    i64 base = 0;
+    if (lane == leader) base = (i64)atomicAdd((unsigned long long *)count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    const u64 lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    if (pred) list[base + __popcll(m & lt)] = v;
+}
+
 __global__ void k_jp_partition(const i64 *off, i64 nv, i64 *small, i64 *large, i64 *counts) {
-    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
-        if (off[v + 1] - off[v] > kBinMaxDeg[6]) {
-            i64 p = (i64)atomicAdd((unsigned long long *)&counts[1], 1ULL);
-            large[p] = v;
-        } else {
-            i64 p = (i64)atomicAdd((unsigned long long *)&counts[0], 1ULL);
-            small[p] = v;
+    // all lanes of a wavefront iterate together so the ballots are complete
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    const i64 start = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+    const i64 trips = (nv + stride - 1) / stride;
+    for (i64 t = 0; t < trips; t++) {
+        const i64 v = start + t * stride;
+        const bool in = v < nv;
+        const bool big = in && (off[v + 1] - off[v] > kBinMaxDeg[6]);
+        wave_append(in && !big, v, small, &counts[0]);
+        wave_append(big, v, large, &counts[1]);
+    }
+}
+
+// Vertices still uncoloured, in increasing id (flags + scan + scatter).
+__global__ void k_flag_uncolored(const int32_t *state, i64 nv, i64 *flags) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        flags[v] = state[v] < 0 ? 1 : 0;
+}
+__global__ void k_scatter_flagged(const i64 *flags, const i64 *pos, i64 nv, i64 *list) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        if (flags[v]) list[pos[v]] = v;
+}
+
+// Sequential first-fit of list[0..n) in order, by one workgroup: the
+// reference's one-thread tentative pass over the vertices JP left.
+constexpr int kTailBlock = 1024;
+__global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const i64 *tails,
+                                                           int32_t *state, const i64 *list, i64 n,
+                                                           int round, int32_t *err) {
+    __shared__ uint32_t mark[kMaxColors / 32];
+    __shared__ int s_color;
+    for (i64 q = 0; q < n; q++) {
+        const i64 v = list[q];
+        for (int i = threadIdx.x; i < kMaxColors / 32; i += kTailBlock) mark[i] = 0;
+        if (threadIdx.x == 0) s_color = kMaxColors;
+        __syncthreads();
+        for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += kTailBlock) {
+            const i64 u = tails[j];
+            if (u == v) continue;
+            const int32_t su = __atomic_load_n(&state[u], __ATOMIC_RELAXED);
+            if (su < 0) continue;
+            const int c = su & (kMaxColors - 1);
+            atomicOr(&mark[c >> 5], 1u << (c & 31));
         }
+        __syncthreads();
+        if (threadIdx.x < kMaxColors / 32) {
+            const uint32_t m = mark[threadIdx.x];
+            if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int color = s_color;
+            if (color >= kMaxColors) {
+                atomicOr(err, 1);
+                color = 0;
+            }
+            __atomic_store_n(&state[v], (int32_t)((round << 12) | color), __ATOMIC_RELAXED);
+        }
+        __syncthreads();
     }
 }
 
@@ -594,24 +755,28 @@ __device__ inline bool blocks_round(int32_t su, int round) {
 // One thread per vertex of degree <= 256.
 __global__ void k_jp_round_thread(const i64 *off, const i64 *tails, int32_t *state, const i64 *work,
                                   i64 nwork, i64 *next, i64 *counts, int round, int32_t *err) {
-    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < nwork; q += (i64)gridDim.x * blockDim.x) {
-        const i64 v = work[q];
-        const u64 pv = jp_priority(v);
-        const i64 b = off[v], e = off[v + 1];
-        bool win = true;
-        for (i64 j = b; j < e; j++) {
-            const i64 u = tails[j];
-            if (u == v) continue;
-            if (blocks_round(ld_relaxed(&state[u]), round) && jp_priority(u) > pv) {
-                win = false;
-                break;
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    const i64 first = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+    const i64 trips = (nwork + stride - 1) / stride;
+    for (i64 t = 0; t < trips; t++) {
+        const i64 q = first + t * stride;
+        const bool in = q < nwork;
+        const i64 v = in ? work[q] : 0;
+        bool win = in;
+        if (in) {
+            const u64 pv = jp_priority(v);
+            for (i64 j = off[v]; j < off[v + 1]; j++) {
+                const i64 u = tails[j];
+                if (u == v) continue;
+                if (blocks_round(ld_relaxed(&state[u]), round) && jp_priority(u) > pv) {
+                    win = false;
+                    break;
+                }
             }
         }
-        if (!win) {
-            i64 p = (i64)atomicAdd((unsigned long long *)&counts[0], 1ULL);
-            next[p] = v;
-            continue;
-        }
+        wave_append(in && !win, v, next, &counts[0]);
+        if (!win) continue;
+        const i64 b = off[v], e = off[v + 1];
         int color = -1;
         for (int base = 0; base < kMaxColors && color < 0; base += 64) {
             u64 mask = 0;
@@ -885,10 +1050,21 @@ void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s) {
     GLV_LAUNCH_CHECK();
 }
 void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys, i64 *vid,
-                     int32_t *hist, i64 *hist_nnz, hipStream_t s) {
+                     int32_t *err, hipStream_t s) {
     if (nv <= 0) return;
-    k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, hist,
-                                                            hist_nnz);
+    k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, err);
+    GLV_LAUNCH_CHECK();
+}
+void launch_slices(const uint32_t *sorted_keys, const i64 *perm, const i64 *off, i64 n, i64 nkeys,
+                   i64 *start, i64 *pnnz, hipStream_t s) {
+    if (n <= 0) return;
+    DBuf<i64> degp(n, s), dscan(n, s);
+    k_slice_starts<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(sorted_keys, n, nkeys, start);
+    GLV_LAUNCH_CHECK();
+    k_perm_degree<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(perm, off, n, degp.p);
+    GLV_LAUNCH_CHECK();
+    scan_exclusive_i64(degp.p, dscan.p, n, s);
+    k_slice_nnz<<<grid_for(nkeys, kBlock, 8192), kBlock, 0, s>>>(start, dscan.p, degp.p, n, nkeys, pnnz);
     GLV_LAUNCH_CHECK();
 }
 void launch_degree(const DevGraph &g, const i64 *big, i64 nbig, double *k, hipStream_t s) {
@@ -928,8 +1104,11 @@ void launch_sum_partials(const double *part, i64 np, double *out, hipStream_t s)
     GLV_LAUNCH_CHECK();
 }
 
-i64 sweep_blocks(int bin, i64 count) {
+static bool use_dense(int bin, i64 nv) { return bin >= 7 && nv <= kDenseMaxNV; }
+
+i64 sweep_blocks(int bin, i64 count, i64 nv) {
     if (count <= 0) return 0;
+    if (use_dense(bin, nv)) return std::min<i64>(count, kDenseBlocks);
     switch (bin) {
         case 1: return (count + kBlock / 4 - 1) / (kBlock / 4);
         case 2: return (count + kBlock / 8 - 1) / (kBlock / 8);
@@ -945,8 +1124,13 @@ i64 sweep_blocks(int bin, i64 count) {
 
 template <bool PLAIN>
 static void sweep_dispatch(int bin, const SweepArgs &A, hipStream_t s) {
-    const unsigned g = (unsigned)sweep_blocks(bin, A.count);
+    const unsigned g = (unsigned)sweep_blocks(bin, A.count, A.nv);
     if (g == 0) return;
+    if (use_dense(bin, A.nv)) {
+        k_sweep_dense<PLAIN><<<g, kBlock, 0, s>>>(A);
+        GLV_LAUNCH_CHECK();
+        return;
+    }
     switch (bin) {
         case 1: k_sweep_group<4, PLAIN><<<g, kBlock, 0, s>>>(A); break;
         case 2: k_sweep_group<8, PLAIN><<<g, kBlock, 0, s>>>(A); break;
@@ -1043,6 +1227,17 @@ void launch_jp_round(const DevGraph &g, int32_t *state, const i64 *small, i64 ns
             g.off, g.tails, state, large, nlarge, nlarge_out, counts, round, err);
         GLV_LAUNCH_CHECK();
     }
+}
+void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
+                       int round, int32_t *err, hipStream_t s) {
+    if (n <= 0) return;
+    k_flag_uncolored<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(state, g.nv, flags);
+    GLV_LAUNCH_CHECK();
+    scan_exclusive_i64(flags, pos, g.nv, s);
+    k_scatter_flagged<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(flags, pos, g.nv, list);
+    GLV_LAUNCH_CHECK();
+    k_color_tail<<<1, kTailBlock, 0, s>>>(g.off, g.tails, state, list, n, round, err);
+    GLV_LAUNCH_CHECK();
 }
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s) {
     if (nv <= 0) return;

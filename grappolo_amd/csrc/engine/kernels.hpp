@@ -25,6 +25,7 @@ struct SweepArgs {
     i64 *gkeys;        // global hash (bin 8): per-block regions of gslots slots
     double *gvals;
     i64 gslots;
+    i64 nv;            // vertices of the graph (community ids are < nv)
 };
 
 // Up to one slice per degree bin of a color class (perm[lo[b] .. hi[b])).
@@ -40,7 +41,10 @@ void launch_fill_i32(int32_t *p, i64 n, int32_t v, hipStream_t s);
 void launch_iota(i64 *p, i64 n, hipStream_t s);
 void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s);  // C[i] = deg? i : -1
 void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
-                     i64 *vid, int32_t *hist, i64 *hist_nnz, hipStream_t s);
+                     i64 *vid, int32_t *err, hipStream_t s);
+// slice starts (nkeys+1) and entries per slice (nkeys) of keys sorted with perm
+void launch_slices(const uint32_t *sorted_keys, const i64 *perm, const i64 *off, i64 n, i64 nkeys,
+                   i64 *start, i64 *pnnz, hipStream_t s);
 void launch_degree(const DevGraph &g, const i64 *big, i64 nbig, double *k, hipStream_t s);
 void launch_init_comm(const double *k, i64 nv, double *a, i64 *n, hipStream_t s);
 void launch_comm_from_labels(const i64 *C, const double *k, i64 nv, double *a, i64 *n,
@@ -55,7 +59,7 @@ void launch_sum_partials(const double *part, i64 np, double *out, hipStream_t s)
 
 // --- sweeps --------------------------------------------------------------
 // number of block partials the sweep of `bin` writes for `count` vertices
-i64 sweep_blocks(int bin, i64 count);
+i64 sweep_blocks(int bin, i64 count, i64 nv);
 void launch_sweep(int bin, bool plain, const SweepArgs &A, hipStream_t s);
 void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hipStream_t s);
 constexpr int kGlobalHashBlocks = 256;
@@ -81,6 +85,11 @@ void launch_jp_round(const DevGraph &g, int32_t *state, const i64 *small, i64 ns
                      const i64 *large, i64 nlarge, i64 *nsmall_out, i64 *nlarge_out, i64 *counts,
                      int round, int32_t *err, hipStream_t s);
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
+// first-fit, one vertex at a time in increasing id, of the n uncoloured vertices
+void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
+                       int round, int32_t *err, hipStream_t s);
+constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many uncoloured vertices
+constexpr i64 kJpMinWon = 32;   // ... or once a round colors fewer vertices than this
 void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);
 
 // --- renumber / compose --------------------------------------------------

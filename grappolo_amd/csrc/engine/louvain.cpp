@@ -50,20 +50,22 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
     P.n.alloc(nv, s_);
     P.perm.alloc(nv, s_);
     DBuf<uint32_t> keys(nv, s_);
-    DBuf<int32_t> hist(nkeys, s_);
-    DBuf<i64> hist_nnz(nkeys, s_);
-    launch_fill_i32(hist.p, nkeys, 0, s_);
-    launch_fill_i64(hist_nnz.p, nkeys, 0, s_);
-    launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, hist.p, hist_nnz.p, s_);
+    DBuf<int32_t> err(1, s_);
+    DBuf<i64> start(nkeys + 1, s_), pnnz(nkeys, s_);
+    launch_fill_i32(err.p, 1, 0, s_);
+    launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, err.p, s_);
     sort_pairs_u32_i64(keys.p, P.perm.p, nv, bits_for(nkeys), s_);
-    std::vector<int32_t> h(nkeys);
-    P.pnnz.assign(nkeys, 0);
-    GLV_HIP(hipMemcpyAsync(h.data(), hist.p, nkeys * sizeof(int32_t), hipMemcpyDeviceToHost, s_));
-    GLV_HIP(hipMemcpyAsync(P.pnnz.data(), hist_nnz.p, nkeys * sizeof(i64), hipMemcpyDeviceToHost, s_));
-    GLV_HIP(hipStreamSynchronize(s_));
+    launch_slices(keys.p, P.perm.p, g.off, nv, nkeys, start.p, pnnz.p, s_);
     P.ptr.assign(nkeys + 1, 0);
-    for (i64 q = 0; q < nkeys; q++) P.ptr[q + 1] = P.ptr[q] + h[q];
-    if (P.ptr[nkeys] != nv) fail(GLV_E_INVALID_ARG, "color values out of range [0, nColors)");
+    P.pnnz.assign(nkeys, 0);
+    int32_t e = 0;
+    GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s_));
+    if (nv > 0) {
+        GLV_HIP(hipMemcpyAsync(P.ptr.data(), start.p, (nkeys + 1) * sizeof(i64), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipMemcpyAsync(P.pnnz.data(), pnnz.p, nkeys * sizeof(i64), hipMemcpyDeviceToHost, s_));
+    }
+    GLV_HIP(hipStreamSynchronize(s_));
+    if (e) fail(GLV_E_INVALID_ARG, "color values out of range [0, nColors)");
     // degrees and 1/(2m)
     const i64 big_lo = P.bin_lo(8), big_hi = P.bin_hi(8);
     launch_degree(g, P.perm.p + big_lo, big_hi - big_lo, P.k.p, s_);
@@ -99,6 +101,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     int round = 0;
     while (ns + nl > 0) {
         if (round >= (1 << 18)) fail(GLV_E_DEVICE, "coloring did not converge");
+        const i64 before = ns + nl;
         launch_fill_i64(counts.p, 2, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
         launch_jp_round(g, colors, small, ns, large, nl, small_next, large_next, counts.p, round, err.p, s_);
@@ -110,6 +113,14 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         std::swap(small, small_next);
         std::swap(large, large_next);
         round++;
+        const i64 rest = ns + nl, won = before - rest;
+        if (rest > 0 && (rest <= kJpTail || won < kJpMinWon)) {
+            // JP rounds have stopped paying for themselves: finish one vertex at a time
+            Profiler::get().begin(K_COLOR, s_);
+            launch_color_tail(g, colors, sB.p, sA.p, lB.p, rest, round, err.p, s_);
+            Profiler::get().end(K_COLOR, 0.0, s_);
+            break;
+        }
     }
     launch_jp_finish(colors, nv, maxc.p, s_);
     if (fetch(err.p)) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
@@ -189,12 +200,18 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     A.gkeys = gkeys.p;
     A.gvals = gvals.p;
     A.gslots = gslots;
+    A.nv = nv;
 
+    std::vector<char> color_nonempty(ncol, 0);
+    for (int c = 0; c < ncol; c++)
+        for (int b = 1; b < kNumBins; b++)
+            if (P.hi(b, c) > P.lo(b, c)) color_nonempty[c] = 1;
     double prevMod = Lower, currMod = -1;
     int numItrs = 0;
     while (true) {
         numItrs++;
         for (int c = 0; c < ncol; c++) {
+            if (!color_nonempty[c]) continue;  // stale colors can leave classes empty
             Slices S;
             S.lo[0] = S.hi[0] = 0;  // isolated vertices were labelled -1 at init
             for (int b = 1; b < kNumBins; b++) {
@@ -243,7 +260,7 @@ double Engine::plain_phase(const DevGraph &g, i64 *C, double Lower, double thres
         launch_iota(curr, nv, s_);
     }
     i64 nparts = kRedBlocks;
-    for (int b = 1; b < kNumBins; b++) nparts += sweep_blocks(b, P.bin_hi(b) - P.bin_lo(b));
+    for (int b = 1; b < kNumBins; b++) nparts += sweep_blocks(b, P.bin_hi(b) - P.bin_lo(b), nv);
     DBuf<double> part(nparts, s_), res(2, s_);
     DBuf<i64> gkeys;
     DBuf<double> gvals;
@@ -272,6 +289,7 @@ double Engine::plain_phase(const DevGraph &g, i64 *C, double Lower, double thres
         A.gkeys = gkeys.p;
         A.gvals = gvals.p;
         A.gslots = gslots;
+    A.nv = nv;
         i64 off = 0;
         for (int b = 1; b < kNumBins; b++) {
             const i64 lo = P.bin_lo(b), cnt = P.bin_hi(b) - lo;
@@ -282,7 +300,7 @@ double Engine::plain_phase(const DevGraph &g, i64 *C, double Lower, double thres
             Profiler::get().begin(K_SWEEP_B1 + b - 1, s_);
             launch_sweep(b, true, A, s_);
             Profiler::get().end(K_SWEEP_B1 + b - 1, 24.0 * P.bin_nnz(b) + 32.0 * cnt, s_);
-            off += sweep_blocks(b, cnt);
+            off += sweep_blocks(b, cnt, nv);
         }
         launch_isolated_targets(P.perm.p + P.bin_lo(0), P.bin_hi(0) - P.bin_lo(0), tgt, s_);
         if (off == 0) launch_fill_f64(res.p, 1, 0.0, s_);
@@ -341,6 +359,7 @@ i64 Engine::color_step(const DevGraph &g, i64 *C, const int32_t *colors, int nco
     A.gkeys = gkeys.p;
     A.gvals = gvals.p;
     A.gslots = gslots;
+    A.nv = nv;
     Slices S;
     S.lo[0] = S.hi[0] = 0;
     for (int b = 1; b < kNumBins; b++) {

@@ -42,7 +42,7 @@ class Stats(ctypes.Structure):
                 ("n_trace", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("phase_nv", _I64P), ("phase_nnz", _I64P), ("phase_iters", _I32P),
                 ("phase_colors", _I32P), ("phase_colored", _I32P), ("phase_mod", _F64P),
-                ("phase_clusters", _I64P), ("trace_q", _F64P)]
+                ("phase_clusters", _I64P), ("trace_q", _F64P), ("phase_seconds", _F64P)]
 
 
 _SIG = {
@@ -129,7 +129,8 @@ def new_stats(phase_cap=256, trace_cap=4096):
                 phase_iters=np.zeros(phase_cap, np.int32),
                 phase_colors=np.zeros(phase_cap, np.int32),
                 phase_colored=np.zeros(phase_cap, np.int32), phase_mod=np.zeros(phase_cap),
-                phase_clusters=np.zeros(phase_cap, np.int64), trace_q=np.zeros(trace_cap))
+                phase_clusters=np.zeros(phase_cap, np.int64), trace_q=np.zeros(trace_cap),
+                phase_seconds=np.zeros(phase_cap))
     st.phase_cap = phase_cap
     st.trace_cap = trace_cap
     types = dict(Stats._fields_)

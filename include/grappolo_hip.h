@@ -67,6 +67,7 @@ typedef struct glv_stats {
     double *phase_mod;
     int64_t *phase_clusters;
     double *trace_q;          /* modularity after every iteration of every phase */
+    double *phase_seconds;    /* wall time of each phase: sweeps, renumbering, coarsening, coloring */
 } glv_stats;
 
 /* ---- device and errors ------------------------------------------------ */

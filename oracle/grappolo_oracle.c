@@ -8,7 +8,7 @@
  * Floating-point expressions are written in the reference's evaluation order
  * and this file is compiled with -ffp-contract=off, so that at one thread the
  * oracle reproduces the reference bit for bit (checked against the compiled
- * reference in tests/test_oracle_vs_reference.py).
+ * reference in tests/test_oracle.py).
  */
 #include "grappolo_oracle.h"
 
@@ -207,11 +207,16 @@ uint64_t orc_jp_priority(i64 v) {
     return z ^ (z >> 31);
 }
 
-/* Restatement of the engine's Jones-Plassmann rounds (grappolo_amd/csrc/
- * k_color.hip): in each round every uncoloured vertex whose priority beats all
- * uncoloured neighbours (self-loops ignored) takes the smallest color absent
- * from its coloured neighbours.  Winners of one round are pairwise
- * non-adjacent, so the result is independent of processing order. */
+/* Restatement of the engine's coloring (grappolo_amd/csrc/engine/
+ * kernels.hip, k_jp_* and k_color_tail): Jones-Plassmann rounds -- every
+ * uncoloured vertex whose priority beats all uncoloured neighbours (self-loops
+ * ignored) takes the smallest color absent from its coloured neighbours;
+ * winners of one round are pairwise non-adjacent, so the order inside a round
+ * does not matter.  Once a round colors fewer than ORC_JP_MIN_WON vertices,
+ * or at most ORC_JP_TAIL remain uncoloured, the rest is colored first-fit one
+ * vertex at a time in increasing id (the reference's one-thread order). */
+#define ORC_JP_TAIL 1024
+#define ORC_JP_MIN_WON 32
 int orc_color_jp(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int32_t *rounds_out) {
     unsigned char mark[ORC_MAX_COLORS];
     i64 *work = (i64 *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(i64));
@@ -267,6 +272,27 @@ int orc_color_jp(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int3
         }
         nwork = nrest;
         rounds++;
+        if (nwork > 0 && (nwork <= ORC_JP_TAIL || nwin < ORC_JP_MIN_WON)) break;
+    }
+    /* sequential first-fit of the remaining vertices in increasing id */
+    for (i64 v = 0; v < nv && nwork > 0; v++) {
+        if (colors[v] >= 0) continue;
+        memset(mark, 0, sizeof(mark));
+        for (i64 j = off[v]; j < off[v + 1]; j++) {
+            i64 u = tails[j];
+            if (u == v) continue;
+            int c = colors[u];
+            if (c >= 0 && c < ORC_MAX_COLORS) mark[c] = 1;
+        }
+        int my = 0;
+        while (my < ORC_MAX_COLORS && mark[my]) my++;
+        if (my >= ORC_MAX_COLORS) {
+            free(work);
+            free(win);
+            return -2;
+        }
+        colors[v] = my;
+        if (my > maxc) maxc = my;
     }
     free(work);
     free(win);

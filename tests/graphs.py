@@ -49,6 +49,8 @@ def catalog():
         "lfr_w": gen_lfr(20000, seed=3, dmax=600, cmax=1500),
         "rmat12": gen_rmat(12, seed=3),
         "hubs": hubs(),
+        # above the dense-accumulator limit: degree > 2048 uses the HBM hash
+        "hubs_big": hubs(seed=8, nv=40000, hub_degrees=(3000, 9000, 2100, 700)),
         "loops_isolated": with_loops_and_isolated(),
     }
 

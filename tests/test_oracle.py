@@ -253,18 +253,31 @@ def test_colorings_valid(name):
         assert oracle.color_conflicts(g, colors) == 0
 
 
-def test_jp_winners_are_local_maxima():
-    """Jones-Plassmann: every vertex's color is the smallest one absent from
-    its neighbours of higher priority (those are colored in earlier rounds)."""
+@pytest.mark.parametrize("name", ["planted_unit", "lfr_w", "rmat12", "hubs"])
+def test_colorings_are_first_fit(name):
+    """Any first-fit coloring (the reference's greedy, and the engine's
+    Jones-Plassmann rounds plus one-at-a-time tail) gives every vertex the
+    smallest color absent from the neighbours colored before it, so every
+    smaller color is present among its neighbours."""
+    g = catalog()[name]
+    for colors in (oracle.color_greedy(g)[0], oracle.color_jp(g)[0]):
+        for v in range(g.numVertices):
+            nb = g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]]
+            have = set(colors[nb[nb != v]].tolist())
+            assert all(c in have for c in range(colors[v])), v
+
+
+def test_jp_round_winners_are_local_maxima():
+    """On a graph where the rounds finish before the tail takes over, a vertex
+    whose priority beats all its neighbours is colored 0."""
     g = catalog()["planted_unit"]
     colors, mc, rounds = oracle.color_jp(g)
     pr = np.array([oracle.jp_priority(v) for v in range(g.numVertices)], dtype=np.uint64)
-    for v in range(0, g.numVertices, 7):
+    for v in range(g.numVertices):
         nb = g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]]
         nb = nb[nb != v]
-        higher = set(colors[nb[pr[nb] > pr[v]]].tolist())
-        first_free = next(c for c in range(mc + 2) if c not in higher)
-        assert colors[v] == first_free
+        if len(nb) and np.all(pr[nb] < pr[v]):
+            assert colors[v] == 0
 
 
 def test_frozen_equals_sequential_with_valid_coloring():
