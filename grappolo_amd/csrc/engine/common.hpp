@@ -33,6 +33,7 @@ constexpr int kBlockHashSlots = 4096;
 // dense LDS accumulator indexed by community id (16384 doubles = 128 KB).
 constexpr i64 kDenseMaxNV = 16384;
 constexpr int kDenseBlocks = 1024;
+constexpr i64 kPersistMaxBlocks = 64;  // cooperative grid of the persistent sweep
 
 __host__ __device__ inline int degree_bin(i64 d) {
     if (d == 0) return 0;

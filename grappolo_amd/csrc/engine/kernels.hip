@@ -560,6 +560,216 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
     if (PLAIN && threadIdx.x == 0) A.eii_part[blockIdx.x] = part;
 }
 
+__device__ inline void move_vertex(i64 t, i64 s, double kv, double *a, i64 *n) {
+    atomicAdd(&a[t], kv);
+    atomicAdd(&a[s], -kv);
+    atomicAdd((unsigned long long *)&n[t], 1ULL);
+    atomicAdd((unsigned long long *)&n[s], ~0ULL);  // -1
+}
+
+// ---------------------------------------------------------- persistent sweep
+// A whole colored iteration of a small graph (at most kDenseMaxNV vertices) in
+// one cooperative launch: for every color, the workgroups share out the
+// vertices of the class (dense LDS accumulator per workgroup), meet at a grid
+// barrier, commit their own vertices' moves, and meet again.  The modularity
+// partials of the iteration follow the last color.  Grid barriers spin on a
+// generation counter with a bounded wait; a workgroup that gives up raises
+// `abort`, which releases every other wait, so every wave always exits.
+constexpr int kPBlock = 1024;
+constexpr unsigned kSpinLimit = 1u << 22;
+
+__device__ inline double block_sum_p(double v, double *scratch) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    __syncthreads();
+    if (l == 0) scratch[w] = v;
+    __syncthreads();
+    double r = 0;
+    for (int q = 0; q < kPBlock / kWave; q++) r += scratch[q];
+    return r;
+}
+
+__device__ inline void block_argmax_p(bool &cand, double &gain, i64 &id, int *sc, double *sg, i64 *si) {
+    group_argmax<kWave>(cand, gain, id);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    __syncthreads();
+    if (l == 0) {
+        sc[w] = cand;
+        sg[w] = gain;
+        si[w] = id;
+    }
+    __syncthreads();
+    bool c = sc[0] != 0;
+    double g = sg[0];
+    i64 d = si[0];
+    for (int q = 1; q < kPBlock / kWave; q++)
+        if (better(sc[q] != 0, sg[q], si[q], c, g, d)) {
+            c = true;
+            g = sg[q];
+            d = si[q];
+        }
+    cand = c;
+    gain = g;
+    id = d;
+}
+
+// Returns false when the barrier was abandoned (abort raised).
+__device__ bool grid_barrier(unsigned *bar, unsigned nblocks) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned gen = __atomic_load_n(&bar[1], __ATOMIC_ACQUIRE);
+        __threadfence();
+        const unsigned arrived = atomicAdd(&bar[0], 1u) + 1;
+        if (arrived == nblocks) {
+            __atomic_store_n(&bar[0], 0u, __ATOMIC_RELAXED);
+            __threadfence();
+            atomicAdd(&bar[1], 1u);
+        } else {
+            unsigned spins = 0;
+            while (__atomic_load_n(&bar[1], __ATOMIC_ACQUIRE) == gen) {
+                if (__atomic_load_n(&bar[2], __ATOMIC_RELAXED) || ++spins > kSpinLimit) {
+                    atomicOr(&bar[2], 1u);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __threadfence();
+        s_ok = ok && !__atomic_load_n(&bar[2], __ATOMIC_RELAXED);
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+__global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 *C, double *a,
+                                                               i64 *n, const i64 *perm,
+                                                               const i64 *start, int ncol,
+                                                               unsigned *bar, double *part) {
+    __shared__ double acc[kDenseMaxNV];
+    __shared__ uint32_t touched[kDenseMaxNV / 32];
+    __shared__ double red[kPBlock / kWave];
+    __shared__ int sc[kPBlock / kWave];
+    __shared__ double sg[kPBlock / kWave];
+    __shared__ i64 si[kPBlock / kWave];
+    const int nv = (int)A.nv;
+    const int nwords = (nv + 31) / 32;
+    const unsigned nb = gridDim.x;
+    for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
+    for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+    __syncthreads();
+    bool alive = true;
+    for (int c = 0; c < ncol && alive; c++) {
+        i64 total = 0;
+        for (int b = 1; b < kNumBins; b++) total += start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c];
+        if (total == 0) continue;
+        // sweep the class: vertex q of the class goes to workgroup q % nb
+        for (i64 q = blockIdx.x; q < total; q += nb) {
+            i64 r = q;
+            int b = 1;
+            while (r >= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c]) {
+                r -= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c];
+                b++;
+            }
+            const i64 v = perm[start[(i64)b * ncol + c] + r];
+            const i64 beg = A.off[v], end = A.off[v + 1];
+            double sl = 0;
+            for (i64 j = beg + threadIdx.x; j < end; j += kPBlock) {
+                const i64 t = A.tails[j];
+                const double ww = A.w[j];
+                if (t == v) sl += ww;
+                const i64 cc = A.C[t];
+                atomicOr(&touched[cc >> 5], 1u << (cc & 31));
+                atomicAdd(&acc[cc], ww);
+            }
+            __syncthreads();
+            sl = block_sum_p(sl, red);
+            const i64 s = A.C[v];
+            const double e_s = acc[s];
+            const double kv = A.k[v];
+            const double eix = e_s - sl, ax = A.a[s] - kv;
+            bool cand = false;
+            double gain = 0;
+            i64 id = INT64_MAX;
+            for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
+                uint32_t m = touched[wi];
+                while (m) {
+                    const int bt = __ffs((int)m) - 1;
+                    m &= m - 1;
+                    const i64 y = (i64)wi * 32 + bt;
+                    if (y == s) continue;
+                    const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
+                    if (better(g > 0, g, y, cand, gain, id)) {
+                        cand = true;
+                        gain = g;
+                        id = y;
+                    }
+                }
+            }
+            block_argmax_p(cand, gain, id, sc, sg, si);
+            if (threadIdx.x == 0) {
+                i64 target = s;
+                if (cand) {
+                    target = id;
+                    if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
+                }
+                A.tgt[v] = target;
+            }
+            __syncthreads();  // every read of acc is done
+            for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
+                uint32_t m = touched[wi];
+                while (m) {
+                    const int bt = __ffs((int)m) - 1;
+                    m &= m - 1;
+                    acc[wi * 32 + bt] = 0.0;
+                }
+                touched[wi] = 0;
+            }
+            __syncthreads();
+        }
+        alive = grid_barrier(bar, nb);
+        if (!alive) break;
+        // commit the moves of the vertices this workgroup swept
+        for (i64 q = blockIdx.x; q < total; q += nb) {
+            if (threadIdx.x != 0) break;
+            i64 r = q;
+            int b = 1;
+            while (r >= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c]) {
+                r -= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c];
+                b++;
+            }
+            const i64 v = perm[start[(i64)b * ncol + c] + r];
+            const i64 t = A.tgt[v], s = A.C[v];
+            if (t != s) {
+                move_vertex(t, s, A.k[v], a, n);
+                C[v] = t;
+            }
+        }
+        __threadfence();
+        alive = grid_barrier(bar, nb);
+    }
+    // modularity partials: internal weight over this workgroup's vertices, sum of a_c^2
+    double exx = 0, a2 = 0;
+    if (alive) {
+        for (i64 v = blockIdx.x; v < nv; v += nb) {
+            const i64 s = A.C[v];
+            if (s < 0) continue;
+            for (i64 j = A.off[v] + threadIdx.x; j < A.off[v + 1]; j += kPBlock)
+                if (A.C[A.tails[j]] == s) exx += A.w[j];
+        }
+        for (i64 i = (i64)blockIdx.x * kPBlock + threadIdx.x; i < nv; i += (i64)nb * kPBlock)
+            a2 += A.a[i] * A.a[i];
+    }
+    exx = block_sum_p(exx, red);
+    a2 = block_sum_p(a2, red);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = exx;
+        part[nb + blockIdx.x] = a2;
+    }
+}
+
 __global__ void k_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out) {
     i64 m = 0;
     for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < count; q += (i64)gridDim.x * blockDim.x) {
@@ -575,12 +785,6 @@ __global__ void k_isolated_targets(const i64 *verts, i64 count, i64 *tgt) {
         tgt[verts[q]] = -1;
 }
 
-__device__ inline void move_vertex(i64 t, i64 s, double kv, double *a, i64 *n) {
-    atomicAdd(&a[t], kv);
-    atomicAdd(&a[s], -kv);
-    atomicAdd((unsigned long long *)&n[t], 1ULL);
-    atomicAdd((unsigned long long *)&n[s], ~0ULL);  // -1
-}
 
 // Commit of one color step: community updates of the movers, then labels.
 __global__ void k_apply_colored(const i64 *perm, Slices S, const i64 *tgt, i64 *C, const double *k,
@@ -1154,6 +1358,38 @@ void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hi
     if (count <= 0) return;
     k_max_degree<<<grid_for(count, kBlock, 4096), kBlock, 0, s>>>(off, verts, count, out);
     GLV_LAUNCH_CHECK();
+}
+int persistent_blocks() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0, cus = 0, per = 0;
+        GLV_HIP(hipGetDevice(&dev));
+        GLV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_colored_iteration, kPBlock, 0));
+        cached = per > 0 ? cus * per : 0;
+    }
+    return cached;
+}
+bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
+                              unsigned *bar, double *part, int nblocks, hipStream_t s) {
+    if (nblocks <= 0 || A.nv > kDenseMaxNV) return false;
+    SweepArgs aa = A;
+    i64 *C = const_cast<i64 *>(A.C);
+    double *ca = const_cast<double *>(A.a);
+    i64 *cn = const_cast<i64 *>(A.n);
+    const i64 *p = perm;
+    const i64 *st = start;
+    int nc = ncol;
+    unsigned *b = bar;
+    double *pt = part;
+    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt};
+    hipError_t e = hipLaunchCooperativeKernel((const void *)k_colored_iteration, dim3(nblocks),
+                                              dim3(kPBlock), args, 0, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
 }
 void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s) {
     if (count <= 0) return;

@@ -63,6 +63,13 @@ i64 sweep_blocks(int bin, i64 count, i64 nv);
 void launch_sweep(int bin, bool plain, const SweepArgs &A, hipStream_t s);
 void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hipStream_t s);
 constexpr int kGlobalHashBlocks = 256;
+// One colored iteration of a graph with at most kDenseMaxNV vertices in one
+// cooperative launch (all colors, commits, modularity partials: part[0..nb)
+// internal weight, part[nb..2nb) sum of a^2).  bar: 3 zeroed words.  Returns
+// false when the cooperative launch is not possible (caller falls back).
+int persistent_blocks();
+bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
+                              unsigned *bar, double *part, int nblocks, hipStream_t s);
 void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s);
 void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 *C, const double *k,
                           double *a, i64 *n, hipStream_t s);

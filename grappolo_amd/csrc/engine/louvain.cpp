@@ -51,17 +51,19 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
     P.perm.alloc(nv, s_);
     DBuf<uint32_t> keys(nv, s_);
     DBuf<int32_t> err(1, s_);
-    DBuf<i64> start(nkeys + 1, s_), pnnz(nkeys, s_);
+    P.dptr.alloc(nkeys + 1, s_);
+    DBuf<i64> pnnz(nkeys, s_);
+    i64 *start = P.dptr.p;
     launch_fill_i32(err.p, 1, 0, s_);
     launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, err.p, s_);
     sort_pairs_u32_i64(keys.p, P.perm.p, nv, bits_for(nkeys), s_);
-    launch_slices(keys.p, P.perm.p, g.off, nv, nkeys, start.p, pnnz.p, s_);
+    launch_slices(keys.p, P.perm.p, g.off, nv, nkeys, start, pnnz.p, s_);
     P.ptr.assign(nkeys + 1, 0);
     P.pnnz.assign(nkeys, 0);
     int32_t e = 0;
     GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s_));
     if (nv > 0) {
-        GLV_HIP(hipMemcpyAsync(P.ptr.data(), start.p, (nkeys + 1) * sizeof(i64), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipMemcpyAsync(P.ptr.data(), start, (nkeys + 1) * sizeof(i64), hipMemcpyDeviceToHost, s_));
         GLV_HIP(hipMemcpyAsync(P.pnnz.data(), pnnz.p, nkeys * sizeof(i64), hipMemcpyDeviceToHost, s_));
     }
     GLV_HIP(hipStreamSynchronize(s_));
@@ -116,9 +118,9 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         const i64 rest = ns + nl, won = before - rest;
         if (rest > 0 && (rest <= kJpTail || won < kJpMinWon)) {
             // JP rounds have stopped paying for themselves: finish one vertex at a time
-            Profiler::get().begin(K_COLOR, s_);
+            Profiler::get().begin(K_COLOR_TAIL, s_);
             launch_color_tail(g, colors, sB.p, sA.p, lB.p, rest, round, err.p, s_);
-            Profiler::get().end(K_COLOR, 0.0, s_);
+            Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
             break;
         }
     }
@@ -206,10 +208,46 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     for (int c = 0; c < ncol; c++)
         for (int b = 1; b < kNumBins; b++)
             if (P.hi(b, c) > P.lo(b, c)) color_nonempty[c] = 1;
+    // small graphs: one cooperative launch per iteration (all colors + modularity)
+    // grid sized to the color classes: every extra workgroup only lengthens the barriers
+    int pblocks = nv <= kDenseMaxNV ? persistent_blocks() : 0;
+    if (pblocks > 0) {
+        int nonempty = 0;
+        for (int c = 0; c < ncol; c++) nonempty += color_nonempty[c];
+        const i64 per_class = (nv + std::max(nonempty, 1) - 1) / std::max(nonempty, 1);
+        pblocks = (int)std::min<i64>({(i64)pblocks, kPersistMaxBlocks, std::max<i64>(4, 2 * per_class)});
+    }
+    DBuf<unsigned> bar(pblocks > 0 ? 3 : 0, s_);
+    DBuf<double> ppart(pblocks > 0 ? 2 * pblocks : 0, s_);
+    bool persistent = pblocks > 0;
     double prevMod = Lower, currMod = -1;
     int numItrs = 0;
     while (true) {
         numItrs++;
+        if (persistent) {
+            launch_fill_i32((int32_t *)bar.p, 3, 0, s_);
+            Profiler::get().begin(K_PERSIST, s_);
+            persistent = launch_colored_iteration(A, P.perm.p, P.dptr.p, ncol, bar.p, ppart.p, pblocks, s_);
+            Profiler::get().end(K_PERSIST, 24.0 * g.nnz + 32.0 * nv, s_);
+            if (persistent) {
+                launch_sum_partials(ppart.p, pblocks, res.p, s_);
+                launch_sum_partials(ppart.p + pblocks, pblocks, res.p + 1, s_);
+                double h[2];
+                unsigned hb[3];
+                GLV_HIP(hipMemcpyAsync(h, res.p, sizeof(h), hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipMemcpyAsync(hb, bar.p, sizeof(hb), hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipStreamSynchronize(s_));
+                if (hb[2]) fail(GLV_E_DEVICE, "grid barrier of the persistent sweep timed out");
+                const double c = P.constant;
+                currMod = h[0] * (double)c - h[1] * (double)c * (double)c;
+                if (trace) trace->push_back(currMod);
+                if ((currMod - prevMod) < thresh) break;
+                if (numItrs >= kMaxPhaseIters) break;
+                prevMod = currMod;
+                continue;
+            }
+            // cooperative launch refused: this phase continues with per-color launches
+        }
         for (int c = 0; c < ncol; c++) {
             if (!color_nonempty[c]) continue;  // stale colors can leave classes empty
             Slices S;

@@ -50,6 +50,7 @@ struct PhaseSetup {
     DBuf<double> k, a;
     DBuf<i64> n;
     DBuf<i64> perm;
+    DBuf<i64> dptr;        // device copy of ptr
     std::vector<i64> ptr;  // (kNumBins*ncol + 1): slice starts in perm, key = bin*ncol + color
     std::vector<i64> pnnz; // entries of the vertices of each slice
     double constant = 0;   // 1 / sum of degrees

@@ -16,7 +16,9 @@ enum KClass {
     K_COLOR = 10,    // one Jones-Plassmann round
     K_COARSEN = 11,  // whole coarsening (sorts included)
     K_RENUMBER = 12,
-    K_NUM = 13
+    K_PERSIST = 13,  // one cooperative launch per colored iteration of a small graph
+    K_COLOR_TAIL = 14,  // sequential first-fit of the vertices JP rounds left
+    K_NUM = 15
 };
 
 struct KStat {

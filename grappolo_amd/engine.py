@@ -90,7 +90,7 @@ _SIG = {
 KERNEL_CLASSES = ["sweep_bin1_deg4", "sweep_bin2_deg8", "sweep_bin3_deg16", "sweep_bin4_deg32",
                   "sweep_bin5_deg64", "sweep_bin6_wavehash", "sweep_bin7_blockhash",
                   "sweep_bin8_hbmhash", "modularity", "commit", "coloring_round", "coarsen",
-                  "renumber"]
+                  "renumber", "sweep_persistent_iteration", "coloring_tail"]
 
 _fns = {}
 

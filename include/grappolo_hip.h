@@ -185,7 +185,9 @@ int glv_count_conflicts_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
  * device time (ms), the summed algorithmic bytes and the launch count of one
  * class: 0..7 sweep kernels of degree bins 1..8 (algorithmic bytes
  * 24*entries + 32*vertices per launch), 8 modularity pass, 9 commits,
- * 10 coloring rounds, 11 coarsening, 12 renumbering. */
+ * 10 coloring rounds, 11 coarsening, 12 renumbering, 13 one whole colored
+ * iteration of a small graph in a single cooperative launch, 14 the
+ * sequential coloring tail. */
 int glv_profile_enable(int on);
 int glv_profile_reset(void);
 int glv_profile_read(int kernel_class, double *ms, double *bytes, int64_t *launches);
