@@ -30,10 +30,15 @@ constexpr i64 kBinMaxDeg[kNumBins] = {0, 4, 8, 16, 32, 64, 256, 2048, INT64_MAX}
 constexpr int kWaveHashSlots = 512;
 constexpr int kBlockHashSlots = 4096;
 // Graphs of at most this many vertices sweep their high-degree bins with a
-// dense LDS accumulator indexed by community id (16384 doubles = 128 KB).
-constexpr i64 kDenseMaxNV = 16384;
+// dense LDS accumulator indexed by community id: 15360 doubles (120 KB), its
+// touched bitmap (1.9 KB) and a uint16 list of touched ids (30 KB) fit the
+// 160 KB of LDS of a gfx950 CU.
+constexpr i64 kDenseMaxNV = 15360;
 constexpr int kDenseBlocks = 1024;
 constexpr i64 kPersistMaxBlocks = 64;  // cooperative grid of the persistent sweep
+constexpr int kBarrierWords = 16 + 16 * 64;  // abort word + one 64-byte flag line per workgroup
+// touched communities of one vertex, listed in LDS: at most min(degree, nv)
+constexpr int kDenseListMax = kDenseMaxNV;
 
 __host__ __device__ inline int degree_bin(i64 d) {
     if (d == 0) return 0;

@@ -33,7 +33,6 @@ __device__ inline u64 jp_priority(i64 v) {
 }
 
 __device__ inline i64 ld_relaxed(const i64 *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
-__device__ inline int32_t ld_relaxed(const int32_t *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
 
 // Gain of moving a vertex of degree kv from community s (internal weight
 // eix excluding its self-loop, degree ax excluding the vertex) to community y
@@ -115,6 +114,50 @@ __device__ void block_argmax(bool &cand, double &gain, i64 &id) {
     gain = g;
     id = d;
     __syncthreads();
+}
+
+// Exclusive prefix sum over a workgroup of BS threads; *total gets the sum.
+template <int BS>
+__device__ inline int block_exscan(int x, int *wsum, int *total) {
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    int incl = x;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    __syncthreads();
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int q = 0; q < BS / kWave; q++) {
+        const int t = wsum[q];
+        if (q < w) before += t;
+        all += t;
+    }
+    *total = all;
+    return before + incl - x;
+}
+
+// Lists the set bits of touched[0..nwords) (community ids) into list[], in
+// increasing id; returns the count.  One word per thread per pass.
+template <int BS>
+__device__ inline int compact_touched(const uint32_t *touched, int nwords, uint16_t *list, int *wsum) {
+    int n = 0;
+    for (int base = 0; base < nwords; base += BS) {
+        const int wi = base + threadIdx.x;
+        uint32_t m = wi < nwords ? touched[wi] : 0u;
+        int total = 0;
+        int at = n + block_exscan<BS>(__popc(m), wsum, &total);
+        while (m) {
+            const int b = __ffs((int)m) - 1;
+            m &= m - 1;
+            list[at++] = (uint16_t)(wi * 32 + b);
+        }
+        n += total;
+    }
+    __syncthreads();
+    return n;
 }
 
 // ---------------------------------------------------------------- elementwise
@@ -493,6 +536,8 @@ template <bool PLAIN>
 __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
     __shared__ double acc[kDenseMaxNV];
     __shared__ uint32_t touched[kDenseMaxNV / 32];
+    __shared__ uint16_t list[kDenseListMax];
+    __shared__ int wsum[kBlock / kWave];
     const int nv = (int)A.nv;
     const int nwords = (nv + 31) / 32;
     // acc[] and touched[] are all zero between vertices
@@ -514,6 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
         }
         __syncthreads();
         sl = block_sum(sl);
+        const int ntouched = compact_touched<kBlock>(touched, nwords, list, wsum);
         const i64 s = A.C[v];
         const double e_s = acc[s];
         const double kv = A.k[v];
@@ -521,19 +567,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
         bool cand = false;
         double gain = 0;
         i64 id = INT64_MAX;
-        for (int wi = threadIdx.x; wi < nwords; wi += kBlock) {
-            uint32_t m = touched[wi];
-            while (m) {
-                const int b = __ffs((int)m) - 1;
-                m &= m - 1;
-                const i64 y = (i64)wi * 32 + b;
-                if (y == s) continue;
-                const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
-                if (better(g > 0, g, y, cand, gain, id)) {
-                    cand = true;
-                    gain = g;
-                    id = y;
-                }
+        for (int k = threadIdx.x; k < ntouched; k += kBlock) {
+            const i64 y = list[k];
+            if (y == s) continue;
+            const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
+            if (better(g > 0, g, y, cand, gain, id)) {
+                cand = true;
+                gain = g;
+                id = y;
             }
         }
         block_argmax(cand, gain, id);  // ends with a barrier: every read of acc is done
@@ -546,15 +587,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
             A.tgt[v] = target;
         }
         part += e_s;
-        for (int wi = threadIdx.x; wi < nwords; wi += kBlock) {
-            uint32_t m = touched[wi];
-            while (m) {
-                const int b = __ffs((int)m) - 1;
-                m &= m - 1;
-                acc[wi * 32 + b] = 0.0;
-            }
-            touched[wi] = 0;
-        }
+        for (int k = threadIdx.x; k < ntouched; k += kBlock) acc[list[k]] = 0.0;
+        for (int i = threadIdx.x; i < nwords; i += kBlock) touched[i] = 0;
         __syncthreads();
     }
     if (PLAIN && threadIdx.x == 0) A.eii_part[blockIdx.x] = part;
@@ -613,43 +647,52 @@ __device__ inline void block_argmax_p(bool &cand, double &gain, i64 &id, int *sc
     id = d;
 }
 
-// Returns false when the barrier was abandoned (abort raised).
-__device__ bool grid_barrier(unsigned *bar, unsigned nblocks) {
+// Grid barrier over nb <= 64 workgroups without a shared counter: workgroup
+// b publishes the barrier's sequence number in its own flag slot (one cache
+// line each), and the first wavefront of every workgroup polls all nb slots at
+// once.  Bounded wait; a workgroup that gives up raises bar[0] (abort), which
+// releases every other wait.  Returns false when the barrier was abandoned.
+__device__ bool grid_barrier(unsigned *bar, unsigned nb, unsigned seq) {
     __shared__ int s_ok;
+    unsigned *flags = bar + 16;  // flag of workgroup b at flags[16 * b]
     __syncthreads();
     if (threadIdx.x == 0) {
+        __threadfence();
+        __atomic_store_n(&flags[16 * blockIdx.x], seq, __ATOMIC_RELEASE);
+    }
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        unsigned spins = 0;
         int ok = 1;
-        const unsigned gen = __atomic_load_n(&bar[1], __ATOMIC_ACQUIRE);
-        __threadfence();
-        const unsigned arrived = atomicAdd(&bar[0], 1u) + 1;
-        if (arrived == nblocks) {
-            __atomic_store_n(&bar[0], 0u, __ATOMIC_RELAXED);
-            __threadfence();
-            atomicAdd(&bar[1], 1u);
-        } else {
-            unsigned spins = 0;
-            while (__atomic_load_n(&bar[1], __ATOMIC_ACQUIRE) == gen) {
-                if (__atomic_load_n(&bar[2], __ATOMIC_RELAXED) || ++spins > kSpinLimit) {
-                    atomicOr(&bar[2], 1u);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
+        while (true) {
+            bool done = true;
+            if ((unsigned)lane < nb) done = __atomic_load_n(&flags[16 * lane], __ATOMIC_ACQUIRE) >= seq;
+            if (__ballot(!done) == 0) break;
+            if (__atomic_load_n(&bar[0], __ATOMIC_RELAXED) || ++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(&bar[0], 1u);
+                ok = 0;
+                break;
             }
+            __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();
-        s_ok = ok && !__atomic_load_n(&bar[2], __ATOMIC_RELAXED);
+        if (lane == 0) {
+            __threadfence();
+            s_ok = ok && !__atomic_load_n(&bar[0], __ATOMIC_RELAXED);
+        }
     }
     __syncthreads();
     return s_ok != 0;
 }
 
+// cperm: vertices grouped by color (cstart[c] .. cstart[c+1]).
 __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 *C, double *a,
-                                                               i64 *n, const i64 *perm,
-                                                               const i64 *start, int ncol,
+                                                               i64 *n, const i64 *cperm,
+                                                               const i64 *cstart, int ncol,
                                                                unsigned *bar, double *part) {
     __shared__ double acc[kDenseMaxNV];
     __shared__ uint32_t touched[kDenseMaxNV / 32];
+    __shared__ uint16_t list[kDenseListMax];
+    __shared__ int wsum[kPBlock / kWave];
     __shared__ double red[kPBlock / kWave];
     __shared__ int sc[kPBlock / kWave];
     __shared__ double sg[kPBlock / kWave];
@@ -657,23 +700,16 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
     const int nv = (int)A.nv;
     const int nwords = (nv + 31) / 32;
     const unsigned nb = gridDim.x;
+    unsigned seq = 0;
     for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
     for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
     __syncthreads();
     bool alive = true;
     for (int c = 0; c < ncol && alive; c++) {
-        i64 total = 0;
-        for (int b = 1; b < kNumBins; b++) total += start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c];
+        const i64 lo = cstart[c], total = cstart[c + 1] - lo;
         if (total == 0) continue;
-        // sweep the class: vertex q of the class goes to workgroup q % nb
         for (i64 q = blockIdx.x; q < total; q += nb) {
-            i64 r = q;
-            int b = 1;
-            while (r >= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c]) {
-                r -= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c];
-                b++;
-            }
-            const i64 v = perm[start[(i64)b * ncol + c] + r];
+            const i64 v = cperm[lo + q];
             const i64 beg = A.off[v], end = A.off[v + 1];
             double sl = 0;
             for (i64 j = beg + threadIdx.x; j < end; j += kPBlock) {
@@ -686,6 +722,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             }
             __syncthreads();
             sl = block_sum_p(sl, red);
+            const int ntouched = compact_touched<kPBlock>(touched, nwords, list, wsum);
             const i64 s = A.C[v];
             const double e_s = acc[s];
             const double kv = A.k[v];
@@ -693,19 +730,14 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             bool cand = false;
             double gain = 0;
             i64 id = INT64_MAX;
-            for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
-                uint32_t m = touched[wi];
-                while (m) {
-                    const int bt = __ffs((int)m) - 1;
-                    m &= m - 1;
-                    const i64 y = (i64)wi * 32 + bt;
-                    if (y == s) continue;
-                    const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
-                    if (better(g > 0, g, y, cand, gain, id)) {
-                        cand = true;
-                        gain = g;
-                        id = y;
-                    }
+            for (int k = threadIdx.x; k < ntouched; k += kPBlock) {
+                const i64 y = list[k];
+                if (y == s) continue;
+                const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
+                if (better(g > 0, g, y, cand, gain, id)) {
+                    cand = true;
+                    gain = g;
+                    id = y;
                 }
             }
             block_argmax_p(cand, gain, id, sc, sg, si);
@@ -718,37 +750,24 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 A.tgt[v] = target;
             }
             __syncthreads();  // every read of acc is done
-            for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
-                uint32_t m = touched[wi];
-                while (m) {
-                    const int bt = __ffs((int)m) - 1;
-                    m &= m - 1;
-                    acc[wi * 32 + bt] = 0.0;
-                }
-                touched[wi] = 0;
-            }
+            for (int k = threadIdx.x; k < ntouched; k += kPBlock) acc[list[k]] = 0.0;
+            for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
             __syncthreads();
         }
-        alive = grid_barrier(bar, nb);
+        alive = grid_barrier(bar, nb, ++seq);
         if (!alive) break;
         // commit the moves of the vertices this workgroup swept
-        for (i64 q = blockIdx.x; q < total; q += nb) {
-            if (threadIdx.x != 0) break;
-            i64 r = q;
-            int b = 1;
-            while (r >= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c]) {
-                r -= start[(i64)b * ncol + c + 1] - start[(i64)b * ncol + c];
-                b++;
-            }
-            const i64 v = perm[start[(i64)b * ncol + c] + r];
-            const i64 t = A.tgt[v], s = A.C[v];
-            if (t != s) {
-                move_vertex(t, s, A.k[v], a, n);
-                C[v] = t;
+        if (threadIdx.x == 0) {
+            for (i64 q = blockIdx.x; q < total; q += nb) {
+                const i64 v = cperm[lo + q];
+                const i64 t = A.tgt[v], s = C[v];
+                if (t != s) {
+                    move_vertex(t, s, A.k[v], a, n);
+                    C[v] = t;
+                }
             }
         }
-        __threadfence();
-        alive = grid_barrier(bar, nb);
+        alive = grid_barrier(bar, nb, ++seq);
     }
     // modularity partials: internal weight over this workgroup's vertices, sum of a_c^2
     double exx = 0, a2 = 0;
@@ -871,8 +890,6 @@ __global__ __launch_bounds__(kBlock) void k_mod_pass(const i64 *off, const i64 *
 }
 
 // ------------------------------------------------------------------- coloring
-// Packed coloring state: -1 uncoloured, else (round << 12) | color.
-constexpr int kColorBits = 12;
 
 // Appends `v` to list[] when `pred`, one atomic per wavefront.
 __device__ inline void wave_append(bool pred, i64 v, i64 *list, i64 *count) {
@@ -887,17 +904,151 @@ __device__ inline void wave_append(bool pred, i64 v, i64 *list, i64 *count) {
     if (pred) list[base + __popcll(m & lt)] = v;
 }
 
-__global__ void k_jp_partition(const i64 *off, i64 nv, i64 *small, i64 *large, i64 *counts) {
-    // all lanes of a wavefront iterate together so the ballots are complete
+// Jones-Plassmann as a topological sweep of the priority DAG: count[v] is the
+// number of uncoloured neighbours with a higher priority; a vertex enters the
+// frontier of round r when its count reaches zero, which is exactly when round
+// r of the classic formulation would find it a local maximum.  Each frontier
+// vertex takes the smallest color absent from its (all coloured) higher
+// neighbours and releases its lower neighbours, so total work is O(nnz).
+__device__ inline void frontier_push(bool pred, i64 u, const i64 *off, i64 *small, i64 *large,
+                                     i64 *counts) {
+    const bool big = pred && (off[u + 1] - off[u] > kBinMaxDeg[6]);
+    wave_append(pred && !big, u, small, &counts[0]);
+    wave_append(big, u, large, &counts[1]);
+}
+
+__global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cnt, i64 *small,
+                           i64 *large, i64 *counts) {
     const i64 stride = (i64)gridDim.x * blockDim.x;
-    const i64 start = blockIdx.x * (i64)blockDim.x + threadIdx.x;
+    const i64 first = blockIdx.x * (i64)blockDim.x + threadIdx.x;
     const i64 trips = (nv + stride - 1) / stride;
     for (i64 t = 0; t < trips; t++) {
-        const i64 v = start + t * stride;
-        const bool in = v < nv;
-        const bool big = in && (off[v + 1] - off[v] > kBinMaxDeg[6]);
-        wave_append(in && !big, v, small, &counts[0]);
-        wave_append(big, v, large, &counts[1]);
+        const i64 v = first + t * stride;
+        int c = 0;
+        if (v < nv) {
+            const u64 pv = jp_priority(v);
+            for (i64 j = off[v]; j < off[v + 1]; j++) {
+                const i64 u = tails[j];
+                if (u != v && jp_priority(u) > pv) c++;
+            }
+            cnt[v] = c;
+        }
+        frontier_push(v < nv && c == 0, v, off, small, large, counts);
+    }
+}
+
+// Frontier vertices of degree <= 256, one wavefront each (a private LDS color
+// bitmap per wavefront); whole workgroups iterate together for the barriers.
+__global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, const i64 *tails,
+                                                             int32_t *colors, int32_t *cnt,
+                                                             const i64 *front, i64 nfront, i64 *small,
+                                                             i64 *large, i64 *counts, int32_t *err) {
+    constexpr int WPB = kBlock / kWave;
+    __shared__ uint32_t mark[WPB][kMaxColors / 32];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    uint32_t *mk = mark[w];
+    const i64 stride = (i64)gridDim.x * WPB;
+    const i64 trips = (nfront + stride - 1) / stride;
+    for (i64 t = 0; t < trips; t++) {
+        const i64 q = (i64)blockIdx.x * WPB + w + t * stride;
+        const bool in = q < nfront;
+        const i64 v = in ? front[q] : 0;
+        const i64 b = in ? off[v] : 0, e = in ? off[v + 1] : 0;
+        for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
+        __syncthreads();
+        for (i64 j = b + lane; j < e; j += kWave) {
+            const i64 u = tails[j];
+            if (u == v) continue;
+            const int c = colors[u];
+            if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+        }
+        __syncthreads();
+        int cand = kMaxColors;
+        for (int i = lane; i < kMaxColors / 32; i += kWave) {
+            const uint32_t m = mk[i];
+            if (m != 0xFFFFFFFFu) {
+                const int c = i * 32 + __ffs((int)~m) - 1;
+                cand = c < cand ? c : cand;
+            }
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const int other = __shfl_xor(cand, o);
+            cand = other < cand ? other : cand;
+        }
+        if (in && lane == 0) {
+            int color = cand;
+            if (color >= kMaxColors) {
+                atomicOr(err, 1);
+                color = 0;
+            }
+            colors[v] = color;
+        }
+        // release the lower-priority neighbours
+        const u64 pv = in ? jp_priority(v) : 0;
+        const i64 deg = e - b;
+        for (i64 j0 = 0; j0 < deg; j0 += kWave) {
+            const i64 j = j0 + lane;
+            bool ready = false;
+            i64 u = 0;
+            if (j < deg) {
+                u = tails[b + j];
+                if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+            }
+            frontier_push(ready, u, off, small, large, counts);
+        }
+        __syncthreads();
+    }
+}
+
+// Frontier vertices of degree > 256, one workgroup each.
+__global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, const i64 *tails,
+                                                              int32_t *colors, int32_t *cnt,
+                                                              const i64 *front, i64 nfront,
+                                                              i64 *small, i64 *large, i64 *counts,
+                                                              int32_t *err) {
+    __shared__ uint32_t s_mark[kMaxColors / 32];
+    __shared__ int s_color;
+    for (i64 q = blockIdx.x; q < nfront; q += gridDim.x) {
+        const i64 v = front[q];
+        const i64 b = off[v], e = off[v + 1];
+        const u64 pv = jp_priority(v);
+        if (threadIdx.x == 0) s_color = kMaxColors;
+        for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) s_mark[i] = 0;
+        __syncthreads();
+        for (i64 j = b + threadIdx.x; j < e; j += kBlock) {
+            const i64 u = tails[j];
+            if (u == v) continue;
+            const int c = colors[u];
+            if (c >= 0) atomicOr(&s_mark[c >> 5], 1u << (c & 31));
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
+            const uint32_t m = s_mark[i];
+            if (m != 0xFFFFFFFFu) atomicMin(&s_color, i * 32 + __ffs((int)~m) - 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int color = s_color;
+            if (color >= kMaxColors) {
+                atomicOr(err, 1);
+                color = 0;
+            }
+            colors[v] = color;
+        }
+        // release lower neighbours; whole wavefronts iterate together for the appends
+        const i64 deg = e - b;
+        const i64 trips = (deg + kBlock - 1) / kBlock;
+        for (i64 t = 0; t < trips; t++) {
+            const i64 j = t * kBlock + threadIdx.x;
+            bool ready = false;
+            i64 u = 0;
+            if (j < deg) {
+                u = tails[b + j];
+                if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+            }
+            frontier_push(ready, u, off, small, large, counts);
+        }
+        __syncthreads();
     }
 }
 
@@ -916,7 +1067,7 @@ __global__ void k_scatter_flagged(const i64 *flags, const i64 *pos, i64 nv, i64 
 constexpr int kTailBlock = 1024;
 __global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const i64 *tails,
                                                            int32_t *state, const i64 *list, i64 n,
-                                                           int round, int32_t *err) {
+                                                           int32_t *err) {
     __shared__ uint32_t mark[kMaxColors / 32];
     __shared__ int s_color;
     for (i64 q = 0; q < n; q++) {
@@ -929,8 +1080,7 @@ __global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const
             if (u == v) continue;
             const int32_t su = __atomic_load_n(&state[u], __ATOMIC_RELAXED);
             if (su < 0) continue;
-            const int c = su & (kMaxColors - 1);
-            atomicOr(&mark[c >> 5], 1u << (c & 31));
+            atomicOr(&mark[su >> 5], 1u << (su & 31));
         }
         __syncthreads();
         if (threadIdx.x < kMaxColors / 32) {
@@ -944,133 +1094,16 @@ __global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const
                 atomicOr(err, 1);
                 color = 0;
             }
-            __atomic_store_n(&state[v], (int32_t)((round << 12) | color), __ATOMIC_RELAXED);
+            __atomic_store_n(&state[v], (int32_t)color, __ATOMIC_RELAXED);
         }
         __syncthreads();
     }
 }
 
-// A neighbour blocks v in round r when it is uncoloured at the start of the
-// round (state < 0 or coloured in round r itself) and has a higher priority.
-__device__ inline bool blocks_round(int32_t su, int round) {
-    return su < 0 || (su >> kColorBits) == round;
-}
-
-// One thread per vertex of degree <= 256.
-__global__ void k_jp_round_thread(const i64 *off, const i64 *tails, int32_t *state, const i64 *work,
-                                  i64 nwork, i64 *next, i64 *counts, int round, int32_t *err) {
-    const i64 stride = (i64)gridDim.x * blockDim.x;
-    const i64 first = blockIdx.x * (i64)blockDim.x + threadIdx.x;
-    const i64 trips = (nwork + stride - 1) / stride;
-    for (i64 t = 0; t < trips; t++) {
-        const i64 q = first + t * stride;
-        const bool in = q < nwork;
-        const i64 v = in ? work[q] : 0;
-        bool win = in;
-        if (in) {
-            const u64 pv = jp_priority(v);
-            for (i64 j = off[v]; j < off[v + 1]; j++) {
-                const i64 u = tails[j];
-                if (u == v) continue;
-                if (blocks_round(ld_relaxed(&state[u]), round) && jp_priority(u) > pv) {
-                    win = false;
-                    break;
-                }
-            }
-        }
-        wave_append(in && !win, v, next, &counts[0]);
-        if (!win) continue;
-        const i64 b = off[v], e = off[v + 1];
-        int color = -1;
-        for (int base = 0; base < kMaxColors && color < 0; base += 64) {
-            u64 mask = 0;
-            for (i64 j = b; j < e; j++) {
-                const i64 u = tails[j];
-                if (u == v) continue;
-                const int32_t su = ld_relaxed(&state[u]);
-                if (su < 0 || (su >> kColorBits) >= round) continue;
-                const int c = su & (kMaxColors - 1);
-                if (c >= base && c < base + 64) mask |= 1ULL << (c - base);
-            }
-            if (~mask) color = base + __ffsll((long long)~mask) - 1;
-        }
-        if (color < 0) {
-            atomicOr(err, 1);
-            color = 0;
-        }
-        __atomic_store_n(&state[v], (int32_t)((round << kColorBits) | color), __ATOMIC_RELAXED);
-    }
-}
-
-// One workgroup per vertex of degree > 256.
-__global__ __launch_bounds__(kBlock) void k_jp_round_block(const i64 *off, const i64 *tails,
-                                                           int32_t *state, const i64 *work, i64 nwork,
-                                                           i64 *next, i64 *counts, int round,
-                                                           int32_t *err) {
-    __shared__ int s_lose;
-    __shared__ uint32_t s_mark[kMaxColors / 32];
-    __shared__ int s_color;
-    for (i64 q = blockIdx.x; q < nwork; q += gridDim.x) {
-        const i64 v = work[q];
-        const u64 pv = jp_priority(v);
-        const i64 b = off[v], e = off[v + 1];
-        if (threadIdx.x == 0) {
-            s_lose = 0;
-            s_color = kMaxColors;
-        }
-        for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) s_mark[i] = 0;
-        __syncthreads();
-        for (i64 j = b + threadIdx.x; j < e; j += kBlock) {
-            if (__atomic_load_n(&s_lose, __ATOMIC_RELAXED)) break;
-            const i64 u = tails[j];
-            if (u == v) continue;
-            if (blocks_round(ld_relaxed(&state[u]), round) && jp_priority(u) > pv) {
-                __atomic_store_n(&s_lose, 1, __ATOMIC_RELAXED);
-                break;
-            }
-        }
-        __syncthreads();
-        const bool lose = s_lose != 0;
-        if (lose) {
-            if (threadIdx.x == 0) {
-                i64 p = (i64)atomicAdd((unsigned long long *)&counts[1], 1ULL);
-                next[p] = v;
-            }
-        } else {
-            for (i64 j = b + threadIdx.x; j < e; j += kBlock) {
-                const i64 u = tails[j];
-                if (u == v) continue;
-                const int32_t su = ld_relaxed(&state[u]);
-                if (su < 0 || (su >> kColorBits) >= round) continue;
-                const int c = su & (kMaxColors - 1);
-                atomicOr(&s_mark[c >> 5], 1u << (c & 31));
-            }
-            __syncthreads();
-            for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
-                uint32_t m = s_mark[i];
-                if (m != 0xFFFFFFFFu) atomicMin(&s_color, i * 32 + __ffs((int)~m) - 1);
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                int color = s_color;
-                if (color >= kMaxColors) {
-                    atomicOr(err, 1);
-                    color = 0;
-                }
-                __atomic_store_n(&state[v], (int32_t)((round << kColorBits) | color), __ATOMIC_RELAXED);
-            }
-        }
-        __syncthreads();
-    }
-}
-
-__global__ void k_jp_finish(int32_t *state, i64 nv, int32_t *maxc) {
+__global__ void k_jp_finish(const int32_t *colors, i64 nv, int32_t *maxc) {
     int m = -1;
-    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
-        int c = state[v] & (kMaxColors - 1);
-        state[v] = c;
-        m = c > m ? c : m;
-    }
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        m = colors[v] > m ? colors[v] : m;
     atomicMax(maxc, m);
 }
 
@@ -1259,6 +1292,25 @@ void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, ui
     k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, err);
     GLV_LAUNCH_CHECK();
 }
+__global__ void k_color_keys(const i64 *perm, const int32_t *colors, i64 n, uint32_t *keys, i64 *vals) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const i64 v = perm[i];
+        keys[i] = (uint32_t)colors[v];
+        vals[i] = v;
+    }
+}
+void launch_color_major(const i64 *perm, const int32_t *colors, i64 n, int ncol, i64 *cperm,
+                        i64 *cstart, hipStream_t s) {
+    if (n <= 0) return;
+    DBuf<uint32_t> keys(n, s);
+    k_color_keys<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(perm, colors, n, keys.p, cperm);
+    GLV_LAUNCH_CHECK();
+    int bits = 1;
+    while ((1 << bits) <= ncol) bits++;
+    sort_pairs_u32_i64(keys.p, cperm, n, bits, s);
+    k_slice_starts<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(keys.p, n, ncol, cstart);
+    GLV_LAUNCH_CHECK();
+}
 void launch_slices(const uint32_t *sorted_keys, const i64 *perm, const i64 *off, i64 n, i64 nkeys,
                    i64 *start, i64 *pnnz, hipStream_t s) {
     if (n <= 0) return;
@@ -1372,7 +1424,7 @@ int persistent_blocks() {
 }
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
                               unsigned *bar, double *part, int nblocks, hipStream_t s) {
-    if (nblocks <= 0 || A.nv > kDenseMaxNV) return false;
+    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || A.nv > kDenseMaxNV) return false;
     SweepArgs aa = A;
     i64 *C = const_cast<i64 *>(A.C);
     double *ca = const_cast<double *>(A.a);
@@ -1445,22 +1497,26 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
     GLV_LAUNCH_CHECK();
 }
 
-void launch_jp_partition(const i64 *off, i64 nv, i64 *small, i64 *large, i64 *counts, hipStream_t s) {
-    if (nv <= 0) return;
-    k_jp_partition<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, small, large, counts);
+void launch_jp_count(const DevGraph &g, int32_t *cnt, i64 *small, i64 *large, i64 *counts,
+                     hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_jp_count<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.nv, cnt, small, large,
+                                                               counts);
     GLV_LAUNCH_CHECK();
 }
-void launch_jp_round(const DevGraph &g, int32_t *state, const i64 *small, i64 nsmall, const i64 *large,
-                     i64 nlarge, i64 *nsmall_out, i64 *nlarge_out, i64 *counts, int round, int32_t *err,
-                     hipStream_t s) {
+void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const i64 *small, i64 nsmall,
+                        const i64 *large, i64 nlarge, i64 *next_small, i64 *next_large, i64 *counts,
+                        int32_t *err, hipStream_t s) {
     if (nsmall > 0) {
-        k_jp_round_thread<<<grid_for(nsmall, kBlock, 8192), kBlock, 0, s>>>(
-            g.off, g.tails, state, small, nsmall, nsmall_out, counts, round, err);
+        const i64 wpb = kBlock / kWave;
+        const unsigned blocks = (unsigned)std::max<i64>(1, std::min<i64>((nsmall + wpb - 1) / wpb, 16384));
+        k_jp_frontier_wave<<<blocks, kBlock, 0, s>>>(g.off, g.tails, colors, cnt, small, nsmall,
+                                                     next_small, next_large, counts, err);
         GLV_LAUNCH_CHECK();
     }
     if (nlarge > 0) {
-        k_jp_round_block<<<(unsigned)std::min<i64>(nlarge, 8192), kBlock, 0, s>>>(
-            g.off, g.tails, state, large, nlarge, nlarge_out, counts, round, err);
+        k_jp_frontier_block<<<(unsigned)std::min<i64>(nlarge, 8192), kBlock, 0, s>>>(
+            g.off, g.tails, colors, cnt, large, nlarge, next_small, next_large, counts, err);
         GLV_LAUNCH_CHECK();
     }
 }
@@ -1472,7 +1528,8 @@ void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, 
     scan_exclusive_i64(flags, pos, g.nv, s);
     k_scatter_flagged<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(flags, pos, g.nv, list);
     GLV_LAUNCH_CHECK();
-    k_color_tail<<<1, kTailBlock, 0, s>>>(g.off, g.tails, state, list, n, round, err);
+    (void)round;
+    k_color_tail<<<1, kTailBlock, 0, s>>>(g.off, g.tails, state, list, n, err);
     GLV_LAUNCH_CHECK();
 }
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s) {

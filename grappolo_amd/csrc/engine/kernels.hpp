@@ -68,6 +68,9 @@ constexpr int kGlobalHashBlocks = 256;
 // internal weight, part[nb..2nb) sum of a^2).  bar: 3 zeroed words.  Returns
 // false when the cooperative launch is not possible (caller falls back).
 int persistent_blocks();
+// vertices grouped by color (stable: within a color in perm's order); cstart: ncol+1
+void launch_color_major(const i64 *perm, const int32_t *colors, i64 n, int ncol, i64 *cperm,
+                        i64 *cstart, hipStream_t s);
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
                               unsigned *bar, double *part, int nblocks, hipStream_t s);
 void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s);
@@ -86,11 +89,13 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
                      double *part, hipStream_t s);
 
 // --- coloring ------------------------------------------------------------
-void launch_jp_partition(const i64 *off, i64 nv, i64 *small, i64 *large, i64 *counts,
-                         hipStream_t s);
-void launch_jp_round(const DevGraph &g, int32_t *state, const i64 *small, i64 nsmall,
-                     const i64 *large, i64 nlarge, i64 *nsmall_out, i64 *nlarge_out, i64 *counts,
-                     int round, int32_t *err, hipStream_t s);
+// count[v] = uncoloured higher-priority neighbours; count-0 vertices form the first frontier
+void launch_jp_count(const DevGraph &g, int32_t *cnt, i64 *small, i64 *large, i64 *counts,
+                     hipStream_t s);
+// color one frontier and build the next one (counts[0] small, counts[1] large appended)
+void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const i64 *small,
+                        i64 nsmall, const i64 *large, i64 nlarge, i64 *next_small,
+                        i64 *next_large, i64 *counts, int32_t *err, hipStream_t s);
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
 // first-fit, one vertex at a time in increasing id, of the n uncoloured vertices
 void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,

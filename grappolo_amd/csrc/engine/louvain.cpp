@@ -90,34 +90,38 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     if (nv == 0) return -1;
     launch_fill_i32(colors, nv, -1, s_);
     DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), counts(2, s_);
-    DBuf<int32_t> err(1, s_), maxc(1, s_);
+    DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
     launch_fill_i64(counts.p, 2, 0, s_);
     launch_fill_i32(err.p, 1, 0, s_);
     launch_fill_i32(maxc.p, 1, -1, s_);
-    launch_jp_partition(g.off, nv, sA.p, lA.p, counts.p, s_);
+    Profiler::get().begin(K_COLOR, s_);
+    launch_jp_count(g, cnt.p, sA.p, lA.p, counts.p, s_);
+    Profiler::get().end(K_COLOR, 0.0, s_);
     i64 hc[2];
     GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
     GLV_HIP(hipStreamSynchronize(s_));
-    i64 ns = hc[0], nl = hc[1];
+    i64 ns = hc[0], nl = hc[1], colored = 0;
     i64 *small = sA.p, *large = lA.p, *small_next = sB.p, *large_next = lB.p;
     int round = 0;
     while (ns + nl > 0) {
-        if (round >= (1 << 18)) fail(GLV_E_DEVICE, "coloring did not converge");
-        const i64 before = ns + nl;
+        if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
+        const i64 won = ns + nl;
         launch_fill_i64(counts.p, 2, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
-        launch_jp_round(g, colors, small, ns, large, nl, small_next, large_next, counts.p, round, err.p, s_);
+        launch_jp_frontier(g, colors, cnt.p, small, ns, large, nl, small_next, large_next, counts.p,
+                           err.p, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
         GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
         GLV_HIP(hipStreamSynchronize(s_));
+        colored += won;
         ns = hc[0];
         nl = hc[1];
         std::swap(small, small_next);
         std::swap(large, large_next);
         round++;
-        const i64 rest = ns + nl, won = before - rest;
+        const i64 rest = nv - colored;
         if (rest > 0 && (rest <= kJpTail || won < kJpMinWon)) {
-            // JP rounds have stopped paying for themselves: finish one vertex at a time
+            // rounds have stopped paying for themselves: finish one vertex at a time
             Profiler::get().begin(K_COLOR_TAIL, s_);
             launch_color_tail(g, colors, sB.p, sA.p, lB.p, rest, round, err.p, s_);
             Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
@@ -217,27 +221,29 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         const i64 per_class = (nv + std::max(nonempty, 1) - 1) / std::max(nonempty, 1);
         pblocks = (int)std::min<i64>({(i64)pblocks, kPersistMaxBlocks, std::max<i64>(4, 2 * per_class)});
     }
-    DBuf<unsigned> bar(pblocks > 0 ? 3 : 0, s_);
+    DBuf<unsigned> bar(pblocks > 0 ? kBarrierWords : 0, s_);
     DBuf<double> ppart(pblocks > 0 ? 2 * pblocks : 0, s_);
+    DBuf<i64> cperm(pblocks > 0 ? nv : 0, s_), cstart(pblocks > 0 ? ncol + 1 : 0, s_);
+    if (pblocks > 0) launch_color_major(P.perm.p, colors, nv, ncol, cperm.p, cstart.p, s_);
     bool persistent = pblocks > 0;
     double prevMod = Lower, currMod = -1;
     int numItrs = 0;
     while (true) {
         numItrs++;
         if (persistent) {
-            launch_fill_i32((int32_t *)bar.p, 3, 0, s_);
+            launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
             Profiler::get().begin(K_PERSIST, s_);
-            persistent = launch_colored_iteration(A, P.perm.p, P.dptr.p, ncol, bar.p, ppart.p, pblocks, s_);
+            persistent = launch_colored_iteration(A, cperm.p, cstart.p, ncol, bar.p, ppart.p, pblocks, s_);
             Profiler::get().end(K_PERSIST, 24.0 * g.nnz + 32.0 * nv, s_);
             if (persistent) {
                 launch_sum_partials(ppart.p, pblocks, res.p, s_);
                 launch_sum_partials(ppart.p + pblocks, pblocks, res.p + 1, s_);
                 double h[2];
-                unsigned hb[3];
+                unsigned abort_flag = 0;
                 GLV_HIP(hipMemcpyAsync(h, res.p, sizeof(h), hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipMemcpyAsync(hb, bar.p, sizeof(hb), hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
                 GLV_HIP(hipStreamSynchronize(s_));
-                if (hb[2]) fail(GLV_E_DEVICE, "grid barrier of the persistent sweep timed out");
+                if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent sweep timed out");
                 const double c = P.constant;
                 currMod = h[0] * (double)c - h[1] * (double)c * (double)c;
                 if (trace) trace->push_back(currMod);
