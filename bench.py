@@ -33,6 +33,9 @@ WORKLOADS = {
     "lfr4m": dict(kind="lfr", nv=4194304, seed=2,
                   desc="LFR-style 4,194,304 vertices, degree exp 2.5 on [10,1000], community "
                        "exp 1.5 on [20,5000], mixing 0.4, weights U[0.5,2], seed 2"),
+    "lfr8m": dict(kind="lfr", nv=8388608, seed=200,
+                  desc="LFR-style 8,388,608 vertices, degree exp 2.5 on [10,1000], community "
+                       "exp 1.5 on [20,5000], mixing 0.4, weights U[0.5,2], seed 200"),
     "planted65k": dict(kind="planted", nv=65536, seed=1,
                        desc="planted partition 65,536 vertices, 64 blocks, mean degree 16, "
                             "mixing 0.3, unit weights, seed 1"),
@@ -111,6 +114,112 @@ def read_traffic(kernel):
     return None
 
 
+def run_learned(args):
+    """Learned active-set pruning against the unpruned colored method: train
+    the MLP on traces of seeded 1M-vertex LFR graphs (seeds 100, 101, ...),
+    pick the threshold on the last graph's trace (held out from training),
+    then time both forms on the held-out LFR graph (seed 200)."""
+    import torch
+    from grappolo_amd import active_set as las
+    from grappolo_amd import engine
+    from grappolo_amd.graph import gen_lfr
+
+    torch.cuda.set_device(0)
+    params = dict(minGraphSize=1000, threshold=1e-9, C_threshold=1e-6)
+    t0 = time.perf_counter()
+    feats, labels, t_trace, t_gen = [], [], 0.0, 0.0
+    ntrain = args.las_train_graphs
+    calib = None
+    for i in range(ntrain + 1):
+        t = time.perf_counter()
+        g = gen_lfr(args.las_train_nv, seed=100 + i)
+        t_gen += time.perf_counter() - t
+        dg = engine.DeviceGraph(g, "cuda:0")
+        C = torch.empty(g.numVertices, dtype=torch.int64, device="cuda:0")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        f, l, _, _, _ = las.trace_dev(dg, C, rate=args.las_rate, seed=1000 + i, cap=1 << 24, **params)
+        torch.cuda.synchronize()
+        t_trace += time.perf_counter() - t
+        if i < ntrain:
+            feats.append(f.clone())
+            labels.append(l.clone())
+        else:
+            calib = (f.clone(), l.clone())
+        del dg, C, f, l
+        print(f"[bench] trace {i}: {feats[-1].shape[0] if i < ntrain else calib[0].shape[0]} rows",
+              file=sys.stderr, flush=True)
+    feats, labels = torch.cat(feats), torch.cat(labels)
+    pred = las.ActiveSetPredictor(seed=1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    losses = pred.fit(feats, labels, steps=args.las_steps, batch=4096, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    tau, kept = pred.calibrate(calib[0], calib[1], recall=args.las_recall)
+    print(f"[bench] trained {args.las_steps} steps in {t_train:.2f}s, loss {losses[:50].mean():.4f} -> "
+          f"{losses[-50:].mean():.4f}, tau {tau:.4f}, kept {kept:.3f}", file=sys.stderr, flush=True)
+
+    g = gen_lfr(args.las_eval_nv, seed=200)
+    dg = engine.DeviceGraph(g, "cuda:0")
+    C = torch.empty(g.numVertices, dtype=torch.int64, device="cuda:0")
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        res, ts = [], []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            res.append(fn())
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return res[-1], sum(ts) / len(ts)
+
+    def phases(st):
+        pp = st["per_phase"]
+        return [{"nv": int(pp["phase_nv"][i]), "iterations": int(pp["phase_iters"][i]),
+                 "ms": round(float(pp["phase_seconds"][i]) * 1e3, 2)} for i in range(len(pp["phase_nv"]))]
+
+    st0, t_plain = timed(lambda: engine.run_colored_dev(dg, C, **params))
+    policies = []
+    for R in [int(x) for x in args.las_rechecks.split(",")]:
+        (st1, ls), t_las = timed(lambda: pred.run(dg, C, max_rechecks=R, **params))
+        q_las = engine.modularity_dev(dg, C)
+        policies.append(dict(max_rechecks=R, ms=t_las * 1e3, speedup=t_plain / t_las,
+                             final_modularity=st1["final_modularity"], partition_modularity=q_las,
+                             dQ=st1["final_modularity"] - st0["final_modularity"],
+                             edges_visited=st1["edges_visited"], per_phase=phases(st1), **ls))
+        print(f"[bench] rechecks {R}: {t_las * 1e3:.1f} ms vs plain {t_plain * 1e3:.1f} ms, "
+              f"dQ {policies[-1]['dQ']:.2e}", file=sys.stderr, flush=True)
+    best = policies[0]
+    t_las = best["ms"] / 1e3
+    out = {
+        "metric": "Louvain time-to-converge, learned-pruned vs plain GPU form (held-out LFR)",
+        "value": t_plain / t_las, "unit": "x", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": True, "dtype": "f64 (MLP bf16/fp32)",
+        "data": "synthetic",
+        "config": {"workload": f"LFR-style {g.numVertices:,} vertices (seed 200), "
+                               f"trained on {ntrain} x {args.las_train_nv:,}-vertex LFR "
+                               f"(seeds 100..{99 + ntrain})", "nnz": g.nnz,
+                   "threshold": 1e-9, "C_threshold": 1e-6, "minGraphSize": 1000},
+        "plain": {"ms": t_plain * 1e3, "final_modularity": st0["final_modularity"],
+                  "edges_visited": st0["edges_visited"], "iterations": st0["total_iterations"],
+                  "edges_per_s": st0["edges_visited"] / t_plain, "per_phase": phases(st0)},
+        "pruned": best,
+        "policies": policies[1:],
+        "dQ": best["dQ"],
+        "train": {"graphs": ntrain, "rows": int(feats.shape[0]),
+                  "positive_rate": float(labels.double().mean().item()), "steps": args.las_steps,
+                  "batch": 4096, "train_s": t_train, "trace_s": t_trace, "generate_s": t_gen,
+                  "loss_first": float(losses[:50].mean().item()),
+                  "loss_last": float(losses[-50:].mean().item()), "tau": tau,
+                  "calibration_kept": kept, "recall_target": args.las_recall,
+                  "total_s": time.perf_counter() - t0},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +229,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="do not time kernels with HIP events in the timed region")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned active-set pruning vs the plain form (held-out LFR graph)")
+    ap.add_argument("--las-train-graphs", type=int, default=32)
+    ap.add_argument("--las-train-nv", type=int, default=1048576)
+    ap.add_argument("--las-eval-nv", type=int, default=8388608)
+    ap.add_argument("--las-rate", type=float, default=0.125)
+    ap.add_argument("--las-steps", type=int, default=3000)
+    ap.add_argument("--las-recall", type=float, default=0.98)
+    ap.add_argument("--las-rechecks", default="1",
+                    help="comma list of per-phase re-check caps to time (first one is reported)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -131,6 +250,9 @@ def main():
     from grappolo_amd import engine
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.learned:
+        run_learned(args)
+        return
     torch.cuda.set_device(0)
     spec = WORKLOADS[args.workload]
     t = time.perf_counter()

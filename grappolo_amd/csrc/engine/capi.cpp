@@ -1,6 +1,7 @@
 // capi.cpp -- the C ABI (include/grappolo_hip.h) and Grappolo's C++ entry
 // points (include/grappolo/*.h) on top of the engine.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -14,6 +15,7 @@
 #include "grappolo_hip.h"
 #include "kernels.hpp"
 #include "louvain.hpp"
+#include "mlp.hpp"
 #include "profiler.hpp"
 
 using glv::DBuf;
@@ -179,6 +181,15 @@ void fill_stats(const glv::RunResult &r, glv_stats *st) {
     st->n_trace = (int32_t)r.trace.size();
     for (int i = 0; i < (int)r.trace.size() && i < st->trace_cap; i++)
         if (st->trace_q) st->trace_q[i] = r.trace[i];
+}
+
+void fill_las(const glv::LasParams &L, glv_las_stats *st) {
+    if (!st) return;
+    st->swept_vertices = L.swept_vertices;
+    st->swept_entries = L.swept_entries;
+    st->full_sweeps = L.full_sweeps;
+    st->pruned_sweeps = L.pruned_sweeps;
+    st->samples = L.count;
 }
 
 // A graph without any entry has no defined modularity (1/(2m) is infinite);
@@ -612,6 +623,131 @@ int glv_count_conflicts_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
         Engine E(s);
         i64 c = E.count_conflicts(g, d_colors);
         if (conflicts) *conflicts = c;
+    });
+}
+
+int glv_mlp_param_count(void) { return glv::kMlpParams; }
+
+int glv_mlp_forward_dev(const float *d_params, const uint16_t *d_feats, int64_t n, float *d_logits,
+                        void *stream) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (!d_params || !d_feats || !d_logits)))
+            glv::fail(GLV_E_INVALID_ARG, "null pointer or negative size");
+        hipStream_t s = engine_stream(stream);
+        glv::launch_mlp_forward(d_params, d_feats, n, d_logits, s);
+        GLV_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int glv_mlp_grad_dev(const float *d_params, const uint16_t *d_feats, const uint8_t *d_labels, int64_t n,
+                     float pos_weight, float *d_grad, void *stream) {
+    return guarded([&] {
+        if (n <= 0 || !d_params || !d_feats || !d_labels || !d_grad)
+            glv::fail(GLV_E_INVALID_ARG, "null pointer or empty sample set");
+        hipStream_t s = engine_stream(stream);
+        glv::MlpBatch B;
+        B.feats = d_feats;
+        B.labels = d_labels;
+        B.n = n;
+        B.batch = n;
+        B.pos_weight = pos_weight;
+        const i64 nw = glv::mlp_grad_waves(n);
+        glv::DBuf<float> gpart(nw * glv::kMlpStride, s);
+        glv::launch_mlp_grad_partials(d_params, B, gpart.p, s);
+        glv::launch_mlp_reduce(gpart.p, nw, d_grad, s);
+        GLV_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int glv_mlp_train_dev(float *d_params, float *d_m, float *d_v, const uint16_t *d_feats,
+                      const uint8_t *d_labels, int64_t n, int64_t steps, int64_t batch, float lr,
+                      float pos_weight, uint64_t seed, int64_t step0, float *d_losses, void *stream) {
+    return guarded([&] {
+        if (n <= 0 || batch <= 0 || steps < 0 || step0 < 0 || !d_params || !d_m || !d_v || !d_feats ||
+            !d_labels)
+            glv::fail(GLV_E_INVALID_ARG, "null pointer or non-positive size");
+        hipStream_t s = engine_stream(stream);
+        glv::MlpBatch B;
+        B.feats = d_feats;
+        B.labels = d_labels;
+        B.n = n;
+        B.batch = batch;
+        B.random = 1;
+        B.seed = seed;
+        B.pos_weight = pos_weight;
+        const i64 nw = glv::mlp_grad_waves(batch);
+        glv::DBuf<float> gpart(nw * glv::kMlpStride, s);
+        const double b1 = 0.9, b2 = 0.999;
+        for (int64_t t = 0; t < steps; t++) {
+            B.step = step0 + t;
+            const double k = (double)(step0 + t + 1);
+            const float c1 = (float)(1.0 / (1.0 - std::pow(b1, k)));
+            const float c2 = (float)(1.0 / (1.0 - std::pow(b2, k)));
+            glv::launch_mlp_grad_partials(d_params, B, gpart.p, s);
+            glv::launch_mlp_adam(d_params, d_m, d_v, gpart.p, nw, lr, (float)b1, (float)b2, 1e-8f, c1, c2,
+                                 d_losses ? d_losses + t : nullptr, s);
+        }
+        GLV_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int glv_las_trace_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                      const double *d_weights, int64_t *d_C_orig, int64_t minGraphSize, double threshold,
+                      double C_threshold, double rate, uint64_t seed, uint16_t *d_feats, int64_t *d_verts,
+                      uint8_t *d_labels, int64_t cap, void *stream, glv_stats *stats,
+                      glv_las_stats *las) {
+    return guarded([&] {
+        if (cap < 0 || (cap > 0 && (!d_feats || !d_verts || !d_labels)) || !(rate >= 0 && rate <= 1))
+            glv::fail(GLV_E_INVALID_ARG, "bad trace buffers or rate");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        glv::LasParams L;
+        L.mode = 1;
+        L.rate = rate;
+        L.seed = seed;
+        L.tfeats = d_feats;
+        L.tverts = d_verts;
+        L.tlabels = d_labels;
+        L.cap = cap;
+        glv::RunParams p;
+        p.minGraphSize = minGraphSize;
+        p.threshold = threshold;
+        p.C_threshold = C_threshold;
+        p.las = &L;
+        glv::RunResult r;
+        run_pipeline(true, g, d_C_orig, p, s, r);
+        fill_stats(r, stats);
+        fill_las(L, las);
+    });
+}
+
+int glv_runMultiPhaseColoring_las_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                                      const int64_t *d_tails, const double *d_weights, int64_t *d_C_orig,
+                                      int64_t minGraphSize, double threshold, double C_threshold,
+                                      const float *d_params, float tau, int max_rechecks,
+                                      void *stream, glv_stats *stats, glv_las_stats *las) {
+    return guarded([&] {
+        if (!d_params) glv::fail(GLV_E_INVALID_ARG, "null MLP parameters");
+        if (!(tau >= 0.0f && tau < 1.0f)) glv::fail(GLV_E_INVALID_ARG, "tau outside [0, 1)");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        glv::LasParams L;
+        L.mode = 2;
+        L.P = d_params;
+        if (max_rechecks < 0) glv::fail(GLV_E_INVALID_ARG, "max_rechecks < 0");
+        L.logit_tau = tau > 0.0f ? std::log(tau / (1.0f - tau)) : -INFINITY;
+        L.max_rechecks = max_rechecks;
+        glv::RunParams p;
+        p.minGraphSize = minGraphSize;
+        p.threshold = threshold;
+        p.C_threshold = C_threshold;
+        p.las = &L;
+        glv::RunResult r;
+        run_pipeline(true, g, d_C_orig, p, s, r);
+        fill_stats(r, stats);
+        fill_las(L, las);
     });
 }
 

@@ -36,6 +36,11 @@ constexpr int kBlockHashSlots = 4096;
 constexpr i64 kDenseMaxNV = 15360;
 constexpr int kDenseBlocks = 1024;
 constexpr i64 kPersistMaxBlocks = 64;  // cooperative grid of the persistent sweep
+// The persistent sweep serves graphs of at most this many vertices whose
+// color classes hold at most kPersistMaxPerColor vertices on average (coarse
+// phases: many small classes, where per-color launches would be latency-bound).
+constexpr i64 kPersistMaxNV = 1 << 20;
+constexpr i64 kPersistMaxPerColor = 256;
 constexpr int kBarrierWords = 16 + 16 * 64;  // abort word + one 64-byte flag line per workgroup
 // touched communities of one vertex, listed in LDS: at most min(degree, nv)
 constexpr int kDenseListMax = kDenseMaxNV;

@@ -17,6 +17,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "kernels.hpp"
+#include "mlp_device.hpp"
 
 namespace glv {
 
@@ -113,6 +114,25 @@ __device__ void block_argmax(bool &cand, double &gain, i64 &id) {
     cand = c;
     gain = g;
     id = d;
+    __syncthreads();
+}
+
+// Block-wide top-two of per-thread (b1, b2), result valid in every thread.
+template <int BS>
+__device__ void block_top2(float &b1, float &b2) {
+    __shared__ float s1[BS / kWave], s2[BS / kWave];
+    top2_group<kWave>(b1, b2);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    __syncthreads();
+    if (l == 0) {
+        s1[w] = b1;
+        s2[w] = b2;
+    }
+    __syncthreads();
+    float x1 = s1[0], x2 = s2[0];
+    for (int q = 1; q < BS / kWave; q++) top2_merge(x1, x2, s1[q], s2[q]);
+    b1 = x1;
+    b2 = x2;
     __syncthreads();
 }
 
@@ -330,12 +350,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
     }
     bool cand = lane < deg && first && c != s;
     double gain = 0;
+    float t1 = -INFINITY, t2 = -INFINITY;
     if (cand) {
         const double kv = A.k[v];
         gain = gain_of(e_mine, e_s - sl, kv, A.a[c], A.a[s] - kv, A.constant);
+        if (A.feat) t1 = (float)(gain / (2.0 * kv));
         cand = gain > 0;  // max() starts from maxGain = 0 and needs a strict increase
     }
     group_argmax<G>(cand, gain, c);
+    if (A.feat) top2_group<G>(t1, t2);
     if (valid && lane == 0) {
         i64 target = s;
         if (cand) {
@@ -343,6 +366,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
             if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;  // swap protection
         }
         A.tgt[v] = target;
+        if (A.feat || A.dein) {
+            const double kv = A.k[v];
+            if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+            if (A.dein)
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * A.constant : 0.0;
+        }
     }
     if (PLAIN) {
         if (lane == 0) s_part[grp] = valid ? e_s : 0.0;
@@ -386,11 +415,13 @@ __device__ inline double hash_lookup(const i64 *keys, const double *vals, u64 ma
 // Score every occupied slot of a table and reduce over the calling group.
 __device__ inline void score_slots(const i64 *keys, const double *vals, i64 slots, int first, int step,
                                    const SweepArgs &A, i64 s, double kv, double eix, double ax,
-                                   bool &cand, double &gain, i64 &id) {
+                                   bool &cand, double &gain, i64 &id, float &t1, float &t2) {
+    const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
     for (i64 h = first; h < slots; h += step) {
         i64 y = keys[h];
         if (y == kEmpty || y == s) continue;
         double g = gain_of(vals[h], eix, kv, A.a[y], ax, A.constant);
+        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
         bool c = g > 0;
         if (better(c, g, y, cand, gain, id)) {
             cand = true;
@@ -435,11 +466,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
     bool cand = false;
     double gain = 0;
     i64 id = INT64_MAX;
+    float t1 = -INFINITY, t2 = -INFINITY;
     if (valid) {
         const double kv = A.k[v];
-        score_slots(keys, vals, SLOTS, lane, kWave, A, s, kv, e_s - sl, A.a[s] - kv, cand, gain, id);
+        score_slots(keys, vals, SLOTS, lane, kWave, A, s, kv, e_s - sl, A.a[s] - kv, cand, gain, id,
+                    t1, t2);
     }
     group_argmax<kWave>(cand, gain, id);
+    if (A.feat) top2_group<kWave>(t1, t2);
     if (valid && lane == 0) {
         i64 target = s;
         if (cand) {
@@ -447,6 +481,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
             if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
         }
         A.tgt[v] = target;
+        if (A.feat || A.dein) {
+            const double kv = A.k[v];
+            if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+            if (A.dein)
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * A.constant : 0.0;
+        }
     }
     if (PLAIN) {
         if (lane == 0) s_part[wid] = valid ? e_s : 0.0;
@@ -511,9 +551,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
         bool cand = false;
         double gain = 0;
         i64 id = INT64_MAX;
+        float t1 = -INFINITY, t2 = -INFINITY;
         score_slots(keys, vals, slots, threadIdx.x, kBlock, A, s, kv, e_s - sl, A.a[s] - kv, cand,
-                    gain, id);
+                    gain, id, t1, t2);
         block_argmax(cand, gain, id);
+        if (A.feat) block_top2<kBlock>(t1, t2);
         if (threadIdx.x == 0) {
             i64 target = s;
             if (cand) {
@@ -521,6 +563,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
                 if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
             }
             A.tgt[v] = target;
+            if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+            if (A.dein)
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * A.constant : 0.0;
         }
         part += e_s;
         __syncthreads();
@@ -567,10 +612,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
         bool cand = false;
         double gain = 0;
         i64 id = INT64_MAX;
+        float t1 = -INFINITY, t2 = -INFINITY;
+        const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
         for (int k = threadIdx.x; k < ntouched; k += kBlock) {
             const i64 y = list[k];
             if (y == s) continue;
             const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
+            if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
             if (better(g > 0, g, y, cand, gain, id)) {
                 cand = true;
                 gain = g;
@@ -578,6 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
             }
         }
         block_argmax(cand, gain, id);  // ends with a barrier: every read of acc is done
+        if (A.feat) block_top2<kBlock>(t1, t2);
         if (threadIdx.x == 0) {
             i64 target = s;
             if (cand) {
@@ -585,6 +634,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
                 if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
             }
             A.tgt[v] = target;
+            if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+            if (A.dein)
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - ax) * A.constant : 0.0;
         }
         part += e_s;
         for (int k = threadIdx.x; k < ntouched; k += kBlock) acc[list[k]] = 0.0;
@@ -684,25 +736,49 @@ __device__ bool grid_barrier(unsigned *bar, unsigned nb, unsigned seq) {
     return s_ok != 0;
 }
 
+// LDS of the persistent sweep: either the dense accumulator of graphs with at
+// most kDenseMaxNV vertices, or (HASH) an open-addressing table of
+// kPHashSlots (community -> weight) slots for vertices of degree up to
+// kPHashMaxDeg; larger vertices use the workgroup's HBM table (A.gkeys).
+constexpr int kPHashSlots = 8192;
+constexpr i64 kPHashMaxDeg = kPHashSlots / 2;
+constexpr size_t kPDenseBytes = kDenseMaxNV * 8 + kDenseMaxNV / 32 * 4 + kDenseListMax * 2;
+constexpr size_t kPHashBytes = (size_t)kPHashSlots * 16;
+constexpr size_t kPSmemBytes = kPDenseBytes > kPHashBytes ? kPDenseBytes : kPHashBytes;
+
 // cperm: vertices grouped by color (cstart[c] .. cstart[c+1]).
+template <bool HASH>
 __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 *C, double *a,
                                                                i64 *n, const i64 *cperm,
                                                                const i64 *cstart, int ncol,
-                                                               unsigned *bar, double *part) {
-    __shared__ double acc[kDenseMaxNV];
-    __shared__ uint32_t touched[kDenseMaxNV / 32];
-    __shared__ uint16_t list[kDenseListMax];
+                                                               unsigned *bar, double *part,
+                                                               uint8_t *stale) {
+    __shared__ __align__(16) unsigned char smem[kPSmemBytes];
     __shared__ int wsum[kPBlock / kWave];
     __shared__ double red[kPBlock / kWave];
     __shared__ int sc[kPBlock / kWave];
     __shared__ double sg[kPBlock / kWave];
     __shared__ i64 si[kPBlock / kWave];
+    // dense view
+    double *acc = reinterpret_cast<double *>(smem);
+    uint32_t *touched = reinterpret_cast<uint32_t *>(smem + kDenseMaxNV * 8);
+    uint16_t *list = reinterpret_cast<uint16_t *>(smem + kDenseMaxNV * 8 + kDenseMaxNV / 32 * 4);
+    // hash view
+    i64 *hkeys = reinterpret_cast<i64 *>(smem);
+    double *hvals = reinterpret_cast<double *>(smem + (size_t)kPHashSlots * 8);
     const int nv = (int)A.nv;
     const int nwords = (nv + 31) / 32;
     const unsigned nb = gridDim.x;
     unsigned seq = 0;
-    for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
-    for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+    if (HASH) {
+        for (int i = threadIdx.x; i < kPHashSlots; i += kPBlock) {
+            hkeys[i] = kEmpty;
+            hvals[i] = 0.0;
+        }
+    } else {
+        for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
+        for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+    }
     __syncthreads();
     bool alive = true;
     for (int c = 0; c < ncol && alive; c++) {
@@ -711,36 +787,75 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
         for (i64 q = blockIdx.x; q < total; q += nb) {
             const i64 v = cperm[lo + q];
             const i64 beg = A.off[v], end = A.off[v + 1];
-            double sl = 0;
+            const i64 s = A.C[v];
+            double sl = 0, e_s = 0;
+            // hash table of this vertex (HASH only)
+            i64 *keys = hkeys;
+            double *vals = hvals;
+            i64 slots = 64;
+            int shift = 58;
+            if (HASH) {
+                while (slots < 2 * (end - beg)) slots <<= 1;
+                if (end - beg > kPHashMaxDeg) {
+                    keys = A.gkeys + (i64)blockIdx.x * A.gslots;
+                    vals = A.gvals + (i64)blockIdx.x * A.gslots;
+                    for (i64 h = threadIdx.x; h < slots; h += kPBlock) {
+                        keys[h] = kEmpty;
+                        vals[h] = 0.0;
+                    }
+                    __threadfence_block();
+                    __syncthreads();
+                }
+                int lg = 0;
+                while (((i64)1 << lg) < slots) lg++;
+                shift = 64 - lg;
+            }
+            const u64 mask = (u64)slots - 1;
             for (i64 j = beg + threadIdx.x; j < end; j += kPBlock) {
                 const i64 t = A.tails[j];
                 const double ww = A.w[j];
                 if (t == v) sl += ww;
                 const i64 cc = A.C[t];
-                atomicOr(&touched[cc >> 5], 1u << (cc & 31));
-                atomicAdd(&acc[cc], ww);
+                if (HASH) {
+                    hash_insert(keys, vals, mask, shift, cc, ww);
+                } else {
+                    atomicOr(&touched[cc >> 5], 1u << (cc & 31));
+                    atomicAdd(&acc[cc], ww);
+                }
             }
+            if (HASH) __threadfence_block();
             __syncthreads();
             sl = block_sum_p(sl, red);
-            const int ntouched = compact_touched<kPBlock>(touched, nwords, list, wsum);
-            const i64 s = A.C[v];
-            const double e_s = acc[s];
             const double kv = A.k[v];
-            const double eix = e_s - sl, ax = A.a[s] - kv;
+            const double ax = A.a[s] - kv;
             bool cand = false;
             double gain = 0;
             i64 id = INT64_MAX;
-            for (int k = threadIdx.x; k < ntouched; k += kPBlock) {
-                const i64 y = list[k];
-                if (y == s) continue;
-                const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
-                if (better(g > 0, g, y, cand, gain, id)) {
-                    cand = true;
-                    gain = g;
-                    id = y;
+            float t1 = -INFINITY, t2 = -INFINITY;
+            int ntouched = 0;
+            if (HASH) {
+                e_s = hash_lookup(keys, vals, mask, shift, s);
+                score_slots(keys, vals, slots, threadIdx.x, kPBlock, A, s, kv, e_s - sl, ax, cand, gain,
+                            id, t1, t2);
+            } else {
+                ntouched = compact_touched<kPBlock>(touched, nwords, list, wsum);
+                e_s = acc[s];
+                const double eix = e_s - sl;
+                const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
+                for (int k = threadIdx.x; k < ntouched; k += kPBlock) {
+                    const i64 y = list[k];
+                    if (y == s) continue;
+                    const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
+                    if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+                    if (better(g > 0, g, y, cand, gain, id)) {
+                        cand = true;
+                        gain = g;
+                        id = y;
+                    }
                 }
             }
             block_argmax_p(cand, gain, id, sc, sg, si);
+            if (A.feat) block_top2<kPBlock>(t1, t2);
             if (threadIdx.x == 0) {
                 i64 target = s;
                 if (cand) {
@@ -748,10 +863,19 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                     if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
                 }
                 A.tgt[v] = target;
+                if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             }
-            __syncthreads();  // every read of acc is done
-            for (int k = threadIdx.x; k < ntouched; k += kPBlock) acc[list[k]] = 0.0;
-            for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+            __syncthreads();  // every read of the accumulator is done
+            if (HASH) {
+                if (keys == hkeys)
+                    for (i64 h = threadIdx.x; h < slots; h += kPBlock) {
+                        hkeys[h] = kEmpty;
+                        hvals[h] = 0.0;
+                    }
+            } else {
+                for (int k = threadIdx.x; k < ntouched; k += kPBlock) acc[list[k]] = 0.0;
+                for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+            }
             __syncthreads();
         }
         alive = grid_barrier(bar, nb, ++seq);
@@ -764,6 +888,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 if (t != s) {
                     move_vertex(t, s, A.k[v], a, n);
                     C[v] = t;
+                    if (stale) stale[v] = 0;
                 }
             }
         }
@@ -807,9 +932,10 @@ __global__ void k_isolated_targets(const i64 *verts, i64 count, i64 *tgt) {
 
 // Commit of one color step: community updates of the movers, then labels.
 __global__ void k_apply_colored(const i64 *perm, Slices S, const i64 *tgt, i64 *C, const double *k,
-                                double *a, i64 *n) {
+                                double *a, i64 *n, uint8_t *stale, const double *dein, double *dsum) {
     i64 total = 0;
     for (int b = 0; b < kNumBins; b++) total += S.hi[b] - S.lo[b];
+    double d = 0;
     for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < total; q += (i64)gridDim.x * blockDim.x) {
         i64 r = q;
         int b = 0;
@@ -822,7 +948,13 @@ __global__ void k_apply_colored(const i64 *perm, Slices S, const i64 *tgt, i64 *
         if (t != s) {
             move_vertex(t, s, k[v], a, n);
             C[v] = t;
+            if (stale) stale[v] = 0;
+            if (dein) d += dein[v];
         }
+    }
+    if (dsum) {
+        d = block_sum(d);
+        if (threadIdx.x == 0 && d != 0.0) atomicAdd(dsum, d);
     }
 }
 // cInfo += cUpdate of a Jacobi iteration (parallelLouvainMethod.cpp:223-236, :275-279).
@@ -1414,17 +1546,22 @@ void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hi
 int persistent_blocks() {
     static int cached = -1;
     if (cached < 0) {
-        int dev = 0, cus = 0, per = 0;
+        int dev = 0, cus = 0, per = 0, per_h = 0;
         GLV_HIP(hipGetDevice(&dev));
         GLV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_colored_iteration, kPBlock, 0));
+        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_colored_iteration<false>, kPBlock, 0));
+        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_h, k_colored_iteration<true>, kPBlock, 0));
+        per = std::min(per, per_h);
         cached = per > 0 ? cus * per : 0;
     }
     return cached;
 }
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
-                              unsigned *bar, double *part, int nblocks, hipStream_t s) {
-    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || A.nv > kDenseMaxNV) return false;
+                              unsigned *bar, double *part, int nblocks, hipStream_t s, uint8_t *stale) {
+    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || A.nv > kPersistMaxNV) return false;
+    const bool hash = A.nv > kDenseMaxNV;
+    // degrees above the LDS table's reach need the workgroup's HBM table
+    if (hash && A.gslots > 0 && (A.gkeys == nullptr || A.gvals == nullptr)) return false;
     SweepArgs aa = A;
     i64 *C = const_cast<i64 *>(A.C);
     double *ca = const_cast<double *>(A.a);
@@ -1434,9 +1571,10 @@ bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *st
     int nc = ncol;
     unsigned *b = bar;
     double *pt = part;
-    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt};
-    hipError_t e = hipLaunchCooperativeKernel((const void *)k_colored_iteration, dim3(nblocks),
-                                              dim3(kPBlock), args, 0, s);
+    uint8_t *sf = stale;
+    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt, &sf};
+    const void *fn = hash ? (const void *)k_colored_iteration<true> : (const void *)k_colored_iteration<false>;
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kPBlock), args, 0, s);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return false;
@@ -1449,11 +1587,13 @@ void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t 
     GLV_LAUNCH_CHECK();
 }
 void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 *C, const double *k,
-                          double *a, i64 *n, hipStream_t s) {
+                          double *a, i64 *n, hipStream_t s, uint8_t *stale, const double *dein,
+                          double *dsum) {
     i64 total = 0;
     for (int b = 0; b < kNumBins; b++) total += S.hi[b] - S.lo[b];
     if (total <= 0) return;
-    k_apply_colored<<<grid_for(total, kBlock, 8192), kBlock, 0, s>>>(perm, S, tgt, C, k, a, n);
+    k_apply_colored<<<grid_for(total, kBlock, 8192), kBlock, 0, s>>>(perm, S, tgt, C, k, a, n, stale, dein,
+                                                                     dsum);
     GLV_LAUNCH_CHECK();
 }
 void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a, i64 *n,

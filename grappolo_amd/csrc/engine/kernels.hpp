@@ -26,6 +26,9 @@ struct SweepArgs {
     double *gvals;
     i64 gslots;
     i64 nv;            // vertices of the graph (community ids are < nv)
+    // learned active set (both null on the plain path):
+    float4 *feat = nullptr;  // per swept vertex: own-community fraction, top-two gains
+    double *dein = nullptr;  // per swept vertex: 2 (e_{v,t} - e_{v,s}) of its move, 0 if none
 };
 
 // Up to one slice per degree bin of a color class (perm[lo[b] .. hi[b])).
@@ -72,10 +75,14 @@ int persistent_blocks();
 void launch_color_major(const i64 *perm, const int32_t *colors, i64 n, int ncol, i64 *cperm,
                         i64 *cstart, hipStream_t s);
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
-                              unsigned *bar, double *part, int nblocks, hipStream_t s);
+                              unsigned *bar, double *part, int nblocks, hipStream_t s,
+                              uint8_t *stale = nullptr);
 void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s);
+// stale (nullable): reset to 0 for every mover; dsum (nullable): += sum of
+// dein over the movers
 void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 *C, const double *k,
-                          double *a, i64 *n, hipStream_t s);
+                          double *a, i64 *n, hipStream_t s, uint8_t *stale = nullptr,
+                          const double *dein = nullptr, double *dsum = nullptr);
 void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a, i64 *n,
                         hipStream_t s);
 void launch_init_comm_ass_opt(const DevGraph &g, const double *k, const double *a, const i64 *n,

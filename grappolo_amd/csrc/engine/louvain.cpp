@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstring>
 
+#include "active_set.hpp"
 #include "kernels.hpp"
 #include "profiler.hpp"
 
@@ -144,7 +145,7 @@ i64 Engine::count_conflicts(const DevGraph &g, const int32_t *colors) {
 // --------------------------------------------------------------- modularity
 // e_xx over all vertices (per-bin passes into block partials), sum of a_c^2.
 double Engine::colored_modularity(const DevGraph &g, const PhaseSetup &P, const i64 *C, double *part,
-                                  double *res) {
+                                  double *res, double *exx) {
     // algorithmic bytes: tail, weight and C[tail] per entry; C[v] per vertex; a_c per community
     Profiler::get().begin(K_MOD, s_);
     i64 off = 0;
@@ -161,6 +162,7 @@ double Engine::colored_modularity(const DevGraph &g, const PhaseSetup &P, const 
     double h[2];
     GLV_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s_));
     GLV_HIP(hipStreamSynchronize(s_));
+    if (exx) *exx = h[0];
     const double c = P.constant;
     return h[0] * (double)c - h[1] * (double)c * (double)c;
 }
@@ -173,8 +175,10 @@ static i64 mod_part_count(const PhaseSetup &P) {
 
 // ----------------------------------------------------------- colored phase
 // algoLouvainWithDistOneColoring, Coloring/parallelLouvainWithColoring.cpp:47-286.
+// With a learned active set (las), see LasParams.
 double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, double Lower,
-                             double thresh, int *iters, std::vector<double> *trace) {
+                             double thresh, int *iters, std::vector<double> *trace, LasParams *las,
+                             int phase_index) {
     const i64 nv = g.nv;
     PhaseSetup P;
     setup(g, colors, ncol, P);
@@ -182,13 +186,32 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     launch_init_comm(P.k.p, nv, P.a.p, P.n.p, s_);
     DBuf<i64> tgt(nv, s_);
     DBuf<double> part(mod_part_count(P), s_), res(2, s_);
+
+    std::vector<char> color_nonempty(ncol, 0);
+    for (int c = 0; c < ncol; c++)
+        for (int b = 1; b < kNumBins; b++)
+            if (P.hi(b, c) > P.lo(b, c)) color_nonempty[c] = 1;
+    // coarse graphs (small, or many small color classes): one cooperative
+    // launch per iteration (all colors + modularity), grid sized to the color
+    // classes: every extra workgroup only lengthens the barriers
+    int pblocks = 0;
+    if (nv <= kPersistMaxNV) {
+        int nonempty = 0;
+        for (int c = 0; c < ncol; c++) nonempty += color_nonempty[c];
+        const i64 per_class = (nv + std::max(nonempty, 1) - 1) / std::max(nonempty, 1);
+        if (nv <= kDenseMaxNV || per_class <= kPersistMaxPerColor) {
+            pblocks = persistent_blocks();
+            if (pblocks > 0)
+                pblocks = (int)std::min<i64>({(i64)pblocks, kPersistMaxBlocks, std::max<i64>(4, 2 * per_class)});
+        }
+    }
     DBuf<i64> gkeys;
     DBuf<double> gvals;
     i64 gslots = 0;
     if (P.maxdeg_global > 0) {
         gslots = 1;
         while (gslots < 2 * P.maxdeg_global) gslots <<= 1;
-        const i64 nb = std::min<i64>(kGlobalHashBlocks, P.bin_hi(8) - P.bin_lo(8));
+        const i64 nb = std::max<i64>(std::min<i64>(kGlobalHashBlocks, P.bin_hi(8) - P.bin_lo(8)), pblocks);
         gkeys.alloc(nb * gslots, s_);
         gvals.alloc(nb * gslots, s_);
     }
@@ -207,33 +230,104 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     A.gvals = gvals.p;
     A.gslots = gslots;
     A.nv = nv;
-
-    std::vector<char> color_nonempty(ncol, 0);
-    for (int c = 0; c < ncol; c++)
-        for (int b = 1; b < kNumBins; b++)
-            if (P.hi(b, c) > P.lo(b, c)) color_nonempty[c] = 1;
-    // small graphs: one cooperative launch per iteration (all colors + modularity)
-    // grid sized to the color classes: every extra workgroup only lengthens the barriers
-    int pblocks = nv <= kDenseMaxNV ? persistent_blocks() : 0;
-    if (pblocks > 0) {
-        int nonempty = 0;
-        for (int c = 0; c < ncol; c++) nonempty += color_nonempty[c];
-        const i64 per_class = (nv + std::max(nonempty, 1) - 1) / std::max(nonempty, 1);
-        pblocks = (int)std::min<i64>({(i64)pblocks, kPersistMaxBlocks, std::max<i64>(4, 2 * per_class)});
-    }
     DBuf<unsigned> bar(pblocks > 0 ? kBarrierWords : 0, s_);
     DBuf<double> ppart(pblocks > 0 ? 2 * pblocks : 0, s_);
     DBuf<i64> cperm(pblocks > 0 ? nv : 0, s_), cstart(pblocks > 0 ? ncol + 1 : 0, s_);
     if (pblocks > 0) launch_color_major(P.perm.p, colors, nv, ncol, cperm.p, cstart.p, s_);
     bool persistent = pblocks > 0;
+
+    // learned active set: recorded features, sweeps since the last move,
+    // active flags and the compacted sweep lists
+    const int mode = las ? las->mode : 0;
+    const bool prune = mode == 2;
+    DBuf<float4> feat(mode ? nv : 0, s_);
+    DBuf<uint8_t> stale(mode ? nv : 0, s_), active(prune ? nv : 0, s_);
+    DBuf<i64> flags(prune ? nv : 0, s_), fpos(prune ? nv : 0, s_), list2(prune ? nv : 0, s_);
+    const i64 nkeys = (i64)kNumBins * ncol;
+    DBuf<i64> start2(prune ? std::max<i64>(nkeys, ncol) + 1 : 0, s_);
+    DBuf<i64> lcount(mode ? 2 : 0, s_);
+    DBuf<double> dein(prune ? nv : 0, s_), dsum(prune ? 1 : 0, s_);
+    std::vector<i64> ptr2;
+    LasView V;
+    if (mode) {
+        launch_fill_i32((int32_t *)feat.p, 4 * nv, 0, s_);
+        GLV_HIP(hipMemsetAsync(stale.p, 0, nv, s_));
+        A.feat = feat.p;
+        V.off = g.off;
+        V.feat = feat.p;
+        V.stale = stale.p;
+        V.colors = colors;
+        V.ncol = ncol;
+        V.C = C;
+        V.n = P.n.p;
+        V.phase = phase_index;
+        V.nv = nv;
+    }
+    i64 nonisolated = 0;
+    for (int b = 1; b < kNumBins; b++) nonisolated += P.bin_hi(b) - P.bin_lo(b);
+
     double prevMod = Lower, currMod = -1;
+    double exx = 0;    // internal weight of the last exact modularity pass
+    bool full = true;  // this iteration sweeps every vertex
+    int rechecks = 0;  // full iterations that followed a pruned one
     int numItrs = 0;
     while (true) {
         numItrs++;
+        // trace capture: features of the previous sweep, labelled after this one
+        i64 tlo = 0, thi = 0;
+        if (mode == 1 && numItrs >= 2 && las->tfeats && las->count < las->cap) {
+            tlo = las->count;
+            GLV_HIP(hipMemcpyAsync(lcount.p, &tlo, sizeof(i64), hipMemcpyHostToDevice, s_));
+            const u64 seed = las->seed + 0x100000000ULL * (u64)phase_index + 0x10000ULL * (u64)numItrs;
+            launch_las_capture(V, seed, las->rate, las->tfeats, las->tverts, lcount.p, las->cap, s_);
+            las->count = fetch(lcount.p);
+            thi = std::min(las->count, las->cap);
+        }
+        const i64 *plist = persistent ? cperm.p : P.perm.p;
+        const i64 *pstart = cstart.p;
+        const std::vector<i64> *hptr = &P.ptr;
+        if (prune && !full) {
+            GLV_HIP(hipMemsetAsync(lcount.p, 0, 2 * sizeof(i64), s_));
+            launch_las_infer(V, las->P, las->logit_tau, active.p, lcount.p, s_);
+            if (persistent) {
+                launch_compact_active(cperm.p, nv, active.p, cstart.p, ncol, flags.p, fpos.p, list2.p,
+                                      start2.p, s_);
+                i64 hc[2];
+                GLV_HIP(hipMemcpyAsync(hc, lcount.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipStreamSynchronize(s_));
+                las->swept_vertices += hc[0];
+                las->swept_entries += hc[1];
+                pstart = start2.p;
+            } else {
+                launch_compact_active(P.perm.p, nv, active.p, P.dptr.p, nkeys, flags.p, fpos.p, list2.p,
+                                      start2.p, s_);
+                ptr2.resize(nkeys + 1);
+                i64 hc[2];
+                GLV_HIP(hipMemcpyAsync(ptr2.data(), start2.p, (nkeys + 1) * sizeof(i64),
+                                       hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipMemcpyAsync(hc, lcount.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipStreamSynchronize(s_));
+                las->swept_vertices += hc[0];
+                las->swept_entries += hc[1];
+                hptr = &ptr2;
+                GLV_HIP(hipMemsetAsync(dsum.p, 0, sizeof(double), s_));
+            }
+            plist = list2.p;
+            las->pruned_sweeps++;
+        } else if (prune) {
+            las->swept_vertices += nonisolated;
+            las->swept_entries += g.nnz;
+            las->full_sweeps++;
+        }
+        if (mode) launch_las_tick(stale.p, nv, s_);
+        const bool incremental = prune && !full && !persistent;
+        A.dein = incremental ? dein.p : nullptr;
+        bool done_iter = false;
         if (persistent) {
             launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
             Profiler::get().begin(K_PERSIST, s_);
-            persistent = launch_colored_iteration(A, cperm.p, cstart.p, ncol, bar.p, ppart.p, pblocks, s_);
+            persistent = launch_colored_iteration(A, plist, pstart, ncol, bar.p, ppart.p, pblocks, s_,
+                                                  mode ? stale.p : nullptr);
             Profiler::get().end(K_PERSIST, 24.0 * g.nnz + 32.0 * nv, s_);
             if (persistent) {
                 launch_sum_partials(ppart.p, pblocks, res.p, s_);
@@ -246,40 +340,77 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent sweep timed out");
                 const double c = P.constant;
                 currMod = h[0] * (double)c - h[1] * (double)c * (double)c;
-                if (trace) trace->push_back(currMod);
-                if ((currMod - prevMod) < thresh) break;
-                if (numItrs >= kMaxPhaseIters) break;
-                prevMod = currMod;
+                done_iter = true;
+            } else if (prune && !full) {
+                // cooperative launch refused: redo this pruned iteration as a full one
+                full = true;
+                numItrs--;
+                las->pruned_sweeps--;
                 continue;
             }
             // cooperative launch refused: this phase continues with per-color launches
         }
-        for (int c = 0; c < ncol; c++) {
-            if (!color_nonempty[c]) continue;  // stale colors can leave classes empty
-            Slices S;
-            S.lo[0] = S.hi[0] = 0;  // isolated vertices were labelled -1 at init
-            for (int b = 1; b < kNumBins; b++) {
-                S.lo[b] = P.lo(b, c);
-                S.hi[b] = P.hi(b, c);
-                if (S.hi[b] <= S.lo[b]) continue;
-                A.verts = P.perm.p + S.lo[b];
-                A.count = S.hi[b] - S.lo[b];
-                Profiler::get().begin(K_SWEEP_B1 + b - 1, s_);
-                launch_sweep(b, false, A, s_);
-                Profiler::get().end(K_SWEEP_B1 + b - 1, 24.0 * P.nnz_of(b, c) + 32.0 * A.count, s_);
+        if (!done_iter) {
+            const std::vector<i64> &ptr = *hptr;
+            auto lo = [&](int b, int c) { return ptr[(size_t)b * ncol + c]; };
+            auto hi = [&](int b, int c) { return ptr[(size_t)b * ncol + c + 1]; };
+            for (int c = 0; c < ncol; c++) {
+                if (!color_nonempty[c]) continue;  // stale colors can leave classes empty
+                Slices S;
+                S.lo[0] = S.hi[0] = 0;  // isolated vertices were labelled -1 at init
+                i64 ncnt = 0;
+                for (int b = 1; b < kNumBins; b++) {
+                    S.lo[b] = lo(b, c);
+                    S.hi[b] = hi(b, c);
+                    ncnt += S.hi[b] - S.lo[b];
+                }
+                if (ncnt == 0) continue;
+                for (int b = 1; b < kNumBins; b++) {
+                    if (S.hi[b] <= S.lo[b]) continue;
+                    A.verts = plist + S.lo[b];
+                    A.count = S.hi[b] - S.lo[b];
+                    Profiler::get().begin(K_SWEEP_B1 + b - 1, s_);
+                    launch_sweep(b, false, A, s_);
+                    Profiler::get().end(K_SWEEP_B1 + b - 1,
+                                        hptr == &P.ptr ? 24.0 * P.nnz_of(b, c) + 32.0 * A.count : 0.0, s_);
+                }
+                Profiler::get().begin(K_APPLY, s_);
+                launch_apply_colored(plist, S, tgt.p, C, P.k.p, P.a.p, P.n.p, s_,
+                                     mode ? stale.p : nullptr, A.dein, incremental ? dsum.p : nullptr);
+                Profiler::get().end(K_APPLY, 24.0 * ncnt, s_);
             }
-            i64 ncnt = 0;
-            for (int b = 1; b < kNumBins; b++) ncnt += S.hi[b] - S.lo[b];
-            Profiler::get().begin(K_APPLY, s_);
-            launch_apply_colored(P.perm.p, S, tgt.p, C, P.k.p, P.a.p, P.n.p, s_);
-            Profiler::get().end(K_APPLY, 24.0 * ncnt, s_);
+            if (incremental) {
+                // moves of one color step are pairwise non-adjacent, so the
+                // internal weight changes by the sum of the movers' 2(e_yt - e_ys)
+                launch_sumsq(P.a.p, nv, part.p, res.p + 1, s_);
+                double h[2];
+                GLV_HIP(hipMemcpyAsync(&h[0], dsum.p, sizeof(double), hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipMemcpyAsync(&h[1], res.p + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
+                GLV_HIP(hipStreamSynchronize(s_));
+                exx += h[0];
+                const double c = P.constant;
+                currMod = exx * (double)c - h[1] * (double)c * (double)c;
+            } else {
+                currMod = colored_modularity(g, P, C, part.p, res.p, &exx);
+            }
         }
-        currMod = colored_modularity(g, P, C, part.p, res.p);
+        if (thi > tlo) launch_las_label(las->tverts, tlo, thi, stale.p, las->tlabels, s_);
         if (trace) trace->push_back(currMod);
-        if ((currMod - prevMod) < thresh) break;  // :260
+        if (prune && full && numItrs > 1) rechecks++;
+        if ((currMod - prevMod) < thresh) {  // :260
+            if (!prune || full) break;
+            full = true;  // pruned sweeps stopped gaining: every vertex gets one more look
+        } else if (prune) {
+            if (full && las->max_rechecks > 0 && rechecks >= las->max_rechecks) {
+                prevMod = currMod;
+                break;
+            }
+            full = false;
+        }
         if (numItrs >= kMaxPhaseIters) break;
         prevMod = currMod;
     }
+    if (mode) sync();
     if (iters) *iters = numItrs;
     return prevMod;  // :284
 }
@@ -542,9 +673,10 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
         rec.nnz = G->nnz;
         int tmpItr = 0;
         t0 = now();
+        const i64 swept0 = p.las ? p.las->swept_entries : 0;
         if (!nonColor) {
             currMod = colored_phase(*G, Cbuf.p, colors.p, nColors, currMod, p.C_threshold, &tmpItr,
-                                    &r.trace);
+                                    &r.trace, p.las, phase - 1);
             rec.colored = 1;
             rec.ncolors = nColors;
             r.edges_visited_mod += (i64)tmpItr * G->nnz;
@@ -557,7 +689,10 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
         totItr += tmpItr;
         rec.iters = tmpItr;
         rec.mod = currMod;
-        r.edges_visited += (i64)tmpItr * G->nnz;
+        if (!nonColor && p.las && p.las->mode == 2)
+            r.edges_visited += p.las->swept_entries - swept0;  // entries the pruned sweeps read
+        else
+            r.edges_visited += (i64)tmpItr * G->nnz;
         t0 = now();
         numClusters = renumber(Cbuf.p, G->nv);
         launch_compose(C_orig, NV, Cbuf.p, phase == 1, s_);

@@ -33,6 +33,29 @@ struct RunResult {
     std::vector<double> trace;   // modularity of every iteration, all phases in order
 };
 
+// Learned active set (no counterpart in the reference).  mode 1 records
+// sweep traces of the unpruned method; mode 2 sweeps, after the first
+// iteration of a colored phase, only the vertices the MLP marks active, and
+// ends a phase only when an iteration over every vertex gains less than the
+// threshold.
+struct LasParams {
+    int mode = 0;               // 0 off, 1 trace capture, 2 prune
+    const float *P = nullptr;   // device MLP parameters (kMlpParams FP32)
+    float logit_tau = 0.0f;     // prune: active when the logit is >= this
+    int max_rechecks = 0;       // prune: a phase ends after this many full re-checks (0: no cap)
+    // trace capture: features (BF16 [cap][8]), vertex ids and next-sweep move labels
+    double rate = 0.0;
+    u64 seed = 0;
+    uint16_t *tfeats = nullptr;
+    i64 *tverts = nullptr;
+    uint8_t *tlabels = nullptr;
+    i64 cap = 0;
+    i64 count = 0;              // samples recorded (may exceed cap: the rest were dropped)
+    // prune statistics
+    i64 swept_vertices = 0, swept_entries = 0;
+    int full_sweeps = 0, pruned_sweeps = 0;
+};
+
 struct RunParams {
     int coloring = 1;
     int replaceMap = 0;  // 1: NoMap plain sweeps (initCommAssOpt start)
@@ -41,6 +64,7 @@ struct RunParams {
     double C_threshold = 1e-6;
     int max_phases = 200;       // runMultiPhaseColoring.cpp:147 (phase > 200)
     int max_total_iters = 10000;  // :147 (totItr > 10000); 100000 for runMultiPhaseBasic
+    LasParams *las = nullptr;     // learned active set of the colored phases (null: off)
 };
 
 // Per-phase device state shared by the sweeps: vertex degrees, community
@@ -81,7 +105,8 @@ public:
 
     // algoLouvainWithDistOneColoring: C (device, nv) receives the labels
     double colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, double Lower,
-                         double thresh, int *iters, std::vector<double> *trace = nullptr);
+                         double thresh, int *iters, std::vector<double> *trace = nullptr,
+                         LasParams *las = nullptr, int phase_index = 0);
     // parallelLouvianMethod(NoMap): C receives pastCommAss
     double plain_phase(const DevGraph &g, i64 *C, double Lower, double thresh, bool nomap,
                        int *iters, std::vector<double> *trace = nullptr);
@@ -104,7 +129,7 @@ private:
     template <class T>
     T fetch(const T *dptr);
     double colored_modularity(const DevGraph &g, const PhaseSetup &P, const i64 *C,
-                              double *part, double *res);
+                              double *part, double *res, double *exx = nullptr);
 };
 
 }  // namespace glv

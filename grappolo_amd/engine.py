@@ -85,6 +85,18 @@ _SIG = {
     "glv_profile_enable": [ctypes.c_int],
     "glv_profile_reset": [],
     "glv_profile_read": [ctypes.c_int, _F64P, _F64P, _I64P],
+    "glv_mlp_param_count": [],
+    "glv_mlp_forward_dev": [_VP, _VP, ctypes.c_int64, _VP, _VP],
+    "glv_mlp_grad_dev": [_VP, _VP, _VP, ctypes.c_int64, ctypes.c_float, _VP, _VP],
+    "glv_mlp_train_dev": [_VP, _VP, _VP, _VP, _VP, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                          ctypes.c_float, ctypes.c_float, ctypes.c_uint64, ctypes.c_int64, _VP, _VP],
+    "glv_las_trace_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, ctypes.c_int64,
+                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _VP, _VP,
+                          _VP, ctypes.c_int64, _VP, ctypes.c_void_p, ctypes.c_void_p],
+    "glv_runMultiPhaseColoring_las_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP,
+                                          ctypes.c_int64, ctypes.c_double, ctypes.c_double, _VP,
+                                          ctypes.c_float, ctypes.c_int, _VP, ctypes.c_void_p,
+                                          ctypes.c_void_p],
 }
 
 KERNEL_CLASSES = ["sweep_bin1_deg4", "sweep_bin2_deg8", "sweep_bin3_deg16", "sweep_bin4_deg32",

@@ -192,6 +192,61 @@ int glv_profile_enable(int on);
 int glv_profile_reset(void);
 int glv_profile_read(int kernel_class, double *ms, double *bytes, int64_t *launches);
 
+/* ---- learned active set (no counterpart in the reference) ----------------
+ * An 8 -> 64 -> 64 -> 1 MLP (ReLU, logistic output; BF16 weights and
+ * activations on the matrix cores, FP32 accumulation) predicts which vertices
+ * the next colored sweep will move.  Parameters: glv_mlp_param_count() FP32
+ * values in device memory, laid out W1[8][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3.  Samples: 8 BF16 features per row, labels 0/1 (uint8). */
+int glv_mlp_param_count(void);
+/* logits[i] of the n feature rows */
+int glv_mlp_forward_dev(const float *d_params, const uint16_t *d_feats, int64_t n, float *d_logits,
+                        void *stream);
+/* gradient of the mean weighted logistic loss over all n rows (label-1 rows
+ * weighted pos_weight) into d_grad[0..param_count), the loss into
+ * d_grad[param_count]; deterministic */
+int glv_mlp_grad_dev(const float *d_params, const uint16_t *d_feats, const uint8_t *d_labels,
+                     int64_t n, float pos_weight, float *d_grad, void *stream);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) from step number step0 on
+ * batches of `batch` rows drawn with replacement (splitmix64 of seed);
+ * d_m, d_v: Adam moments (zero before the first step); d_losses[steps]
+ * (nullable) receives the loss of every step */
+int glv_mlp_train_dev(float *d_params, float *d_m, float *d_v, const uint16_t *d_feats,
+                      const uint8_t *d_labels, int64_t n, int64_t steps, int64_t batch, float lr,
+                      float pos_weight, uint64_t seed, int64_t step0, float *d_losses, void *stream);
+
+typedef struct glv_las_stats {
+    int64_t swept_vertices;  /* vertex visits of every sweep */
+    int64_t swept_entries;   /* CSR entries those visits read */
+    int32_t full_sweeps;     /* iterations over every vertex */
+    int32_t pruned_sweeps;   /* iterations over the active vertices only */
+    int64_t samples;         /* trace rows recorded (may exceed the capacity) */
+} glv_las_stats;
+
+/* The unpruned colored run (same partition as glv_runMultiPhaseColoring_dev)
+ * that also records sweep traces: from the second iteration of every colored
+ * phase, a `rate` fraction of the vertices (splitmix64 of seed) gets its
+ * features (as recorded by the previous sweep) written to d_feats[row][8] and
+ * the vertex id to d_verts[row]; d_labels[row] = 1 when the iteration then
+ * moves it.  At most cap rows. */
+int glv_las_trace_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                      const double *d_weights, int64_t *d_C_orig, int64_t minGraphSize,
+                      double threshold, double C_threshold, double rate, uint64_t seed,
+                      uint16_t *d_feats, int64_t *d_verts, uint8_t *d_labels, int64_t cap,
+                      void *stream, glv_stats *stats, glv_las_stats *las);
+/* runMultiPhaseColoring with learned pruning: after the first iteration of a
+ * colored phase only vertices whose predicted move probability is >= tau are
+ * swept.  When a pruned iteration gains less than C_threshold, the next
+ * iteration sweeps every vertex; the phase ends when such a full iteration
+ * gains less than C_threshold, or after max_rechecks of them (0: no cap).
+ * stats->edges_visited counts the entries actually read. */
+int glv_runMultiPhaseColoring_las_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
+                                      const int64_t *d_tails, const double *d_weights,
+                                      int64_t *d_C_orig, int64_t minGraphSize, double threshold,
+                                      double C_threshold, const float *d_params, float tau,
+                                      int max_rechecks, void *stream, glv_stats *stats,
+                                      glv_las_stats *las);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,31 @@ def with_loops_and_isolated(seed=11, nv=3000):
     return Graph.from_pairs(nv, pairs, w)  # vertices nv-100.. have no edges
 
 
+def cliques(seed=13, nv=20000, size=200, inter=8, hub_degrees=(5000, 4500), weighted=False):
+    """Dense near-cliques (every coloring needs ~size colors, so classes hold
+    ~nv/size vertices) joined by random inter-clique edges, plus hubs: a graph
+    above the dense-accumulator limit that the persistent sweep serves with its
+    LDS hash (and, past kPHashMaxDeg, the HBM table)."""
+    rng = np.random.default_rng(seed)
+    blocks = nv // size
+    pairs = []
+    for b in range(blocks):
+        base = b * size
+        iu = np.triu_indices(size, 1)
+        keep = rng.random(len(iu[0])) < 0.97
+        pairs.append(np.stack([iu[0][keep] + base, iu[1][keep] + base], 1))
+    m = nv * inter // 2
+    u, v = rng.integers(0, nv, m), rng.integers(0, nv, m)
+    pairs.append(np.stack([u, v], 1)[u != v])
+    for h, d in enumerate(hub_degrees):
+        others = rng.choice(np.arange(nv), size=d, replace=False)
+        others = others[others != h]
+        pairs.append(np.stack([np.full(len(others), h), others], 1))
+    p = np.unique(np.sort(np.concatenate(pairs), 1), axis=0)
+    w = rng.uniform(0.5, 2.0, len(p)) if weighted else None
+    return Graph.from_pairs(nv, p, w)
+
+
 def catalog():
     return {
         "karate": karate(),

@@ -1,0 +1,268 @@
+"""Learned active-set predictor: the MLP oracle pinned by calculus, the GPU
+kernels against the oracle, and the pruned Louvain run against the unpruned
+one (the fallback and judge).  The MLP has no counterpart in the reference."""
+import numpy as np
+import pytest
+
+import oracle.mlp as om
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-1, 1, (n, 8))
+    X[:, 0] = rng.uniform(0, 1, n)
+    y = (X[:, 2] + 0.3 * X[:, 4] > 0.1).astype(np.float64)
+    return X, y
+
+
+# ----------------------------------------------------------------- CPU pins
+def test_oracle_gradient_matches_finite_differences():
+    P = om.init(3).astype(np.float64)
+    P[om.B1:om.W2] = 0.05
+    P[om.B2:om.W3] = 0.05
+    X, y = _batch(37, 4)
+    _, g = om.loss_grad(P, X, y, pos_weight=3.0, rounding=False)
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([rng.choice(np.arange(om.W1, om.B1), 12, replace=False),
+                          rng.choice(np.arange(om.B1, om.W2), 6, replace=False),
+                          rng.choice(np.arange(om.W2, om.B2), 24, replace=False),
+                          rng.choice(np.arange(om.B2, om.W3), 6, replace=False),
+                          rng.choice(np.arange(om.W3, om.B3), 8, replace=False), [om.B3]])
+    h = 1e-6
+    for i in idx:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += h
+        Pm[i] -= h
+        lp, _ = om.loss_grad(Pp, X, y, 3.0, rounding=False)
+        lm, _ = om.loss_grad(Pm, X, y, 3.0, rounding=False)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_oracle_loss_closed_form():
+    # all-zero network: z = b3 for every row; loss = mean c (softplus(b3) - y b3)
+    P = np.zeros(om.NPARAMS)
+    P[om.B3] = 0.7
+    X, y = _batch(50, 1)
+    loss, g = om.loss_grad(P, X, y, pos_weight=2.0, rounding=False)
+    c = np.where(y > 0.5, 2.0, 1.0) / len(y)
+    assert loss == pytest.approx(np.sum(c * (np.log1p(np.exp(0.7)) - y * 0.7)), rel=1e-12)
+    s = 1 / (1 + np.exp(-0.7))
+    assert g[om.B3] == pytest.approx(np.sum(c * (s - y)), rel=1e-12)
+    assert np.all(g[:om.B3] == 0)
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(0, 1, 5000).astype(np.float32),
+                        # exact ties between two bf16 values round to even
+                        (np.arange(1, 200, dtype=np.uint32) * 0x10000 + 0x8000).view(np.float32),
+                        np.array([0.0, -0.0, 1e-30, -3.5e38], np.float32)])
+    ours = om.bf16(x)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(ours, ref)
+    bits = om.float_to_bf16_bits(x)
+    np.testing.assert_array_equal(om.bf16_bits_to_float(bits), ref)
+
+
+def test_adam_oracle_first_step_is_signed_lr():
+    P = np.zeros(4)
+    g = np.array([1e-3, -2.0, 0.5, 0.0])
+    P1, m, v = om.adam_step(P, np.zeros(4), np.zeros(4), g, 1, lr=0.01, eps=0.0)
+    np.testing.assert_allclose(P1[:3], -0.01 * np.sign(g[:3]), rtol=1e-12)
+    assert P1[3] == 0 or np.isnan(P1[3])
+
+
+# ----------------------------------------------------------------- GPU tests
+def _dev_batch(n, seed):
+    import torch
+    X, y = _batch(n, seed)
+    bits = om.float_to_bf16_bits(X).astype(np.int16)
+    return (torch.from_numpy(bits).cuda(), torch.from_numpy(y.astype(np.uint8)).cuda(),
+            om.bf16_bits_to_float(bits), y)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_oracle():
+    import torch
+    from grappolo_amd import active_set as las
+    P = om.init(7)
+    P[om.B1:om.W2] = np.linspace(-0.1, 0.1, 64)
+    P[om.B3] = 0.2
+    feats, _, X, _ = _dev_batch(1000, 2)
+    z = las.forward_dev(torch.from_numpy(P).cuda(), feats).cpu().numpy().astype(np.float64)
+    zr, _ = om.forward(P, X, rounding=True)
+    assert np.max(np.abs(z - zr) / (1 + np.abs(zr))) < 2e-3
+    # almost every row agrees to FP32 accumulation error
+    assert np.mean(np.abs(z - zr) <= 1e-4 * (1 + np.abs(zr))) > 0.95
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [16, 1000, 9000])
+def test_gpu_gradient_matches_oracle(n):
+    import torch
+    from grappolo_amd import active_set as las
+    P = om.init(11)
+    P[om.B1:om.W2] = 0.02
+    P[om.B2:om.W3] = -0.01
+    feats, labels, X, y = _dev_batch(n, 3)
+    loss, g = las.grad_dev(torch.from_numpy(P).cuda(), feats, labels, pos_weight=4.0)
+    g = g.cpu().numpy().astype(np.float64)
+    lr, gr = om.loss_grad(P, X, y, 4.0, rounding=True)
+    assert loss == pytest.approx(lr, rel=1e-4)
+    for lo, hi in [(om.W1, om.B1), (om.B1, om.W2), (om.W2, om.B2), (om.B2, om.W3), (om.W3, om.B3),
+                   (om.B3, om.NPARAMS)]:
+        err = np.linalg.norm(g[lo:hi] - gr[lo:hi]) / max(np.linalg.norm(gr[lo:hi]), 1e-12)
+        assert err < 5e-3, (lo, hi, err)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_is_deterministic():
+    import torch
+    from grappolo_amd import active_set as las
+    P = torch.from_numpy(om.init(1)).cuda()
+    feats, labels, _, _ = _dev_batch(5000, 9)
+    _, g1 = las.grad_dev(P, feats, labels, 2.0)
+    _, g2 = las.grad_dev(P, feats, labels, 2.0)
+    assert torch.equal(g1, g2)
+
+
+def _splitmix(z):
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+@pytest.mark.gpu
+def test_gpu_adam_step_matches_oracle():
+    """Two training steps on drawn batches: the rows drawn (splitmix64 of
+    seed + step*batch + j, mod n), the loss of each step and the Adam update
+    follow the oracle."""
+    import torch
+    from grappolo_amd import active_set as las
+    P0 = om.init(5)
+    n, batch, seed, lr, pw = 3000, 1024, 77, 1e-2, 2.5
+    feats, labels, X, y = _dev_batch(n, 6)
+    P = torch.from_numpy(P0.copy()).cuda()
+    m, v = torch.zeros_like(P), torch.zeros_like(P)
+    losses = las.train_dev(P, m, v, feats, labels, steps=2, batch=batch, lr=lr, pos_weight=pw,
+                           seed=seed).cpu().numpy()
+    Po, mo, vo = P0.astype(np.float64), np.zeros(om.NPARAMS), np.zeros(om.NPARAMS)
+    for step in range(2):
+        j = np.arange(batch, dtype=np.uint64) + np.uint64(step * batch)
+        idx = (_splitmix(np.uint64(seed) + j) % np.uint64(n)).astype(np.int64)
+        lo, go = om.loss_grad(Po, X[idx], y[idx], pw, rounding=True)
+        assert losses[step] == pytest.approx(lo, rel=1e-4)
+        Po, mo, vo = om.adam_step(Po, mo, vo, go, step + 1, lr)
+    d_gpu = P.cpu().numpy().astype(np.float64) - P0
+    d_ora = Po - P0
+    # parameters with a clear gradient sign move identically (Adam's first steps are
+    # sign-like, so this pins direction and step size)
+    clear = np.abs(d_ora) > 0.5 * lr
+    assert clear.mean() > 0.5
+    np.testing.assert_allclose(d_gpu[clear], d_ora[clear], rtol=2e-2, atol=1e-4)
+
+
+@pytest.mark.gpu
+def test_gpu_training_learns_a_rule():
+    import torch
+    from grappolo_amd import active_set as las
+    feats, labels, _, y = _dev_batch(50000, 12)
+    pred = las.ActiveSetPredictor(seed=3)
+    losses = pred.fit(feats, labels, steps=600, batch=2048, lr=3e-3, pos_weight=1.0).cpu().numpy()
+    assert losses[-50:].mean() < 0.5 * losses[:20].mean()
+    z = las.forward_dev(pred.P, feats).cpu().numpy()
+    acc = np.mean((z > 0) == (y > 0.5))
+    assert acc > 0.95
+
+
+def _lfr(nv, seed):
+    from grappolo_amd.graph import gen_lfr
+    return gen_lfr(nv, seed=seed)
+
+
+@pytest.mark.gpu
+def test_trace_run_equals_plain_run():
+    import torch
+    from grappolo_amd import active_set as las
+    from grappolo_amd import engine
+    g = _lfr(30000, 21)
+    dg = engine.DeviceGraph(g, "cuda")
+    C0 = torch.empty(g.numVertices, dtype=torch.int64, device="cuda")
+    C1 = torch.empty_like(C0)
+    st0 = engine.run_colored_dev(dg, C0)
+    feats, labels, verts, st1, ls = las.trace_dev(dg, C1, rate=0.25, seed=5, cap=1 << 20)
+    assert torch.equal(C0, C1)
+    assert st0["final_modularity"] == st1["final_modularity"]
+    assert feats.shape[0] == ls["samples"] > 1000
+    y = labels.cpu().numpy()
+    assert set(np.unique(y)) <= {0, 1} and 0 < y.mean() < 0.9
+    X = om.bf16_bits_to_float(feats.cpu().numpy())
+    assert np.all(X[:, 0] >= 0) and np.all(X[:, 1] >= 0) and np.all(X[:, 1] <= 1 + 1e-2)
+    assert np.all(np.abs(X[:, 2:4]) <= 1.0 + 1e-6) and np.all(X[:, 2] >= X[:, 3])
+    assert np.all((X[:, 4] >= 0) & (X[:, 4] <= 1)) and np.all((X[:, 5] >= 0) & (X[:, 5] < 1))
+    v = verts.cpu().numpy()
+    assert v.min() >= 0
+
+
+@pytest.mark.gpu
+def test_pruned_run_everything_active_matches_plain_quality():
+    import torch
+    from grappolo_amd import active_set as las
+    from grappolo_amd import engine
+    g = _lfr(40000, 22)
+    dg = engine.DeviceGraph(g, "cuda")
+    C0 = torch.empty(g.numVertices, dtype=torch.int64, device="cuda")
+    C1 = torch.empty_like(C0)
+    st0 = engine.run_colored_dev(dg, C0)
+    P = las.init_params(0)
+    st1, ls = las.run_colored_las_dev(dg, C1, P, tau=0.0)
+    assert abs(st1["final_modularity"] - st0["final_modularity"]) <= 1e-3
+    assert ls["pruned_sweeps"] > 0 and ls["full_sweeps"] > 0
+    # with every vertex active the first phase repeats the unpruned sweeps; its
+    # modularity after a pruned sweep comes from the movers' internal-weight
+    # deltas instead of a pass over all entries, and must agree
+    n1 = int(st0["per_phase"]["phase_iters"][0])
+    q0 = np.asarray(st0["trace"][:n1])
+    q1 = np.asarray(st1["trace"][:n1])
+    assert n1 >= 3 and np.max(np.abs(q0 - q1)) <= 1e-9, (q0, q1)
+    q = engine.modularity_dev(dg, C1)
+    assert abs(q - st1["final_modularity"]) <= 1e-4
+
+
+@pytest.mark.gpu
+def test_trained_predictor_prunes_work_at_equal_modularity():
+    """What the trained network gains: on a held-out graph, fewer CSR entries
+    read than the unpruned method, final modularity within 1e-3."""
+    import torch
+    from grappolo_amd import active_set as las
+    from grappolo_amd import engine
+    fs, ls_ = [], []
+    for seed in (31, 32, 33, 34):
+        g = _lfr(50000, seed)
+        dg = engine.DeviceGraph(g, "cuda")
+        C = torch.empty(g.numVertices, dtype=torch.int64, device="cuda")
+        f, l, _, _, _ = las.trace_dev(dg, C, rate=0.5, seed=seed, cap=1 << 21)
+        fs.append(f)
+        ls_.append(l)
+    feats, labels = torch.cat(fs), torch.cat(ls_)
+    pred = las.ActiveSetPredictor(seed=1)
+    pred.fit(feats, labels, steps=800, batch=4096, lr=2e-3)
+    tau, kept = pred.calibrate(feats, labels, recall=0.98)
+    assert 0 < tau < 1 and kept < 0.8
+    g = _lfr(120000, 40)
+    dg = engine.DeviceGraph(g, "cuda")
+    C0 = torch.empty(g.numVertices, dtype=torch.int64, device="cuda")
+    C1 = torch.empty_like(C0)
+    st0 = engine.run_colored_dev(dg, C0)
+    st1, ls = pred.run(dg, C1, max_rechecks=1)
+    assert abs(st1["final_modularity"] - st0["final_modularity"]) <= 1e-3
+    assert ls["swept_entries"] < 0.8 * st0["edges_visited"], (ls, st0["edges_visited"])
+    # the final plain phase raises its starting value to the previous phase's
+    # result (parallelLouvainMethod.cpp:272-273), so the reported value may
+    # exceed the partition's own modularity by that phase's last change
+    q = engine.modularity_dev(dg, C1)
+    assert abs(q - st1["final_modularity"]) <= 1e-4

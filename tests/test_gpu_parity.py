@@ -157,6 +157,25 @@ def test_plain_phase(name, nomap):
         assert abs(q - qo) < 1e-3
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_colored_phase_persistent_hash(weighted):
+    """A graph above the dense-accumulator limit with small color classes: the
+    persistent sweep with per-workgroup hash tables (LDS, and HBM for the two
+    hubs) against the oracle's frozen color steps."""
+    from graphs import cliques
+    g = cliques(weighted=weighted)
+    colors, mc = oracle.color_greedy(g)
+    assert g.numVertices > 15360 and g.numVertices / (mc + 1) <= 256
+    assert g.degrees().max() > 4096
+    q, C, it = engine.algoLouvainWithDistOneColoring(g, colors, mc + 1, -1.0, 1e-6)
+    qo, Co, ito, _ = oracle.louvain_colored(g, colors, mc + 1, -1.0, 1e-6, frozen=True)
+    if not weighted:
+        assert it == ito and q == qo
+        np.testing.assert_array_equal(C, Co)
+    else:
+        assert abs(q - qo) < 1e-3
+
+
 @pytest.mark.parametrize("name", NAMES)
 def test_pipeline_colored(name):
     g = GRAPHS[name]
