@@ -220,6 +220,94 @@ def run_learned(args):
     print(json.dumps(out))
 
 
+def batch_specs(n, seed=5):
+    """Graph parameters of the batched workload: vertices log-uniform on
+    [100, 50000], blocks log-uniform on [4, 256] (at most the vertex count),
+    mean degree uniform on [8, 32], mixing uniform on [0.1, 0.5], graph g
+    generated with seed 1000 + g; the draws come from numpy's PCG64 (seed 5)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    nv = np.exp(rng.uniform(np.log(100), np.log(50000), n)).astype(np.int64)
+    blocks = np.exp(rng.uniform(np.log(4), np.log(256), n)).astype(np.int64)
+    deg = rng.uniform(8, 32, n)
+    mix = rng.uniform(0.1, 0.5, n)
+    return [(int(nv[g]), int(min(blocks[g], nv[g])), float(deg[g]), float(mix[g]), 1000 + g)
+            for g in range(n)]
+
+
+def run_batched(args):
+    """Batched runMultiPhaseBasic over thousands of independent planted-
+    partition graphs (one union CSR, every graph's run exact) against one
+    launch sequence per graph."""
+    import numpy as np
+    import torch
+    from grappolo_amd import engine
+    from grappolo_amd.graph import gen_planted, union
+
+    torch.cuda.set_device(0)
+    t = time.perf_counter()
+    specs = batch_specs(args.batch_graphs)
+    graphs = [gen_planted(nv, b, d, m, seed=sd) for nv, b, d, m, sd in specs]
+    U, vb = union(graphs)
+    t_gen = time.perf_counter() - t
+    print(f"[bench] batch of {len(graphs)} graphs: {U} generated in {t_gen:.1f}s", file=sys.stderr,
+          flush=True)
+    dg = engine.DeviceGraph(U, "cuda:0")
+    C = torch.empty(U.numVertices, dtype=torch.int64, device="cuda:0")
+    default_bins = engine.get_bins()
+    bins, t_def, t_tuned = engine.autotune_bins_dev(dg, reps=3)
+    engine.set_bins(bins)
+    print(f"[bench] bins {bins[1:7].tolist()}: sweep {t_def:.2f} -> {t_tuned:.2f} ms", file=sys.stderr,
+          flush=True)
+    for _ in range(args.warmup):
+        engine.run_basic_batch_dev(dg, vb, C)
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = engine.run_basic_batch_dev(dg, vb, C)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    t_batch = sum(times) / len(times)
+    Cb = C.cpu().numpy()
+    # one launch sequence per graph, on an evenly spaced sample
+    sample = list(range(0, len(graphs), max(1, len(graphs) // args.batch_sample)))
+    t_single, agree, worst = 0.0, 0, 0.0
+    for g in sample:
+        dgi = engine.DeviceGraph(graphs[g], "cuda:0")
+        Ci = torch.empty(graphs[g].numVertices, dtype=torch.int64, device="cuda:0")
+        engine.run_basic_dev(dgi, Ci)  # warm
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = engine.run_basic_dev(dgi, Ci)
+        torch.cuda.synchronize()
+        t_single += time.perf_counter() - t
+        same = np.array_equal(Ci.cpu().numpy(), Cb[vb[g]:vb[g + 1]])
+        agree += int(same and st["final_modularity"] == res["final_modularity"][g])
+        worst = max(worst, abs(st["final_modularity"] - res["final_modularity"][g]))
+    per_graph = t_single / len(sample)
+    out = {
+        "metric": "batched Louvain graphs/s vs per-graph launches",
+        "value": len(graphs) / t_batch, "unit": "graphs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(graphs)} planted-partition graphs, vertices log-uniform "
+                               "[100, 50000], blocks log-uniform [4, 256], mean degree U[8, 32], "
+                               "mixing U[0.1, 0.5], unit weights, seeds 1000+g",
+                   "vertices": U.numVertices, "nnz": U.nnz, "method": "runMultiPhaseBasic",
+                   "threshold": 1e-9},
+        "batch_s": t_batch, "levels": res["levels"], "batched_iterations": res["iterations"],
+        "per_graph_launches": {"graphs_per_s": 1.0 / per_graph, "sample": len(sample),
+                               "mean_ms": per_graph * 1e3},
+        "speedup": (len(graphs) / t_batch) * per_graph,
+        "sample_identical": f"{agree}/{len(sample)}", "sample_max_dQ": worst,
+        "autotune": {"bins": bins.tolist(), "default_bins": default_bins.tolist(),
+                     "sweep_ms_default": t_def, "sweep_ms_tuned": t_tuned},
+        "mean_final_modularity": float(np.mean(res["final_modularity"])),
+        "generate_s": t_gen,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +325,11 @@ def main():
     ap.add_argument("--las-rate", type=float, default=0.125)
     ap.add_argument("--las-steps", type=int, default=3000)
     ap.add_argument("--las-recall", type=float, default=0.98)
+    ap.add_argument("--batched", action="store_true",
+                    help="batched Louvain over many planted-partition graphs vs per-graph launches")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--batch-sample", type=int, default=128,
+                    help="graphs timed with per-graph launches")
     ap.add_argument("--las-rechecks", default="1",
                     help="comma list of per-phase re-check caps to time (first one is reported)")
     args = ap.parse_args()
@@ -252,6 +345,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.learned:
         run_learned(args)
+        return
+    if args.batched:
+        run_batched(args)
         return
     torch.cuda.set_device(0)
     spec = WORKLOADS[args.workload]

@@ -53,7 +53,9 @@ __global__ __launch_bounds__(kLasBlock) void k_las_infer(LasView S, const float 
         MlpAct A;
         mlp_forward_tile(W, vertex_x(S, v, rec), A);
         if (inb && lane < 16) {
-            const bool act = has && (!rec || A.z >= logit_tau);
+            // giant vertices always sweep: their HBM tables are laid out for the full class
+            const bool giant = has && S.off[v + 1] - S.off[v] > kGiantDeg;
+            const bool act = has && (!rec || giant || A.z >= logit_tau);
             active[v] = act ? 1 : 0;
             if (act) {
                 na++;
@@ -191,6 +193,12 @@ void launch_compact_active(const i64 *list, i64 n, const uint8_t *active, const 
     if (n <= 0) return;
     k_flag_active<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(list, n, active, flags);
     GLV_LAUNCH_CHECK();
+    compact_flagged(list, n, flags, pos, start, nkeys, out, out_start, s);
+}
+
+void compact_flagged(const i64 *list, i64 n, i64 *flags, i64 *pos, const i64 *start, i64 nkeys, i64 *out,
+                     i64 *out_start, hipStream_t s) {
+    if (n <= 0) return;
     scan_exclusive_i64(flags, pos, n, s);
     k_compact_list<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(list, n, flags, pos, out);
     GLV_LAUNCH_CHECK();

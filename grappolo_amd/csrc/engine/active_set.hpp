@@ -32,5 +32,8 @@ void launch_las_features(const LasView &S, const i64 *verts, i64 n, uint16_t *fe
 // starts of the kept entries in out_start[0..nkeys].  flags, pos: n scratch.
 void launch_compact_active(const i64 *list, i64 n, const uint8_t *active, const i64 *start, i64 nkeys,
                            i64 *flags, i64 *pos, i64 *out, i64 *out_start, hipStream_t s);
+// the same from flags[0..n) already set (0/1); pos: n scratch
+void compact_flagged(const i64 *list, i64 n, i64 *flags, i64 *pos, const i64 *start, i64 nkeys, i64 *out,
+                     i64 *out_start, hipStream_t s);
 
 }  // namespace glv

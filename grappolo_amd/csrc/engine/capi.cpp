@@ -12,6 +12,7 @@
 #include "grappolo/basic_util.h"
 #include "grappolo/color_comm.h"
 #include "grappolo/coloring.h"
+#include "batch.hpp"
 #include "grappolo_hip.h"
 #include "kernels.hpp"
 #include "louvain.hpp"
@@ -181,6 +182,21 @@ void fill_stats(const glv::RunResult &r, glv_stats *st) {
     st->n_trace = (int32_t)r.trace.size();
     for (int i = 0; i < (int)r.trace.size() && i < st->trace_cap; i++)
         if (st->trace_q) st->trace_q[i] = r.trace[i];
+}
+
+// No entry of the union may join two graphs of the batch.
+void check_batch_blocks(const DevGraph &g, const std::vector<i64> &vb, hipStream_t s) {
+    const i64 ng = (i64)vb.size() - 1;
+    if (ng <= 0 || g.nnz == 0) return;
+    DBuf<i64> d_vb(ng + 1, s);
+    DBuf<int32_t> err(1, s);
+    GLV_HIP(hipMemcpyAsync(d_vb.p, vb.data(), (ng + 1) * sizeof(i64), hipMemcpyHostToDevice, s));
+    glv::launch_fill_i32(err.p, 1, 0, s);
+    glv::launch_check_blocks(g.off, g.tails, d_vb.p, ng, err.p, s);
+    int32_t e = 0;
+    GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s));
+    GLV_HIP(hipStreamSynchronize(s));
+    if (e) glv::fail(GLV_E_INVALID_GRAPH, "an entry joins two graphs of the batch");
 }
 
 void fill_las(const glv::LasParams &L, glv_las_stats *st) {
@@ -748,6 +764,75 @@ int glv_runMultiPhaseColoring_las_dev(int64_t nv, int64_t nnz, const int64_t *d_
         run_pipeline(true, g, d_C_orig, p, s, r);
         fill_stats(r, stats);
         fill_las(L, las);
+    });
+}
+
+int glv_runMultiPhaseBasic_batch_dev(int64_t ngraphs, const int64_t *vbase, int64_t nv, int64_t nnz,
+                                     const int64_t *d_offsets, const int64_t *d_tails, const double *d_weights,
+                                     int64_t *d_C_orig, int basicOpt, double threshold, void *stream,
+                                     glv_batch_stats *stats) {
+    return guarded([&] {
+        if (ngraphs < 0 || (ngraphs > 0 && !vbase)) glv::fail(GLV_E_INVALID_ARG, "bad graph table");
+        if (ngraphs > 0 && (vbase[0] != 0 || vbase[ngraphs] != nv))
+            glv::fail(GLV_E_INVALID_ARG, "vbase must start at 0 and end at nv");
+        for (int64_t g = 0; g < ngraphs; g++)
+            if (vbase[g + 1] < vbase[g]) glv::fail(GLV_E_INVALID_ARG, "vbase not monotone");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        std::vector<i64> vb(vbase, vbase + ngraphs + 1);
+        if (ngraphs == 0) vb.assign(1, 0);
+        check_batch_blocks(g, vb, s);
+        glv::BatchParams p;
+        p.basicOpt = basicOpt;
+        p.threshold = threshold;
+        glv::BatchResult r;
+        Engine E(s);
+        glv::run_basic_batch(E, g, vb, d_C_orig, p, r);
+        if (stats) {
+            for (int64_t i = 0; i < ngraphs; i++) {
+                if (stats->final_modularity) stats->final_modularity[i] = r.final_modularity[i];
+                if (stats->last_modularity) stats->last_modularity[i] = r.last_modularity[i];
+                if (stats->phases) stats->phases[i] = r.phases[i];
+                if (stats->total_iterations) stats->total_iterations[i] = r.total_iterations[i];
+                if (stats->final_clusters) stats->final_clusters[i] = r.clusters[i];
+            }
+            stats->levels = r.levels;
+            stats->iterations = r.iterations;
+            stats->time_total = r.t_total;
+        }
+    });
+}
+
+int glv_get_bins(int64_t *max_deg) {
+    return guarded([&] {
+        if (!max_deg) glv::fail(GLV_E_INVALID_ARG, "null array");
+        for (int b = 0; b < glv::kNumBins; b++) max_deg[b] = glv::current_bins().max_deg[b];
+    });
+}
+
+int glv_set_bins(const int64_t *max_deg) {
+    return guarded([&] {
+        if (!max_deg) glv::fail(GLV_E_INVALID_ARG, "null array");
+        glv::BinConfig c;
+        for (int b = 0; b < glv::kNumBins; b++) c.max_deg[b] = max_deg[b];
+        if (!glv::valid_bins(c)) glv::fail(GLV_E_INVALID_ARG, "bin ranges outside the kernels' capacities");
+        glv::current_bins() = c;
+    });
+}
+
+int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                          const double *d_weights, int reps, int64_t *max_deg, double *ms_default,
+                          double *ms_tuned, void *stream) {
+    return guarded([&] {
+        if (!max_deg || reps < 1) glv::fail(GLV_E_INVALID_ARG, "null array or reps < 1");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        if (nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        Engine E(s);
+        const glv::BinConfig c = glv::autotune_bins(E, g, reps, ms_default, ms_tuned);
+        for (int b = 0; b < glv::kNumBins; b++) max_deg[b] = c.max_deg[b];
     });
 }
 

@@ -23,10 +23,14 @@ constexpr i64 kEmpty = -1;        // empty hash slot key
 //  1..5: deg <= 4/8/16/32/64 -> sub-wave groups of G lanes, one vertex each
 //  6: deg <= 256      -> one wavefront per vertex, LDS hash of 512 slots
 //  7: deg <= 2048     -> one workgroup per vertex, LDS hash of 4096 slots
-//  8: larger          -> one workgroup per vertex, hash table in HBM
-constexpr int kNumBins = 9;
+//  8: deg <= kGiantDeg -> one workgroup per vertex, hash table in HBM
+//  9: larger (giant)  -> kGiantBlocks workgroups per vertex, chunks pre-summed
+//                        in LDS, one HBM hash table per vertex of the class
+constexpr int kNumBins = 10;
+constexpr i64 kGiantDeg = 16384;
+constexpr int kGiantBlocks = 64;
 constexpr int kBinGroup[6] = {0, 4, 8, 16, 32, 64};
-constexpr i64 kBinMaxDeg[kNumBins] = {0, 4, 8, 16, 32, 64, 256, 2048, INT64_MAX};
+constexpr i64 kBinMaxDeg[kNumBins] = {0, 4, 8, 16, 32, 64, 256, 2048, kGiantDeg, INT64_MAX};
 constexpr int kWaveHashSlots = 512;
 constexpr int kBlockHashSlots = 4096;
 // Graphs of at most this many vertices sweep their high-degree bins with a
@@ -39,11 +43,39 @@ constexpr i64 kPersistMaxBlocks = 64;  // cooperative grid of the persistent swe
 // The persistent sweep serves graphs of at most this many vertices whose
 // color classes hold at most kPersistMaxPerColor vertices on average (coarse
 // phases: many small classes, where per-color launches would be latency-bound).
-constexpr i64 kPersistMaxNV = 1 << 20;
+constexpr i64 kPersistMaxNV = 1 << 18;
 constexpr i64 kPersistMaxPerColor = 256;
 constexpr int kBarrierWords = 16 + 16 * 64;  // abort word + one 64-byte flag line per workgroup
 // touched communities of one vertex, listed in LDS: at most min(degree, nv)
 constexpr int kDenseListMax = kDenseMaxNV;
+
+// Degree ranges of the sweep bins as used at run time: bin b takes degrees
+// in (max_deg[b-1], max_deg[b]].  The upper ends of bins 1..6 may be lowered
+// (a bin may become empty) by the autotuner, within the capacity of their
+// kernels (kBinMaxDeg); bins 7..9 end where kBinMaxDeg says.
+struct BinConfig {
+    i64 max_deg[kNumBins];
+};
+inline BinConfig default_bins() {
+    BinConfig c;
+    for (int b = 0; b < kNumBins; b++) c.max_deg[b] = kBinMaxDeg[b];
+    return c;
+}
+// true when c is a legal configuration
+inline bool valid_bins(const BinConfig &c) {
+    if (c.max_deg[0] != 0) return false;
+    for (int b = 1; b < kNumBins; b++) {
+        if (c.max_deg[b] < c.max_deg[b - 1] || c.max_deg[b] > kBinMaxDeg[b]) return false;
+        if (b >= 7 && c.max_deg[b] != kBinMaxDeg[b]) return false;
+    }
+    return true;
+}
+__host__ __device__ inline int degree_bin_of(i64 d, const BinConfig &c) {
+    if (d == 0) return 0;
+    for (int b = 1; b < kNumBins - 1; b++)
+        if (d <= c.max_deg[b]) return b;
+    return kNumBins - 1;
+}
 
 __host__ __device__ inline int degree_bin(i64 d) {
     if (d == 0) return 0;
@@ -54,7 +86,8 @@ __host__ __device__ inline int degree_bin(i64 d) {
     if (d <= 64) return 5;
     if (d <= 256) return 6;
     if (d <= 2048) return 7;
-    return 8;
+    if (d <= kGiantDeg) return 8;
+    return 9;
 }
 
 // Error carried from the HIP layer up to the C ABI.

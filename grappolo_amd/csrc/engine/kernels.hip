@@ -34,6 +34,14 @@ __device__ inline u64 jp_priority(i64 v) {
 }
 
 __device__ inline i64 ld_relaxed(const i64 *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
+// Loads of tables that other lanes fill with atomics (performed in L2): agent
+// scope, so a line cached in the CU's L1 before the atomics cannot be served.
+__device__ inline double ld_agent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline i64 ld_agent(const i64 *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Gain of moving a vertex of degree kv from community s (internal weight
 // eix excluding its self-loop, degree ax excluding the vertex) to community y
@@ -42,6 +50,11 @@ __device__ inline i64 ld_relaxed(const i64 *p) { return __atomic_load_n(p, __ATO
 __device__ inline double gain_of(double eiy, double eix, double kv, double ay, double ax,
                                  double constant) {
     return 2 * (eiy - eix) - 2 * kv * (ay - ax) * constant;
+}
+
+// 1/(2m) of the graph vertex v belongs to (batched graphs carry one per graph)
+__device__ inline double vconst(const SweepArgs &A, i64 v) {
+    return A.gid ? A.gconst[A.gid[v]] : A.constant;
 }
 
 // "better" of two candidates: larger gain, then smaller community id.
@@ -206,9 +219,9 @@ __global__ void k_init_labels(const i64 *off, i64 nv, i64 *C) {
 }
 
 __global__ void k_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
-                           i64 *vid, int32_t *err) {
+                           i64 *vid, int32_t *err, BinConfig bins) {
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
-        const int b = degree_bin(off[i + 1] - off[i]);
+        const int b = degree_bin_of(off[i + 1] - off[i], bins);
         const int c = colors ? colors[i] : 0;
         vid[i] = i;
         if (c < 0 || c >= ncol) {
@@ -351,9 +364,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
     bool cand = lane < deg && first && c != s;
     double gain = 0;
     float t1 = -INFINITY, t2 = -INFINITY;
+    const double cst = valid ? vconst(A, v) : A.constant;
     if (cand) {
         const double kv = A.k[v];
-        gain = gain_of(e_mine, e_s - sl, kv, A.a[c], A.a[s] - kv, A.constant);
+        gain = gain_of(e_mine, e_s - sl, kv, A.a[c], A.a[s] - kv, cst);
         if (A.feat) t1 = (float)(gain / (2.0 * kv));
         cand = gain > 0;  // max() starts from maxGain = 0 and needs a strict increase
     }
@@ -370,8 +384,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
             const double kv = A.k[v];
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * A.constant : 0.0;
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * cst : 0.0;
         }
+        if (A.eii_v) A.eii_v[v] = e_s;
     }
     if (PLAIN) {
         if (lane == 0) s_part[grp] = valid ? e_s : 0.0;
@@ -405,8 +420,8 @@ __device__ inline void hash_insert(i64 *keys, double *vals, u64 mask, int shift,
 __device__ inline double hash_lookup(const i64 *keys, const double *vals, u64 mask, int shift, i64 c) {
     u64 h = ((u64)c * kFib) >> shift;
     while (true) {
-        i64 k = keys[h];
-        if (k == c) return vals[h];
+        i64 k = ld_agent(&keys[h]);
+        if (k == c) return ld_agent(&vals[h]);
         if (k == kEmpty) return 0.0;
         h = (h + 1) & mask;
     }
@@ -414,13 +429,13 @@ __device__ inline double hash_lookup(const i64 *keys, const double *vals, u64 ma
 
 // Score every occupied slot of a table and reduce over the calling group.
 __device__ inline void score_slots(const i64 *keys, const double *vals, i64 slots, int first, int step,
-                                   const SweepArgs &A, i64 s, double kv, double eix, double ax,
+                                   const SweepArgs &A, i64 s, double kv, double eix, double ax, double cst,
                                    bool &cand, double &gain, i64 &id, float &t1, float &t2) {
     const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
     for (i64 h = first; h < slots; h += step) {
-        i64 y = keys[h];
+        i64 y = ld_agent(&keys[h]);
         if (y == kEmpty || y == s) continue;
-        double g = gain_of(vals[h], eix, kv, A.a[y], ax, A.constant);
+        double g = gain_of(ld_agent(&vals[h]), eix, kv, A.a[y], ax, cst);
         if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
         bool c = g > 0;
         if (better(c, g, y, cand, gain, id)) {
@@ -467,9 +482,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
     double gain = 0;
     i64 id = INT64_MAX;
     float t1 = -INFINITY, t2 = -INFINITY;
+    const double cst = valid ? vconst(A, v) : A.constant;
     if (valid) {
         const double kv = A.k[v];
-        score_slots(keys, vals, SLOTS, lane, kWave, A, s, kv, e_s - sl, A.a[s] - kv, cand, gain, id,
+        score_slots(keys, vals, SLOTS, lane, kWave, A, s, kv, e_s - sl, A.a[s] - kv, cst, cand, gain, id,
                     t1, t2);
     }
     group_argmax<kWave>(cand, gain, id);
@@ -485,8 +501,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
             const double kv = A.k[v];
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * A.constant : 0.0;
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * cst : 0.0;
         }
+        if (A.eii_v) A.eii_v[v] = e_s;
     }
     if (PLAIN) {
         if (lane == 0) s_part[wid] = valid ? e_s : 0.0;
@@ -552,7 +569,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
         double gain = 0;
         i64 id = INT64_MAX;
         float t1 = -INFINITY, t2 = -INFINITY;
-        score_slots(keys, vals, slots, threadIdx.x, kBlock, A, s, kv, e_s - sl, A.a[s] - kv, cand,
+        const double cst = vconst(A, v);
+        score_slots(keys, vals, slots, threadIdx.x, kBlock, A, s, kv, e_s - sl, A.a[s] - kv, cst, cand,
                     gain, id, t1, t2);
         block_argmax(cand, gain, id);
         if (A.feat) block_top2<kBlock>(t1, t2);
@@ -565,7 +583,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
             A.tgt[v] = target;
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * A.constant : 0.0;
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * cst : 0.0;
+            if (A.eii_v) A.eii_v[v] = e_s;
         }
         part += e_s;
         __syncthreads();
@@ -609,6 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
         const double e_s = acc[s];
         const double kv = A.k[v];
         const double eix = e_s - sl, ax = A.a[s] - kv;
+        const double cst = vconst(A, v);
         bool cand = false;
         double gain = 0;
         i64 id = INT64_MAX;
@@ -617,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
         for (int k = threadIdx.x; k < ntouched; k += kBlock) {
             const i64 y = list[k];
             if (y == s) continue;
-            const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
+            const double g = gain_of(acc[y], eix, kv, A.a[y], ax, cst);
             if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
             if (better(g > 0, g, y, cand, gain, id)) {
                 cand = true;
@@ -636,7 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
             A.tgt[v] = target;
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - ax) * A.constant : 0.0;
+                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - ax) * cst : 0.0;
+            if (A.eii_v) A.eii_v[v] = e_s;
         }
         part += e_s;
         for (int k = threadIdx.x; k < ntouched; k += kBlock) acc[list[k]] = 0.0;
@@ -644,6 +665,154 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
         __syncthreads();
     }
     if (PLAIN && threadIdx.x == 0) A.eii_part[blockIdx.x] = part;
+}
+
+// Giant vertices (degree > kGiantDeg), kGiantBlocks workgroups each
+// (blockIdx.y = vertex of the slice).  Accumulate: every workgroup sums
+// chunks of kGiantChunk entries per community in an LDS table, then adds the
+// chunk's sums into the vertex's HBM table; score: the workgroups split the
+// vertex's table, score and clear their slots; decide: one wavefront per
+// vertex merges the workgroups' candidates.
+constexpr int kGiantChunk = 2048;
+constexpr int kGiantLds = 2 * kGiantChunk;
+
+__global__ __launch_bounds__(kBlock) void k_giant_accum(SweepArgs A) {
+    __shared__ i64 lk[kGiantLds];
+    __shared__ double lv[kGiantLds];
+    const i64 q = blockIdx.y;
+    const i64 v = A.verts[q];
+    const i64 beg = A.off[v], end = A.off[v + 1];
+    const i64 s = A.C[v];
+    const i64 tb = A.gt_tab[q], slots = A.gt_tab[q + 1] - tb;
+    i64 *keys = A.gt_keys + tb;
+    double *vals = A.gt_vals + tb;
+    int lg = 0;
+    while (((i64)1 << lg) < slots) lg++;
+    const int gshift = 64 - lg;
+    const u64 gmask = (u64)slots - 1;
+    constexpr int LSHIFT = 64 - 12;  // log2(kGiantLds) = 12
+    double sl = 0, es = 0;
+    for (i64 c0 = beg + (i64)blockIdx.x * kGiantChunk; c0 < end; c0 += (i64)gridDim.x * kGiantChunk) {
+        for (int h = threadIdx.x; h < kGiantLds; h += kBlock) {
+            lk[h] = kEmpty;
+            lv[h] = 0.0;
+        }
+        __syncthreads();
+        const i64 c1 = c0 + kGiantChunk < end ? c0 + kGiantChunk : end;
+        for (i64 j = c0 + threadIdx.x; j < c1; j += kBlock) {
+            const i64 t = A.tails[j];
+            const double ww = A.w[j];
+            const i64 cc = A.C[t];
+            if (t == v) sl += ww;
+            if (cc == s) es += ww;
+            hash_insert(lk, lv, kGiantLds - 1, LSHIFT, cc, ww);
+        }
+        __syncthreads();
+        for (int h = threadIdx.x; h < kGiantLds; h += kBlock) {
+            const i64 k = lk[h];
+            if (k != kEmpty) hash_insert(keys, vals, gmask, gshift, k, lv[h]);
+        }
+        __syncthreads();
+    }
+    sl = block_sum(sl);
+    es = block_sum(es);
+    if (threadIdx.x == 0) {
+        if (es != 0.0) atomicAdd(&A.gt_sc[2 * q], es);
+        if (sl != 0.0) atomicAdd(&A.gt_sc[2 * q + 1], sl);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_giant_score(SweepArgs A) {
+    const i64 q = blockIdx.y;
+    const i64 v = A.verts[q];
+    const i64 s = A.C[v];
+    const i64 tb = A.gt_tab[q], slots = A.gt_tab[q + 1] - tb;
+    i64 *keys = A.gt_keys + tb;
+    double *vals = A.gt_vals + tb;
+    const double kv = A.k[v];
+    const double e_s = ld_agent(&A.gt_sc[2 * q]), sl = ld_agent(&A.gt_sc[2 * q + 1]);
+    const double eix = e_s - sl, ax = A.a[s] - kv;
+    const double cst = vconst(A, v);
+    const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
+    const i64 per = slots / gridDim.x;
+    const i64 h0 = (i64)blockIdx.x * per;
+    bool cand = false;
+    double gain = 0;
+    i64 id = INT64_MAX;
+    float t1 = -INFINITY, t2 = -INFINITY;
+    for (i64 h = h0 + threadIdx.x; h < h0 + per; h += kBlock) {
+        const i64 y = ld_agent(&keys[h]);
+        if (y == kEmpty) continue;
+        const double ey = ld_agent(&vals[h]);
+        keys[h] = kEmpty;
+        vals[h] = 0.0;
+        if (y == s) continue;
+        const double g = gain_of(ey, eix, kv, A.a[y], ax, cst);
+        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+        if (better(g > 0, g, y, cand, gain, id)) {
+            cand = true;
+            gain = g;
+            id = y;
+        }
+    }
+    block_argmax(cand, gain, id);
+    if (A.feat) block_top2<kBlock>(t1, t2);
+    if (threadIdx.x == 0) {
+        const i64 r = q * kGiantBlocks + blockIdx.x;
+        A.gt_rg[r] = cand ? gain : -1.0;
+        A.gt_ri[r] = id;
+        A.gt_rt[2 * r] = t1;
+        A.gt_rt[2 * r + 1] = t2;
+    }
+}
+
+template <bool PLAIN>
+__global__ __launch_bounds__(kWave) void k_giant_decide(SweepArgs A) {
+    const i64 q = blockIdx.x;
+    const int lane = threadIdx.x;
+    const i64 v = A.verts[q];
+    const i64 s = A.C[v];
+    const i64 r = q * kGiantBlocks + lane;
+    bool cand = false;
+    double gain = 0;
+    i64 id = INT64_MAX;
+    float t1 = -INFINITY, t2 = -INFINITY;
+    if (lane < kGiantBlocks) {
+        const double g = A.gt_rg[r];
+        cand = g > 0;
+        gain = cand ? g : 0.0;
+        id = A.gt_ri[r];
+        t1 = A.gt_rt[2 * r];
+        t2 = A.gt_rt[2 * r + 1];
+    }
+    group_argmax<kWave>(cand, gain, id);
+    if (A.feat) top2_group<kWave>(t1, t2);
+    if (lane == 0) {
+        const double e_s = A.gt_sc[2 * q];
+        i64 target = s;
+        if (cand) {
+            target = id;
+            if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
+        }
+        A.tgt[v] = target;
+        const double kv = A.k[v];
+        if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+        if (A.dein) A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * vconst(A, v) : 0.0;
+        if (PLAIN) A.eii_part[q] = e_s;
+        if (A.eii_v) A.eii_v[v] = e_s;
+        A.gt_sc[2 * q] = 0.0;
+        A.gt_sc[2 * q + 1] = 0.0;
+    }
+}
+
+__global__ void k_giant_table_sizes(const i64 *off, const i64 *verts, i64 count, i64 *tab) {
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < count; q += (i64)gridDim.x * blockDim.x) {
+        const i64 v = verts[q];
+        const i64 d = off[v + 1] - off[v];
+        i64 sl = 64;
+        while (sl < 2 * d) sl <<= 1;
+        tab[q] = sl;
+    }
 }
 
 __device__ inline void move_vertex(i64 t, i64 s, double kv, double *a, i64 *n) {
@@ -739,11 +908,13 @@ __device__ bool grid_barrier(unsigned *bar, unsigned nb, unsigned seq) {
 // LDS of the persistent sweep: either the dense accumulator of graphs with at
 // most kDenseMaxNV vertices, or (HASH) an open-addressing table of
 // kPHashSlots (community -> weight) slots for vertices of degree up to
-// kPHashMaxDeg; larger vertices use the workgroup's HBM table (A.gkeys).
-constexpr int kPHashSlots = 8192;
+// kPHashMaxDeg plus a touched-community bitmap of the whole graph for larger
+// vertices, which accumulate into the workgroup's dense HBM row (A.dacc).
+constexpr int kPHashSlots = 4096;
 constexpr i64 kPHashMaxDeg = kPHashSlots / 2;
+constexpr int kPBitmapWords = (int)(kPersistMaxNV / 32);
 constexpr size_t kPDenseBytes = kDenseMaxNV * 8 + kDenseMaxNV / 32 * 4 + kDenseListMax * 2;
-constexpr size_t kPHashBytes = (size_t)kPHashSlots * 16;
+constexpr size_t kPHashBytes = (size_t)kPHashSlots * 16 + (size_t)kPBitmapWords * 4;
 constexpr size_t kPSmemBytes = kPDenseBytes > kPHashBytes ? kPDenseBytes : kPHashBytes;
 
 // cperm: vertices grouped by color (cstart[c] .. cstart[c+1]).
@@ -766,6 +937,8 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
     // hash view
     i64 *hkeys = reinterpret_cast<i64 *>(smem);
     double *hvals = reinterpret_cast<double *>(smem + (size_t)kPHashSlots * 8);
+    uint32_t *hbits = reinterpret_cast<uint32_t *>(smem + (size_t)kPHashSlots * 16);
+    double *dacc = HASH ? A.dacc + (i64)blockIdx.x * A.nv : nullptr;
     const int nv = (int)A.nv;
     const int nwords = (nv + 31) / 32;
     const unsigned nb = gridDim.x;
@@ -775,6 +948,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             hkeys[i] = kEmpty;
             hvals[i] = 0.0;
         }
+        for (int i = threadIdx.x; i < nwords; i += kPBlock) hbits[i] = 0;
     } else {
         for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
         for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
@@ -789,23 +963,12 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             const i64 beg = A.off[v], end = A.off[v + 1];
             const i64 s = A.C[v];
             double sl = 0, e_s = 0;
-            // hash table of this vertex (HASH only)
-            i64 *keys = hkeys;
-            double *vals = hvals;
+            // HASH: LDS table of this vertex, or the dense HBM row when it is large
+            const bool big = HASH && end - beg > kPHashMaxDeg;
             i64 slots = 64;
             int shift = 58;
-            if (HASH) {
+            if (HASH && !big) {
                 while (slots < 2 * (end - beg)) slots <<= 1;
-                if (end - beg > kPHashMaxDeg) {
-                    keys = A.gkeys + (i64)blockIdx.x * A.gslots;
-                    vals = A.gvals + (i64)blockIdx.x * A.gslots;
-                    for (i64 h = threadIdx.x; h < slots; h += kPBlock) {
-                        keys[h] = kEmpty;
-                        vals[h] = 0.0;
-                    }
-                    __threadfence_block();
-                    __syncthreads();
-                }
                 int lg = 0;
                 while (((i64)1 << lg) < slots) lg++;
                 shift = 64 - lg;
@@ -817,7 +980,12 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 if (t == v) sl += ww;
                 const i64 cc = A.C[t];
                 if (HASH) {
-                    hash_insert(keys, vals, mask, shift, cc, ww);
+                    if (big) {
+                        atomicOr(&hbits[cc >> 5], 1u << (cc & 31));
+                        atomicAdd(&dacc[cc], ww);
+                    } else {
+                        hash_insert(hkeys, hvals, mask, shift, cc, ww);
+                    }
                 } else {
                     atomicOr(&touched[cc >> 5], 1u << (cc & 31));
                     atomicAdd(&acc[cc], ww);
@@ -833,10 +1001,30 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             i64 id = INT64_MAX;
             float t1 = -INFINITY, t2 = -INFINITY;
             int ntouched = 0;
-            if (HASH) {
-                e_s = hash_lookup(keys, vals, mask, shift, s);
-                score_slots(keys, vals, slots, threadIdx.x, kPBlock, A, s, kv, e_s - sl, ax, cand, gain,
-                            id, t1, t2);
+            if (HASH && big) {
+                e_s = ld_agent(&dacc[s]);
+                const double eix = e_s - sl;
+                const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
+                for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
+                    uint32_t m = hbits[wi];
+                    while (m) {
+                        const int b = __ffs((int)m) - 1;
+                        m &= m - 1;
+                        const i64 y = (i64)wi * 32 + b;
+                        if (y == s) continue;
+                        const double g = gain_of(ld_agent(&dacc[y]), eix, kv, A.a[y], ax, A.constant);
+                        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+                        if (better(g > 0, g, y, cand, gain, id)) {
+                            cand = true;
+                            gain = g;
+                            id = y;
+                        }
+                    }
+                }
+            } else if (HASH) {
+                e_s = hash_lookup(hkeys, hvals, mask, shift, s);
+                score_slots(hkeys, hvals, slots, threadIdx.x, kPBlock, A, s, kv, e_s - sl, ax, A.constant, cand,
+                            gain, id, t1, t2);
             } else {
                 ntouched = compact_touched<kPBlock>(touched, nwords, list, wsum);
                 e_s = acc[s];
@@ -866,12 +1054,22 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             }
             __syncthreads();  // every read of the accumulator is done
-            if (HASH) {
-                if (keys == hkeys)
-                    for (i64 h = threadIdx.x; h < slots; h += kPBlock) {
-                        hkeys[h] = kEmpty;
-                        hvals[h] = 0.0;
+            if (HASH && big) {
+                for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
+                    uint32_t m = hbits[wi];
+                    while (m) {
+                        const int b = __ffs((int)m) - 1;
+                        m &= m - 1;
+                        dacc[(i64)wi * 32 + b] = 0.0;
                     }
+                    hbits[wi] = 0;
+                }
+                __threadfence_block();
+            } else if (HASH) {
+                for (i64 h = threadIdx.x; h < slots; h += kPBlock) {
+                    hkeys[h] = kEmpty;
+                    hvals[h] = 0.0;
+                }
             } else {
                 for (int k = threadIdx.x; k < ntouched; k += kPBlock) acc[list[k]] = 0.0;
                 for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
@@ -970,9 +1168,10 @@ __global__ void k_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const dou
 // vertex, quirks kept (integer-truncated self-loop in selfLoop, last
 // self-loop weight as the own-cluster counter, duplicates not merged).
 __global__ void k_init_comm_ass_opt(const i64 *off, const i64 *tails, const double *w, i64 nv,
-                                    const double *k, const double *a, const i64 *n, double constant,
-                                    i64 *curr) {
+                                    const double *k, const double *a, const i64 *n, double constant0,
+                                    const int32_t *gid, const double *gconst, i64 *curr) {
     for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
+        const double constant = gid ? gconst[gid[v]] : constant0;
         double selfLoop = 0, cself = 0;
         for (i64 j = off[v]; j < off[v + 1]; j++)
             if (tails[j] == v) {
@@ -1021,6 +1220,21 @@ __global__ __launch_bounds__(kBlock) void k_mod_pass(const i64 *off, const i64 *
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// Internal weight of giant vertices: kGiantBlocks workgroups per vertex.
+__global__ __launch_bounds__(kBlock) void k_mod_giant(const i64 *off, const i64 *tails, const double *w,
+                                                   const i64 *C, const i64 *verts, double *part) {
+    const i64 q = blockIdx.y;
+    const i64 v = verts[q];
+    const i64 s = C[v];
+    double acc = 0;
+    if (s >= 0)
+        for (i64 j = off[v] + (i64)blockIdx.x * kBlock + threadIdx.x; j < off[v + 1];
+             j += (i64)gridDim.x * kBlock)
+            if (C[tails[j]] == s) acc += w[j];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = acc;
+}
+
 // ------------------------------------------------------------------- coloring
 
 // Appends `v` to list[] when `pred`, one atomic per wavefront.
@@ -1042,30 +1256,62 @@ __device__ inline void wave_append(bool pred, i64 v, i64 *list, i64 *count) {
 // r of the classic formulation would find it a local maximum.  Each frontier
 // vertex takes the smallest color absent from its (all coloured) higher
 // neighbours and releases its lower neighbours, so total work is O(nnz).
-__device__ inline void frontier_push(bool pred, i64 u, const i64 *off, i64 *small, i64 *large,
-                                     i64 *counts) {
-    const bool big = pred && (off[u + 1] - off[u] > kBinMaxDeg[6]);
-    wave_append(pred && !big, u, small, &counts[0]);
-    wave_append(big, u, large, &counts[1]);
+__device__ inline void frontier_push(bool pred, i64 u, const i64 *off, const JpLists &L) {
+    const i64 d = pred ? off[u + 1] - off[u] : 0;
+    const bool big = pred && d > kBinMaxDeg[6];
+    const bool giant = big && d > kGiantDeg;
+    wave_append(pred && !big, u, L.small, &L.counts[0]);
+    wave_append(big && !giant, u, L.large, &L.counts[1]);
+    wave_append(giant, u, L.giant, &L.counts[2]);
 }
 
-__global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cnt, i64 *small,
-                           i64 *large, i64 *counts) {
+// count[v] of vertices up to 256 entries, one thread each; heavier ones are
+// listed in heavy[] (heavy_count) for k_jp_count_heavy.
+__global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cnt, JpLists L, i64 *heavy,
+                           i64 *heavy_count) {
     const i64 stride = (i64)gridDim.x * blockDim.x;
     const i64 first = blockIdx.x * (i64)blockDim.x + threadIdx.x;
     const i64 trips = (nv + stride - 1) / stride;
     for (i64 t = 0; t < trips; t++) {
         const i64 v = first + t * stride;
         int c = 0;
+        bool heavy_v = false;
         if (v < nv) {
-            const u64 pv = jp_priority(v);
-            for (i64 j = off[v]; j < off[v + 1]; j++) {
-                const i64 u = tails[j];
-                if (u != v && jp_priority(u) > pv) c++;
+            const i64 b = off[v], e = off[v + 1];
+            if (e - b > kBinMaxDeg[6]) {
+                heavy_v = true;
+            } else {
+                const u64 pv = jp_priority(v);
+                for (i64 j = b; j < e; j++) {
+                    const i64 u = tails[j];
+                    if (u != v && jp_priority(u) > pv) c++;
+                }
+                cnt[v] = c;
             }
-            cnt[v] = c;
         }
-        frontier_push(v < nv && c == 0, v, off, small, large, counts);
+        wave_append(heavy_v, v, heavy, heavy_count);
+        frontier_push(v < nv && !heavy_v && c == 0, v, off, L);
+    }
+}
+// count[v] of the heavy vertices, one wavefront each
+__global__ __launch_bounds__(kBlock) void k_jp_count_heavy(const i64 *off, const i64 *tails, int32_t *cnt,
+                                                           const i64 *heavy, const i64 *heavy_count,
+                                                           JpLists L) {
+    constexpr int WPB = kBlock / kWave;
+    const i64 nheavy = *heavy_count;
+    const int lane = threadIdx.x % kWave;
+    const i64 stride = (i64)gridDim.x * WPB;
+    for (i64 q = (i64)blockIdx.x * WPB + threadIdx.x / kWave; q < nheavy; q += stride) {
+        const i64 v = heavy[q];
+        const u64 pv = jp_priority(v);
+        int c = 0;
+        for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
+            const i64 u = tails[j];
+            if (u != v && jp_priority(u) > pv) c++;
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (lane == 0) cnt[v] = c;
+        frontier_push(lane == 0 && c == 0, v, off, L);
     }
 }
 
@@ -1073,8 +1319,8 @@ __global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cn
 // bitmap per wavefront); whole workgroups iterate together for the barriers.
 __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, const i64 *tails,
                                                              int32_t *colors, int32_t *cnt,
-                                                             const i64 *front, i64 nfront, i64 *small,
-                                                             i64 *large, i64 *counts, int32_t *err) {
+                                                             const i64 *front, i64 nfront, JpLists L,
+                                                             int32_t *err) {
     constexpr int WPB = kBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -1126,7 +1372,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
                 u = tails[b + j];
                 if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
             }
-            frontier_push(ready, u, off, small, large, counts);
+            frontier_push(ready, u, off, L);
         }
         __syncthreads();
     }
@@ -1135,8 +1381,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
 // Frontier vertices of degree > 256, one workgroup each.
 __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, const i64 *tails,
                                                               int32_t *colors, int32_t *cnt,
-                                                              const i64 *front, i64 nfront,
-                                                              i64 *small, i64 *large, i64 *counts,
+                                                              const i64 *front, i64 nfront, JpLists L,
                                                               int32_t *err) {
     __shared__ uint32_t s_mark[kMaxColors / 32];
     __shared__ int s_color;
@@ -1178,9 +1423,64 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                 u = tails[b + j];
                 if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
             }
-            frontier_push(ready, u, off, small, large, counts);
+            frontier_push(ready, u, off, L);
         }
         __syncthreads();
+    }
+}
+
+// Giant frontier vertices, kGiantBlocks workgroups each (blockIdx.y): mark
+// the colors of the (coloured, higher) neighbours in the vertex's HBM bitmap
+// gmark[y][kMaxColors/32] (zero on entry) ...
+__global__ __launch_bounds__(kBlock) void k_jp_giant_mark(const i64 *off, const i64 *tails,
+                                                          const int32_t *colors, const i64 *front,
+                                                          uint32_t *gmark) {
+    const i64 v = front[blockIdx.y];
+    uint32_t *mk = gmark + (i64)blockIdx.y * (kMaxColors / 32);
+    for (i64 j = off[v] + (i64)blockIdx.x * kBlock + threadIdx.x; j < off[v + 1]; j += (i64)gridDim.x * kBlock) {
+        const i64 u = tails[j];
+        if (u == v) continue;
+        const int c = colors[u];
+        if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+    }
+}
+// ... then take the first free color and release the lower neighbours.
+__global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, const i64 *tails,
+                                                             int32_t *colors, int32_t *cnt,
+                                                             const i64 *front, const uint32_t *gmark,
+                                                             JpLists L, int32_t *err) {
+    __shared__ int s_color;
+    const i64 v = front[blockIdx.y];
+    if (blockIdx.x == 0) {
+        const uint32_t *mk = gmark + (i64)blockIdx.y * (kMaxColors / 32);
+        if (threadIdx.x == 0) s_color = kMaxColors;
+        __syncthreads();
+        for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
+            const uint32_t m = mk[i];
+            if (m != 0xFFFFFFFFu) atomicMin(&s_color, i * 32 + __ffs((int)~m) - 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int color = s_color;
+            if (color >= kMaxColors) {
+                atomicOr(err, 1);
+                color = 0;
+            }
+            colors[v] = color;
+        }
+    }
+    const u64 pv = jp_priority(v);
+    const i64 b = off[v], deg = off[v + 1] - b;
+    const i64 step = (i64)gridDim.x * kBlock;
+    for (i64 j0 = (i64)blockIdx.x * kBlock; j0 < deg; j0 += step) {
+        const i64 j = j0 + threadIdx.x;
+        bool ready = false;
+        i64 u = 0;
+        if (j < deg) {
+            u = tails[b + j];
+            if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+        }
+        frontier_push(ready, u, off, L);
     }
 }
 
@@ -1230,6 +1530,182 @@ __global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const
         }
         __syncthreads();
     }
+}
+
+// Jones-Plassmann of a small graph in one cooperative launch: the same
+// topological sweep as the host-driven rounds (count of higher-priority
+// uncoloured neighbours, frontier = count zero, first-fit against the
+// coloured higher neighbours), with a grid barrier between rounds instead of
+// a launch and a host round trip.  When at most tail_at vertices remain,
+// workgroup 0 colours them by sequential first-fit in increasing id (the
+// host-driven path's tail).  ring[0..2]: frontier counts of rounds r (mod 3),
+// zero on entry; ring[3] receives the number of rounds.
+constexpr int kJpPBlock = 1024;
+__global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, const i64 *tails, i64 nv,
+                                                             int32_t *colors, int32_t *cnt, i64 *listA,
+                                                             i64 *listB, i64 *ring, unsigned *bar,
+                                                             int32_t *err, i64 tail_at) {
+    constexpr int WPB = kJpPBlock / kWave;
+    __shared__ uint32_t mark[WPB][kMaxColors / 32];
+    __shared__ i64 tail_list[kJpPBlock];
+    __shared__ int wsum[kJpPBlock / kWave];
+    __shared__ int s_color;
+    const unsigned nb = gridDim.x;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    uint32_t *mk = mark[w];
+    unsigned seq = 0;
+    // counts: one thread per vertex up to 256 entries; heavier ones are listed
+    // in listB (ring[2]) and counted by a wavefront each
+    for (i64 base = (i64)blockIdx.x * kJpPBlock; base < nv; base += (i64)nb * kJpPBlock) {
+        const i64 v = base + threadIdx.x;
+        int c = 0;
+        bool heavy = false;
+        if (v < nv) {
+            const i64 b = off[v], e = off[v + 1];
+            if (e - b > kBinMaxDeg[6]) {
+                heavy = true;
+            } else {
+                const u64 pv = jp_priority(v);
+                for (i64 j = b; j < e; j++) {
+                    const i64 u = tails[j];
+                    if (u != v && jp_priority(u) > pv) c++;
+                }
+                cnt[v] = c;
+            }
+        }
+        wave_append(heavy, v, listB, &ring[2]);
+        wave_append(v < nv && !heavy && c == 0, v, listA, &ring[0]);
+    }
+    bool alive = grid_barrier(bar, nb, ++seq);
+    if (alive) {
+        const i64 nheavy = __atomic_load_n(&ring[2], __ATOMIC_RELAXED);
+        for (i64 q = (i64)blockIdx.x * WPB + w; q < nheavy; q += (i64)nb * WPB) {
+            const i64 v = listB[q];
+            const u64 pv = jp_priority(v);
+            int c = 0;
+            for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
+                const i64 u = tails[j];
+                if (u != v && jp_priority(u) > pv) c++;
+            }
+            for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
+            if (lane == 0) cnt[v] = c;
+            wave_append(lane == 0 && c == 0, v, listA, &ring[0]);
+        }
+        alive = grid_barrier(bar, nb, ++seq);
+    }
+    i64 colored = 0;
+    int r = 0;
+    i64 *front = listA, *next = listB;
+    while (alive) {
+        const i64 nfront = __atomic_load_n(&ring[r % 3], __ATOMIC_RELAXED);
+        if (nfront == 0) break;
+        if (blockIdx.x == 0 && threadIdx.x == 0) __atomic_store_n(&ring[(r + 2) % 3], (i64)0, __ATOMIC_RELAXED);
+        i64 *ncount = &ring[(r + 1) % 3];
+        for (i64 base = (i64)blockIdx.x * WPB; base < nfront; base += (i64)nb * WPB) {
+            const i64 q = base + w;
+            const bool in = q < nfront;
+            const i64 v = in ? front[q] : 0;
+            const i64 b = in ? off[v] : 0, e = in ? off[v + 1] : 0;
+            for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
+            __syncthreads();
+            for (i64 j = b + lane; j < e; j += kWave) {
+                const i64 u = tails[j];
+                if (u == v) continue;
+                const int c = colors[u];
+                if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+            }
+            __syncthreads();
+            int cand = kMaxColors;
+            for (int i = lane; i < kMaxColors / 32; i += kWave) {
+                const uint32_t m = mk[i];
+                if (m != 0xFFFFFFFFu) {
+                    const int c = i * 32 + __ffs((int)~m) - 1;
+                    cand = c < cand ? c : cand;
+                }
+            }
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const int other = __shfl_xor(cand, o);
+                cand = other < cand ? other : cand;
+            }
+            if (in && lane == 0) {
+                int color = cand;
+                if (color >= kMaxColors) {
+                    atomicOr(err, 1);
+                    color = 0;
+                }
+                colors[v] = color;
+            }
+            const u64 pv = in ? jp_priority(v) : 0;
+            const i64 deg = e - b;
+            for (i64 j0 = 0; j0 < deg; j0 += kWave) {
+                const i64 j = j0 + lane;
+                bool ready = false;
+                i64 u = 0;
+                if (j < deg) {
+                    u = tails[b + j];
+                    if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+                }
+                wave_append(ready, u, next, ncount);
+            }
+            __syncthreads();
+        }
+        alive = grid_barrier(bar, nb, ++seq);
+        colored += nfront;
+        r++;
+        const i64 rest = nv - colored;
+        if (rest == 0) break;
+        if (rest <= tail_at) {
+            if (blockIdx.x != 0) break;
+            // workgroup 0: the uncoloured vertices in increasing id, then first-fit one by one
+            int n = 0;
+            for (i64 base = 0; base < nv; base += kJpPBlock) {
+                const i64 v = base + threadIdx.x;
+                const bool un = v < nv && __atomic_load_n(&colors[v], __ATOMIC_RELAXED) < 0;
+                int total = 0;
+                const int at = n + block_exscan<kJpPBlock>(un ? 1 : 0, wsum, &total);
+                if (un && at < kJpPBlock) tail_list[at] = v;
+                n += total;
+            }
+            __syncthreads();
+            if (n > kJpPBlock) {
+                if (threadIdx.x == 0) atomicOr(err, 2);
+                break;
+            }
+            for (int q = 0; q < n; q++) {
+                const i64 v = tail_list[q];
+                for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
+                if (threadIdx.x == 0) s_color = kMaxColors;
+                __syncthreads();
+                for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += kJpPBlock) {
+                    const i64 u = tails[j];
+                    if (u == v) continue;
+                    const int32_t su = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
+                    if (su < 0) continue;
+                    atomicOr(&mark[0][su >> 5], 1u << (su & 31));
+                }
+                __syncthreads();
+                if (threadIdx.x < kMaxColors / 32) {
+                    const uint32_t m = mark[0][threadIdx.x];
+                    if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    int color = s_color;
+                    if (color >= kMaxColors) {
+                        atomicOr(err, 1);
+                        color = 0;
+                    }
+                    __atomic_store_n(&colors[v], (int32_t)color, __ATOMIC_RELAXED);
+                }
+                __syncthreads();
+            }
+            break;
+        }
+        i64 *t = front;
+        front = next;
+        next = t;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ring[3] = r;
 }
 
 __global__ void k_jp_finish(const int32_t *colors, i64 nv, int32_t *maxc) {
@@ -1419,9 +1895,9 @@ void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s) {
     GLV_LAUNCH_CHECK();
 }
 void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys, i64 *vid,
-                     int32_t *err, hipStream_t s) {
+                     int32_t *err, hipStream_t s, const BinConfig &bins) {
     if (nv <= 0) return;
-    k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, err);
+    k_bin_keys<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(off, nv, colors, ncol, keys, vid, err, bins);
     GLV_LAUNCH_CHECK();
 }
 __global__ void k_color_keys(const i64 *perm, const int32_t *colors, i64 n, uint32_t *keys, i64 *vals) {
@@ -1506,6 +1982,7 @@ i64 sweep_blocks(int bin, i64 count, i64 nv) {
         case 6: return (count + kBlock / kWave - 1) / (kBlock / kWave);
         case 7: return count;
         case 8: return std::min<i64>(count, kGlobalHashBlocks);
+        case 9: return count;  // one partial per giant vertex (written by its decision)
         default: return 0;
     }
 }
@@ -1528,6 +2005,15 @@ static void sweep_dispatch(int bin, const SweepArgs &A, hipStream_t s) {
         case 6: k_sweep_wavehash<PLAIN><<<g, kBlock, 0, s>>>(A); break;
         case 7: k_sweep_blockhash<PLAIN, false><<<g, kBlock, 0, s>>>(A); break;
         case 8: k_sweep_blockhash<PLAIN, true><<<g, kBlock, 0, s>>>(A); break;
+        case 9: {
+            const dim3 grid(kGiantBlocks, (unsigned)A.count);
+            k_giant_accum<<<grid, kBlock, 0, s>>>(A);
+            GLV_LAUNCH_CHECK();
+            k_giant_score<<<grid, kBlock, 0, s>>>(A);
+            GLV_LAUNCH_CHECK();
+            k_giant_decide<PLAIN><<<(unsigned)A.count, kWave, 0, s>>>(A);
+            break;
+        }
         default: return;
     }
     GLV_LAUNCH_CHECK();
@@ -1537,6 +2023,11 @@ void launch_sweep(int bin, bool plain, const SweepArgs &A, hipStream_t s) {
         sweep_dispatch<true>(bin, A, s);
     else
         sweep_dispatch<false>(bin, A, s);
+}
+void launch_giant_table_sizes(const i64 *off, const i64 *verts, i64 count, i64 *tab, hipStream_t s) {
+    if (count <= 0) return;
+    k_giant_table_sizes<<<grid_for(count, kBlock, 1024), kBlock, 0, s>>>(off, verts, count, tab);
+    GLV_LAUNCH_CHECK();
 }
 void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hipStream_t s) {
     if (count <= 0) return;
@@ -1560,8 +2051,8 @@ bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *st
                               unsigned *bar, double *part, int nblocks, hipStream_t s, uint8_t *stale) {
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || A.nv > kPersistMaxNV) return false;
     const bool hash = A.nv > kDenseMaxNV;
-    // degrees above the LDS table's reach need the workgroup's HBM table
-    if (hash && A.gslots > 0 && (A.gkeys == nullptr || A.gvals == nullptr)) return false;
+    // degrees above the LDS table's reach need the workgroup's dense HBM row
+    if (hash && A.dacc == nullptr) return false;
     SweepArgs aa = A;
     i64 *C = const_cast<i64 *>(A.C);
     double *ca = const_cast<double *>(A.a);
@@ -1603,10 +2094,11 @@ void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k
     GLV_LAUNCH_CHECK();
 }
 void launch_init_comm_ass_opt(const DevGraph &g, const double *k, const double *a, const i64 *n,
-                              double constant, i64 *curr, hipStream_t s) {
+                              double constant, i64 *curr, hipStream_t s, const int32_t *gid,
+                              const double *gconst) {
     if (g.nv <= 0) return;
     k_init_comm_ass_opt<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.w, g.nv, k, a,
-                                                                        n, constant, curr);
+                                                                        n, constant, gid, gconst, curr);
     GLV_LAUNCH_CHECK();
 }
 void launch_update_ax_opt(i64 nv, const i64 *curr, const double *k, double *a, i64 *n, hipStream_t s) {
@@ -1617,6 +2109,7 @@ void launch_update_ax_opt(i64 nv, const i64 *curr, const double *k, double *a, i
 
 i64 mod_blocks(int bin, i64 count) {
     if (count <= 0 || bin == 0) return 0;
+    if (bin == 9) return count * kGiantBlocks;
     const int G = bin <= 5 ? kBinGroup[bin] : (bin == 6 ? kWave : kBlock);
     const i64 vpb = kBlock / G;
     return std::min<i64>((count + vpb - 1) / vpb, 8192);
@@ -1625,6 +2118,11 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
                      hipStream_t s) {
     const unsigned gr = (unsigned)mod_blocks(bin, count);
     if (gr == 0) return;
+    if (bin == 9) {
+        k_mod_giant<<<dim3(kGiantBlocks, (unsigned)count), kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, part);
+        GLV_LAUNCH_CHECK();
+        return;
+    }
     switch (bin) {
         case 1: k_mod_pass<4><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
         case 2: k_mod_pass<8><<<gr, kBlock, 0, s>>>(g.off, g.tails, g.w, C, verts, count, part); break;
@@ -1637,26 +2135,34 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
     GLV_LAUNCH_CHECK();
 }
 
-void launch_jp_count(const DevGraph &g, int32_t *cnt, i64 *small, i64 *large, i64 *counts,
+void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *heavy, i64 *heavy_count,
                      hipStream_t s) {
     if (g.nv <= 0) return;
-    k_jp_count<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.nv, cnt, small, large,
-                                                               counts);
+    k_jp_count<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.nv, cnt, L, heavy,
+                                                               heavy_count);
+    GLV_LAUNCH_CHECK();
+    k_jp_count_heavy<<<4096, kBlock, 0, s>>>(g.off, g.tails, cnt, heavy, heavy_count, L);
     GLV_LAUNCH_CHECK();
 }
-void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const i64 *small, i64 nsmall,
-                        const i64 *large, i64 nlarge, i64 *next_small, i64 *next_large, i64 *counts,
-                        int32_t *err, hipStream_t s) {
-    if (nsmall > 0) {
+void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const JpLists &cur,
+                        const i64 n[3], const JpLists &next, uint32_t *gmark, int32_t *err, hipStream_t s) {
+    if (n[0] > 0) {
         const i64 wpb = kBlock / kWave;
-        const unsigned blocks = (unsigned)std::max<i64>(1, std::min<i64>((nsmall + wpb - 1) / wpb, 16384));
-        k_jp_frontier_wave<<<blocks, kBlock, 0, s>>>(g.off, g.tails, colors, cnt, small, nsmall,
-                                                     next_small, next_large, counts, err);
+        const unsigned blocks = (unsigned)std::max<i64>(1, std::min<i64>((n[0] + wpb - 1) / wpb, 16384));
+        k_jp_frontier_wave<<<blocks, kBlock, 0, s>>>(g.off, g.tails, colors, cnt, cur.small, n[0], next, err);
         GLV_LAUNCH_CHECK();
     }
-    if (nlarge > 0) {
-        k_jp_frontier_block<<<(unsigned)std::min<i64>(nlarge, 8192), kBlock, 0, s>>>(
-            g.off, g.tails, colors, cnt, large, nlarge, next_small, next_large, counts, err);
+    if (n[1] > 0) {
+        k_jp_frontier_block<<<(unsigned)std::min<i64>(n[1], 8192), kBlock, 0, s>>>(
+            g.off, g.tails, colors, cnt, cur.large, n[1], next, err);
+        GLV_LAUNCH_CHECK();
+    }
+    if (n[2] > 0) {
+        launch_fill_i32((int32_t *)gmark, n[2] * (kMaxColors / 32), 0, s);
+        const dim3 grid(kGiantBlocks, (unsigned)n[2]);
+        k_jp_giant_mark<<<grid, kBlock, 0, s>>>(g.off, g.tails, colors, cur.giant, gmark);
+        GLV_LAUNCH_CHECK();
+        k_jp_giant_release<<<grid, kBlock, 0, s>>>(g.off, g.tails, colors, cnt, cur.giant, gmark, next, err);
         GLV_LAUNCH_CHECK();
     }
 }
@@ -1671,6 +2177,31 @@ void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, 
     (void)round;
     k_color_tail<<<1, kTailBlock, 0, s>>>(g.off, g.tails, state, list, n, err);
     GLV_LAUNCH_CHECK();
+}
+int jp_persistent_blocks() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0, cus = 0, per = 0;
+        GLV_HIP(hipGetDevice(&dev));
+        GLV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_jp_persistent, kJpPBlock, 0));
+        cached = per > 0 ? cus * per : 0;
+    }
+    return cached;
+}
+bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *listA, i64 *listB,
+                          i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, int nblocks, hipStream_t s) {
+    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0) return false;
+    const i64 *off = g.off, *tails = g.tails;
+    i64 nv = g.nv, ta = tail_at;
+    void *args[] = {&off, &tails, &nv, &colors, &cnt, &listA, &listB, &ring, &bar, &err, &ta};
+    hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_persistent, dim3(nblocks), dim3(kJpPBlock),
+                                              args, 0, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
 }
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s) {
     if (nv <= 0) return;

@@ -29,6 +29,27 @@ struct SweepArgs {
     // learned active set (both null on the plain path):
     float4 *feat = nullptr;  // per swept vertex: own-community fraction, top-two gains
     double *dein = nullptr;  // per swept vertex: 2 (e_{v,t} - e_{v,s}) of its move, 0 if none
+    // persistent sweep above the dense-accumulator limit: nv zeros per workgroup
+    // (kept zero between vertices) for the vertices too large for its LDS table
+    double *dacc = nullptr;
+    // giant vertices (bin 9) of the slice: HBM hash table of vertex q at
+    // gt_keys/gt_vals + gt_tab[q], gt_tab[q+1] - gt_tab[q] slots (kEmpty / 0
+    // between uses); gt_sc[2q..2q+1]: weight to its own community and
+    // self-loop weight (0 between uses); gt_rg/gt_ri/gt_rt: the best move of
+    // each of its kGiantBlocks workgroups
+    // batched graphs: vertex v belongs to graph gid[v] whose 1/(2m) is
+    // gconst[gid[v]] (null: one graph, `constant`); eii_v[v] receives the
+    // weight from v to its own community (plain sweeps, per-graph modularity)
+    const int32_t *gid = nullptr;
+    const double *gconst = nullptr;
+    double *eii_v = nullptr;
+    const i64 *gt_tab = nullptr;
+    i64 *gt_keys = nullptr;
+    double *gt_vals = nullptr;
+    double *gt_sc = nullptr;
+    double *gt_rg = nullptr;
+    i64 *gt_ri = nullptr;
+    float *gt_rt = nullptr;
 };
 
 // Up to one slice per degree bin of a color class (perm[lo[b] .. hi[b])).
@@ -44,7 +65,7 @@ void launch_fill_i32(int32_t *p, i64 n, int32_t v, hipStream_t s);
 void launch_iota(i64 *p, i64 n, hipStream_t s);
 void launch_init_labels(const i64 *off, i64 nv, i64 *C, hipStream_t s);  // C[i] = deg? i : -1
 void launch_bin_keys(const i64 *off, i64 nv, const int32_t *colors, int ncol, uint32_t *keys,
-                     i64 *vid, int32_t *err, hipStream_t s);
+                     i64 *vid, int32_t *err, hipStream_t s, const BinConfig &bins);
 // slice starts (nkeys+1) and entries per slice (nkeys) of keys sorted with perm
 void launch_slices(const uint32_t *sorted_keys, const i64 *perm, const i64 *off, i64 n, i64 nkeys,
                    i64 *start, i64 *pnnz, hipStream_t s);
@@ -86,9 +107,19 @@ void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 
 void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a, i64 *n,
                         hipStream_t s);
 void launch_init_comm_ass_opt(const DevGraph &g, const double *k, const double *a, const i64 *n,
-                              double constant, i64 *curr, hipStream_t s);
+                              double constant, i64 *curr, hipStream_t s, const int32_t *gid = nullptr,
+                              const double *gconst = nullptr);
 void launch_update_ax_opt(i64 nv, const i64 *curr, const double *k, double *a, i64 *n,
                           hipStream_t s);
+
+// slots of the HBM hash table of a giant vertex of degree d
+inline i64 giant_table_slots(i64 d) {
+    i64 s = 64;
+    while (s < 2 * d) s <<= 1;
+    return s;
+}
+// per giant vertex q of verts[0..count): tab[q] = table slots (for a scan)
+void launch_giant_table_sizes(const i64 *off, const i64 *verts, i64 count, i64 *tab, hipStream_t s);
 
 // --- modularity ----------------------------------------------------------
 i64 mod_blocks(int bin, i64 count);
@@ -96,19 +127,34 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
                      double *part, hipStream_t s);
 
 // --- coloring ------------------------------------------------------------
-// count[v] = uncoloured higher-priority neighbours; count-0 vertices form the first frontier
-void launch_jp_count(const DevGraph &g, int32_t *cnt, i64 *small, i64 *large, i64 *counts,
+// Frontier lists of the Jones-Plassmann rounds by degree: small (<= 256, a
+// wavefront each), large (<= kGiantDeg, a workgroup each), giant
+// (kGiantBlocks workgroups each); counts[0..2] their lengths.
+struct JpLists {
+    i64 *small, *large, *giant;
+    i64 *counts;
+};
+// count[v] = uncoloured higher-priority neighbours; count-0 vertices form the
+// first frontier.  heavy: nv scratch, heavy_count: one zeroed word.
+void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *heavy, i64 *heavy_count,
                      hipStream_t s);
-// color one frontier and build the next one (counts[0] small, counts[1] large appended)
-void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const i64 *small,
-                        i64 nsmall, const i64 *large, i64 nlarge, i64 *next_small,
-                        i64 *next_large, i64 *counts, int32_t *err, hipStream_t s);
+// color one frontier (n[0..2] vertices in cur's lists) and build the next
+// one into next (whose counts are zero); gmark: n[2] * kMaxColors/32 words
+void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const JpLists &cur,
+                        const i64 n[3], const JpLists &next, uint32_t *gmark, int32_t *err,
+                        hipStream_t s);
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
 // first-fit, one vertex at a time in increasing id, of the n uncoloured vertices
 void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
                        int round, int32_t *err, hipStream_t s);
 constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many uncoloured vertices
-constexpr i64 kJpMinWon = 32;   // ... or once a round colors fewer vertices than this
+// the whole coloring of a graph of at most kPersistMaxNV vertices in one
+// cooperative launch (same result as the host-driven rounds); false when the
+// launch is refused.  ring: 4 zeroed words, ring[3] receives the rounds;
+// bar: kBarrierWords zeroed words; err: 1 too many colors.
+int jp_persistent_blocks();
+bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *listA, i64 *listB,
+                          i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, int nblocks, hipStream_t s);
 void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);
 
 // --- renumber / compose --------------------------------------------------

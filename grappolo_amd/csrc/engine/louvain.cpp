@@ -13,6 +13,11 @@ namespace glv {
 
 [[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
 
+BinConfig &current_bins() {
+    static BinConfig c = default_bins();
+    return c;
+}
+
 namespace {
 
 double now() {
@@ -56,7 +61,7 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
     DBuf<i64> pnnz(nkeys, s_);
     i64 *start = P.dptr.p;
     launch_fill_i32(err.p, 1, 0, s_);
-    launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, err.p, s_);
+    launch_bin_keys(g.off, nv, colors, ncol, keys.p, P.perm.p, err.p, s_, bins_);
     sort_pairs_u32_i64(keys.p, P.perm.p, nv, bits_for(nkeys), s_);
     launch_slices(keys.p, P.perm.p, g.off, nv, nkeys, start, pnnz.p, s_);
     P.ptr.assign(nkeys + 1, 0);
@@ -70,19 +75,67 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
     GLV_HIP(hipStreamSynchronize(s_));
     if (e) fail(GLV_E_INVALID_ARG, "color values out of range [0, nColors)");
     // degrees and 1/(2m)
-    const i64 big_lo = P.bin_lo(8), big_hi = P.bin_hi(8);
+    const i64 big_lo = P.bin_lo(8), big_hi = P.bin_hi(9);
     launch_degree(g, P.perm.p + big_lo, big_hi - big_lo, P.k.p, s_);
     DBuf<double> part(kRedBlocks, s_), res(1, s_);
     launch_sum(P.k.p, nv, part.p, res.p, s_);
     P.total_weight = fetch(res.p);
     P.constant = (double)1 / P.total_weight;
     P.maxdeg_global = 0;
-    if (big_hi > big_lo) {
+    if (P.bin_hi(8) > big_lo) {
         DBuf<i64> md(1, s_);
         launch_fill_i64(md.p, 1, 0, s_);
-        launch_max_degree(g.off, P.perm.p + big_lo, big_hi - big_lo, md.p, s_);
+        launch_max_degree(g.off, P.perm.p + big_lo, P.bin_hi(8) - big_lo, md.p, s_);
         P.maxdeg_global = fetch(md.p);
     }
+    // giant vertices: class-relative table offsets and the table memory
+    const i64 g_lo = P.bin_lo(9), ng = P.bin_hi(9) - g_lo;
+    P.gt_seg.assign(ncol, 0);
+    if (ng > 0 && nv > kDenseMaxNV) {
+        DBuf<i64> sz(ng, s_);
+        launch_giant_table_sizes(g.off, P.perm.p + g_lo, ng, sz.p, s_);
+        std::vector<i64> hsz(ng);
+        GLV_HIP(hipMemcpyAsync(hsz.data(), sz.p, ng * sizeof(i64), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipStreamSynchronize(s_));
+        std::vector<i64> tab;
+        tab.reserve(ng + ncol);
+        i64 most = 0;
+        for (int c = 0; c < ncol; c++) {
+            P.gt_seg[c] = (i64)tab.size();
+            i64 at = 0;
+            tab.push_back(0);
+            for (i64 i = P.lo(9, c) - g_lo; i < P.hi(9, c) - g_lo; i++) {
+                at += hsz[i];
+                tab.push_back(at);
+            }
+            most = std::max(most, at);
+        }
+        P.gt_tab.alloc(tab.size(), s_);
+        GLV_HIP(hipMemcpyAsync(P.gt_tab.p, tab.data(), tab.size() * sizeof(i64), hipMemcpyHostToDevice, s_));
+        P.gt_keys.alloc(most, s_);
+        P.gt_vals.alloc(most, s_);
+        launch_fill_i64(P.gt_keys.p, most, kEmpty, s_);
+        launch_fill_f64(P.gt_vals.p, most, 0.0, s_);
+        P.gt_sc.alloc(2 * ng, s_);
+        launch_fill_f64(P.gt_sc.p, 2 * ng, 0.0, s_);
+        P.gt_rg.alloc(ng * kGiantBlocks, s_);
+        P.gt_ri.alloc(ng * kGiantBlocks, s_);
+        P.gt_rt.alloc(2 * ng * kGiantBlocks, s_);
+        GLV_HIP(hipStreamSynchronize(s_));  // `tab` is pageable host memory
+    }
+}
+
+// Sweep arguments of the giant vertices of color class c (bin 9).
+void giant_args(const PhaseSetup &P, int c, SweepArgs &A) {
+    if (P.gt_tab.p == nullptr) return;
+    const i64 qb = P.lo(9, c) - P.bin_lo(9);
+    A.gt_tab = P.gt_tab.p + P.gt_seg[c];
+    A.gt_keys = P.gt_keys.p;
+    A.gt_vals = P.gt_vals.p;
+    A.gt_sc = P.gt_sc.p + 2 * qb;
+    A.gt_rg = P.gt_rg.p + qb * kGiantBlocks;
+    A.gt_ri = P.gt_ri.p + qb * kGiantBlocks;
+    A.gt_rt = P.gt_rt.p + 2 * qb * kGiantBlocks;
 }
 
 // ----------------------------------------------------------------- coloring
@@ -90,41 +143,72 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     const i64 nv = g.nv;
     if (nv == 0) return -1;
     launch_fill_i32(colors, nv, -1, s_);
-    DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), counts(2, s_);
+    if (nv <= kPersistMaxNV) {
+        // small graph: every round inside one cooperative launch
+        int nb = jp_persistent_blocks();
+        nb = (int)std::min<i64>({(i64)nb, kPersistMaxBlocks, std::max<i64>(4, nv / 2048)});
+        DBuf<i64> lA(nv, s_), lB(nv, s_), ring(4, s_);
+        DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
+        DBuf<unsigned> bar(kBarrierWords, s_);
+        launch_fill_i64(ring.p, 4, 0, s_);
+        launch_fill_i32(err.p, 1, 0, s_);
+        launch_fill_i32(maxc.p, 1, -1, s_);
+        launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
+        Profiler::get().begin(K_COLOR, s_);
+        const bool ok = nb > 0 && launch_jp_persistent(g, colors, cnt.p, lA.p, lB.p, ring.p, bar.p, err.p,
+                                                       kJpTail, nb, s_);
+        Profiler::get().end(K_COLOR, 0.0, s_);
+        if (ok) {
+            launch_jp_finish(colors, nv, maxc.p, s_);
+            i64 rounds = 0;
+            unsigned abort_flag = 0;
+            int32_t e = 0, mc = -1;
+            GLV_HIP(hipMemcpyAsync(&rounds, ring.p + 3, sizeof(i64), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipMemcpyAsync(&mc, maxc.p, sizeof(mc), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipStreamSynchronize(s_));
+            if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
+            if (e & 1) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
+            if (e) fail(GLV_E_DEVICE, "persistent coloring tail overflow");
+            if (rounds_out) *rounds_out = (int)rounds;
+            return mc;
+        }
+        launch_fill_i32(colors, nv, -1, s_);  // launch refused: host-driven rounds below
+    }
+    DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
     DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
-    launch_fill_i64(counts.p, 2, 0, s_);
+    DBuf<uint32_t> gmark;
+    launch_fill_i64(counts.p, 8, 0, s_);
     launch_fill_i32(err.p, 1, 0, s_);
     launch_fill_i32(maxc.p, 1, -1, s_);
+    JpLists cur{sA.p, lA.p, gA.p, counts.p}, next{sB.p, lB.p, gB.p, counts.p + 4};
     Profiler::get().begin(K_COLOR, s_);
-    launch_jp_count(g, cnt.p, sA.p, lA.p, counts.p, s_);
+    launch_jp_count(g, cnt.p, cur, sB.p, counts.p + 3, s_);  // heavy list in the next round's buffer
     Profiler::get().end(K_COLOR, 0.0, s_);
-    i64 hc[2];
+    i64 hc[3];
     GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
     GLV_HIP(hipStreamSynchronize(s_));
-    i64 ns = hc[0], nl = hc[1], colored = 0;
-    i64 *small = sA.p, *large = lA.p, *small_next = sB.p, *large_next = lB.p;
+    i64 colored = 0;
     int round = 0;
-    while (ns + nl > 0) {
+    while (hc[0] + hc[1] + hc[2] > 0) {
         if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
-        const i64 won = ns + nl;
-        launch_fill_i64(counts.p, 2, 0, s_);
+        const i64 won = hc[0] + hc[1] + hc[2];
+        if (hc[2] > 0 && (i64)gmark.n < hc[2] * (kMaxColors / 32)) gmark.alloc(hc[2] * (kMaxColors / 32), s_);
+        launch_fill_i64(next.counts, 4, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
-        launch_jp_frontier(g, colors, cnt.p, small, ns, large, nl, small_next, large_next, counts.p,
-                           err.p, s_);
+        launch_jp_frontier(g, colors, cnt.p, cur, hc, next, gmark.p, err.p, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
-        GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipMemcpyAsync(hc, next.counts, sizeof(hc), hipMemcpyDeviceToHost, s_));
         GLV_HIP(hipStreamSynchronize(s_));
         colored += won;
-        ns = hc[0];
-        nl = hc[1];
-        std::swap(small, small_next);
-        std::swap(large, large_next);
+        std::swap(cur, next);
         round++;
         const i64 rest = nv - colored;
-        if (rest > 0 && (rest <= kJpTail || won < kJpMinWon)) {
-            // rounds have stopped paying for themselves: finish one vertex at a time
+        if (rest > 0 && rest <= kJpTail) {
+            // the last vertices one at a time (scratch: lists not holding a frontier)
             Profiler::get().begin(K_COLOR_TAIL, s_);
-            launch_color_tail(g, colors, sB.p, sA.p, lB.p, rest, round, err.p, s_);
+            launch_color_tail(g, colors, next.small, next.large, next.giant, rest, round, err.p, s_);
             Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
             break;
         }
@@ -211,7 +295,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     if (P.maxdeg_global > 0) {
         gslots = 1;
         while (gslots < 2 * P.maxdeg_global) gslots <<= 1;
-        const i64 nb = std::max<i64>(std::min<i64>(kGlobalHashBlocks, P.bin_hi(8) - P.bin_lo(8)), pblocks);
+        const i64 nb = std::min<i64>(kGlobalHashBlocks, P.bin_hi(8) - P.bin_lo(8));
         gkeys.alloc(nb * gslots, s_);
         gvals.alloc(nb * gslots, s_);
     }
@@ -230,6 +314,11 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     A.gvals = gvals.p;
     A.gslots = gslots;
     A.nv = nv;
+    DBuf<double> dacc(pblocks > 0 && nv > kDenseMaxNV ? (size_t)pblocks * nv : 0, s_);
+    if (dacc.n) {
+        launch_fill_f64(dacc.p, (i64)dacc.n, 0.0, s_);
+        A.dacc = dacc.p;
+    }
     DBuf<unsigned> bar(pblocks > 0 ? kBarrierWords : 0, s_);
     DBuf<double> ppart(pblocks > 0 ? 2 * pblocks : 0, s_);
     DBuf<i64> cperm(pblocks > 0 ? nv : 0, s_), cstart(pblocks > 0 ? ncol + 1 : 0, s_);
@@ -369,9 +458,10 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                     if (S.hi[b] <= S.lo[b]) continue;
                     A.verts = plist + S.lo[b];
                     A.count = S.hi[b] - S.lo[b];
-                    Profiler::get().begin(K_SWEEP_B1 + b - 1, s_);
+                    if (b == 9) giant_args(P, c, A);
+                    Profiler::get().begin(sweep_class(b), s_);
                     launch_sweep(b, false, A, s_);
-                    Profiler::get().end(K_SWEEP_B1 + b - 1,
+                    Profiler::get().end(sweep_class(b),
                                         hptr == &P.ptr ? 24.0 * P.nnz_of(b, c) + 32.0 * A.count : 0.0, s_);
                 }
                 Profiler::get().begin(K_APPLY, s_);
@@ -472,9 +562,10 @@ double Engine::plain_phase(const DevGraph &g, i64 *C, double Lower, double thres
             A.verts = P.perm.p + lo;
             A.count = cnt;
             A.eii_part = part.p + off;
-            Profiler::get().begin(K_SWEEP_B1 + b - 1, s_);
+            if (b == 9) giant_args(P, 0, A);
+            Profiler::get().begin(sweep_class(b), s_);
             launch_sweep(b, true, A, s_);
-            Profiler::get().end(K_SWEEP_B1 + b - 1, 24.0 * P.bin_nnz(b) + 32.0 * cnt, s_);
+            Profiler::get().end(sweep_class(b), 24.0 * P.bin_nnz(b) + 32.0 * cnt, s_);
             off += sweep_blocks(b, cnt, nv);
         }
         launch_isolated_targets(P.perm.p + P.bin_lo(0), P.bin_hi(0) - P.bin_lo(0), tgt, s_);
@@ -543,6 +634,7 @@ i64 Engine::color_step(const DevGraph &g, i64 *C, const int32_t *colors, int nco
         if (S.hi[b] <= S.lo[b]) continue;
         A.verts = P.perm.p + S.lo[b];
         A.count = S.hi[b] - S.lo[b];
+        if (b == 9) giant_args(P, which, A);
         launch_sweep(b, false, A, s_);
     }
     // count movers before committing: compare on the host after the commit

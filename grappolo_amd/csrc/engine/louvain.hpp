@@ -80,6 +80,12 @@ struct PhaseSetup {
     double constant = 0;   // 1 / sum of degrees
     double total_weight = 0;
     i64 maxdeg_global = 0;  // largest degree in the HBM-hash bin
+    // giant vertices (bin 9): the HBM table offsets of color c's class are
+    // gt_tab[gt_seg[c] ..] (class-relative); the tables hold the largest class
+    DBuf<i64> gt_tab, gt_keys, gt_ri;
+    DBuf<double> gt_vals, gt_sc, gt_rg;
+    DBuf<float> gt_rt;
+    std::vector<i64> gt_seg;
     i64 lo(int bin, int color) const { return ptr[(size_t)bin * ncol + color]; }
     i64 hi(int bin, int color) const { return ptr[(size_t)bin * ncol + color + 1]; }
     i64 bin_lo(int bin) const { return ptr[(size_t)bin * ncol]; }
@@ -92,10 +98,21 @@ struct PhaseSetup {
     }
 };
 
+struct SweepArgs;
+// the bin-9 (giant vertex) fields of A for color class c of P
+void giant_args(const PhaseSetup &P, int c, SweepArgs &A);
+
+// The bin ranges every engine uses (process-wide; the autotuner's result).
+BinConfig &current_bins();
+
 class Engine {
 public:
-    explicit Engine(hipStream_t s) : s_(s) {}
+    explicit Engine(hipStream_t s) : s_(s), bins_(current_bins()) {}
     hipStream_t stream() const { return s_; }
+    void set_bins(const BinConfig &b) { bins_ = b; }
+    // Times one plain sweep of g over every vertex (best of `reps`) with the
+    // given bins, in ms.
+    double time_sweep(const DevGraph &g, const BinConfig &b, int reps);
 
     void setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetup &P);
 
@@ -125,11 +142,16 @@ public:
 
 private:
     hipStream_t s_;
+    BinConfig bins_;
     // small pinned-free scratch for scalars copied back to the host
     template <class T>
     T fetch(const T *dptr);
     double colored_modularity(const DevGraph &g, const PhaseSetup &P, const i64 *C,
                               double *part, double *res, double *exx = nullptr);
 };
+
+// Sweep bin ranges that time fastest on g (autotune.cpp); the default
+// ranges' and the result's sweep times in ms.
+BinConfig autotune_bins(Engine &E, const DevGraph &g, int reps, double *ms_default, double *ms_tuned);
 
 }  // namespace glv

@@ -18,8 +18,10 @@ enum KClass {
     K_RENUMBER = 12,
     K_PERSIST = 13,  // one cooperative launch per colored iteration of a small graph
     K_COLOR_TAIL = 14,  // sequential first-fit of the vertices JP rounds left
-    K_NUM = 15
+    K_SWEEP_GIANT = 15,  // sweep of degree bin 9 (giant vertices, several workgroups each)
+    K_NUM = 16
 };
+inline int sweep_class(int bin) { return bin <= 8 ? K_SWEEP_B1 + bin - 1 : K_SWEEP_GIANT; }
 
 struct KStat {
     double ms = 0;

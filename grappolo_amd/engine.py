@@ -85,6 +85,12 @@ _SIG = {
     "glv_profile_enable": [ctypes.c_int],
     "glv_profile_reset": [],
     "glv_profile_read": [ctypes.c_int, _F64P, _F64P, _I64P],
+    "glv_runMultiPhaseBasic_batch_dev": [ctypes.c_int64, _I64P, ctypes.c_int64, ctypes.c_int64, _VP, _VP,
+                                         _VP, _VP, ctypes.c_int, ctypes.c_double, _VP, ctypes.c_void_p],
+    "glv_get_bins": [_I64P],
+    "glv_set_bins": [_I64P],
+    "glv_autotune_bins_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, _I64P, _F64P,
+                              _F64P, _VP],
     "glv_mlp_param_count": [],
     "glv_mlp_forward_dev": [_VP, _VP, ctypes.c_int64, _VP, _VP],
     "glv_mlp_grad_dev": [_VP, _VP, _VP, ctypes.c_int64, ctypes.c_float, _VP, _VP],
@@ -102,7 +108,7 @@ _SIG = {
 KERNEL_CLASSES = ["sweep_bin1_deg4", "sweep_bin2_deg8", "sweep_bin3_deg16", "sweep_bin4_deg32",
                   "sweep_bin5_deg64", "sweep_bin6_wavehash", "sweep_bin7_blockhash",
                   "sweep_bin8_hbmhash", "modularity", "commit", "coloring_round", "coarsen",
-                  "renumber", "sweep_persistent_iteration", "coloring_tail"]
+                  "renumber", "sweep_persistent_iteration", "coloring_tail", "sweep_bin9_giant"]
 
 _fns = {}
 
@@ -387,6 +393,53 @@ def coarsen_dev(dg, C, k):
     for p in (po, pt, pw):
         _call("glv_dev_free", p)
     return Graph(off, tails[:nnz.value], w[:nnz.value], numEdges=ne.value)
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("final_modularity", _F64P), ("last_modularity", _F64P), ("phases", _I32P),
+                ("total_iterations", _I32P), ("final_clusters", _I64P), ("levels", ctypes.c_int32),
+                ("iterations", ctypes.c_int32), ("time_total", ctypes.c_double)]
+
+
+def run_basic_batch_dev(dg, vbase, C_orig, basicOpt=0, threshold=1e-9):
+    """runMultiPhaseBasic on every graph of a block-diagonal union at once
+    (``graph.union``); C_orig receives graph-local labels.  Returns per-graph
+    arrays and the batch's level and iteration counts."""
+    vb = np.ascontiguousarray(vbase, dtype=np.int64)
+    ng = len(vb) - 1
+    out = dict(final_modularity=np.zeros(ng), last_modularity=np.zeros(ng),
+               phases=np.zeros(ng, np.int32), total_iterations=np.zeros(ng, np.int32),
+               final_clusters=np.zeros(ng, np.int64))
+    st = BatchStats(_p(out["final_modularity"], _F64P), _p(out["last_modularity"], _F64P),
+                    _p(out["phases"], _I32P), _p(out["total_iterations"], _I32P),
+                    _p(out["final_clusters"], _I64P), 0, 0, 0.0)
+    _call("glv_runMultiPhaseBasic_batch_dev", ng, _p(vb, _I64P), *dg.ptrs(), _ptr(C_orig),
+          int(basicOpt), float(threshold), _stream(), ctypes.byref(st))
+    out.update(levels=st.levels, iterations=st.iterations, time_total=st.time_total)
+    return out
+
+
+def get_bins():
+    """Upper degree of each sweep bin (10 values) used by later calls."""
+    b = np.zeros(10, np.int64)
+    _call("glv_get_bins", _p(b, _I64P))
+    return b
+
+
+def set_bins(max_deg):
+    b = np.ascontiguousarray(max_deg, dtype=np.int64)
+    if b.shape != (10,):
+        raise ValueError("10 bin bounds expected")
+    _call("glv_set_bins", _p(b, _I64P))
+
+
+def autotune_bins_dev(dg, reps=3):
+    """(bins, ms with the default bins, ms with the returned bins) of one plain sweep of dg."""
+    b = np.zeros(10, np.int64)
+    t0, t1 = ctypes.c_double(0), ctypes.c_double(0)
+    _call("glv_autotune_bins_dev", *dg.ptrs(), int(reps), _p(b, _I64P), ctypes.byref(t0),
+          ctypes.byref(t1), _stream())
+    return b, t0.value, t1.value
 
 
 _hip_rt = None

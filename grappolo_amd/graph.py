@@ -86,6 +86,25 @@ class Graph:
         return f"Graph(numVertices={self.numVertices}, nnz={self.nnz}, numEdges={self.numEdges})"
 
 
+def union(graphs):
+    """Block-diagonal union of independent graphs: (Graph, vbase) where graph
+    i owns the vertices vbase[i]..vbase[i+1]-1 (the batched entry points'
+    input)."""
+    vb = np.zeros(len(graphs) + 1, dtype=np.int64)
+    eb = np.zeros(len(graphs) + 1, dtype=np.int64)
+    for i, g in enumerate(graphs):
+        vb[i + 1] = vb[i] + g.numVertices
+        eb[i + 1] = eb[i] + g.nnz
+    ptr = np.zeros(vb[-1] + 1, dtype=np.int64)
+    tails = np.empty(eb[-1], dtype=np.int64)
+    w = np.empty(eb[-1], dtype=np.float64)
+    for i, g in enumerate(graphs):
+        ptr[vb[i] + 1:vb[i + 1] + 1] = g.edgeListPtrs[1:] + eb[i]
+        tails[eb[i]:eb[i + 1]] = g.tails + vb[i]
+        w[eb[i]:eb[i + 1]] = g.weights
+    return Graph(ptr, tails, w, numEdges=sum(g.numEdges for g in graphs)), vb
+
+
 def _fn(name, *argtypes):
     f = getattr(host_lib(), name)
     f.argtypes = list(argtypes)

@@ -187,7 +187,8 @@ int glv_count_conflicts_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
  * 24*entries + 32*vertices per launch), 8 modularity pass, 9 commits,
  * 10 coloring rounds, 11 coarsening, 12 renumbering, 13 one whole colored
  * iteration of a small graph in a single cooperative launch, 14 the
- * sequential coloring tail. */
+ * sequential coloring tail, 15 sweep kernels of degree bin 9 (more than
+ * 16384 entries, several workgroups per vertex). */
 int glv_profile_enable(int on);
 int glv_profile_reset(void);
 int glv_profile_read(int kernel_class, double *ms, double *bytes, int64_t *launches);
@@ -246,6 +247,44 @@ int glv_runMultiPhaseColoring_las_dev(int64_t nv, int64_t nnz, const int64_t *d_
                                       double C_threshold, const float *d_params, float tau,
                                       int max_rechecks, void *stream, glv_stats *stats,
                                       glv_las_stats *las);
+
+/* ---- batched graphs (no counterpart in the reference) ---------------------
+ * runMultiPhaseBasic (runMultiPhaseBasic.cpp:52-170) on ngraphs independent
+ * graphs at once.  The graphs are one device-resident union CSR: graph g owns
+ * the vertices [vbase[g], vbase[g+1]) (vbase on the host, vbase[0] = 0,
+ * vbase[ngraphs] = nv) and no entry joins two graphs.  Each graph follows its
+ * own iterations, phases and termination exactly as a separate
+ * glv_runMultiPhaseBasic_dev call would; all graphs of a phase are swept by
+ * the same launches.  d_C_orig (nv) receives graph-local cluster ids.
+ * Per-graph results go to the caller's host arrays (each nullable, ngraphs
+ * entries). */
+typedef struct glv_batch_stats {
+    double *final_modularity;
+    double *last_modularity;
+    int32_t *phases;
+    int32_t *total_iterations;
+    int64_t *final_clusters;
+    int32_t levels;      /* phases of the longest-running graph */
+    int32_t iterations;  /* batched iterations over all levels */
+    double time_total;   /* seconds */
+} glv_batch_stats;
+int glv_runMultiPhaseBasic_batch_dev(int64_t ngraphs, const int64_t *vbase, int64_t nv, int64_t nnz,
+                                     const int64_t *d_offsets, const int64_t *d_tails,
+                                     const double *d_weights, int64_t *d_C_orig, int basicOpt,
+                                     double threshold, void *stream, glv_batch_stats *stats);
+
+/* ---- sweep bin ranges (no counterpart in the reference) ------------------
+ * max_deg[0..9]: upper degree of each bin of the sweep (bin 0 = no entries:
+ * 0; defaults 4, 8, 16, 32, 64, 256, 2048, 16384, INT64_MAX).  The upper ends
+ * of bins 1..6 may be lowered (never raised); bins 7..9 are fixed.
+ * glv_autotune_bins_dev times one plain sweep of the given graph per
+ * candidate and writes the fastest ranges (it does not apply them);
+ * glv_set_bins applies ranges to every later call in the process. */
+int glv_get_bins(int64_t *max_deg);
+int glv_set_bins(const int64_t *max_deg);
+int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                          const double *d_weights, int reps, int64_t *max_deg, double *ms_default,
+                          double *ms_tuned, void *stream);
 
 #ifdef __cplusplus
 }

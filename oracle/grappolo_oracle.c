@@ -212,11 +212,10 @@ uint64_t orc_jp_priority(i64 v) {
  * uncoloured vertex whose priority beats all uncoloured neighbours (self-loops
  * ignored) takes the smallest color absent from its coloured neighbours;
  * winners of one round are pairwise non-adjacent, so the order inside a round
- * does not matter.  Once a round colors fewer than ORC_JP_MIN_WON vertices,
- * or at most ORC_JP_TAIL remain uncoloured, the rest is colored first-fit one
- * vertex at a time in increasing id (the reference's one-thread order). */
+ * does not matter.  Once at most ORC_JP_TAIL vertices remain uncoloured, the
+ * rest is colored first-fit one vertex at a time in increasing id (the
+ * reference's one-thread order). */
 #define ORC_JP_TAIL 1024
-#define ORC_JP_MIN_WON 32
 int orc_color_jp(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int32_t *rounds_out) {
     unsigned char mark[ORC_MAX_COLORS];
     i64 *work = (i64 *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(i64));
@@ -272,7 +271,7 @@ int orc_color_jp(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int3
         }
         nwork = nrest;
         rounds++;
-        if (nwork > 0 && (nwork <= ORC_JP_TAIL || nwin < ORC_JP_MIN_WON)) break;
+        if (nwork > 0 && nwork <= ORC_JP_TAIL) break;
     }
     /* sequential first-fit of the remaining vertices in increasing id */
     for (i64 v = 0; v < nv && nwork > 0; v++) {

@@ -76,6 +76,8 @@ def catalog():
         "hubs": hubs(),
         # above the dense-accumulator limit: degree > 2048 uses the HBM hash
         "hubs_big": hubs(seed=8, nv=40000, hub_degrees=(3000, 9000, 2100, 700)),
+        # degree > 16384: several workgroups per vertex (sweep, modularity, coloring)
+        "hubs_giant": hubs(seed=9, nv=60000, hub_degrees=(20000, 31000, 17000, 5000)),
         "loops_isolated": with_loops_and_isolated(),
     }
 

@@ -45,6 +45,21 @@ def test_coloring_valid_and_equal_to_restated_jp(name):
     np.testing.assert_array_equal(colors, ref)
 
 
+@pytest.mark.parametrize("which", ["rmat19", "cliques"])
+def test_coloring_rounds_paths_equal_restated_jp(which):
+    """Above 2^18 vertices the rounds are launched from the host, below they
+    run inside one cooperative launch; both equal the restated coloring, on
+    a skewed graph (long priority chains through hubs) and a dense one."""
+    from graphs import cliques
+    from grappolo_amd.graph import gen_rmat
+    g = gen_rmat(19, seed=5) if which == "rmat19" else cliques()
+    colors, mc = engine.algoDistanceOneVertexColoringOpt(g)
+    ref, rmc, rounds = oracle.color_jp(g)
+    assert oracle.color_conflicts(g, colors) == 0
+    assert mc == rmc
+    np.testing.assert_array_equal(colors, ref)
+
+
 @pytest.mark.parametrize("name", NAMES)
 def test_modularity_of_labelling(name):
     g = GRAPHS[name]
