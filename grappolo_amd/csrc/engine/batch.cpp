@@ -26,6 +26,7 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 constexpr int kMaxPhaseIters = 100000;
+constexpr int kReplays = 2;  // iterations between two compactions of the sweep lists
 }  // namespace
 
 void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0, i64 *C_orig,
@@ -130,6 +131,13 @@ void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0,
             GLV_HIP(hipMemcpyAsync(hstart.data(), start.p, (nkeys + 1) * sizeof(i64), hipMemcpyDeviceToHost, s));
             GLV_HIP(hipStreamSynchronize(s));
             if (hstart[nkeys] == 0) break;  // every graph's phase has ended
+            // One iteration (sweeps, per-graph modularity, decisions, commits) is
+            // captured once per list and replayed kReplays times: a graph whose
+            // phase ends meanwhile is still swept (its targets are not used),
+            // but nothing of it is committed.
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            GLV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             SweepArgs A;
             A.off = G->off;
             A.tails = G->tails;
@@ -154,16 +162,22 @@ void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0,
                 A.count = cnt;
                 A.eii_part = part.p;
                 if (b == 9) giant_args(P, 0, A);
-                Profiler::get().begin(sweep_class(b), s);
                 launch_sweep(b, true, A, s);
-                Profiler::get().end(sweep_class(b), 0.0, s);
             }
             launch_isolated_targets(P.perm.p + P.bin_lo(0), P.bin_hi(0) - P.bin_lo(0), tgt, s);
             launch_graph_mod(d_vb.p, ng, d_active.p, eii.p, P.a.p, d_exx.p, d_a2.p, s);
             launch_batch_decide(ng, d_exx.p, d_a2.p, d_gconst.p, d_lower.p, p.threshold, kMaxPhaseIters,
                                 d_prev.p, d_curr.p, d_itrs.p, d_active.p, d_apply.p, s);
             launch_batch_apply(nv, gid.p, d_apply.p, tgt, curr, past, P.k.p, P.a.p, P.n.p, s);
-            r.iterations++;
+            GLV_HIP(hipStreamEndCapture(s, &graph));
+            GLV_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            for (int rep = 0; rep < kReplays; rep++) {
+                GLV_HIP(hipGraphLaunch(exec, s));
+                r.iterations++;
+            }
+            GLV_HIP(hipStreamSynchronize(s));
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
         }
         r.levels++;
         // per graph: the phase's value (prevMod at exit) and its iterations

@@ -1532,19 +1532,24 @@ __global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const
     }
 }
 
-// Jones-Plassmann of a small graph in one cooperative launch: the same
-// topological sweep as the host-driven rounds (count of higher-priority
-// uncoloured neighbours, frontier = count zero, first-fit against the
-// coloured higher neighbours), with a grid barrier between rounds instead of
-// a launch and a host round trip.  When at most tail_at vertices remain,
-// workgroup 0 colours them by sequential first-fit in increasing id (the
-// host-driven path's tail).  ring[0..2]: frontier counts of rounds r (mod 3),
-// zero on entry; ring[3] receives the number of rounds.
+// Jones-Plassmann rounds in one cooperative launch: the same topological
+// sweep as the host-driven rounds (count of higher-priority uncoloured
+// neighbours, frontier = count zero, first-fit against the coloured higher
+// neighbours), with a grid barrier between rounds instead of a launch and a
+// host round trip.  Frontier vertices of degree <= 256 take a wavefront each,
+// heavier ones a workgroup each.  do_count: start from scratch (counts and
+// the first frontier); otherwise continue from the frontier in sA/lA (counts
+// ring[0], ring[1]) after colored0 vertices were coloured.  When at most
+// tail_at vertices remain, workgroup 0 colours them by sequential first-fit
+// in increasing id (the host-driven path's tail).  ring: 8 words, [2q],
+// [2q+1] the small and large frontier counts of rounds r = q (mod 3), [6]
+// receives the rounds run here, [7] counts heavy vertices while counting.
 constexpr int kJpPBlock = 1024;
 __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, const i64 *tails, i64 nv,
-                                                             int32_t *colors, int32_t *cnt, i64 *listA,
-                                                             i64 *listB, i64 *ring, unsigned *bar,
-                                                             int32_t *err, i64 tail_at) {
+                                                             int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB,
+                                                             i64 *lA, i64 *lB, i64 *ring, unsigned *bar,
+                                                             int32_t *err, i64 tail_at, int do_count,
+                                                             i64 colored0) {
     constexpr int WPB = kJpPBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
     __shared__ i64 tail_list[kJpPBlock];
@@ -1554,57 +1559,66 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     uint32_t *mk = mark[w];
     unsigned seq = 0;
-    // counts: one thread per vertex up to 256 entries; heavier ones are listed
-    // in listB (ring[2]) and counted by a wavefront each
-    for (i64 base = (i64)blockIdx.x * kJpPBlock; base < nv; base += (i64)nb * kJpPBlock) {
-        const i64 v = base + threadIdx.x;
-        int c = 0;
-        bool heavy = false;
-        if (v < nv) {
-            const i64 b = off[v], e = off[v + 1];
-            if (e - b > kBinMaxDeg[6]) {
-                heavy = true;
-            } else {
+    bool alive = true;
+    if (do_count) {
+        // counts: one thread per vertex up to 256 entries; heavier ones are listed
+        // in lB (ring[7]) and counted by a wavefront each
+        for (i64 base = (i64)blockIdx.x * kJpPBlock; base < nv; base += (i64)nb * kJpPBlock) {
+            const i64 v = base + threadIdx.x;
+            int c = 0;
+            bool heavy = false;
+            if (v < nv) {
+                const i64 b = off[v], e = off[v + 1];
+                if (e - b > kBinMaxDeg[6]) {
+                    heavy = true;
+                } else {
+                    const u64 pv = jp_priority(v);
+                    for (i64 j = b; j < e; j++) {
+                        const i64 u = tails[j];
+                        if (u != v && jp_priority(u) > pv) c++;
+                    }
+                    cnt[v] = c;
+                }
+            }
+            wave_append(heavy, v, lB, &ring[7]);
+            wave_append(v < nv && !heavy && c == 0, v, sA, &ring[0]);
+        }
+        alive = grid_barrier(bar, nb, ++seq);
+        if (alive) {
+            const i64 nheavy = __atomic_load_n(&ring[7], __ATOMIC_RELAXED);
+            for (i64 q = (i64)blockIdx.x * WPB + w; q < nheavy; q += (i64)nb * WPB) {
+                const i64 v = lB[q];
                 const u64 pv = jp_priority(v);
-                for (i64 j = b; j < e; j++) {
+                int c = 0;
+                for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
                     const i64 u = tails[j];
                     if (u != v && jp_priority(u) > pv) c++;
                 }
-                cnt[v] = c;
+                for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
+                if (lane == 0) cnt[v] = c;
+                wave_append(lane == 0 && c == 0, v, lA, &ring[1]);
             }
+            alive = grid_barrier(bar, nb, ++seq);
         }
-        wave_append(heavy, v, listB, &ring[2]);
-        wave_append(v < nv && !heavy && c == 0, v, listA, &ring[0]);
     }
-    bool alive = grid_barrier(bar, nb, ++seq);
-    if (alive) {
-        const i64 nheavy = __atomic_load_n(&ring[2], __ATOMIC_RELAXED);
-        for (i64 q = (i64)blockIdx.x * WPB + w; q < nheavy; q += (i64)nb * WPB) {
-            const i64 v = listB[q];
-            const u64 pv = jp_priority(v);
-            int c = 0;
-            for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
-                const i64 u = tails[j];
-                if (u != v && jp_priority(u) > pv) c++;
-            }
-            for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
-            if (lane == 0) cnt[v] = c;
-            wave_append(lane == 0 && c == 0, v, listA, &ring[0]);
-        }
-        alive = grid_barrier(bar, nb, ++seq);
-    }
-    i64 colored = 0;
+    i64 colored = colored0;
     int r = 0;
-    i64 *front = listA, *next = listB;
+    i64 *fs = sA, *fl = lA, *ns_ = sB, *nl_ = lB;
     while (alive) {
-        const i64 nfront = __atomic_load_n(&ring[r % 3], __ATOMIC_RELAXED);
-        if (nfront == 0) break;
-        if (blockIdx.x == 0 && threadIdx.x == 0) __atomic_store_n(&ring[(r + 2) % 3], (i64)0, __ATOMIC_RELAXED);
-        i64 *ncount = &ring[(r + 1) % 3];
-        for (i64 base = (i64)blockIdx.x * WPB; base < nfront; base += (i64)nb * WPB) {
-            const i64 q = base + w;
-            const bool in = q < nfront;
-            const i64 v = in ? front[q] : 0;
+        const int q = r % 3, nq = (r + 1) % 3, zq = (r + 2) % 3;
+        const i64 nsmall = __atomic_load_n(&ring[2 * q], __ATOMIC_RELAXED);
+        const i64 nlarge = __atomic_load_n(&ring[2 * q + 1], __ATOMIC_RELAXED);
+        if (nsmall + nlarge == 0) break;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            __atomic_store_n(&ring[2 * zq], (i64)0, __ATOMIC_RELAXED);
+            __atomic_store_n(&ring[2 * zq + 1], (i64)0, __ATOMIC_RELAXED);
+        }
+        i64 *cs = &ring[2 * nq], *cl = &ring[2 * nq + 1];
+        // small frontier vertices: a wavefront each
+        for (i64 base = (i64)blockIdx.x * WPB; base < nsmall; base += (i64)nb * WPB) {
+            const i64 qq = base + w;
+            const bool in = qq < nsmall;
+            const i64 v = in ? fs[qq] : 0;
             const i64 b = in ? off[v] : 0, e = in ? off[v + 1] : 0;
             for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
             __syncthreads();
@@ -1645,12 +1659,57 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                     u = tails[b + j];
                     if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
                 }
-                wave_append(ready, u, next, ncount);
+                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
+                wave_append(ready && !big, u, ns_, cs);
+                wave_append(big, u, nl_, cl);
+            }
+            __syncthreads();
+        }
+        // large frontier vertices: a workgroup each
+        for (i64 qq = blockIdx.x; qq < nlarge; qq += nb) {
+            const i64 v = fl[qq];
+            const i64 b = off[v], e = off[v + 1];
+            for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
+            if (threadIdx.x == 0) s_color = kMaxColors;
+            __syncthreads();
+            for (i64 j = b + threadIdx.x; j < e; j += kJpPBlock) {
+                const i64 u = tails[j];
+                if (u == v) continue;
+                const int c = colors[u];
+                if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
+            }
+            __syncthreads();
+            if (threadIdx.x < kMaxColors / 32) {
+                const uint32_t m = mark[0][threadIdx.x];
+                if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int color = s_color;
+                if (color >= kMaxColors) {
+                    atomicOr(err, 1);
+                    color = 0;
+                }
+                colors[v] = color;
+            }
+            const u64 pv = jp_priority(v);
+            const i64 deg = e - b;
+            for (i64 j0 = 0; j0 < deg; j0 += kJpPBlock) {
+                const i64 j = j0 + threadIdx.x;
+                bool ready = false;
+                i64 u = 0;
+                if (j < deg) {
+                    u = tails[b + j];
+                    if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+                }
+                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
+                wave_append(ready && !big, u, ns_, cs);
+                wave_append(big, u, nl_, cl);
             }
             __syncthreads();
         }
         alive = grid_barrier(bar, nb, ++seq);
-        colored += nfront;
+        colored += nsmall + nlarge;
         r++;
         const i64 rest = nv - colored;
         if (rest == 0) break;
@@ -1671,8 +1730,8 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 if (threadIdx.x == 0) atomicOr(err, 2);
                 break;
             }
-            for (int q = 0; q < n; q++) {
-                const i64 v = tail_list[q];
+            for (int qq = 0; qq < n; qq++) {
+                const i64 v = tail_list[qq];
                 for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
                 if (threadIdx.x == 0) s_color = kMaxColors;
                 __syncthreads();
@@ -1701,11 +1760,14 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             }
             break;
         }
-        i64 *t = front;
-        front = next;
-        next = t;
+        i64 *t = fs;
+        fs = ns_;
+        ns_ = t;
+        t = fl;
+        fl = nl_;
+        nl_ = t;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ring[3] = r;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ring[6] = r;
 }
 
 __global__ void k_jp_finish(const int32_t *colors, i64 nv, int32_t *maxc) {
@@ -2189,12 +2251,14 @@ int jp_persistent_blocks() {
     }
     return cached;
 }
-bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *listA, i64 *listB,
-                          i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, int nblocks, hipStream_t s) {
+bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
+                          i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
+                          i64 colored0, int nblocks, hipStream_t s) {
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0) return false;
     const i64 *off = g.off, *tails = g.tails;
-    i64 nv = g.nv, ta = tail_at;
-    void *args[] = {&off, &tails, &nv, &colors, &cnt, &listA, &listB, &ring, &bar, &err, &ta};
+    i64 nv = g.nv, ta = tail_at, c0 = colored0;
+    int dc = count ? 1 : 0;
+    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0};
     hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_persistent, dim3(nblocks), dim3(kJpPBlock),
                                               args, 0, s);
     if (e != hipSuccess) {

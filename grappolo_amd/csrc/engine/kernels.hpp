@@ -148,13 +148,18 @@ void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
 void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
                        int round, int32_t *err, hipStream_t s);
 constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many uncoloured vertices
-// the whole coloring of a graph of at most kPersistMaxNV vertices in one
-// cooperative launch (same result as the host-driven rounds); false when the
-// launch is refused.  ring: 4 zeroed words, ring[3] receives the rounds;
-// bar: kBarrierWords zeroed words; err: 1 too many colors.
+// Jones-Plassmann rounds in one cooperative launch (same result as the
+// host-driven rounds): from scratch (count) or continuing from the frontier
+// in sA (small, count ring[0]) and lA (large, ring[1]) after colored0
+// vertices; false when the launch is refused.  ring: 8 words (zero except the
+// frontier counts), ring[6] receives the rounds; bar: kBarrierWords zeroed
+// words; err: 1 too many colors.  sA..lB: nv each.
 int jp_persistent_blocks();
-bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *listA, i64 *listB,
-                          i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, int nblocks, hipStream_t s);
+bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
+                          i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
+                          i64 colored0, int nblocks, hipStream_t s);
+// frontiers below this many vertices finish in one cooperative launch
+constexpr i64 kJpPersistFrontier = 16384;
 void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);
 
 // --- renumber / compose --------------------------------------------------

@@ -143,78 +143,89 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     const i64 nv = g.nv;
     if (nv == 0) return -1;
     launch_fill_i32(colors, nv, -1, s_);
-    if (nv <= kPersistMaxNV) {
-        // small graph: every round inside one cooperative launch
-        int nb = jp_persistent_blocks();
-        nb = (int)std::min<i64>({(i64)nb, kPersistMaxBlocks, std::max<i64>(4, nv / 2048)});
-        DBuf<i64> lA(nv, s_), lB(nv, s_), ring(4, s_);
-        DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
-        DBuf<unsigned> bar(kBarrierWords, s_);
-        launch_fill_i64(ring.p, 4, 0, s_);
-        launch_fill_i32(err.p, 1, 0, s_);
-        launch_fill_i32(maxc.p, 1, -1, s_);
-        launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
-        Profiler::get().begin(K_COLOR, s_);
-        const bool ok = nb > 0 && launch_jp_persistent(g, colors, cnt.p, lA.p, lB.p, ring.p, bar.p, err.p,
-                                                       kJpTail, nb, s_);
-        Profiler::get().end(K_COLOR, 0.0, s_);
-        if (ok) {
-            launch_jp_finish(colors, nv, maxc.p, s_);
-            i64 rounds = 0;
-            unsigned abort_flag = 0;
-            int32_t e = 0, mc = -1;
-            GLV_HIP(hipMemcpyAsync(&rounds, ring.p + 3, sizeof(i64), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipMemcpyAsync(&mc, maxc.p, sizeof(mc), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipStreamSynchronize(s_));
-            if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
-            if (e & 1) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
-            if (e) fail(GLV_E_DEVICE, "persistent coloring tail overflow");
-            if (rounds_out) *rounds_out = (int)rounds;
-            return mc;
-        }
-        launch_fill_i32(colors, nv, -1, s_);  // launch refused: host-driven rounds below
-    }
-    DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
+    const int pnb = (int)std::min<i64>({(i64)jp_persistent_blocks(), kPersistMaxBlocks,
+                                         std::max<i64>(4, nv / 2048)});
+    DBuf<i64> ring(8, s_);
+    DBuf<unsigned> bar(kBarrierWords, s_);
     DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
-    DBuf<uint32_t> gmark;
-    launch_fill_i64(counts.p, 8, 0, s_);
     launch_fill_i32(err.p, 1, 0, s_);
     launch_fill_i32(maxc.p, 1, -1, s_);
-    JpLists cur{sA.p, lA.p, gA.p, counts.p}, next{sB.p, lB.p, gB.p, counts.p + 4};
-    Profiler::get().begin(K_COLOR, s_);
-    launch_jp_count(g, cnt.p, cur, sB.p, counts.p + 3, s_);  // heavy list in the next round's buffer
-    Profiler::get().end(K_COLOR, 0.0, s_);
-    i64 hc[3];
-    GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
-    GLV_HIP(hipStreamSynchronize(s_));
-    i64 colored = 0;
-    int round = 0;
-    while (hc[0] + hc[1] + hc[2] > 0) {
-        if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
-        const i64 won = hc[0] + hc[1] + hc[2];
-        if (hc[2] > 0 && (i64)gmark.n < hc[2] * (kMaxColors / 32)) gmark.alloc(hc[2] * (kMaxColors / 32), s_);
-        launch_fill_i64(next.counts, 4, 0, s_);
+    // the rounds run in one cooperative launch once the frontier no longer fills the GPU
+    auto persistent = [&](i64 *sA, i64 *sB, i64 *lA, i64 *lB, const i64 n0[2], bool count, i64 colored0,
+                          int *rounds) -> bool {
+        i64 h[8] = {n0[0], n0[1], 0, 0, 0, 0, 0, 0};
+        GLV_HIP(hipMemcpyAsync(ring.p, h, sizeof(h), hipMemcpyHostToDevice, s_));
+        launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
-        launch_jp_frontier(g, colors, cnt.p, cur, hc, next, gmark.p, err.p, s_);
+        const bool ok = pnb > 0 && launch_jp_persistent(g, colors, cnt.p, sA, sB, lA, lB, ring.p, bar.p, err.p,
+                                                        kJpTail, count, colored0, pnb, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
-        GLV_HIP(hipMemcpyAsync(hc, next.counts, sizeof(hc), hipMemcpyDeviceToHost, s_));
+        unsigned abort_flag = 0;
+        i64 r = 0;
+        GLV_HIP(hipMemcpyAsync(&r, ring.p + 6, sizeof(i64), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipStreamSynchronize(s_));  // `h` is pageable host memory
+        if (!ok) return false;
+        if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
+        *rounds += (int)r;
+        return true;
+    };
+    int round = 0;
+    DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
+    bool done = false;
+    if (nv <= kPersistMaxNV) {
+        const i64 z[2] = {0, 0};
+        done = persistent(sA.p, sB.p, lA.p, lB.p, z, true, 0, &round);
+        if (!done) launch_fill_i32(colors, nv, -1, s_);  // launch refused: host-driven rounds below
+    }
+    if (!done) {
+        DBuf<uint32_t> gmark;
+        launch_fill_i64(counts.p, 8, 0, s_);
+        JpLists cur{sA.p, lA.p, gA.p, counts.p}, next{sB.p, lB.p, gB.p, counts.p + 4};
+        Profiler::get().begin(K_COLOR, s_);
+        launch_jp_count(g, cnt.p, cur, sB.p, counts.p + 3, s_);  // heavy list in the next round's buffer
+        Profiler::get().end(K_COLOR, 0.0, s_);
+        i64 hc[3];
+        GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
         GLV_HIP(hipStreamSynchronize(s_));
-        colored += won;
-        std::swap(cur, next);
-        round++;
-        const i64 rest = nv - colored;
-        if (rest > 0 && rest <= kJpTail) {
-            // the last vertices one at a time (scratch: lists not holding a frontier)
-            Profiler::get().begin(K_COLOR_TAIL, s_);
-            launch_color_tail(g, colors, next.small, next.large, next.giant, rest, round, err.p, s_);
-            Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
-            break;
+        i64 colored = 0;
+        while (hc[0] + hc[1] + hc[2] > 0) {
+            if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
+            const i64 won = hc[0] + hc[1] + hc[2];
+            if (won < kJpPersistFrontier && colored > 0) {
+                // hand the remaining rounds to one cooperative launch: giant
+                // frontier vertices join the large list
+                if (hc[2] > 0)
+                    GLV_HIP(hipMemcpyAsync(cur.large + hc[1], cur.giant, hc[2] * sizeof(i64),
+                                           hipMemcpyDeviceToDevice, s_));
+                hc[1] += hc[2];
+                hc[2] = 0;
+                if (persistent(cur.small, next.small, cur.large, next.large, hc, false, colored, &round)) break;
+            }
+            if (hc[2] > 0 && (i64)gmark.n < hc[2] * (kMaxColors / 32)) gmark.alloc(hc[2] * (kMaxColors / 32), s_);
+            launch_fill_i64(next.counts, 4, 0, s_);
+            Profiler::get().begin(K_COLOR, s_);
+            launch_jp_frontier(g, colors, cnt.p, cur, hc, next, gmark.p, err.p, s_);
+            Profiler::get().end(K_COLOR, 0.0, s_);
+            GLV_HIP(hipMemcpyAsync(hc, next.counts, sizeof(hc), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipStreamSynchronize(s_));
+            colored += won;
+            std::swap(cur, next);
+            round++;
+            const i64 rest = nv - colored;
+            if (rest > 0 && rest <= kJpTail) {
+                // the last vertices one at a time (scratch: lists not holding a frontier)
+                Profiler::get().begin(K_COLOR_TAIL, s_);
+                launch_color_tail(g, colors, next.small, next.large, next.giant, rest, round, err.p, s_);
+                Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
+                break;
+            }
         }
     }
     launch_jp_finish(colors, nv, maxc.p, s_);
-    if (fetch(err.p)) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
+    const int32_t e = fetch(err.p);
+    if (e & 1) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
+    if (e) fail(GLV_E_DEVICE, "coloring tail overflow");
     if (rounds_out) *rounds_out = round;
     return fetch(maxc.p);
 }
