@@ -317,6 +317,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="do not time kernels with HIP events in the timed region")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy a MIB-sized device buffer (a known-traffic kernel for counters)")
     ap.add_argument("--learned", action="store_true",
                     help="learned active-set pruning vs the plain form (held-out LFR graph)")
     ap.add_argument("--las-train-graphs", type=int, default=32)
@@ -350,6 +352,12 @@ def main():
         run_batched(args)
         return
     torch.cuda.set_device(0)
+    if args.calibrate_copy > 0:
+        src = torch.ones(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     spec = WORKLOADS[args.workload]
     t = time.perf_counter()
     g = make_graph(spec)
