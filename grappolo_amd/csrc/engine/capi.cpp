@@ -392,10 +392,9 @@ int glv_renumberClustersContiguously(int64_t *C, int64_t size, int64_t *numClust
     });
 }
 
-int glv_buildNextLevelGraphOpt(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
-                               const int64_t *C, int64_t numUniqueClusters,
-                               int64_t **outEdgeListPtrs, glv_edge **outEdgeList,
-                               int64_t *outNumEdges) {
+static int coarsen_host(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList, const int64_t *C,
+                        int64_t numUniqueClusters, int64_t **outEdgeListPtrs, glv_edge **outEdgeList,
+                        int64_t *outNumEdges, bool vf) {
     return guarded([&] {
         if (!outEdgeListPtrs || !outEdgeList || (nv > 0 && !C))
             glv::fail(GLV_E_INVALID_ARG, "null argument");
@@ -406,7 +405,7 @@ int glv_buildNextLevelGraphOpt(int64_t nv, const int64_t *edgeListPtrs, const gl
         if (nv > 0) GLV_HIP(hipMemcpyAsync(dC.p, C, nv * sizeof(i64), hipMemcpyHostToDevice, s));
         check_labels(g, dC.p, numUniqueClusters, s);
         Engine E(s);
-        E.coarsen(g, dC.p, numUniqueClusters, out);
+        E.coarsen(g, dC.p, numUniqueClusters, out, vf);
         const i64 k = out.nv, tot = out.nnz;
         int64_t *hp = (int64_t *)malloc((k + 1) * sizeof(int64_t));
         glv_edge *he = (glv_edge *)malloc((tot > 0 ? tot : 1) * sizeof(glv_edge));
@@ -428,6 +427,37 @@ int glv_buildNextLevelGraphOpt(int64_t nv, const int64_t *edgeListPtrs, const gl
         *outEdgeListPtrs = hp;
         *outEdgeList = he;
         if (outNumEdges) *outNumEdges = out.num_edges;
+    });
+}
+
+int glv_buildNextLevelGraphOpt(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                               const int64_t *C, int64_t numUniqueClusters,
+                               int64_t **outEdgeListPtrs, glv_edge **outEdgeList,
+                               int64_t *outNumEdges) {
+    return coarsen_host(nv, edgeListPtrs, edgeList, C, numUniqueClusters, outEdgeListPtrs, outEdgeList,
+                        outNumEdges, false);
+}
+
+int glv_buildNewGraphVF(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList, const int64_t *C,
+                        int64_t numUniqueClusters, int64_t **outEdgeListPtrs, glv_edge **outEdgeList,
+                        int64_t *outNumEdges) {
+    return coarsen_host(nv, edgeListPtrs, edgeList, C, numUniqueClusters, outEdgeListPtrs, outEdgeList,
+                        outNumEdges, true);
+}
+
+int glv_vertexFollowing(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList, int64_t *C,
+                        int64_t *numNode) {
+    return guarded([&] {
+        if (nv > 0 && !C) glv::fail(GLV_E_INVALID_ARG, "null argument");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        DBuf<i64> dC(nv, s);
+        Engine E(s);
+        const i64 n = E.vertex_following(g, dC.p);
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(C, dC.p, nv * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        if (numNode) *numNode = n;
     });
 }
 
@@ -596,6 +626,78 @@ int glv_coarsen_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int
         out.off_buf.p = nullptr;
         out.tails_buf.p = nullptr;
         out.w_buf.p = nullptr;
+        if (out_nnz) *out_nnz = out.nnz;
+        if (out_num_edges) *out_num_edges = out.num_edges;
+    });
+}
+
+int glv_vertexFollowing_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                            int64_t *d_C, int64_t *numNode, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, nullptr);
+        Engine E(s);
+        const i64 n = E.vertex_following(g, d_C);
+        if (numNode) *numNode = n;
+    });
+}
+
+int glv_buildNewGraphVF_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                            const double *d_weights, const int64_t *d_C, int64_t k, int64_t **d_out_offsets,
+                            int64_t **d_out_tails, double **d_out_weights, int64_t *out_nnz,
+                            int64_t *out_num_edges, void *stream) {
+    return guarded([&] {
+        if (!d_out_offsets || !d_out_tails || !d_out_weights) glv::fail(GLV_E_INVALID_ARG, "null output");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights), out;
+        check_labels(g, d_C, k, s);
+        Engine E(s);
+        E.coarsen(g, d_C, k, out, true);
+        E.sync();
+        *d_out_offsets = out.off_buf.p;
+        *d_out_tails = out.tails_buf.p;
+        *d_out_weights = out.w_buf.p;
+        out.off_buf.p = nullptr;
+        out.tails_buf.p = nullptr;
+        out.w_buf.p = nullptr;
+        if (out_nnz) *out_nnz = out.nnz;
+        if (out_num_edges) *out_num_edges = out.num_edges;
+    });
+}
+
+int glv_vertexFollowingPrepass_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                                   const double *d_weights, int64_t **d_out_offsets, int64_t **d_out_tails,
+                                   double **d_out_weights, int64_t *out_nv, int64_t *out_nnz,
+                                   int64_t *out_num_edges, int64_t *numNode, void *stream) {
+    return guarded([&] {
+        if (!d_out_offsets || !d_out_tails || !d_out_weights || !out_nv)
+            glv::fail(GLV_E_INVALID_ARG, "null output");
+        *d_out_offsets = nullptr;
+        *d_out_tails = nullptr;
+        *d_out_weights = nullptr;
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights), out;
+        Engine E(s);
+        DBuf<i64> C(std::max<i64>(nv, 1), s);
+        const i64 n = E.vertex_following(g, C.p);
+        if (numNode) *numNode = n;
+        *out_nv = nv;
+        if (out_nnz) *out_nnz = nnz;
+        if (out_num_edges) *out_num_edges = nnz / 2;
+        if (n == 0) return;  // graph unchanged (driverForGraphClustering.cpp:118)
+        const i64 k = E.renumber(C.p, nv);
+        E.coarsen(g, C.p, k, out, true);
+        E.sync();
+        *d_out_offsets = out.off_buf.p;
+        *d_out_tails = out.tails_buf.p;
+        *d_out_weights = out.w_buf.p;
+        out.off_buf.p = nullptr;
+        out.tails_buf.p = nullptr;
+        out.w_buf.p = nullptr;
+        *out_nv = k;
         if (out_nnz) *out_nnz = out.nnz;
         if (out_num_edges) *out_num_edges = out.num_edges;
     });
@@ -963,6 +1065,30 @@ long renumberClustersContiguously(long *C, long size) {
     int rc = glv_renumberClustersContiguously((int64_t *)C, size, &k);
     if (rc) die("renumberClustersContiguously", rc);
     return k;
+}
+
+long vertexFollowing(graph *G, long *C) {
+    int64_t n = 0;
+    int rc = glv_vertexFollowing(G->numVertices, G->edgeListPtrs, (const glv_edge *)G->edgeList,
+                                 (int64_t *)C, &n);
+    if (rc) die("vertexFollowing", rc);
+    return n;
+}
+
+double buildNewGraphVF(graph *Gin, graph *Gout, long *C, long numUniqueClusters) {
+    int64_t *ptrs = nullptr, ne = 0;
+    glv_edge *edges = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = glv_buildNewGraphVF(Gin->numVertices, Gin->edgeListPtrs, (const glv_edge *)Gin->edgeList,
+                                 (const int64_t *)C, numUniqueClusters, &ptrs, &edges, &ne);
+    auto t1 = std::chrono::steady_clock::now();
+    if (rc) die("buildNewGraphVF", rc);
+    Gout->numVertices = numUniqueClusters;
+    Gout->sVertices = numUniqueClusters;
+    Gout->numEdges = ne;
+    Gout->edgeListPtrs = (long *)ptrs;
+    Gout->edgeList = (edge *)edges;
+    return std::chrono::duration<double>(t1 - t0).count();
 }
 
 double buildNextLevelGraphOpt(graph *Gin, graph *Gout, long *C, long numUniqueClusters, int nThreads) {

@@ -1837,13 +1837,16 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_count(const i64 *off, const 
         if (lane == 0) cnt[v] = c;
     }
 }
+// nseeds: k (buildNextLevelGraphOpt: one zero self-loop per cluster) or 0
+// (buildNewGraphVF, vertexFollowing.cpp:127-131, which also sums weights
+// truncated to integers, :147-151: trunc).
 __global__ __launch_bounds__(kBlock) void k_coarsen_emit(const i64 *off, const i64 *tails,
                                                       const double *w, const i64 *C, i64 nv,
                                                       const i64 *pos, i64 k, int kb, u64 *keys,
-                                                      double *vals) {
+                                                      double *vals, i64 nseeds, bool trunc) {
     const int lane = threadIdx.x % kWave;
     // seeds: one zero self-loop per cluster (buildNextPhase.cpp:122)
-    for (i64 c = blockIdx.x * (i64)kBlock + threadIdx.x; c < k; c += (i64)gridDim.x * kBlock) {
+    for (i64 c = blockIdx.x * (i64)kBlock + threadIdx.x; c < nseeds; c += (i64)gridDim.x * kBlock) {
         keys[c] = ((u64)c << kb) | (u64)c;
         vals[c] = 0.0;
     }
@@ -1852,7 +1855,7 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_emit(const i64 *off, const i
          v += (i64)gridDim.x * (kBlock / kWave)) {
         const i64 cv = C[v];
         if (cv < 0) continue;
-        i64 base = k + pos[v];
+        i64 base = nseeds + pos[v];
         for (i64 j0 = off[v]; j0 < off[v + 1]; j0 += kWave) {
             const i64 j = j0 + lane;
             i64 ct = -1;
@@ -1865,7 +1868,7 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_emit(const i64 *off, const i
             if (p) {
                 const i64 at = base + __popcll(m & lt);
                 keys[at] = ((u64)cv << kb) | (u64)ct;
-                vals[at] = w[j];
+                vals[at] = trunc ? (double)(long)w[j] : w[j];
             }
             base += __popcll(m);
         }
@@ -1874,7 +1877,7 @@ __global__ __launch_bounds__(kBlock) void k_coarsen_emit(const i64 *off, const i
 // Each unique (a,b) becomes the entry of row a, and of row b when a != b;
 // the second slot of a self-loop gets an all-ones key that sorts last.
 __global__ void k_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, int kb, u64 *keys,
-                                 double *vals) {
+                                 double *vals, unsigned long long *nself) {
     const u64 mask = ((u64)1 << kb) - 1;
     for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < nu; q += (i64)gridDim.x * blockDim.x) {
         const u64 key = ukeys[q];
@@ -1887,6 +1890,7 @@ __global__ void k_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, 
         } else {
             keys[2 * q + 1] = ~0ULL;
             vals[2 * q + 1] = 0.0;
+            if (nself) atomicAdd(nself, 1ULL);
         }
     }
 }
@@ -1898,9 +1902,37 @@ __global__ void k_coarsen_rows(const u64 *keys, const double *vals, i64 total, i
         const i64 r = (i64)(key >> kb);
         tails[i] = (i64)(key & mask);
         w[i] = vals[i];
-        if (i == 0 || (i64)(keys[i - 1] >> kb) != r) off[r] = i;
-        if (i == 0) off[k] = total;
+        // row starts, empty rows included (the contraction without seeds can
+        // leave a cluster without entries): every off[] slot has one writer
+        const i64 rp = i == 0 ? -1 : (i64)(keys[i - 1] >> kb);
+        for (i64 rr = rp + 1; rr <= r; ++rr) off[rr] = i;
+        if (i == total - 1)
+            for (i64 rr = r + 1; rr <= k; ++rr) off[rr] = total;
     }
+}
+
+// vertexFollowing (Utility/vertexFollowing.cpp:46-89): -1 for vertices
+// without entries; a vertex with one entry follows its neighbour when that
+// neighbour has more than one entry or a smaller id; *count += both kinds.
+__global__ void k_vertex_following(const i64 *off, const i64 *tails, i64 nv, i64 *C,
+                                   unsigned long long *count) {
+    unsigned long long local = 0;
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
+        const i64 a = off[i], b = off[i + 1];
+        i64 c = i;
+        if (a == b) {
+            c = -1;
+            local++;
+        } else if (b - a == 1) {
+            const i64 t = tails[a];
+            if (off[t + 1] - off[t] > 1 || i > t) {
+                c = t;
+                local++;
+            }
+        }
+        C[i] = c;
+    }
+    if (local) atomicAdd(count, local);
 }
 
 // ----------------------------------------------------------------- AoS/checks
@@ -2307,28 +2339,38 @@ void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *cnt, hipStream_t
     GLV_LAUNCH_CHECK();
 }
 void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *pos, i64 k, int kb, u64 *keys,
-                         double *vals, hipStream_t s) {
+                         double *vals, hipStream_t s, bool vf) {
     const i64 vpb = kBlock / kWave;
     i64 blocks = std::max<i64>((g.nv + vpb - 1) / vpb, (k + kBlock - 1) / kBlock);
     blocks = std::max<i64>(1, std::min<i64>(blocks, 16384));
     k_coarsen_emit<<<(unsigned)blocks, kBlock, 0, s>>>(g.off, g.tails, g.w, C, g.nv, pos, k, kb, keys,
-                                                       vals);
+                                                       vals, vf ? 0 : k, vf);
     GLV_LAUNCH_CHECK();
 }
 void launch_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, int kb, u64 *keys, double *vals,
-                           hipStream_t s) {
+                           hipStream_t s, i64 *nself) {
     if (nu <= 0) return;
-    k_coarsen_mirror<<<grid_for(nu, kBlock, 8192), kBlock, 0, s>>>(ukeys, uvals, nu, kb, keys, vals);
+    k_coarsen_mirror<<<grid_for(nu, kBlock, 8192), kBlock, 0, s>>>(ukeys, uvals, nu, kb, keys, vals,
+                                                                   (unsigned long long *)nself);
     GLV_LAUNCH_CHECK();
 }
 void launch_coarsen_rows(const u64 *keys, const double *vals, i64 total, i64 k, int kb, i64 *off,
                          i64 *tails, double *w, hipStream_t s) {
-    if (total <= 0) return;
+    if (total <= 0) {
+        launch_fill_i64(off, k + 1, 0, s);
+        return;
+    }
     k_coarsen_rows<<<grid_for(total, kBlock, 8192), kBlock, 0, s>>>(keys, vals, total, k, kb, off, tails,
                                                                     w);
     GLV_LAUNCH_CHECK();
 }
 
+void launch_vertex_following(const DevGraph &g, i64 *C, i64 *count, hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_vertex_following<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.nv, C,
+                                                                       (unsigned long long *)count);
+    GLV_LAUNCH_CHECK();
+}
 void launch_split_edges(const void *aos, i64 nnz, i64 *tails, double *w, hipStream_t s) {
     if (nnz <= 0) return;
     k_split_edges<<<grid_for(nnz, kBlock, 8192), kBlock, 0, s>>>((const i64 *)aos, nnz, tails, w);

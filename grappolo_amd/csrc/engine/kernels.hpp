@@ -170,10 +170,14 @@ void launch_compose(i64 *C_orig, i64 n0, const i64 *C, bool first_phase, hipStre
 
 // --- coarsen -------------------------------------------------------------
 void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *cnt, hipStream_t s);
+// vf: no seeded self-loops and integer-truncated weights (buildNewGraphVF)
 void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *pos, i64 k, int kb,
-                         u64 *keys, double *vals, hipStream_t s);
+                         u64 *keys, double *vals, hipStream_t s, bool vf = false);
+// nself (nullable, zeroed): += number of self-loop keys
 void launch_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, int kb, u64 *keys,
-                           double *vals, hipStream_t s);
+                           double *vals, hipStream_t s, i64 *nself = nullptr);
+// vertexFollowing: C[i] and *count (zeroed) += vertices to remove
+void launch_vertex_following(const DevGraph &g, i64 *C, i64 *count, hipStream_t s);
 void launch_coarsen_rows(const u64 *keys, const double *vals, i64 total, i64 k, int kb, i64 *off,
                          i64 *tails, double *w, hipStream_t s);
 

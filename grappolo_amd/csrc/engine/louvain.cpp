@@ -691,7 +691,22 @@ i64 Engine::renumber(i64 *C, i64 n) {
 // ------------------------------------------------------------------ coarsen
 // buildNextLevelGraphOpt, Utility/buildNextPhase.cpp:83-255, as sort plus
 // segmented reduce.  Output rows are sorted by tail.
-void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out) {
+i64 Engine::vertex_following(const DevGraph &g, i64 *C) {
+    DBuf<i64> cnt(1, s_);
+    launch_fill_i64(cnt.p, 1, 0, s_);
+    launch_vertex_following(g, C, cnt.p, s_);
+    return fetch(cnt.p);
+}
+
+void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out, bool vf) {
+    if (vf && k == 0) {
+        // every vertex isolated and labelled -1: buildNewGraphVF yields an empty graph
+        out.nv = out.nnz = out.num_edges = 0;
+        out.off_buf.alloc(1, s_);
+        launch_fill_i64(out.off_buf.p, 1, 0, s_);
+        out.adopt();
+        return;
+    }
     if (k <= 0) fail(GLV_E_INVALID_ARG, "coarsening needs at least one cluster");
     const int kb = bits_for(k);
     if (2 * kb > 64) fail(GLV_E_UNSUPPORTED, "more than 2^32 clusters");
@@ -708,10 +723,10 @@ void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out) {
         GLV_HIP(hipStreamSynchronize(s_));
         m = h[0] + h[1];
     }
-    const i64 tot = m + k;
+    const i64 tot = m + (vf ? 0 : k);
     DBuf<u64> keys(tot, s_);
     DBuf<double> vals(tot, s_);
-    launch_coarsen_emit(g, C, pos.p, k, kb, keys.p, vals.p, s_);
+    launch_coarsen_emit(g, C, pos.p, k, kb, keys.p, vals.p, s_, vf);
     cnt.release();
     pos.release();
     sort_pairs_u64_f64(keys.p, vals.p, tot, 2 * kb, s_);
@@ -722,11 +737,14 @@ void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out) {
     vals.release();
     DBuf<u64> k2(2 * nu, s_);
     DBuf<double> v2(2 * nu, s_);
-    launch_coarsen_mirror(ukeys.p, usums.p, nu, kb, k2.p, v2.p, s_);
+    DBuf<i64> nself(1, s_);
+    launch_fill_i64(nself.p, 1, 0, s_);
+    launch_coarsen_mirror(ukeys.p, usums.p, nu, kb, k2.p, v2.p, s_, nself.p);
     ukeys.release();
     usums.release();
     sort_pairs_u64_f64(k2.p, v2.p, 2 * nu, 2 * kb, s_);
-    const i64 total = 2 * nu - k;
+    // self-loop entries appear once (k of them with the seeds)
+    const i64 total = 2 * nu - (vf ? fetch(nself.p) : k);
     out.nv = k;
     out.nnz = total;
     out.num_edges = nu;  // NE_out + k (buildNextPhase.cpp:237)

@@ -131,7 +131,11 @@ public:
     i64 color_step(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, int which);
 
     i64 renumber(i64 *C, i64 n);
-    void coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out);
+    // vf: buildNewGraphVF (Utility/vertexFollowing.cpp:93-258) instead of
+    // buildNextLevelGraphOpt: no zero self-loops, weights truncated to integers
+    void coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out, bool vf = false);
+    // vertexFollowing (Utility/vertexFollowing.cpp:46-89); returns the vertices to remove
+    i64 vertex_following(const DevGraph &g, i64 *C);
     // modularity of a labelling with community degrees summed from C
     double modularity(const DevGraph &g, const i64 *C);
 

@@ -62,6 +62,9 @@ _SIG = {
     "glv_renumberClustersContiguously": [_I64P, ctypes.c_int64, _I64P],
     "glv_buildNextLevelGraphOpt": [ctypes.c_int64, _I64P, _VP, _I64P, ctypes.c_int64,
                                    ctypes.POINTER(_I64P), ctypes.POINTER(_VP), _I64P],
+    "glv_buildNewGraphVF": [ctypes.c_int64, _I64P, _VP, _I64P, ctypes.c_int64,
+                            ctypes.POINTER(_I64P), ctypes.POINTER(_VP), _I64P],
+    "glv_vertexFollowing": [ctypes.c_int64, _I64P, _VP, _I64P, _I64P],
     "glv_modularity": [ctypes.c_int64, _I64P, _VP, _I64P, _F64P],
     "glv_runMultiPhaseColoring_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
@@ -80,6 +83,13 @@ _SIG = {
     "glv_coarsen_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, ctypes.c_int64,
                         ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I64P,
                         _I64P, _VP],
+    "glv_vertexFollowing_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _I64P, _VP],
+    "glv_buildNewGraphVF_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP, ctypes.c_int64,
+                                ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I64P,
+                                _I64P, _VP],
+    "glv_vertexFollowingPrepass_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.POINTER(_VP),
+                                       ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I64P, _I64P, _I64P, _I64P,
+                                       _VP],
     "glv_dev_free": [_VP],
     "glv_count_conflicts_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _I64P, _VP],
     "glv_profile_enable": [ctypes.c_int],
@@ -259,10 +269,30 @@ def renumberClustersContiguously(C):
 
 def buildNextLevelGraphOpt(G, C, numUniqueClusters):
     """basic_util.h:6.  Returns the coarsened Graph (rows sorted by tail)."""
+    return _host_contract("glv_buildNextLevelGraphOpt", G, C, numUniqueClusters)
+
+
+def buildNewGraphVF(G, C, numUniqueClusters):
+    """basic_util.h:15.  Contracts without zero self-loops, weights truncated to
+    integers (vertexFollowing.cpp:93-258).  Rows sorted by tail."""
+    return _host_contract("glv_buildNewGraphVF", G, C, numUniqueClusters)
+
+
+def vertexFollowing(G):
+    """basic_util.h:14.  Returns (C, numNode): C[i] = -1 for isolated vertices,
+    the neighbour a degree-one vertex follows, else i."""
+    args, keep = _host_graph(G)
+    C = np.empty(G.numVertices, np.int64)
+    n = ctypes.c_int64(0)
+    _call("glv_vertexFollowing", *args, _p(C, _I64P), ctypes.byref(n))
+    return C, n.value
+
+
+def _host_contract(fn, G, C, numUniqueClusters):
     args, keep = _host_graph(G)
     C = np.ascontiguousarray(C, dtype=np.int64)
     po, pe, ne = _I64P(), _VP(), ctypes.c_int64(0)
-    _call("glv_buildNextLevelGraphOpt", *args, _p(C, _I64P), int(numUniqueClusters),
+    _call(fn, *args, _p(C, _I64P), int(numUniqueClusters),
           ctypes.byref(po), ctypes.byref(pe), ctypes.byref(ne))
     k = int(numUniqueClusters)
     ptrs = np.ctypeslib.as_array(po, shape=(k + 1,)).copy()
@@ -299,9 +329,23 @@ class DeviceGraph:
         self.tails = torch.from_numpy(G.tails).to(device)
         self.w = torch.from_numpy(G.weights).to(device)
 
+    @classmethod
+    def from_tensors(cls, off, tails, w, numEdges):
+        """Wraps CSR tensors already on the device."""
+        dg = cls.__new__(cls)
+        dg.numVertices = off.shape[0] - 1
+        dg.nnz = tails.shape[0]
+        dg.numEdges = numEdges
+        dg.off, dg.tails, dg.w = off, tails, w
+        return dg
+
     def ptrs(self):
         return [self.numVertices, self.nnz, _VP(self.off.data_ptr()), _VP(self.tails.data_ptr()),
                 _VP(self.w.data_ptr())]
+
+    def to_host(self):
+        return Graph(self.off.cpu().numpy(), self.tails.cpu().numpy(), self.w.cpu().numpy(),
+                     numEdges=self.numEdges)
 
 
 def _stream():
@@ -375,13 +419,15 @@ def renumber_dev(C):
     return k.value
 
 
-def coarsen_dev(dg, C, k):
-    """Returns a host Graph copied from the engine's coarsened device graph."""
+def coarsen_dev(dg, C, k, vf=False):
+    """Returns a host Graph copied from the engine's coarsened device graph
+    (vf: the buildNewGraphVF contraction)."""
     import torch
     po, pt, pw = _VP(), _VP(), _VP()
     nnz, ne = ctypes.c_int64(0), ctypes.c_int64(0)
-    _call("glv_coarsen_dev", *dg.ptrs(), _ptr(C), int(k), ctypes.byref(po), ctypes.byref(pt),
-          ctypes.byref(pw), ctypes.byref(nnz), ctypes.byref(ne), _stream())
+    _call("glv_buildNewGraphVF_dev" if vf else "glv_coarsen_dev", *dg.ptrs(), _ptr(C), int(k),
+          ctypes.byref(po), ctypes.byref(pt), ctypes.byref(pw), ctypes.byref(nnz), ctypes.byref(ne),
+          _stream())
     torch.cuda.synchronize()
     off = np.empty(int(k) + 1, np.int64)
     tails = np.empty(max(nnz.value, 1), np.int64)
@@ -393,6 +439,38 @@ def coarsen_dev(dg, C, k):
     for p in (po, pt, pw):
         _call("glv_dev_free", p)
     return Graph(off, tails[:nnz.value], w[:nnz.value], numEdges=ne.value)
+
+
+def vertex_following_prepass_dev(dg):
+    """The driver's vertex-following hook on the device (vertexFollowing,
+    renumber, buildNewGraphVF).  Returns (graph, numNode); the graph is `dg`
+    itself when nothing was folded."""
+    import torch
+    po, pt, pw = _VP(), _VP(), _VP()
+    nv, nnz, ne, n = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    _call("glv_vertexFollowingPrepass_dev", *dg.ptrs(), ctypes.byref(po), ctypes.byref(pt), ctypes.byref(pw),
+          ctypes.byref(nv), ctypes.byref(nnz), ctypes.byref(ne), ctypes.byref(n), _stream())
+    if n.value == 0:
+        return dg, 0
+    torch.cuda.synchronize()
+    dev = dg.off.device
+    off = torch.empty(nv.value + 1, dtype=torch.int64, device=dev)
+    tails = torch.empty(nnz.value, dtype=torch.int64, device=dev)
+    w = torch.empty(nnz.value, dtype=torch.float64, device=dev)
+    for t, p in ((off, po), (tails, pt), (w, pw)):
+        hip_copy(t.data_ptr(), p.value, t.numel() * 8, 3)  # hipMemcpyDeviceToDevice
+    for p in (po, pt, pw):
+        if p.value:
+            _call("glv_dev_free", p)
+    return DeviceGraph.from_tensors(off, tails, w, ne.value), n.value
+
+
+def vertex_following_dev(dg, C):
+    """Fills device labels C (vertexFollowing); returns numNode."""
+    n = ctypes.c_int64(0)
+    nv, nnz, off, tails, _w = dg.ptrs()
+    _call("glv_vertexFollowing_dev", nv, nnz, off, tails, _ptr(C), ctypes.byref(n), _stream())
+    return n.value
 
 
 class BatchStats(ctypes.Structure):
@@ -447,6 +525,11 @@ _hip_rt = None
 
 def hip_copy_d2h(dst, src, nbytes):
     """Blocking device-to-host copy of engine-owned memory (HIP runtime)."""
+    hip_copy(dst, src, nbytes, 2)  # hipMemcpyDeviceToHost
+
+
+def hip_copy(dst, src, nbytes, kind):
+    """Blocking hipMemcpy of `kind` (2: device to host, 3: device to device)."""
     global _hip_rt
     if nbytes == 0:
         return
@@ -454,6 +537,6 @@ def hip_copy_d2h(dst, src, nbytes):
         _hip_rt = ctypes.CDLL("libamdhip64.so")
         _hip_rt.hipMemcpy.argtypes = [_VP, _VP, ctypes.c_size_t, ctypes.c_int]
         _hip_rt.hipMemcpy.restype = ctypes.c_int
-    rc = _hip_rt.hipMemcpy(_VP(dst), _VP(src), nbytes, 2)  # hipMemcpyDeviceToHost
+    rc = _hip_rt.hipMemcpy(_VP(dst), _VP(src), nbytes, kind)
     if rc != 0:
         raise RuntimeError(f"hipMemcpy failed ({rc})")

@@ -10,4 +10,11 @@ long renumberClustersContiguously(long *C, long size);
    Gout->edgeListPtrs and Gout->edgeList are malloc()ed; the caller frees them. */
 double buildNextLevelGraphOpt(graph *Gin, graph *Gout, long *C, long numUniqueClusters,
                               int nThreads);
+/* basic_util.h:14 -- vertexFollowing.cpp:46-89: -1 for isolated vertices, the
+   neighbour for degree-one vertices that follow it; returns the count. */
+long vertexFollowing(graph *G, long *C);
+
+/* basic_util.h:15 -- vertexFollowing.cpp:93-258: contract without zero
+   self-loops, integer-truncated weights; outputs malloc()ed as above. */
+double buildNewGraphVF(graph *Gin, graph *Gout, long *C, long numUniqueClusters);
 #endif

@@ -124,6 +124,20 @@ int glv_parallelLouvianMethod(int64_t nv, const int64_t *edgeListPtrs, const glv
  * Utility/buildNextPhase.cpp:48-79. */
 int glv_renumberClustersContiguously(int64_t *C, int64_t size, int64_t *numClusters);
 
+/* vertexFollowing, DefineStructure/basic_util.h:14 and
+ * Utility/vertexFollowing.cpp:46-89: C[i] = -1 for vertices without entries,
+ * the neighbour's id for a vertex with one entry whose neighbour has more than
+ * one entry or a smaller id, else i; *numNode = vertices to remove. */
+int glv_vertexFollowing(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList, int64_t *C,
+                        int64_t *numNode);
+/* buildNewGraphVF, DefineStructure/basic_util.h:15 and
+ * Utility/vertexFollowing.cpp:93-258: like glv_buildNextLevelGraphOpt but
+ * without the zero self-loop per cluster and with every weight truncated to
+ * an integer before summing; *outNumEdges = NE_out + NE_self. */
+int glv_buildNewGraphVF(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList, const int64_t *C,
+                        int64_t numUniqueClusters, int64_t **outEdgeListPtrs, glv_edge **outEdgeList,
+                        int64_t *outNumEdges);
+
 /* buildNextLevelGraphOpt, DefineStructure/basic_util.h:6 and
  * Utility/buildNextPhase.cpp:83-255.  Outputs are malloc()ed (free with free()
  * or glv_free); rows are sorted by tail; *outNumEdges = NE_out + k. */
@@ -174,6 +188,21 @@ int glv_coarsen_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int
                     const double *d_weights, const int64_t *d_C, int64_t k, int64_t **d_out_offsets,
                     int64_t **d_out_tails, double **d_out_weights, int64_t *out_nnz,
                     int64_t *out_num_edges, void *stream);
+/* device forms of glv_vertexFollowing and glv_buildNewGraphVF (outputs as glv_coarsen_dev) */
+int glv_vertexFollowing_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                            int64_t *d_C, int64_t *numNode, void *stream);
+int glv_buildNewGraphVF_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                            const double *d_weights, const int64_t *d_C, int64_t k, int64_t **d_out_offsets,
+                            int64_t **d_out_tails, double **d_out_weights, int64_t *out_nnz,
+                            int64_t *out_num_edges, void *stream);
+/* The vertex-following hook of driverForGraphClustering.cpp:112-132 on the
+ * device: vertexFollowing, renumberClustersContiguously and buildNewGraphVF.
+ * When *numNode is 0 the graph is unchanged: the outputs are NULL and
+ * *out_nv = nv.  Otherwise the folded graph is returned like glv_coarsen_dev. */
+int glv_vertexFollowingPrepass_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                                   const double *d_weights, int64_t **d_out_offsets, int64_t **d_out_tails,
+                                   double **d_out_weights, int64_t *out_nv, int64_t *out_nnz,
+                                   int64_t *out_num_edges, int64_t *numNode, void *stream);
 int glv_dev_free(void *d_ptr);
 int glv_count_conflicts_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
                             const int64_t *d_tails, const int32_t *d_colors, int64_t *conflicts,

@@ -99,6 +99,10 @@ def _declare(L):
     L.orc_renumber.restype = ctypes.c_int64
     L.orc_coarsen.argtypes = G + [_I64P, ctypes.c_int64, _I64P, _I64P, _F64P, _I64P]
     L.orc_coarsen.restype = ctypes.c_int64
+    L.orc_build_vf.argtypes = G + [_I64P, ctypes.c_int64, _I64P, _I64P, _F64P, _I64P]
+    L.orc_build_vf.restype = ctypes.c_int64
+    L.orc_vertex_following.argtypes = [ctypes.c_int64, _I64P, _I64P, _I64P]
+    L.orc_vertex_following.restype = ctypes.c_int64
     L.orc_run_colored.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(Stats)]
@@ -122,6 +126,11 @@ def _declare_ref(R):
     R.ref_coarsen.argtypes = G + [_I64P, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_I64P),
                                   ctypes.POINTER(_I64P), ctypes.POINTER(_F64P), _I64P]
     R.ref_coarsen.restype = ctypes.c_int64
+    R.ref_build_vf.argtypes = G + [_I64P, ctypes.c_int64, ctypes.POINTER(_I64P), ctypes.POINTER(_I64P),
+                                   ctypes.POINTER(_F64P), _I64P]
+    R.ref_build_vf.restype = ctypes.c_int64
+    R.ref_vertex_following.argtypes = [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]
+    R.ref_vertex_following.restype = ctypes.c_int64
     R.ref_free.argtypes = [ctypes.c_void_p]
     R.ref_free.restype = None
     R.ref_run_colored.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
@@ -226,17 +235,33 @@ def renumber(C):
     return C, k
 
 
-def coarsen(g, C, k):
+def _contract(fn, g, C, k):
     from grappolo_amd.graph import Graph
     C = np.ascontiguousarray(C, dtype=np.int64)
     ne = ctypes.c_int64(0)
-    tot = lib().orc_coarsen(*_g(g), _p(C, _I64P), k, None, None, None, ctypes.byref(ne))
+    tot = fn(*_g(g), _p(C, _I64P), k, None, None, None, ctypes.byref(ne))
     off = np.empty(k + 1, dtype=np.int64)
     tails = np.empty(max(tot, 1), dtype=np.int64)
     w = np.empty(max(tot, 1))
-    lib().orc_coarsen(*_g(g), _p(C, _I64P), k, _p(off, _I64P), _p(tails, _I64P), _p(w, _F64P),
-                      ctypes.byref(ne))
+    fn(*_g(g), _p(C, _I64P), k, _p(off, _I64P), _p(tails, _I64P), _p(w, _F64P), ctypes.byref(ne))
     return Graph(off, tails[:tot], w[:tot], numEdges=ne.value)
+
+
+def coarsen(g, C, k):
+    return _contract(lib().orc_coarsen, g, C, k)
+
+
+def build_vf(g, C, k):
+    """buildNewGraphVF restated; rows sorted by tail."""
+    return _contract(lib().orc_build_vf, g, C, k)
+
+
+def vertex_following(g):
+    """vertexFollowing restated: (C, numNode)."""
+    C = np.empty(g.numVertices, np.int64)
+    n = lib().orc_vertex_following(g.numVertices, _p(g.edgeListPtrs, _I64P), _p(g.tails, _I64P),
+                                   _p(C, _I64P))
+    return C, n
 
 
 def _stats(cap=256):
@@ -321,13 +346,28 @@ def ref_renumber(C):
     return C, k
 
 
+def ref_vertex_following(g):
+    C = np.empty(g.numVertices, np.int64)
+    n = ref().ref_vertex_following(g.numVertices, g.numEdges, _p(g.edgeListPtrs, _I64P),
+                                   _p(g.tails, _I64P), _p(C, _I64P))
+    return C, n
+
+
+def ref_build_vf(g, C, k):
+    return _ref_contract(lambda *a: ref().ref_build_vf(*a), g, C, k, ())
+
+
 def ref_coarsen(g, C, k, nthreads=1):
+    return _ref_contract(lambda *a: ref().ref_coarsen(*a), g, C, k, (nthreads,))
+
+
+def _ref_contract(fn, g, C, k, extra):
     from grappolo_amd.graph import Graph
     C = np.array(C, dtype=np.int64)
     po, pt, pw = _I64P(), _I64P(), _F64P()
     ne = ctypes.c_int64(0)
-    tot = ref().ref_coarsen(*_rg(g), _p(C, _I64P), k, nthreads, ctypes.byref(po),
-                            ctypes.byref(pt), ctypes.byref(pw), ctypes.byref(ne))
+    tot = fn(*_rg(g), _p(C, _I64P), k, *extra, ctypes.byref(po), ctypes.byref(pt), ctypes.byref(pw),
+             ctypes.byref(ne))
     off = np.ctypeslib.as_array(po, shape=(k + 1,)).copy()
     tails = np.ctypeslib.as_array(pt, shape=(max(tot, 1),))[:tot].copy()
     w = np.ctypeslib.as_array(pw, shape=(max(tot, 1),))[:tot].copy()

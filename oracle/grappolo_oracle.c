@@ -610,20 +610,25 @@ i64 orc_renumber(i64 *C, i64 size) {
  * order, :145-174); every cluster gets a self-loop entry even when its weight
  * is 0 (:122); a cut pair is emitted in both rows with the same weight
  * (:205-223).  Rows are returned sorted by tail (the canonical form; the
- * reference's row order is thread-dependent). */
-i64 orc_coarsen(i64 nv, const i64 *off, const i64 *tails, const double *w, const i64 *C, i64 k,
-                i64 *out_off, i64 *out_tails, double *out_w, i64 *out_num_edges) {
+ * reference's row order is thread-dependent).
+ *
+ * vf != 0 gives buildNewGraphVF, vertexFollowing.cpp:93-258: no self-loop
+ * seeds (:126-129), every weight truncated to an integer before it is summed
+ * (:162-166), numEdges = NE_out + NE_self (:228). */
+static i64 coarsen_impl(i64 nv, const i64 *off, const i64 *tails, const double *w, const i64 *C, i64 k,
+                        int vf, i64 *out_off, i64 *out_tails, double *out_w, i64 *out_num_edges) {
     /* pass 1: number of aggregated (a,b) entries, via a two-key counting sort */
     i64 m = 0;
     for (i64 i = 0; i < nv; i++)
         for (i64 j = off[i]; j < off[i + 1]; j++)
             if (C[i] >= 0 && C[i] >= C[tails[j]]) m++;
-    i64 tot = m + k; /* plus one zero self-loop seed per cluster */
+    const i64 nseed = vf ? 0 : k;
+    i64 tot = m + nseed; /* plus one zero self-loop seed per cluster */
     i64 *ka = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
     i64 *kb = (i64 *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(i64));
     double *kw = (double *)malloc((size_t)(tot > 0 ? tot : 1) * sizeof(double));
     i64 p = 0;
-    for (i64 c = 0; c < k; c++) {
+    for (i64 c = 0; c < nseed; c++) {
         ka[p] = c;
         kb[p] = c;
         kw[p] = 0;
@@ -634,7 +639,7 @@ i64 orc_coarsen(i64 nv, const i64 *off, const i64 *tails, const double *w, const
             if (C[i] >= 0 && C[i] >= C[tails[j]]) {
                 ka[p] = C[i];
                 kb[p] = C[tails[j]];
-                kw[p] = w[j];
+                kw[p] = vf ? (double)(long long)w[j] : w[j];
                 p++;
             }
     /* stable LSD counting sort by b then by a */
@@ -670,8 +675,9 @@ i64 orc_coarsen(i64 nv, const i64 *off, const i64 *tails, const double *w, const
     i64 ne_out = 0;
     for (i64 q = 0; q < nu; q++)
         if (ua[q] != ub[q]) ne_out++;
-    i64 total_entries = 2 * ne_out + k;
-    if (out_num_edges) *out_num_edges = ne_out + k;
+    const i64 ne_self = nu - ne_out; /* k when seeded */
+    i64 total_entries = 2 * ne_out + ne_self;
+    if (out_num_edges) *out_num_edges = ne_out + ne_self;
     if (out_off) {
         /* row a: (a,b) for b<=a in increasing b, then mirrors (b',a) for b'>a */
         i64 *deg = (i64 *)calloc((size_t)k + 1, sizeof(i64));
@@ -707,6 +713,39 @@ i64 orc_coarsen(i64 nv, const i64 *off, const i64 *tails, const double *w, const
     free(ub);
     free(uw);
     return total_entries;
+}
+
+i64 orc_coarsen(i64 nv, const i64 *off, const i64 *tails, const double *w, const i64 *C, i64 k,
+                i64 *out_off, i64 *out_tails, double *out_w, i64 *out_num_edges) {
+    return coarsen_impl(nv, off, tails, w, C, k, 0, out_off, out_tails, out_w, out_num_edges);
+}
+
+i64 orc_build_vf(i64 nv, const i64 *off, const i64 *tails, const double *w, const i64 *C, i64 k,
+                 i64 *out_off, i64 *out_tails, double *out_w, i64 *out_num_edges) {
+    return coarsen_impl(nv, off, tails, w, C, k, 1, out_off, out_tails, out_w, out_num_edges);
+}
+
+/* vertexFollowing, vertexFollowing.cpp:46-89: an isolated vertex gets -1; a
+ * vertex with exactly one entry, whose neighbour t has more than one entry or
+ * t < i, follows t; every other vertex keeps its own id.  Returns how many
+ * vertices got -1 or a neighbour. */
+i64 orc_vertex_following(i64 nv, const i64 *off, const i64 *tails, i64 *C) {
+    i64 n = 0;
+    for (i64 i = 0; i < nv; i++) {
+        C[i] = i;
+        const i64 d = off[i + 1] - off[i];
+        if (d == 0) {
+            C[i] = -1;
+            n++;
+        } else if (d == 1) {
+            const i64 t = tails[off[i]];
+            if (off[t + 1] - off[t] > 1 || i > t) {
+                C[i] = t;
+                n++;
+            }
+        }
+    }
+    return n;
 }
 
 /* ------------------------------------------------------------------------ */

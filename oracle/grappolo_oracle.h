@@ -83,6 +83,16 @@ int64_t orc_coarsen(int64_t nv, const int64_t *off, const int64_t *tails, const 
                     const int64_t *C, int64_t k, int64_t *out_off, int64_t *out_tails,
                     double *out_w, int64_t *out_num_edges);
 
+/* vertexFollowing.cpp:93-258 (buildNewGraphVF), same calling form as
+ * orc_coarsen: no self-loop seeds, weights truncated to integers,
+ * *out_num_edges = NE_out + NE_self. */
+int64_t orc_build_vf(int64_t nv, const int64_t *off, const int64_t *tails, const double *w,
+                     const int64_t *C, int64_t k, int64_t *out_off, int64_t *out_tails,
+                     double *out_w, int64_t *out_num_edges);
+
+/* vertexFollowing.cpp:46-89; returns numNode */
+int64_t orc_vertex_following(int64_t nv, const int64_t *off, const int64_t *tails, int64_t *C);
+
 /* runMultiPhaseColoring.cpp:51-238.  coloring_mode 0: greedy (reference at one
  * thread), 1: Jones-Plassmann (engine).  Returns 0 or a negative error. */
 int orc_run_colored(int64_t nv, const int64_t *off, const int64_t *tails, const double *w,

@@ -37,7 +37,7 @@ graph *make_graph(int64_t nv, int64_t num_edges, const int64_t *off, const int64
         for (int64_t j = off[i]; j < off[i + 1]; j++) {
             G->edgeList[j].head = i;
             G->edgeList[j].tail = tails[j];
-            G->edgeList[j].weight = w[j];
+            G->edgeList[j].weight = w ? w[j] : 1.0;
         }
     return G;
 }
@@ -141,6 +141,45 @@ int64_t ref_coarsen(int64_t nv, int64_t num_edges, const int64_t *off, const int
     {
         Quiet q(nullptr);
         buildNextLevelGraphOpt(G, &Gout, (long *)C, k, nthreads);
+    }
+    int64_t tot = Gout.edgeListPtrs[Gout.numVertices];
+    *out_off = (int64_t *)malloc((k + 1) * sizeof(int64_t));
+    *out_tails = (int64_t *)malloc((tot > 0 ? tot : 1) * sizeof(int64_t));
+    *out_w = (double *)malloc((tot > 0 ? tot : 1) * sizeof(double));
+    for (int64_t i = 0; i <= k; i++) (*out_off)[i] = Gout.edgeListPtrs[i];
+    for (int64_t j = 0; j < tot; j++) {
+        (*out_tails)[j] = Gout.edgeList[j].tail;
+        (*out_w)[j] = Gout.edgeList[j].weight;
+    }
+    *out_num_edges = Gout.numEdges;
+    free(Gout.edgeListPtrs);
+    free(Gout.edgeList);
+    free_graph(G);
+    return tot;
+}
+
+// vertexFollowing (vertexFollowing.cpp:46-89) on the reference's own code.
+int64_t ref_vertex_following(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                             int64_t *C) {
+    graph *G = make_graph(nv, num_edges, off, tails, nullptr);
+    long n;
+    {
+        Quiet q(nullptr);
+        n = vertexFollowing(G, (long *)C);
+    }
+    free_graph(G);
+    return n;
+}
+
+// buildNewGraphVF (vertexFollowing.cpp:93-258); outputs as ref_coarsen.
+int64_t ref_build_vf(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails,
+                     const double *w, int64_t *C, int64_t k, int64_t **out_off, int64_t **out_tails,
+                     double **out_w, int64_t *out_num_edges) {
+    graph *G = make_graph(nv, num_edges, off, tails, w);
+    graph Gout;
+    {
+        Quiet q(nullptr);
+        buildNewGraphVF(G, &Gout, (long *)C, k);
     }
     int64_t tot = Gout.edgeListPtrs[Gout.numVertices];
     *out_off = (int64_t *)malloc((k + 1) * sizeof(int64_t));

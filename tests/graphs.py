@@ -41,6 +41,33 @@ def with_loops_and_isolated(seed=11, nv=3000):
     return Graph.from_pairs(nv, pairs, w)  # vertices nv-100.. have no edges
 
 
+def pendants(seed=17, nv=5000, weighted=True):
+    """What vertex following folds: a planted core with pendant (degree-one)
+    vertices hung on it, disjoint pairs of degree-one vertices, vertices whose
+    only entry is a self-loop, and isolated vertices.  Weighted graphs carry
+    fractional weights, which the folded graph truncates."""
+    rng = np.random.default_rng(seed)
+    core = nv // 2
+    g = gen_planted(core, 8, 8, 0.3, seed=seed)
+    rows = np.repeat(np.arange(g.numVertices), g.degrees())
+    keep = rows < g.tails
+    pairs = [np.stack([rows[keep], g.tails[keep]], 1)]
+    v = core
+    npend = nv // 4
+    pairs.append(np.stack([rng.integers(0, core, npend), np.arange(v, v + npend)], 1))
+    v += npend
+    npair = (nv // 10) // 2
+    a = np.arange(v, v + 2 * npair, 2)
+    pairs.append(np.stack([a, a + 1], 1))
+    v += 2 * npair
+    nloop = nv // 20
+    pairs.append(np.stack([np.arange(v, v + nloop), np.arange(v, v + nloop)], 1))
+    v += nloop  # the rest stay isolated
+    p = np.concatenate(pairs)
+    w = rng.uniform(0.4, 3.7, len(p)) if weighted else None
+    return Graph.from_pairs(nv, p, w)
+
+
 def cliques(seed=13, nv=20000, size=200, inter=8, hub_degrees=(5000, 4500), weighted=False):
     """Dense near-cliques (every coloring needs ~size colors, so classes hold
     ~nv/size vertices) joined by random inter-clique edges, plus hubs: a graph

@@ -54,7 +54,9 @@ def test_hip_library_exports_reference_cpp_api():
               "_Z21parallelLouvianMethodP5graphPliddPdPi",
               "_Z26parallelLouvianMethodNoMapP5graphPliddPdPi",
               "_Z28renumberClustersContiguouslyPll",
-              "_Z22buildNextLevelGraphOptP5graphS0_Plli"]:
+              "_Z22buildNextLevelGraphOptP5graphS0_Plli",
+              "_Z15vertexFollowingP5graphPl",
+              "_Z15buildNewGraphVFP5graphS0_Pll"]:
         assert m in syms, m
 
 
@@ -167,3 +169,27 @@ def test_graph_container_edge_list_round_trip():
     h = Graph.from_edge_list(g.edgeListPtrs, e, g.numEdges)
     np.testing.assert_array_equal(h.tails, g.tails)
     np.testing.assert_array_equal(e["head"], np.repeat(np.arange(512), g.degrees()))
+
+
+# ------------------------------------------------ driver output and input files
+def test_clust_info_writer_format(tmp_path):
+    """driverForGraphClustering.cpp:195-202: `<inFile>_clustInfo`, one "%ld\n"
+    per vertex."""
+    from grappolo_amd import driver
+    C = np.array([3, -1, 0, 12345678901, 7], np.int64)
+    path = driver.write_clust_info(str(tmp_path / "g.bin"), C)
+    assert path == str(tmp_path / "g.bin_clustInfo")
+    assert open(path, "rb").read() == b"3\n-1\n0\n12345678901\n7\n"
+    np.testing.assert_array_equal(driver.read_communities(path), C)
+
+
+def test_read_communities_keeps_the_reference_parse(tmp_path):
+    """driverForClusterComparison.cpp:59-83: tokens until end of file; a file
+    without trailing whitespace loses its last value, an empty file gives none."""
+    from grappolo_amd import driver
+    cases = {b"1 2\n3\n": [1, 2, 3], b"1\n2\n3": [1, 2], b"": [], b"\n": [], b"5": [],
+             b"  4\t-1\r\n": [4, -1]}
+    for i, (text, exp) in enumerate(cases.items()):
+        p = tmp_path / f"c{i}"
+        p.write_bytes(text)
+        np.testing.assert_array_equal(driver.read_communities(p), np.array(exp, np.int64))

@@ -294,3 +294,77 @@ def test_karate_modularity_near_known_optimum():
     land just below it."""
     C, st = oracle.run_colored(karate(), coloring="greedy", minGraphSize=10)
     assert 0.40 <= st["final_modularity"] <= 0.4198 + 1e-4
+
+
+# ------------------------------------------------------- vertex following
+def _pairs_graph(nv, pairs, w=None):
+    return Graph.from_pairs(nv, np.array(pairs, np.int64).reshape(-1, 2), w)
+
+
+def test_vertex_following_closed_forms():
+    """vertexFollowing.cpp:46-89 on hand-built cases: a star's leaves follow
+    the centre, in a lone pair the larger id follows the smaller, a vertex
+    whose only entry is its own loop and an isolated vertex (-1) do not."""
+    # 0: centre of leaves 1..4; 5-6 a pair; 7 self-loop only; 8 isolated; 9-10-11 a path
+    g = _pairs_graph(12, [(0, 1), (0, 2), (0, 3), (0, 4), (5, 6), (7, 7), (9, 10), (10, 11)])
+    C, n = oracle.vertex_following(g)
+    np.testing.assert_array_equal(C, [0, 0, 0, 0, 0, 5, 5, 7, -1, 10, 10, 10])
+    assert n == 4 + 1 + 1 + 2
+    Cr, k = oracle.renumber(C)
+    assert k == 4
+    vf = oracle.build_vf(g, Cr, k)
+    # star -> one vertex with loop 2*4; pair -> loop 2; self-loop kept; path -> loop 4
+    assert vf.numVertices == 4 and vf.numEdges == 4
+    np.testing.assert_array_equal(vf.edgeListPtrs, [0, 1, 2, 3, 4])
+    np.testing.assert_array_equal(vf.tails, [0, 1, 2, 3])
+    np.testing.assert_array_equal(vf.weights, [8.0, 2.0, 1.0, 4.0])
+
+
+def test_build_vf_invariants():
+    """buildNewGraphVF: no zero self-loop seeds, every weight truncated to an
+    integer before summing, numEdges = NE_out + NE_self."""
+    from graphs import pendants
+    g = pendants(seed=3, nv=3000)
+    C, n = oracle.vertex_following(g)
+    assert n > 0
+    Cr, k = oracle.renumber(C)
+    vf = oracle.build_vf(g, Cr, k)
+    assert vf.numVertices == k
+    rows = np.repeat(np.arange(k), vf.degrees())
+    loops = rows == vf.tails
+    ne_self = int(loops.sum())
+    assert vf.numEdges == (vf.nnz - ne_self) // 2 + ne_self
+    assert np.all(np.diff(rows * k + vf.tails) > 0)
+    order = np.lexsort((rows, vf.tails))
+    np.testing.assert_array_equal(vf.tails[order], rows)
+    np.testing.assert_array_equal(vf.weights[order], vf.weights)
+    # total truncated weight is kept: every input entry lands in exactly one
+    # output entry (self pairs counted from both endpoints, as in the input)
+    tw = np.trunc(g.weights)
+    assert vf.weights[~loops].sum() + vf.weights[loops].sum() == tw[Cr[np.repeat(
+        np.arange(g.numVertices), g.degrees())] >= 0].sum()
+    assert np.all(vf.weights == np.trunc(vf.weights))
+    # no seeds: a cluster without internal entries has no loop
+    Cs = Cr[np.repeat(np.arange(g.numVertices), g.degrees())]
+    internal = np.zeros(k, bool)
+    internal[Cs[Cs == Cr[g.tails]]] = True
+    assert np.array_equal(np.bincount(rows[loops], minlength=k) > 0, internal)
+
+
+@need_ref
+@pytest.mark.parametrize("name", ["karate", "loops_isolated", "rmat12", "hubs", "pendants_w", "pendants"])
+def test_vertex_following_matches_reference(name):
+    from graphs import pendants
+    g = {"pendants_w": lambda: pendants(seed=5), "pendants": lambda: pendants(seed=6, weighted=False)}.get(
+        name, lambda: catalog()[name])()
+    C, n = oracle.vertex_following(g)
+    Cr, nr = oracle.ref_vertex_following(g)
+    assert n == nr
+    np.testing.assert_array_equal(C, Cr)
+    C2, k = oracle.renumber(C)
+    a = oracle.build_vf(g, C2, k)
+    b = oracle.canonical(oracle.ref_build_vf(g, C2, k))
+    np.testing.assert_array_equal(a.edgeListPtrs, b.edgeListPtrs)
+    np.testing.assert_array_equal(a.tails, b.tails)
+    np.testing.assert_array_equal(a.weights, b.weights)  # integer sums: order-free
+    assert a.numEdges == b.numEdges
