@@ -13,6 +13,8 @@
 #include "grappolo/color_comm.h"
 #include "grappolo/coloring.h"
 #include "batch.hpp"
+#include "compare.hpp"
+#include "grappolo/utilityClusteringFunctions.h"
 #include "grappolo_hip.h"
 #include "kernels.hpp"
 #include "louvain.hpp"
@@ -938,6 +940,72 @@ int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, con
     });
 }
 
+static void fill_comparison(const glv::CompareResult &r, glv_cluster_comparison *out) {
+    out->nC1 = r.nC1;
+    out->nC2 = r.nC2;
+    out->SameSame = r.SameSame;
+    out->SameDiff = r.SameDiff;
+    out->DiffSame = r.DiffSame;
+    out->Agree = r.Agree;
+    out->precision = r.precision;
+    out->recall = r.recall;
+    out->fScore = r.fScore;
+    out->gini1 = r.gini1;
+    out->gini2 = r.gini2;
+}
+
+int glv_computeCommunityComparisons_dev(int64_t *d_C1, int64_t N1, int64_t *d_C2, int64_t N2,
+                                        glv_cluster_comparison *out, void *stream) {
+    return guarded([&] {
+        if (!out) glv::fail(GLV_E_INVALID_ARG, "null output");
+        if (N1 != N2) glv::fail(GLV_E_INVALID_ARG, "the two labellings differ in length");
+        hipStream_t s = engine_stream(stream);
+        glv::CompareResult r;
+        glv::compare_communities(d_C1, d_C2, N1, r, s);
+        fill_comparison(r, out);
+    });
+}
+
+int glv_computeCommunityComparisons(int64_t *C1, int64_t N1, int64_t *C2, int64_t N2,
+                                    glv_cluster_comparison *out) {
+    return guarded([&] {
+        if (!out || (N1 > 0 && (!C1 || !C2))) glv::fail(GLV_E_INVALID_ARG, "null argument");
+        if (N1 != N2) glv::fail(GLV_E_INVALID_ARG, "the two labellings differ in length");
+        if (N1 <= 0) glv::fail(GLV_E_INVALID_ARG, "comparison needs at least one vertex");
+        hipStream_t s = engine_stream(nullptr);
+        DBuf<i64> a(N1, s), b(N1, s);
+        GLV_HIP(hipMemcpyAsync(a.p, C1, N1 * sizeof(i64), hipMemcpyHostToDevice, s));
+        GLV_HIP(hipMemcpyAsync(b.p, C2, N1 * sizeof(i64), hipMemcpyHostToDevice, s));
+        glv::CompareResult r;
+        glv::compare_communities(a.p, b.p, N1, r, s);
+        GLV_HIP(hipMemcpyAsync(C1, a.p, N1 * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipMemcpyAsync(C2, b.p, N1 * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        fill_comparison(r, out);
+    });
+}
+
+int glv_computeGiniCoefficient_dev(int64_t *d_colorSize, int numColors, double *gini, void *stream) {
+    return guarded([&] {
+        if (!gini || (numColors > 0 && !d_colorSize)) glv::fail(GLV_E_INVALID_ARG, "null argument");
+        hipStream_t s = engine_stream(stream);
+        *gini = glv::gini_coefficient(d_colorSize, std::max(numColors, 0), numColors, s);
+    });
+}
+
+int glv_computeGiniCoefficient(int64_t *colorSize, int numColors, double *gini) {
+    return guarded([&] {
+        if (!gini || (numColors > 0 && !colorSize)) glv::fail(GLV_E_INVALID_ARG, "null argument");
+        hipStream_t s = engine_stream(nullptr);
+        const i64 n = std::max(numColors, 0);
+        DBuf<i64> d(n, s);
+        if (n) GLV_HIP(hipMemcpyAsync(d.p, colorSize, n * sizeof(i64), hipMemcpyHostToDevice, s));
+        *gini = glv::gini_coefficient(d.p, n, numColors, s);
+        if (n) GLV_HIP(hipMemcpyAsync(colorSize, d.p, n * sizeof(i64), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+    });
+}
+
 }  // extern "C"
 
 // ===================================================== C++ drop-in (reference API)
@@ -1065,6 +1133,40 @@ long renumberClustersContiguously(long *C, long size) {
     int rc = glv_renumberClustersContiguously((int64_t *)C, size, &k);
     if (rc) die("renumberClustersContiguously", rc);
     return k;
+}
+
+void computeCommunityComparisons(std::vector<long> &C1, long N1, std::vector<long> &C2, long N2) {
+    if ((long)C1.size() < N1 || (long)C2.size() < N2) die("computeCommunityComparisons", GLV_E_INVALID_ARG);
+    glv_cluster_comparison r;
+    int rc = glv_computeCommunityComparisons((int64_t *)C1.data(), N1, (int64_t *)C2.data(), N2, &r);
+    if (rc) die("computeCommunityComparisons", rc);
+    // the report of utilityClusterComparisonMetrics.cpp:247-264
+    printf("*******************************************\n");
+    printf("Cluster comparison statistics: \n");
+    printf("*******************************************\n");
+    printf("|C1| (truth)          : %ld\n", N1);
+    printf("Num communities in C1 : %ld\n", (long)r.nC1);
+    printf("|C2| (output)         : %ld\n", N2);
+    printf("Num communities in C2 : %ld\n", (long)r.nC2);
+    printf("-------------------------------------------\n");
+    printf("Same-Same (True positive)  : %ld\n", (long)r.SameSame);
+    printf("Same-Diff (False negative) : %ld\n", (long)r.SameDiff);
+    printf("Diff-Same (False positive) : %ld\n", (long)r.DiffSame);
+    printf("-------------------------------------------\n");
+    printf("Precision             :  %lf (%3.2lf%%)\n", r.precision, r.precision * 100);
+    printf("Recall                :  %lf (%3.2lf%%)\n", r.recall, r.recall * 100);
+    printf("F-score               :  %lf\n", r.fScore);
+    printf("-------------------------------------------\n");
+    printf("Gini coefficient, C1  :  %lf \n", r.gini1);
+    printf("Gini coefficient, C2  :  %lf \n", r.gini2);
+    printf("*******************************************\n");
+}
+
+double computeGiniCoefficient(long *colorSize, int numColors) {
+    double g = 0;
+    int rc = glv_computeGiniCoefficient((int64_t *)colorSize, numColors, &g);
+    if (rc) die("computeGiniCoefficient", rc);
+    return g;
 }
 
 long vertexFollowing(graph *G, long *C) {

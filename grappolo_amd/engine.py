@@ -90,6 +90,10 @@ _SIG = {
     "glv_vertexFollowingPrepass_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.POINTER(_VP),
                                        ctypes.POINTER(_VP), ctypes.POINTER(_VP), _I64P, _I64P, _I64P, _I64P,
                                        _VP],
+    "glv_computeCommunityComparisons": [_I64P, ctypes.c_int64, _I64P, ctypes.c_int64, _VP],
+    "glv_computeCommunityComparisons_dev": [_VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, _VP],
+    "glv_computeGiniCoefficient": [_I64P, ctypes.c_int, _F64P],
+    "glv_computeGiniCoefficient_dev": [_VP, ctypes.c_int, _F64P, _VP],
     "glv_dev_free": [_VP],
     "glv_count_conflicts_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _I64P, _VP],
     "glv_profile_enable": [ctypes.c_int],
@@ -471,6 +475,51 @@ def vertex_following_dev(dg, C):
     nv, nnz, off, tails, _w = dg.ptrs()
     _call("glv_vertexFollowing_dev", nv, nnz, off, tails, _ptr(C), ctypes.byref(n), _stream())
     return n.value
+
+
+class ClusterComparison(ctypes.Structure):
+    """glv_cluster_comparison"""
+    _fields_ = [("nC1", ctypes.c_int64), ("nC2", ctypes.c_int64), ("SameSame", ctypes.c_int64),
+                ("SameDiff", ctypes.c_int64), ("DiffSame", ctypes.c_int64), ("Agree", ctypes.c_int64),
+                ("precision", ctypes.c_double), ("recall", ctypes.c_double), ("fScore", ctypes.c_double),
+                ("gini1", ctypes.c_double), ("gini2", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def computeCommunityComparisons(C1, C2):
+    """utilityClusteringFunctions.h:71 (C1 = truth).  Returns (metrics dict,
+    relabelled C1, relabelled C2) -- the reference relabels -1 in place."""
+    a = np.array(C1, dtype=np.int64)
+    b = np.array(C2, dtype=np.int64)
+    out = ClusterComparison()
+    _call("glv_computeCommunityComparisons", _p(a, _I64P), a.shape[0], _p(b, _I64P), b.shape[0],
+          ctypes.byref(out))
+    return out.as_dict(), a, b
+
+
+def compare_dev(C1, C2):
+    """computeCommunityComparisons on device label tensors (relabelled in place)."""
+    out = ClusterComparison()
+    _call("glv_computeCommunityComparisons_dev", _ptr(C1), C1.shape[0], _ptr(C2), C2.shape[0],
+          ctypes.byref(out), _stream())
+    return out.as_dict()
+
+
+def computeGiniCoefficient(sizes):
+    """basic_util.h:18.  Returns (gini, sizes sorted as the reference leaves them)."""
+    s = np.array(sizes, dtype=np.int64)
+    g = ctypes.c_double(0)
+    _call("glv_computeGiniCoefficient", _p(s, _I64P), s.shape[0], ctypes.byref(g))
+    return g.value, s
+
+
+def gini_dev(sizes):
+    """computeGiniCoefficient of a device int64 tensor (sorted in place)."""
+    g = ctypes.c_double(0)
+    _call("glv_computeGiniCoefficient_dev", _ptr(sizes), sizes.shape[0], ctypes.byref(g), _stream())
+    return g.value
 
 
 class BatchStats(ctypes.Structure):

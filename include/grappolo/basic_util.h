@@ -17,4 +17,9 @@ long vertexFollowing(graph *G, long *C);
 /* basic_util.h:15 -- vertexFollowing.cpp:93-258: contract without zero
    self-loops, integer-truncated weights; outputs malloc()ed as above. */
 double buildNewGraphVF(graph *Gin, graph *Gout, long *C, long numUniqueClusters);
+
+/* basic_util.h:18 -- utilityClusterComparisonMetrics.cpp:309-330: sorts
+   colorSize[0..numColors) ascending in place and returns the Gini
+   coefficient of the sizes. */
+double computeGiniCoefficient(long *colorSize, int numColors);
 #endif

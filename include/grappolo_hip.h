@@ -315,6 +315,35 @@ int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, con
                           const double *d_weights, int reps, int64_t *max_deg, double *ms_default,
                           double *ms_tuned, void *stream);
 
+/* ---- cluster comparison (the -r metrics and driverForClusterComparison) --
+ * computeCommunityComparisons, Utility/utilityClusteringFunctions.h:71 and
+ * utilityClusterComparisonMetrics.cpp:50-304, C1 = truth.  N1 must equal N2.
+ * Both label arrays are relabelled in place as the reference does: -1 in C1
+ * becomes a new last community; -1 in C2 becomes N2 + 1.  A labelling is
+ * zero-based when it holds a 0, else one-based.  The pair counts are those of
+ * the reference at one thread (its nested parallel region multiplies the
+ * printed counts by its thread count; precision, recall and F-score do not
+ * change).  Labels below -1 are rejected. */
+typedef struct glv_cluster_comparison {
+    int64_t nC1, nC2;     /* "Num communities in C1/C2" */
+    int64_t SameSame;     /* pairs together in both (true positives) */
+    int64_t SameDiff;     /* together in C1 only (false negatives) */
+    int64_t DiffSame;     /* together in C2 only (false positives) */
+    int64_t Agree;        /* = SameSame, counted from C2's side */
+    double precision, recall, fScore;
+    double gini1, gini2;  /* computeGiniCoefficient of the community sizes */
+} glv_cluster_comparison;
+int glv_computeCommunityComparisons(int64_t *C1, int64_t N1, int64_t *C2, int64_t N2,
+                                    glv_cluster_comparison *out);
+int glv_computeCommunityComparisons_dev(int64_t *d_C1, int64_t N1, int64_t *d_C2, int64_t N2,
+                                        glv_cluster_comparison *out, void *stream);
+/* computeGiniCoefficient, basic_util.h:18 and
+ * utilityClusterComparisonMetrics.cpp:309-330: sorts colorSize[0..numColors)
+ * ascending in place and returns 2*sum((i+1)*s_i)/(numColors*sum(s_i)) -
+ * (numColors+1)/numColors (exact while the sums stay below 2^53). */
+int glv_computeGiniCoefficient(int64_t *colorSize, int numColors, double *gini);
+int glv_computeGiniCoefficient_dev(int64_t *d_colorSize, int numColors, double *gini, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

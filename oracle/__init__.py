@@ -60,6 +60,17 @@ def ref():
     return _ref
 
 
+class Comparison(ctypes.Structure):
+    _fields_ = [("nC1", ctypes.c_int64), ("nC2", ctypes.c_int64), ("same_same", ctypes.c_int64),
+                ("same_diff", ctypes.c_int64), ("diff_same", ctypes.c_int64), ("agree", ctypes.c_int64),
+                ("precision", ctypes.c_double), ("recall", ctypes.c_double), ("f_score", ctypes.c_double),
+                ("gini1", ctypes.c_double), ("gini2", ctypes.c_double)]
+
+
+def comparison_dict(c):
+    return {f: getattr(c, f) for f, _ in c._fields_}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("phases", ctypes.c_int32), ("total_iterations", ctypes.c_int32),
                 ("final_clusters", ctypes.c_int64), ("final_modularity", ctypes.c_double),
@@ -103,6 +114,10 @@ def _declare(L):
     L.orc_build_vf.restype = ctypes.c_int64
     L.orc_vertex_following.argtypes = [ctypes.c_int64, _I64P, _I64P, _I64P]
     L.orc_vertex_following.restype = ctypes.c_int64
+    L.orc_compare.argtypes = [_I64P, _I64P, ctypes.c_int64, ctypes.POINTER(Comparison)]
+    L.orc_compare.restype = ctypes.c_int
+    L.orc_gini.argtypes = [_I64P, ctypes.c_int]
+    L.orc_gini.restype = ctypes.c_double
     L.orc_run_colored.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(Stats)]
@@ -131,6 +146,10 @@ def _declare_ref(R):
     R.ref_build_vf.restype = ctypes.c_int64
     R.ref_vertex_following.argtypes = [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]
     R.ref_vertex_following.restype = ctypes.c_int64
+    R.ref_compare.argtypes = [_I64P, _I64P, ctypes.c_int64, ctypes.c_int, ctypes.c_char_p]
+    R.ref_compare.restype = None
+    R.ref_gini.argtypes = [_I64P, ctypes.c_int]
+    R.ref_gini.restype = ctypes.c_double
     R.ref_free.argtypes = [ctypes.c_void_p]
     R.ref_free.restype = None
     R.ref_run_colored.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
@@ -338,6 +357,56 @@ def ref_plain_phase(g, lower=-1.0, thresh=1e-6, nomap=False, nthreads=1):
     mod = ref().ref_plain_phase(*_rg(g), _p(C, _I64P), lower, thresh, int(nomap), nthreads,
                                 ctypes.byref(it))
     return mod, C, it.value
+
+
+def compare(C1, C2):
+    """computeCommunityComparisons restated: (dict, relabelled C1, C2)."""
+    a = np.array(C1, dtype=np.int64)
+    b = np.array(C2, dtype=np.int64)
+    out = Comparison()
+    if lib().orc_compare(_p(a, _I64P), _p(b, _I64P), a.shape[0], ctypes.byref(out)) != 0:
+        raise ValueError("labels the reference cannot compare")
+    return comparison_dict(out), a, b
+
+
+def gini(sizes):
+    """computeGiniCoefficient restated: (gini, sorted sizes)."""
+    s = np.array(sizes, dtype=np.int64)
+    return lib().orc_gini(_p(s, _I64P), s.shape[0]), s
+
+
+_REPORT = {"Same-Same (True positive)": ("same_same", int), "Same-Diff (False negative)": ("same_diff", int),
+           "Diff-Same (False positive)": ("diff_same", int), "Num communities in C1": ("nC1", int),
+           "Num communities in C2": ("nC2", int), "Precision": ("precision", float),
+           "Recall": ("recall", float), "F-score": ("f_score", float),
+           "Gini coefficient, C1": ("gini1", float), "Gini coefficient, C2": ("gini2", float)}
+
+
+def ref_compare(C1, C2, nthreads=1):
+    """The compiled reference's computeCommunityComparisons; its printed
+    report parsed into a dict (floats as printed, %lf), plus the relabelled
+    vectors."""
+    import tempfile
+    a = np.array(C1, dtype=np.int64)
+    b = np.array(C2, dtype=np.int64)
+    with tempfile.NamedTemporaryFile("r", suffix=".log") as f:
+        ref().ref_compare(_p(a, _I64P), _p(b, _I64P), a.shape[0], nthreads, f.name.encode())
+        text = open(f.name).read()
+    out = {}
+    for line in text.splitlines():
+        if ":" not in line:
+            continue
+        key, val = line.split(":", 1)
+        key = key.strip()
+        if key in _REPORT:
+            name, typ = _REPORT[key]
+            out[name] = typ(val.split()[0])
+    return out, a, b
+
+
+def ref_gini(sizes):
+    s = np.array(sizes, dtype=np.int64)
+    return ref().ref_gini(_p(s, _I64P), s.shape[0]), s
 
 
 def ref_renumber(C):

@@ -749,6 +749,151 @@ i64 orc_vertex_following(i64 nv, const i64 *off, const i64 *tails, i64 *C) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Cluster comparison                                                        */
+/* ------------------------------------------------------------------------ */
+
+static int cmp_i64(const void *a, const void *b) {
+    const i64 x = *(const i64 *)a, y = *(const i64 *)b;
+    return (x > y) - (x < y);
+}
+
+/* computeGiniCoefficient, utilityClusterComparisonMetrics.cpp:309-330: sort
+ * ascending, then 2*sum((i+1)*s_i) / (n*sum(s_i)) - (n+1)/n, the sums taken in
+ * double in index order. */
+double orc_gini(i64 *colorSize, int numColors) {
+    if (numColors > 0) qsort(colorSize, (size_t)numColors, sizeof(i64), cmp_i64);
+    double num = 0.0, den = 0.0;
+    for (i64 i = 0; i < numColors; i++) {
+        num = num + (double)((i + 1) * colorSize[i]);
+        den = den + (double)colorSize[i];
+    }
+    return ((2 * num) / (numColors * den)) - ((double)(numColors + 1) / (double)numColors);
+}
+
+/* One side of the comparison: the community count of :62-78 / :82-104 and
+ * the CSR of communities of :108-146 (vertices grouped by community in
+ * increasing vertex order -- the one-thread order).  `zero` says whether 0 is
+ * a label (zero-based) or not (one-based, label c at index c - 1). */
+typedef struct {
+    i64 nC;
+    int zero;
+    i64 *ptr, *idx, *dist;
+} CommCSR;
+
+static int comm_csr(const i64 *C, i64 N, i64 nC, int zero, CommCSR *o) {
+    o->nC = nC;
+    o->zero = zero;
+    o->ptr = (i64 *)calloc((size_t)nC + 1, sizeof(i64));
+    o->idx = (i64 *)malloc((size_t)(N > 0 ? N : 1) * sizeof(i64));
+    o->dist = (i64 *)malloc((size_t)nC * sizeof(i64));
+    i64 *fill = (i64 *)calloc((size_t)nC, sizeof(i64));
+    if (!o->ptr || !o->idx || !o->dist || !fill) {
+        free(fill);
+        return -1;
+    }
+    for (i64 i = 0; i < N; i++) {
+        const i64 c = zero ? C[i] : C[i] - 1;
+        if (c < 0 || c >= nC) {
+            free(fill);
+            return -1;
+        }
+        o->ptr[c + 1]++;
+    }
+    for (i64 c = 0; c < nC; c++) o->dist[c] = o->ptr[c + 1];
+    for (i64 c = 0; c < nC; c++) o->ptr[c + 1] += o->ptr[c];
+    for (i64 i = 0; i < N; i++) {
+        const i64 c = zero ? C[i] : C[i] - 1;
+        o->idx[o->ptr[c] + fill[c]++] = i;
+    }
+    free(fill);
+    return 0;
+}
+
+static void comm_free(CommCSR *o) {
+    free(o->ptr);
+    free(o->idx);
+    free(o->dist);
+}
+
+/* computeCommunityComparisons, utilityClusterComparisonMetrics.cpp:50-304.
+ * C1 (truth): nC1 = max label, +1 when 0 occurs; a -1 label becomes nC1 and
+ * then nC1 grows by one (:62-85).  C2: a -1 label becomes N + 1 before the
+ * count (:90-91), so isolated vertices of the output form one community.
+ * Pairs inside each community of C1 are same-same when C2 agrees, else
+ * same-diff; pairs inside each community of C2 are counted into agree or
+ * diff-same by C1 (:166-232).  The reference runs those loops in a nested
+ * parallel region whose every outer thread walks all communities, so its
+ * printed counts are the thread count times these (ratios unchanged). */
+int orc_compare(i64 *C1, i64 *C2, i64 N, orc_comparison *out) {
+    if (N <= 0) return -1;
+    for (i64 i = 0; i < N; i++)
+        if (C1[i] < -1 || C2[i] < -1) return -1;
+    i64 nC1 = -1;
+    int zero1 = 0;
+    for (i64 i = 0; i < N; i++) {
+        if (C1[i] == 0) zero1 = 1;
+        if (C1[i] > nC1) nC1 = C1[i];
+    }
+    if (zero1) nC1++;
+    if (nC1 <= 0) return -1;
+    int found = 0;
+    for (i64 i = 0; i < N; i++)
+        if (C1[i] == -1) {
+            C1[i] = nC1;
+            found = 1;
+        }
+    if (found) nC1++;
+    i64 nC2 = -1;
+    int zero2 = 0;
+    for (i64 i = 0; i < N; i++) {
+        if (C2[i] == -1) C2[i] = N + 1;
+        if (C2[i] == 0) zero2 = 1;
+        if (C2[i] > nC2) nC2 = C2[i];
+    }
+    if (zero2) nC2++;
+    if (nC2 <= 0) return -1;
+    CommCSR a, b;
+    memset(&a, 0, sizeof a);
+    memset(&b, 0, sizeof b);
+    if (comm_csr(C1, N, nC1, zero1, &a) || comm_csr(C2, N, nC2, zero2, &b)) {
+        comm_free(&a);
+        comm_free(&b);
+        return -1;
+    }
+    i64 ss = 0, sd = 0, ds = 0, ag = 0;
+    for (i64 c = 0; c < nC1; c++)
+        for (i64 i = a.ptr[c]; i < a.ptr[c + 1]; i++)
+            for (i64 j = i + 1; j < a.ptr[c + 1]; j++) {
+                if (C2[a.idx[i]] == C2[a.idx[j]])
+                    ss++;
+                else
+                    sd++;
+            }
+    for (i64 c = 0; c < nC2; c++)
+        for (i64 i = b.ptr[c]; i < b.ptr[c + 1]; i++)
+            for (i64 j = i + 1; j < b.ptr[c + 1]; j++) {
+                if (C1[b.idx[i]] == C1[b.idx[j]])
+                    ag++;
+                else
+                    ds++;
+            }
+    out->nC1 = nC1;
+    out->nC2 = nC2;
+    out->same_same = ss;
+    out->same_diff = sd;
+    out->diff_same = ds;
+    out->agree = ag;
+    out->precision = (double)ss / (double)(ss + ds);
+    out->recall = (double)ss / (double)(ss + sd);
+    out->f_score = 2 * ((out->precision * out->recall) / (out->precision + out->recall));
+    out->gini1 = orc_gini(a.dist, (int)nC1);
+    out->gini2 = orc_gini(b.dist, (int)nC2);
+    comm_free(&a);
+    comm_free(&b);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Multi-phase drivers                                                       */
 /* ------------------------------------------------------------------------ */
 

@@ -93,6 +93,22 @@ int64_t orc_build_vf(int64_t nv, const int64_t *off, const int64_t *tails, const
 /* vertexFollowing.cpp:46-89; returns numNode */
 int64_t orc_vertex_following(int64_t nv, const int64_t *off, const int64_t *tails, int64_t *C);
 
+/* Results of computeCommunityComparisons (utilityClusterComparisonMetrics.cpp:50-304). */
+typedef struct {
+    int64_t nC1, nC2;
+    int64_t same_same, same_diff, diff_same, agree;
+    double precision, recall, f_score, gini1, gini2;
+} orc_comparison;
+
+/* utilityClusterComparisonMetrics.cpp:50-304 at one thread.  C1 (truth) and
+ * C2 are relabelled in place as the reference does.  Returns 0, or -1 where
+ * the reference asserts (N <= 0, no positive community count) or indexes out
+ * of range (a label below -1). */
+int orc_compare(int64_t *C1, int64_t *C2, int64_t N, orc_comparison *out);
+
+/* utilityClusterComparisonMetrics.cpp:309-330; sorts colorSize in place */
+double orc_gini(int64_t *colorSize, int numColors);
+
 /* runMultiPhaseColoring.cpp:51-238.  coloring_mode 0: greedy (reference at one
  * thread), 1: Jones-Plassmann (engine).  Returns 0 or a negative error. */
 int orc_run_colored(int64_t nv, const int64_t *off, const int64_t *tails, const double *w,

@@ -13,11 +13,15 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
+
+#include <omp.h>
 
 #include "defs.h"  // must precede the other reference headers
 #include "basic_comm.h"
 #include "basic_util.h"
 #include "color_comm.h"
+#include "utilityClusteringFunctions.h"
 #include "coloring.h"
 #include "input_output.h"
 
@@ -195,6 +199,31 @@ int64_t ref_build_vf(int64_t nv, int64_t num_edges, const int64_t *off, const in
     free(Gout.edgeList);
     free_graph(G);
     return tot;
+}
+
+// computeCommunityComparisons (utilityClusterComparisonMetrics.cpp:50-304)
+// with `nthreads` OpenMP threads; its report goes to `log`.  The vectors carry
+// one element past N, as driverForClusterComparison.cpp's reader leaves them.
+// C1 and C2 receive the relabelled vectors.
+void ref_compare(int64_t *C1, int64_t *C2, int64_t N, int nthreads, const char *log) {
+    std::vector<long> a(C1, C1 + N), b(C2, C2 + N);
+    a.push_back(0);
+    b.push_back(0);
+    omp_set_num_threads(nthreads);
+    {
+        Quiet q(log);
+        computeCommunityComparisons(a, N, b, N);
+    }
+    for (int64_t i = 0; i < N; i++) {
+        C1[i] = a[i];
+        C2[i] = b[i];
+    }
+}
+
+// computeGiniCoefficient (utilityClusterComparisonMetrics.cpp:309-330); sorts in place.
+double ref_gini(int64_t *sizes, int n) {
+    Quiet q(nullptr);
+    return computeGiniCoefficient((long *)sizes, n);
 }
 
 void ref_free(void *p) { free(p); }

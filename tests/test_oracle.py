@@ -368,3 +368,117 @@ def test_vertex_following_matches_reference(name):
     np.testing.assert_array_equal(a.tails, b.tails)
     np.testing.assert_array_equal(a.weights, b.weights)  # integer sums: order-free
     assert a.numEdges == b.numEdges
+
+
+# ------------------------------------------------------ cluster comparison
+def _brute_compare(C1, C2):
+    """Pair counts by enumerating every vertex pair after the reference's -1
+    relabelling (utilityClusterComparisonMetrics.cpp:62-107)."""
+    a = np.array(C1, np.int64)
+    b = np.array(C2, np.int64)
+    n = len(a)
+    nC1 = a.max() + (1 if (a == 0).any() else 0)
+    if (a == -1).any():
+        a[a == -1] = nC1
+    b[b == -1] = n + 1
+    ss = sd = ds = 0
+    for i, j in itertools.combinations(range(n), 2):
+        s1, s2 = a[i] == a[j], b[i] == b[j]
+        ss += s1 and s2
+        sd += s1 and not s2
+        ds += s2 and not s1
+    return int(ss), int(sd), int(ds), a, b
+
+
+def _gini_exact(sizes):
+    s = sorted(int(x) for x in sizes)
+    n = len(s)
+    return Fraction(2 * sum((i + 1) * v for i, v in enumerate(s)), n * sum(s)) - Fraction(n + 1, n)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compare_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 60))
+    C1 = rng.integers(seed % 2, 6, n)  # odd seeds: one-based truth
+    C2 = rng.integers(0, 9, n)
+    if seed >= 2:
+        C1[rng.random(n) < 0.2] = -1
+        C2[rng.random(n) < 0.2] = -1
+    if (C1 == -1).all():
+        C1[0] = 1
+    out, a, b = oracle.compare(C1, C2)
+    ss, sd, ds, ea, eb = _brute_compare(C1, C2)
+    assert (out["same_same"], out["same_diff"], out["diff_same"], out["agree"]) == (ss, sd, ds, ss)
+    np.testing.assert_array_equal(a, ea)
+    np.testing.assert_array_equal(b, eb)
+    assert out["precision"] == ss / (ss + ds) if ss + ds else np.isnan(out["precision"])
+    assert out["recall"] == ss / (ss + sd) if ss + sd else np.isnan(out["recall"])
+
+
+def test_gini_closed_forms():
+    """Equal sizes give 0; one non-empty community among n gives (n-1)/n;
+    random sizes match exact rational arithmetic; the input is sorted in place."""
+    g, s = oracle.gini([7] * 9)
+    assert g == pytest.approx(0.0, abs=1e-15)
+    g, s = oracle.gini([0, 0, 0, 12, 0])
+    assert g == pytest.approx(4 / 5, abs=1e-15)
+    np.testing.assert_array_equal(s, [0, 0, 0, 0, 12])
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        sz = rng.integers(0, 10000, int(rng.integers(2, 500)))
+        g, s = oracle.gini(sz)
+        assert g == pytest.approx(float(_gini_exact(sz)), rel=1e-14, abs=1e-15)
+        np.testing.assert_array_equal(s, np.sort(sz))
+
+
+@need_ref
+@pytest.mark.parametrize("case", range(8))
+def test_compare_matches_reference(case):
+    """Counts exactly, the floats as the reference prints them (%lf), the
+    relabelled vectors, for zero- and one-based labels with and without -1."""
+    rng = np.random.default_rng(100 + case)
+    n = int(rng.integers(10, 4000))
+    C1 = rng.integers(case % 2, int(rng.integers(2, 80)), n)
+    C2 = rng.integers((case // 2) % 2, int(rng.integers(2, 120)), n)
+    if case >= 4:
+        C1[rng.random(n) < 0.05] = -1
+    if case >= 6:
+        C2[rng.random(n) < 0.05] = -1
+    out, a, b = oracle.compare(C1, C2)
+    ref, ra, rb = oracle.ref_compare(C1, C2, nthreads=1)
+    np.testing.assert_array_equal(a, ra)
+    np.testing.assert_array_equal(b, rb)
+    for k, v in ref.items():
+        if isinstance(v, int):
+            assert out[k] == v, k
+        else:
+            assert f"{out[k]:f}" == f"{v:f}", k
+
+
+@need_ref
+def test_reference_compare_counts_scale_with_threads():
+    """The reference's pair loops sit in a nested parallel region whose every
+    outer thread walks all communities: with t threads it prints t times the
+    pair counts, with the same ratios.  The restatement keeps the one-thread
+    counts."""
+    rng = np.random.default_rng(7)
+    C1 = rng.integers(0, 30, 2000)
+    C2 = rng.integers(0, 50, 2000)
+    one, _, _ = oracle.ref_compare(C1, C2, nthreads=1)
+    four, _, _ = oracle.ref_compare(C1, C2, nthreads=4)
+    for k in ("same_same", "same_diff", "diff_same"):
+        assert four[k] == 4 * one[k]
+    for k in ("precision", "recall", "f_score", "gini1", "gini2"):
+        assert four[k] == one[k]
+
+
+@need_ref
+def test_gini_matches_reference():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 17, 1000):
+        sz = rng.integers(-5, 100000, n)
+        g, s = oracle.gini(sz)
+        rg, rs = oracle.ref_gini(sz)
+        assert g == rg
+        np.testing.assert_array_equal(s, rs)
