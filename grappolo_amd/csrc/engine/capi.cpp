@@ -13,6 +13,7 @@
 #include "grappolo/color_comm.h"
 #include "grappolo/coloring.h"
 #include "batch.hpp"
+#include "coloring_modes.hpp"
 #include "compare.hpp"
 #include "grappolo/utilityClusteringFunctions.h"
 #include "grappolo_hip.h"
@@ -261,11 +262,9 @@ int glv_runMultiPhaseColoring(int64_t nv, const int64_t *edgeListPtrs, const glv
                               int64_t *C_orig, int coloring, int numColors, int replaceMap,
                               int64_t minGraphSize, double threshold, double C_threshold,
                               int numThreads, int threadsOpt, glv_stats *stats) {
-    (void)numColors;
     (void)numThreads;
     (void)threadsOpt;
     return guarded([&] {
-        if (coloring != 1) glv::fail(GLV_E_UNSUPPORTED, "only coloring == 1 is implemented");
         if (nv > 0 && !C_orig) glv::fail(GLV_E_INVALID_ARG, "null C_orig");
         hipStream_t s = engine_stream(nullptr);
         DevGraph g;
@@ -273,6 +272,7 @@ int glv_runMultiPhaseColoring(int64_t nv, const int64_t *edgeListPtrs, const glv
         DBuf<i64> dC(nv, s);
         glv::RunParams p;
         p.coloring = coloring;
+        p.numColors = numColors;
         p.replaceMap = replaceMap;
         p.minGraphSize = minGraphSize;
         p.threshold = threshold;
@@ -482,15 +482,16 @@ int glv_modularity(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edge
 // ---------------------------------------------------------------- device forms
 int glv_runMultiPhaseColoring_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
                                   const int64_t *d_tails, const double *d_weights, int64_t *d_C_orig,
-                                  int coloring, int replaceMap, int64_t minGraphSize,
+                                  int coloring, int numColors, int replaceMap, int64_t minGraphSize,
                                   double threshold, double C_threshold, void *stream,
                                   glv_stats *stats) {
     return guarded([&] {
-        if (coloring != 1) glv::fail(GLV_E_UNSUPPORTED, "only coloring == 1 is implemented");
         hipStream_t s = engine_stream(stream);
         check_dev_csr(nv, nnz, d_offsets, d_tails, s);
         DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
         glv::RunParams p;
+        p.coloring = coloring;
+        p.numColors = numColors;
         p.replaceMap = replaceMap;
         p.minGraphSize = minGraphSize;
         p.threshold = threshold;
@@ -940,6 +941,61 @@ int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, con
     });
 }
 
+int glv_algoColoringMultiHashMaxMin_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                                        int32_t *d_colors, int nHash, int nItrs, int32_t *maxColor, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, nullptr);
+        const int mc = glv::color_multihash(g, d_colors, nHash, nItrs, s);
+        GLV_HIP(hipStreamSynchronize(s));
+        if (maxColor) *maxColor = mc;
+    });
+}
+
+int glv_algoColoringMultiHashMaxMin(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                                    int32_t *vtxColor, int nHash, int nItrs, int32_t *maxColor) {
+    return guarded([&] {
+        if (nv > 0 && !vtxColor) glv::fail(GLV_E_INVALID_ARG, "null colors");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        DBuf<int32_t> c(nv, s);
+        const int mc = glv::color_multihash(g, c.p, nHash, nItrs, s);
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(vtxColor, c.p, nv * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        if (maxColor) *maxColor = mc;
+    });
+}
+
+int glv_vBaseRedistribution_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                                int32_t *d_colors, int ncolors, int type, int32_t *rounds, void *stream) {
+    return guarded([&] {
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, nullptr);
+        const int r = glv::balance_colors(g, d_colors, ncolors, type, s);
+        GLV_HIP(hipStreamSynchronize(s));
+        if (rounds) *rounds = r;
+    });
+}
+
+int glv_vBaseRedistribution(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList, int32_t *vtxColor,
+                            int ncolors, int type, int32_t *rounds) {
+    return guarded([&] {
+        if (nv > 0 && !vtxColor) glv::fail(GLV_E_INVALID_ARG, "null colors");
+        hipStream_t s = engine_stream(nullptr);
+        DevGraph g;
+        upload(nv, edgeListPtrs, edgeList, g, s);
+        DBuf<int32_t> c(nv, s);
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(c.p, vtxColor, nv * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        const int r = glv::balance_colors(g, c.p, ncolors, type, s);
+        if (nv > 0) GLV_HIP(hipMemcpyAsync(vtxColor, c.p, nv * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GLV_HIP(hipStreamSynchronize(s));
+        if (rounds) *rounds = r;
+    });
+}
+
 static void fill_comparison(const glv::CompareResult &r, glv_cluster_comparison *out) {
     out->nC1 = r.nC1;
     out->nC2 = r.nC2;
@@ -1037,7 +1093,7 @@ void print_summary(const glv_stats &st, int coloring, int numThreads) {
     if (coloring == 1) printf("Total time for coloring        : %lf\n", st.time_coloring);
     printf("********************************************\n");
     printf("TOTAL TIME                     : %lf\n",
-           st.time_clustering + st.time_building + (coloring == 1 ? st.time_coloring : 0.0));
+           st.time_clustering + st.time_building + st.time_coloring);
     printf("********************************************\n");
 }
 
@@ -1133,6 +1189,26 @@ long renumberClustersContiguously(long *C, long size) {
     int rc = glv_renumberClustersContiguously((int64_t *)C, size, &k);
     if (rc) die("renumberClustersContiguously", rc);
     return k;
+}
+
+int algoColoringMultiHashMaxMin(graph *G, int *vtxColor, int nThreads, double *totTime, int nHash, int nItrs) {
+    (void)nThreads;
+    int32_t mc = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = glv_algoColoringMultiHashMaxMin(G->numVertices, G->edgeListPtrs, (const glv_edge *)G->edgeList,
+                                             (int32_t *)vtxColor, nHash, nItrs, &mc);
+    auto t1 = std::chrono::steady_clock::now();
+    if (rc) die("algoColoringMultiHashMaxMin", rc);
+    if (totTime) *totTime = std::chrono::duration<double>(t1 - t0).count();
+    return mc;
+}
+
+int vBaseRedistribution(graph *G, int *vtxColor, int ncolors, int type) {
+    int32_t rounds = 0;
+    int rc = glv_vBaseRedistribution(G->numVertices, G->edgeListPtrs, (const glv_edge *)G->edgeList,
+                                     (int32_t *)vtxColor, ncolors, type, &rounds);
+    if (rc) die("vBaseRedistribution", rc);
+    return rounds;
 }
 
 void computeCommunityComparisons(std::vector<long> &C1, long N1, std::vector<long> &C2, long N2) {

@@ -1,5 +1,6 @@
 // louvain.cpp -- host orchestration of the engine (see louvain.hpp).
 #include "louvain.hpp"
+#include "coloring_modes.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -766,9 +767,24 @@ double Engine::modularity(const DevGraph &g, const i64 *C) {
 }
 
 // ------------------------------------------------------------ multi-phase
-// runMultiPhaseColoring, Coloring/runMultiPhaseColoring.cpp:51-238 (coloring == 1).
+// runMultiPhaseColoring, Coloring/runMultiPhaseColoring.cpp:51-238 (coloring 1, 2 or 3).
+int Engine::color_for_run(const DevGraph &g, int32_t *colors, const RunParams &p) {
+    if (p.coloring == 3) {
+        // runMultiPhaseColoring.cpp:78-86: two hashes, numColors / 4 iterations
+        const int nItrs = std::max(p.numColors / 4, 1);
+        const int mc = color_multihash(g, colors, 2, nItrs, s_);
+        if (mc + 1 > kMaxColors) fail(GLV_E_TOO_MANY_COLORS, "multi-hash coloring needs more than 4096 colors");
+        return mc + 1;
+    }
+    const int n = color(g, colors) + 1;
+    if (p.coloring == 2) balance_colors(g, colors, n, 0, s_);  // :72-73
+    return n;
+}
+
 void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, RunResult &r) {
-    if (p.coloring != 1) fail(GLV_E_UNSUPPORTED, "only coloring == 1 (distance-1) is implemented");
+    if (p.coloring < 1 || p.coloring > 3) fail(GLV_E_INVALID_ARG, "coloring must be 1, 2 or 3");
+    if (p.las && p.coloring != 1)
+        fail(GLV_E_UNSUPPORTED, "the learned active set needs independent color classes (coloring == 1)");
     const double t_start = now();
     const i64 NV = g0.nv;
     r = RunResult();
@@ -776,7 +792,7 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
     DBuf<int32_t> colors(NV, s_);
     DBuf<i64> Cbuf(NV, s_);
     double t0 = now();
-    int nColors = color(g0, colors.p) + 1;
+    int nColors = color_for_run(g0, colors.p, p);
     r.t_coloring += now() - t0;
     r.colors_first = nColors;
 
@@ -839,7 +855,7 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
             phase++;
             if (G->nv > p.minGraphSize && !nonColor) {
                 t0 = now();
-                nColors = color(*G, colors.p) + 1;
+                nColors = color_for_run(*G, colors.p, p);
                 rec.t_color = now() - t0;
                 r.t_coloring += rec.t_color;
             }

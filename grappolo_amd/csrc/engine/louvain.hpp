@@ -57,7 +57,8 @@ struct LasParams {
 };
 
 struct RunParams {
-    int coloring = 1;
+    int coloring = 1;    // 1: distance-1, 2: distance-1 then balanced, 3: incomplete multi-hash
+    int numColors = 16;  // coloring 3: nItrs = max(numColors / 4, 1) iterations of 2 hashes
     int replaceMap = 0;  // 1: NoMap plain sweeps (initCommAssOpt start)
     i64 minGraphSize = 1000;
     double threshold = 1e-9;
@@ -118,6 +119,9 @@ public:
 
     // algoDistanceOneVertexColoringOpt: returns the largest color
     int color(const DevGraph &g, int32_t *colors, int *rounds = nullptr);
+    // the coloring step of runMultiPhaseColoring.cpp:68-87 for p.coloring;
+    // returns nColors (largest color + 1)
+    int color_for_run(const DevGraph &g, int32_t *colors, const RunParams &p);
     i64 count_conflicts(const DevGraph &g, const int32_t *colors);
 
     // algoLouvainWithDistOneColoring: C (device, nv) receives the labels

@@ -22,7 +22,7 @@ class ClusteringParameters:
     of the reference do not apply to one device."""
     VF: bool = False           # -v
     output: bool = False       # -o
-    coloring: int = 0          # -c: 0 none, 1 distance-1 (2 and 3 are rejected by the engine)
+    coloring: int = 0          # -c: 0 none, 1 distance-1, 2 balanced, 3 incomplete multi-hash
     numColors: int = 16        # -n
     basicOpt: int = 0          # -b: 1 = replaceMap / NoMap sweeps
     syncType: int = 0          # -y (runMultiPhaseSyncType is not provided)
@@ -50,9 +50,9 @@ def cluster(G, opts=None, inFile=None, device="cuda"):
     if dg.numVertices == 0:
         st = {}
     elif opts.coloring != 0:
-        st = engine.run_colored_dev(dg, C, coloring=opts.coloring, replaceMap=replaceMap,
-                                    minGraphSize=opts.minGraphSize, threshold=opts.threshold,
-                                    C_threshold=opts.C_thresh)
+        st = engine.run_colored_dev(dg, C, coloring=opts.coloring, numColors=opts.numColors,
+                                    replaceMap=replaceMap, minGraphSize=opts.minGraphSize,
+                                    threshold=opts.threshold, C_threshold=opts.C_thresh)
     else:
         st = engine.run_basic_dev(dg, C, basicOpt=opts.basicOpt, threshold=opts.threshold)
     st["vertices_folded"] = folded

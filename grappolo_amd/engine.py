@@ -67,7 +67,7 @@ _SIG = {
     "glv_vertexFollowing": [ctypes.c_int64, _I64P, _VP, _I64P, _I64P],
     "glv_modularity": [ctypes.c_int64, _I64P, _VP, _I64P, _F64P],
     "glv_runMultiPhaseColoring_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP,
-                                      ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
                                       ctypes.c_double, _VP, ctypes.POINTER(Stats)],
     "glv_runMultiPhaseBasic_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _VP,
                                    ctypes.c_int, ctypes.c_double, _VP, ctypes.POINTER(Stats)],
@@ -94,6 +94,12 @@ _SIG = {
     "glv_computeCommunityComparisons_dev": [_VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, _VP],
     "glv_computeGiniCoefficient": [_I64P, ctypes.c_int, _F64P],
     "glv_computeGiniCoefficient_dev": [_VP, ctypes.c_int, _F64P, _VP],
+    "glv_algoColoringMultiHashMaxMin": [ctypes.c_int64, _I64P, _VP, _I32P, ctypes.c_int, ctypes.c_int, _I32P],
+    "glv_algoColoringMultiHashMaxMin_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int,
+                                            ctypes.c_int, _I32P, _VP],
+    "glv_vBaseRedistribution": [ctypes.c_int64, _I64P, _VP, _I32P, ctypes.c_int, ctypes.c_int, _I32P],
+    "glv_vBaseRedistribution_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int,
+                                    _I32P, _VP],
     "glv_dev_free": [_VP],
     "glv_count_conflicts_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, _I64P, _VP],
     "glv_profile_enable": [ctypes.c_int],
@@ -242,6 +248,38 @@ def algoDistanceOneVertexColoringOpt(G):
     return colors, mc.value
 
 
+def algoColoringMultiHashMaxMin(G, nHash=2, nItrs=4):
+    """coloring.h:12.  Returns (colors, maxColor)."""
+    args, keep = _host_graph(G)
+    c = np.empty(G.numVertices, np.int32)
+    mc = ctypes.c_int32(0)
+    _call("glv_algoColoringMultiHashMaxMin", *args, _p(c, _I32P), nHash, nItrs, ctypes.byref(mc))
+    return c, mc.value
+
+
+def vBaseRedistribution(G, colors, ncolors, type=0):
+    """coloring.h:15.  Returns (recolored copy, rounds)."""
+    args, keep = _host_graph(G)
+    c = np.array(colors, dtype=np.int32)
+    r = ctypes.c_int32(0)
+    _call("glv_vBaseRedistribution", *args, _p(c, _I32P), int(ncolors), int(type), ctypes.byref(r))
+    return c, r.value
+
+
+def multihash_dev(dg, colors, nHash=2, nItrs=4):
+    mc = ctypes.c_int32(0)
+    _call("glv_algoColoringMultiHashMaxMin_dev", dg.numVertices, dg.nnz, _ptr(dg.off), _ptr(dg.tails),
+          _ptr(colors), nHash, nItrs, ctypes.byref(mc), _stream())
+    return mc.value
+
+
+def balance_dev(dg, colors, ncolors, type=0):
+    r = ctypes.c_int32(0)
+    _call("glv_vBaseRedistribution_dev", dg.numVertices, dg.nnz, _ptr(dg.off), _ptr(dg.tails), _ptr(colors),
+          int(ncolors), int(type), ctypes.byref(r), _stream())
+    return r.value
+
+
 def algoLouvainWithDistOneColoring(G, color, numColor, Lower=-1.0, thresh=1e-6):
     """color_comm.h:10.  Returns (modularity, C, numItr)."""
     args, keep = _host_graph(G)
@@ -362,9 +400,9 @@ def _ptr(t):
 
 
 def run_colored_dev(dg, C_orig, coloring=1, replaceMap=0, minGraphSize=1000, threshold=1e-9,
-                    C_threshold=1e-6, stats=None):
+                    C_threshold=1e-6, stats=None, numColors=16):
     st = stats if stats is not None else new_stats()
-    _call("glv_runMultiPhaseColoring_dev", *dg.ptrs(), _ptr(C_orig), coloring, replaceMap,
+    _call("glv_runMultiPhaseColoring_dev", *dg.ptrs(), _ptr(C_orig), coloring, numColors, replaceMap,
           minGraphSize, threshold, C_threshold, _stream(), ctypes.byref(st))
     return stats_dict(st)
 

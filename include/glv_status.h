@@ -10,7 +10,7 @@ enum {
     GLV_E_OOM = 2,            /* host or device allocation failed */
     GLV_E_DEVICE = 3,         /* a HIP call failed or no device is present */
     GLV_E_TOO_MANY_COLORS = 4,/* a coloring needed more than 4096 colors */
-    GLV_E_UNSUPPORTED = 5,    /* option outside the implemented path (e.g. coloring 2/3) */
+    GLV_E_UNSUPPORTED = 5,    /* option outside the implemented path (e.g. syncType) */
     GLV_E_INVALID_ARG = 6,    /* bad scalar argument or null pointer */
     GLV_E_IO = 7              /* file could not be read or written */
 };

@@ -8,4 +8,15 @@
    rounds with hash priorities, so they differ from the reference's speculative
    coloring; only validity is part of the contract. */
 int algoDistanceOneVertexColoringOpt(graph *G, int *vtxColor, int nThreads, double *totTime);
+
+/* coloring.h:12 -- coloringMultiHashMaxMin.cpp:60-226: incomplete coloring by
+   nItrs iterations of nHash max/min passes; vertices left uncoloured hold the
+   returned maxColor.  Random values come from a hash of the vertex id (the
+   reference draws rand()). */
+int algoColoringMultiHashMaxMin(graph *G, int *vtxColor, int nThreads, double *totTime, int nHash, int nItrs);
+
+/* coloring.h:15 -- vBase.cpp:6-168: rebalances a proper coloring of ncolors
+   colors in place (type 0 first fit, 1 least used) in deterministic rounds;
+   returns the number of rounds. */
+int vBaseRedistribution(graph *G, int *vtxColor, int ncolors, int type);
 #endif

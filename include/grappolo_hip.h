@@ -80,7 +80,10 @@ const char *glv_version(void);
 /* ---- whole-pipeline entry points (host arrays) --------------------------- */
 
 /* runMultiPhaseColoring, DefineStructure/color_comm.h:7 and
- * Coloring/runMultiPhaseColoring.cpp:51-238.  coloring must be 1.
+ * Coloring/runMultiPhaseColoring.cpp:51-238.  coloring 1: distance-1
+ * (Jones-Plassmann); 2: distance-1 then balanced (glv_vBaseRedistribution,
+ * type 0); 3: incomplete multi-hash with nHash 2 and nItrs = max(numColors/4,
+ * 1) (glv_algoColoringMultiHashMaxMin); numColors is used by coloring 3 only.
  * C_orig[NV] receives the partition (-1 for vertices without edges).
  * The graph is not modified (the C++ drop-in frees it, as the reference does).
  * numThreads/threadsOpt are accepted and ignored (no host threads are used). */
@@ -157,7 +160,7 @@ void glv_free(void *p);
 
 int glv_runMultiPhaseColoring_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
                                   const int64_t *d_tails, const double *d_weights, int64_t *d_C_orig,
-                                  int coloring, int replaceMap, int64_t minGraphSize,
+                                  int coloring, int numColors, int replaceMap, int64_t minGraphSize,
                                   double threshold, double C_threshold, void *stream,
                                   glv_stats *stats);
 int glv_runMultiPhaseBasic_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets,
@@ -314,6 +317,36 @@ int glv_set_bins(const int64_t *max_deg);
 int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
                           const double *d_weights, int reps, int64_t *max_deg, double *ms_default,
                           double *ms_tuned, void *stream);
+
+/* ---- the other colorings of runMultiPhaseColoring ------------------------
+ * algoColoringMultiHashMaxMin, DefineStructure/coloring.h:12 and
+ * coloringMultiHashMaxMin.cpp:60-226 (coloring == 3): nItrs iterations of
+ * nHash passes; in pass (itr, h) an uncoloured vertex whose hash value is a
+ * strict maximum (minimum) among its uncoloured or just-coloured neighbours
+ * takes colour 2*itr*nHash + 2h (+1).  Vertices left uncoloured hold
+ * *maxColor = 2*nHash*nItrs (smaller when an iteration finds every vertex
+ * coloured, :165-170).  The hash value of vertex v under hash h is the top
+ * 53 bits of splitmix64(v ^ ((h+1) << 56)); the reference draws rand(). */
+int glv_algoColoringMultiHashMaxMin(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList,
+                                    int32_t *vtxColor, int nHash, int nItrs, int32_t *maxColor);
+int glv_algoColoringMultiHashMaxMin_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                                        int32_t *d_colors, int nHash, int nItrs, int32_t *maxColor, void *stream);
+/* vBaseRedistribution, DefineStructure/coloring.h:15 and vBase.cpp:6-168
+ * (coloring == 2 applies it with type 0 after the distance-1 coloring).
+ * avg = ceil(NV / ncolors); colours whose class starts above avg are
+ * oversize.  In rounds, every vertex of an oversize base class whose current
+ * class still holds more than avg wishes for an eligible colour (not
+ * oversize, fewer than avg vertices, held by no neighbour): the first one
+ * (type 0) or the least used, smaller on ties (type 1).  Adjacent vertices
+ * with the same wish keep only the higher priority (splitmix64 of the id);
+ * each wished colour admits up to avg - size wishers and each class lets go
+ * of up to size - avg of them, in decreasing priority.  Rounds end when one
+ * moves nothing.  The reference applies the same rules one vertex at a time
+ * inside a parallel loop (its outcome depends on the schedule). */
+int glv_vBaseRedistribution(int64_t nv, const int64_t *edgeListPtrs, const glv_edge *edgeList, int32_t *vtxColor,
+                            int ncolors, int type, int32_t *rounds);
+int glv_vBaseRedistribution_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                                int32_t *d_colors, int ncolors, int type, int32_t *rounds, void *stream);
 
 /* ---- cluster comparison (the -r metrics and driverForClusterComparison) --
  * computeCommunityComparisons, Utility/utilityClusteringFunctions.h:71 and

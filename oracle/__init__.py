@@ -118,9 +118,18 @@ def _declare(L):
     L.orc_compare.restype = ctypes.c_int
     L.orc_gini.argtypes = [_I64P, ctypes.c_int]
     L.orc_gini.restype = ctypes.c_double
-    L.orc_run_colored.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+    L.orc_run_colored.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(Stats)]
+    L.orc_mh_value.argtypes = [ctypes.c_int, ctypes.c_int64]
+    L.orc_mh_value.restype = ctypes.c_double
+    L.orc_color_multihash.argtypes = [ctypes.c_int64, _I64P, _I64P, ctypes.c_int, ctypes.c_int, _I32P]
+    L.orc_color_multihash.restype = ctypes.c_int
+    L.orc_color_multihash_vals.argtypes = [ctypes.c_int64, _I64P, _I64P, ctypes.c_int, ctypes.c_int, _F64P,
+                                           _I32P]
+    L.orc_color_multihash_vals.restype = ctypes.c_int
+    L.orc_color_balance.argtypes = [ctypes.c_int64, _I64P, _I64P, _I32P, ctypes.c_int, ctypes.c_int]
+    L.orc_color_balance.restype = ctypes.c_int
     L.orc_run_colored.restype = ctypes.c_int
     L.orc_run_basic.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Stats)]
     L.orc_run_basic.restype = ctypes.c_int
@@ -150,6 +159,9 @@ def _declare_ref(R):
     R.ref_compare.restype = None
     R.ref_gini.argtypes = [_I64P, ctypes.c_int]
     R.ref_gini.restype = ctypes.c_double
+    R.ref_multihash.argtypes = [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_uint, _I32P, _F64P]
+    R.ref_multihash.restype = ctypes.c_int
     R.ref_free.argtypes = [ctypes.c_void_p]
     R.ref_free.restype = None
     R.ref_run_colored.argtypes = G + [_I64P, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
@@ -303,18 +315,61 @@ def _stats_dict(st, arrays):
     return d
 
 
+_COLORING_MODES = {"greedy": 0, "jp": 1, "balanced": 2, "multihash": 3}
+
+
 def run_colored(g, coloring="greedy", frozen=False, replaceMap=0, minGraphSize=1000,
-                threshold=1e-9, C_threshold=1e-6):
-    """runMultiPhaseColoring (coloring=1).  coloring='greedy' with frozen=False
-    is the reference at one thread; 'jp' with frozen=True is the engine."""
+                threshold=1e-9, C_threshold=1e-6, numColors=16):
+    """runMultiPhaseColoring.  coloring='greedy' with frozen=False is the
+    reference at one thread (coloring=1); 'jp', 'balanced' and 'multihash'
+    with frozen=True are the engine's coloring=1, 2 and 3."""
     C_orig = np.empty(g.numVertices, dtype=np.int64)
     st, arrays = _stats()
-    rc = lib().orc_run_colored(*_g(g), _p(C_orig, _I64P), 1 if coloring == "jp" else 0,
+    rc = lib().orc_run_colored(*_g(g), _p(C_orig, _I64P), _COLORING_MODES[coloring], int(numColors),
                                int(frozen), int(replaceMap), int(minGraphSize), threshold,
                                C_threshold, ctypes.byref(st))
     if rc != 0:
         raise RuntimeError(f"orc_run_colored failed ({rc})")
     return C_orig, _stats_dict(st, arrays)
+
+
+def color_multihash(g, nHash=2, nItrs=4):
+    """(colors, maxColor) of the restated incomplete coloring."""
+    c = np.empty(g.numVertices, np.int32)
+    mc = lib().orc_color_multihash(g.numVertices, _p(g.edgeListPtrs, _I64P), _p(g.tails, _I64P), nHash,
+                                   nItrs, _p(c, _I32P))
+    return c, mc
+
+
+def color_balance(g, colors, ncolors, type=0):
+    """(recolored copy, rounds) of the restated balanced recoloring."""
+    c = np.array(colors, dtype=np.int32)
+    r = lib().orc_color_balance(g.numVertices, _p(g.edgeListPtrs, _I64P), _p(g.tails, _I64P), _p(c, _I32P),
+                                int(ncolors), int(type))
+    if r < 0:
+        raise ValueError(f"orc_color_balance failed ({r})")
+    return c, r
+
+
+def color_multihash_vals(g, nHash, nItrs, vals):
+    c = np.empty(g.numVertices, np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    mc = lib().orc_color_multihash_vals(g.numVertices, _p(g.edgeListPtrs, _I64P), _p(g.tails, _I64P), nHash,
+                                        nItrs, _p(vals, _F64P), _p(c, _I32P))
+    return c, mc
+
+
+def ref_multihash(g, nHash, nItrs, seed=1):
+    """(colors, maxColor, the rand() values drawn) of the compiled reference."""
+    c = np.empty(g.numVertices, np.int32)
+    vals = np.empty(max(nHash * g.numVertices, 1))
+    mc = ref().ref_multihash(g.numVertices, g.numEdges, _p(g.edgeListPtrs, _I64P), _p(g.tails, _I64P), nHash,
+                             nItrs, seed, _p(c, _I32P), _p(vals, _F64P))
+    return c, mc, vals[:nHash * g.numVertices]
+
+
+def mh_value(h, v):
+    return lib().orc_mh_value(h, v)
 
 
 def run_basic(g, basicOpt=0, threshold=1e-9):

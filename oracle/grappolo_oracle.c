@@ -12,6 +12,7 @@
  */
 #include "grappolo_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -297,6 +298,203 @@ int orc_color_jp(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int3
     free(win);
     if (rounds_out) *rounds_out = (int32_t)rounds;
     return maxc;
+}
+
+/* ---- incomplete coloring: coloringMultiHashMaxMin.cpp:60-226 ------------- */
+
+/* The random value of vertex v under hash h.  The reference draws rand()
+ * values per call (:54-60, "need to fix this with a hash function"); the
+ * engine uses this hash: the top 53 bits of splitmix64 of (v, h). */
+double orc_mh_value(int h, i64 v) {
+    return (double)(orc_jp_priority(v ^ ((i64)(h + 1) << 56)) >> 11);
+}
+
+/* algoColoringMultiHashMaxMin: nItrs iterations of nHash passes; in pass
+ * (itr, h) every uncoloured vertex whose value is a strict maximum (minimum)
+ * over its neighbours that are uncoloured or coloured in this pass
+ * (self-loops ignored) takes color 2*itr*nHash + 2h (+1).  Uncoloured
+ * vertices keep maxColor = 2*nHash*nItrs; an iteration that colors nothing
+ * after every vertex got a color ends the loop with maxColor =
+ * 2*(itr-1)*nHash + 2*nHash + 1 (:165-170).  Returns maxColor, the largest
+ * color index in use. */
+int orc_color_multihash(i64 nv, const i64 *off, const i64 *tails, int nHash, int nItrs, int32_t *colors) {
+    if (nHash <= 0 || nItrs <= 0) return -1;
+    double *vals = (double *)malloc((size_t)(nv > 0 ? nv : 1) * (size_t)nHash * sizeof(double));
+    if (!vals) return -3;
+    for (int h = 0; h < nHash; h++)
+        for (i64 v = 0; v < nv; v++) vals[(i64)h * nv + v] = orc_mh_value(h, v);
+    int mc = orc_color_multihash_vals(nv, off, tails, nHash, nItrs, vals, colors);
+    free(vals);
+    return mc;
+}
+
+/* The same with the random values given: vals[h * nv + v]. */
+int orc_color_multihash_vals(i64 nv, const i64 *off, const i64 *tails, int nHash, int nItrs,
+                             const double *vals, int32_t *colors) {
+    if (nHash <= 0 || nItrs <= 0) return -1;
+    int maxColor = 2 * nHash * nItrs;
+    i64 totalColored = 0, iterFreq = 0;
+    for (i64 v = 0; v < nv; v++) colors[v] = maxColor;
+    for (int itr = 0; itr < nItrs; itr++) {
+        for (int h = 0; h < nHash; h++) {
+            const int cur = 2 * itr * nHash + 2 * h;
+            /* decisions of one pass read only colors < cur, which the pass
+             * never writes, so they can be applied as they are made */
+            for (i64 v = 0; v < nv; v++) {
+                if (colors[v] != maxColor) continue;
+                const double rv = vals[(i64)h * nv + v];
+                int isMax = 1, isMin = 1;
+                for (i64 j = off[v]; j < off[v + 1]; j++) {
+                    const i64 u = tails[j];
+                    if (u == v) continue;
+                    if (colors[u] < cur) continue;
+                    const double ru = vals[(i64)h * nv + u];
+                    if (rv <= ru) isMax = 0;
+                    if (rv >= ru) isMin = 0;
+                }
+                if (isMax) {
+                    colors[v] = cur;
+                    iterFreq++;
+                } else if (isMin) {
+                    colors[v] = cur + 1;
+                    iterFreq++;
+                }
+            }
+        }
+        totalColored += iterFreq;
+        if (iterFreq == 0) {
+            if (totalColored == nv) {
+                maxColor = (2 * (itr - 1) * nHash) + 2 * nHash + 1;
+                break;
+            }
+        } else {
+            iterFreq = 0;
+        }
+    }
+    return maxColor;
+}
+
+/* ---- balanced recoloring: vBase.cpp:6-168 (type 0, first fit) ------------
+ * The reference moves vertices one at a time inside a parallel loop whose
+ * counters and neighbour colours are read while other threads write them, so
+ * its result depends on the schedule.  The engine runs a deterministic
+ * round-synchronous form of the same rules:
+ *   avg = ceil(NV / ncolors); a colour is oversize when its class starts
+ *   above avg (fixed for the run, :92-100);
+ *   a vertex is a candidate while its base class is oversize and its current
+ *   class holds more than avg vertices (:109-112);
+ *   its wish is the first colour c with no neighbour of colour c (self-loops
+ *   ignored), size(c) < avg and c not oversize (:118-122).
+ * In a round every candidate computes its wish from the state at the start
+ * of the round.  A wish is dropped when an adjacent candidate with the same
+ * wish has a higher priority (orc_jp_priority).  The remaining wishers of
+ * colour c are admitted in decreasing priority up to avg - size(c) of them;
+ * then the admitted movers leaving class X are kept in decreasing priority up
+ * to size(X) - avg of them.  The kept moves are applied and the sizes
+ * updated; rounds repeat until a round moves nothing.  As in the reference
+ * at one thread: the result is a proper colouring, no colour that was not
+ * oversize ends above avg, an oversize class never grows nor drops below
+ * avg, and no remaining candidate has a colour it could move to.
+ * Returns the number of rounds, or a negative error. */
+typedef struct {
+    uint64_t pri;
+    i64 v;
+    int32_t seg;
+} BalItem;
+
+static int bal_cmp(const void *a, const void *b) {
+    const BalItem *x = (const BalItem *)a, *y = (const BalItem *)b;
+    if (x->seg != y->seg) return x->seg < y->seg ? -1 : 1;
+    if (x->pri != y->pri) return x->pri > y->pri ? -1 : 1; /* decreasing priority */
+    return 0;
+}
+
+int orc_color_balance(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int ncolors, int type) {
+    if (ncolors <= 0 || ncolors > ORC_MAX_COLORS || (type != 0 && type != 1)) return -2;
+    const i64 avg = (i64)ceil((double)nv / (double)ncolors);
+    i64 *freq = (i64 *)calloc((size_t)ncolors, sizeof(i64));
+    unsigned char *over = (unsigned char *)calloc((size_t)ncolors, 1);
+    int32_t *base = (int32_t *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(int32_t));
+    int32_t *wish = (int32_t *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(int32_t));
+    BalItem *items = (BalItem *)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(BalItem));
+    unsigned char mark[ORC_MAX_COLORS];
+    if (!freq || !over || !base || !wish || !items) {
+        free(freq); free(over); free(base); free(wish); free(items);
+        return -3;
+    }
+    for (i64 v = 0; v < nv; v++) {
+        if (colors[v] < 0 || colors[v] >= ncolors) {
+            free(freq); free(over); free(base); free(wish); free(items);
+            return -1;
+        }
+        base[v] = colors[v];
+        freq[colors[v]]++;
+    }
+    for (int c = 0; c < ncolors; c++) over[c] = freq[c] > avg;
+    int rounds = 0;
+    for (;;) {
+        /* wishes from the state at the start of the round */
+        for (i64 v = 0; v < nv; v++) {
+            wish[v] = -1;
+            if (!over[base[v]] || freq[colors[v]] <= avg) continue;
+            memset(mark, 0, (size_t)ncolors);
+            for (i64 j = off[v]; j < off[v + 1]; j++) {
+                const i64 u = tails[j];
+                if (u != v) mark[colors[u]] = 1;
+            }
+            for (int c = 0; c < ncolors; c++)
+                if (!mark[c] && freq[c] < avg && !over[c]) {
+                    /* type 0 (first fit): the first; type 1 (least used, :124-131):
+                     * the fewest vertices, the smaller colour on ties */
+                    if (wish[v] < 0 || freq[c] < freq[wish[v]]) wish[v] = c;
+                    if (type == 0) break;
+                }
+        }
+        /* same-wish neighbours: the higher priority keeps its wish */
+        i64 n = 0;
+        for (i64 v = 0; v < nv; v++) {
+            if (wish[v] < 0) continue;
+            const uint64_t pv = orc_jp_priority(v);
+            int ok = 1;
+            for (i64 j = off[v]; j < off[v + 1] && ok; j++) {
+                const i64 u = tails[j];
+                if (u != v && wish[u] == wish[v] && orc_jp_priority(u) > pv) ok = 0;
+            }
+            if (ok) items[n++] = (BalItem){orc_jp_priority(v), v, wish[v]};
+        }
+        /* capacity of the target colour */
+        qsort(items, (size_t)n, sizeof(BalItem), bal_cmp);
+        i64 m = 0;
+        for (i64 q = 0, rank = 0, prev = -1; q < n; q++) {
+            const int32_t c = items[q].seg;
+            rank = (c == prev) ? rank + 1 : 0;
+            prev = c;
+            if (rank < avg - freq[c]) {
+                items[m] = items[q];
+                items[m].seg = colors[items[m].v]; /* next key: the class left */
+                m++;
+            }
+        }
+        /* departures of the source class */
+        qsort(items, (size_t)m, sizeof(BalItem), bal_cmp);
+        i64 moved = 0;
+        for (i64 q = 0, rank = 0, prev = -1; q < m; q++) {
+            const int32_t c = items[q].seg;
+            rank = (c == prev) ? rank + 1 : 0;
+            prev = c;
+            if (rank < freq[c] - avg) items[moved++] = items[q];
+        }
+        if (moved == 0) break;
+        for (i64 q = 0; q < moved; q++) {
+            const i64 v = items[q].v;
+            freq[colors[v]]--;
+            freq[wish[v]]++;
+            colors[v] = wish[v];
+        }
+        rounds++;
+    }
+    free(freq); free(over); free(base); free(wish); free(items);
+    return rounds;
 }
 
 /* The verification pass of coloringDistanceOne.cpp:207-226 (conflicting
@@ -945,23 +1143,39 @@ static void st_phase(orc_stats *st, i64 nv, i64 nnz, int iters, int ncol, int co
 
 /* runMultiPhaseColoring, Coloring/runMultiPhaseColoring.cpp:51-238, for
  * coloring == 1. */
+/* One coloring step of runMultiPhaseColoring.cpp:68-87 / :183-197 for
+ * coloring_mode 0 (greedy), 1 (JP), 2 (JP then balanced), 3 (multi-hash,
+ * nHash = 2, nItrs = max(numColors / 4, 1)).  Returns nColors or < 0. */
+static int color_step(const OGraph *G, int coloring_mode, int numColors, int32_t *colors) {
+    for (i64 i = 0; i < G->nv; i++) colors[i] = -1;
+    int mc;
+    if (coloring_mode == 3) {
+        int nItrs = numColors / 4;
+        if (nItrs <= 0) nItrs = 1;
+        mc = orc_color_multihash(G->nv, G->off, G->tails, 2, nItrs, colors);
+        return mc < 0 ? -2 : mc + 1;
+    }
+    mc = coloring_mode ? orc_color_jp(G->nv, G->off, G->tails, colors, NULL)
+                       : orc_color_greedy(G->nv, G->off, G->tails, colors);
+    if (mc < -1) return -2;
+    if (coloring_mode == 2 && orc_color_balance(G->nv, G->off, G->tails, colors, mc + 1, 0) < 0) return -2;
+    return mc + 1;
+}
+
 int orc_run_colored(i64 nv, const i64 *off, const i64 *tails, const double *w, i64 *C_orig,
-                    int coloring_mode, int frozen, int replaceMap, i64 minGraphSize,
+                    int coloring_mode, int numColors, int frozen, int replaceMap, i64 minGraphSize,
                     double threshold, double C_threshold, orc_stats *st) {
     i64 NV = nv;
     OGraph G = {nv, (i64 *)off, (i64 *)tails, (double *)w, 0};
     int32_t *colors = (int32_t *)malloc((size_t)(NV > 0 ? NV : 1) * sizeof(int32_t));
     i64 *C = (i64 *)malloc((size_t)(NV > 0 ? NV : 1) * sizeof(i64));
     if (!colors || !C) return -3;
-    for (i64 i = 0; i < NV; i++) colors[i] = -1;
-    int mc = coloring_mode ? orc_color_jp(G.nv, G.off, G.tails, colors, NULL)
-                           : orc_color_greedy(G.nv, G.off, G.tails, colors);
-    if (mc < -1) {
+    int nColors = color_step(&G, coloring_mode, numColors, colors);
+    if (nColors < 0) {
         free(colors);
         free(C);
         return -2;
     }
-    int nColors = mc + 1;
     if (st) st->colors_first = nColors;
     double prevMod = -1, currMod = -1;
     i64 phase = 1, numClusters = 0;
@@ -1001,11 +1215,8 @@ int orc_run_colored(i64 nv, const i64 *off, const i64 *tails, const double *w, i
             for (i64 i = 0; i < numClusters; i++) C[i] = -1;
             phase++;
             if ((G.nv > minGraphSize) && (nonColor == 0)) {
-                for (i64 i = 0; i < G.nv; i++) colors[i] = -1;
-                mc = coloring_mode ? orc_color_jp(G.nv, G.off, G.tails, colors, NULL)
-                                   : orc_color_greedy(G.nv, G.off, G.tails, colors);
-                if (mc < -1) return -2;
-                nColors = mc + 1;
+                nColors = color_step(&G, coloring_mode, numColors, colors);
+                if (nColors < 0) return -2;
             }
         } else {
             if (nonColor == 0)

@@ -109,10 +109,22 @@ int orc_compare(int64_t *C1, int64_t *C2, int64_t N, orc_comparison *out);
 /* utilityClusterComparisonMetrics.cpp:309-330; sorts colorSize in place */
 double orc_gini(int64_t *colorSize, int numColors);
 
+/* coloringMultiHashMaxMin.cpp:60-226 with hashed random values; returns maxColor */
+double orc_mh_value(int h, int64_t v);
+int orc_color_multihash(int64_t nv, const int64_t *off, const int64_t *tails, int nHash, int nItrs,
+                        int32_t *colors);
+int orc_color_multihash_vals(int64_t nv, const int64_t *off, const int64_t *tails, int nHash, int nItrs,
+                             const double *vals, int32_t *colors);
+/* vBase.cpp:6-168 (type 0) as deterministic rounds; recolors in place,
+ * returns the number of rounds or < 0 */
+int orc_color_balance(int64_t nv, const int64_t *off, const int64_t *tails, int32_t *colors, int ncolors,
+                      int type);
+
 /* runMultiPhaseColoring.cpp:51-238.  coloring_mode 0: greedy (reference at one
- * thread), 1: Jones-Plassmann (engine).  Returns 0 or a negative error. */
+ * thread), 1: Jones-Plassmann (engine), 2: Jones-Plassmann then balanced,
+ * 3: multi-hash incomplete coloring with numColors.  Returns 0 or < 0. */
 int orc_run_colored(int64_t nv, const int64_t *off, const int64_t *tails, const double *w,
-                    int64_t *C_orig, int coloring_mode, int frozen, int replaceMap,
+                    int64_t *C_orig, int coloring_mode, int numColors, int frozen, int replaceMap,
                     int64_t minGraphSize, double threshold, double C_threshold, orc_stats *st);
 
 /* runMultiPhaseBasic.cpp:52-170 */

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iostream>
 #include <vector>
 
 #include <omp.h>
@@ -66,6 +67,7 @@ struct Quiet {
         }
     }
     ~Quiet() {
+        std::cout.flush();
         fflush(stdout);
         if (saved >= 0) {
             dup2(saved, 1);
@@ -224,6 +226,26 @@ void ref_compare(int64_t *C1, int64_t *C2, int64_t N, int nthreads, const char *
 double ref_gini(int64_t *sizes, int n) {
     Quiet q(nullptr);
     return computeGiniCoefficient((long *)sizes, n);
+}
+
+// algoColoringMultiHashMaxMin (coloringMultiHashMaxMin.cpp:60-226) at one
+// thread after srand(seed); vals receives the rand() values it drew,
+// vals[h * nv + v] (generateRandomNumbers2, :52-58, hash by hash).
+int ref_multihash(int64_t nv, int64_t num_edges, const int64_t *off, const int64_t *tails, int nHash,
+                  int nItrs, unsigned seed, int32_t *colors, double *vals) {
+    graph *G = make_graph(nv, num_edges, off, tails, nullptr);
+    double t = 0;
+    int mc;
+    srand(seed);
+    {
+        Quiet q(nullptr);
+        mc = algoColoringMultiHashMaxMin(G, colors, 1, &t, nHash, nItrs);
+    }
+    srand(seed);
+    for (int h = 0; h < nHash; h++)
+        for (int64_t v = 0; v < nv; v++) vals[h * nv + v] = (double)rand();
+    free_graph(G);
+    return mc;
 }
 
 void ref_free(void *p) { free(p); }

@@ -287,8 +287,8 @@ def test_invalid_inputs_raise():
         engine.runMultiPhaseColoring(bad)
     assert e.value.code == 1
     with pytest.raises(engine.GlvError) as e:
-        engine.runMultiPhaseColoring(g, coloring=3)
-    assert e.value.code == 5
+        engine.runMultiPhaseColoring(g, coloring=4)  # runMultiPhaseColoring.cpp:55 asserts 0 < coloring < 4
+    assert e.value.code == 6
     colors = np.full(g.numVertices, 7, np.int32)
     with pytest.raises(engine.GlvError):
         engine.algoLouvainWithDistOneColoring(g, colors, 3)

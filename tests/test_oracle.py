@@ -482,3 +482,81 @@ def test_gini_matches_reference():
         rg, rs = oracle.ref_gini(sz)
         assert g == rg
         np.testing.assert_array_equal(s, rs)
+
+
+# ------------------------------------------------------- coloring 2 and 3
+def _graphs_for_coloring():
+    from graphs import pendants
+    c = catalog()
+    return [karate(), c["planted_unit"], c["loops_isolated"], c["rmat12"], c["hubs"], pendants(seed=4)]
+
+
+@need_ref
+@pytest.mark.parametrize("nItrs", [1, 2, 4])
+def test_multihash_matches_reference_with_its_random_values(nItrs):
+    """coloringMultiHashMaxMin.cpp:60-226: given the rand() values the
+    reference drew, the restatement colors every vertex the same and returns
+    the same maxColor (including the early-exit value of :165-170)."""
+    for g in _graphs_for_coloring():
+        c, mc, vals = oracle.ref_multihash(g, 2, nItrs, seed=11)
+        o, omc = oracle.color_multihash_vals(g, 2, nItrs, vals)
+        assert omc == mc
+        np.testing.assert_array_equal(o, c)
+
+
+def test_multihash_classes_are_independent():
+    """Every class but the uncoloured one (maxColor) is an independent set;
+    a vertex stays uncoloured only if every pass saw a neighbour on both sides."""
+    for g in _graphs_for_coloring():
+        for nItrs in (1, 3):
+            c, mc = oracle.color_multihash(g, 2, nItrs)
+            assert c.min() >= 0 and c.max() <= mc
+            rows = np.repeat(np.arange(g.numVertices), g.degrees())
+            bad = (rows != g.tails) & (c[rows] == c[g.tails]) & (c[rows] != mc)
+            assert not bad.any()
+
+
+def _balance_invariants(g, base, out, ncolors, type):
+    import math
+    nv = g.numVertices
+    avg = math.ceil(nv / ncolors)
+    f0 = np.bincount(base, minlength=ncolors)
+    f1 = np.bincount(out, minlength=ncolors)
+    over = f0 > avg
+    rows = np.repeat(np.arange(nv), g.degrees())
+    nl = rows != g.tails
+    assert not np.any(out[rows[nl]] == out[g.tails[nl]])          # proper
+    assert np.all(f1[~over] <= avg)                                # non-oversize capped
+    assert np.all(f1[over] <= f0[over]) and np.all(f1[over] >= avg)
+    moved = out != base
+    assert np.all(over[base[moved]]) and not np.any(over[out[moved]])
+    elig = (~over) & (f1 < avg)
+    for v in np.flatnonzero(over[base] & (f1[out] > avg)):        # nothing left to move
+        held = set(out[g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]]].tolist())
+        assert not any(elig[c] and c not in held for c in range(ncolors)), v
+
+
+@pytest.mark.parametrize("type", [0, 1])
+def test_balance_invariants(type):
+    """vBase.cpp:6-168: what its rules fix -- a proper coloring, colors that
+    were not oversize end at most avg, oversize classes neither grow nor drop
+    below avg, only vertices of oversize classes move, and no vertex that
+    could still move is left.  (The compiled reference's vBaseRedistribution
+    cannot be run as a checker: when a vertex finds no eligible color it reads
+    and writes one past its count arrays, and on karate at one thread it
+    aborts with a corrupted heap.)"""
+    for g in _graphs_for_coloring():
+        for base, mc in (oracle.color_greedy(g), oracle.color_jp(g)[:2]):
+            out, rounds = oracle.color_balance(g, base, mc + 1, type)
+            _balance_invariants(g, base, out, mc + 1, type)
+            assert rounds >= 1 or np.array_equal(out, base)
+
+
+def test_balance_least_used_picks_smallest_class():
+    """type 1 on a star whose leaves are all in one oversize class: the
+    leaves go to the emptiest eligible color first."""
+    g = Graph.from_pairs(9, np.array([(0, i) for i in range(1, 9)], np.int64))
+    out, _ = oracle.color_balance(g, np.array([0, 1, 1, 1, 1, 1, 1, 1, 2], np.int32), 3, 1)
+    # avg = 3; class 1 (7 leaves) is oversize; color 0 is the centre's, so
+    # color 2 takes leaves until it holds 3
+    assert np.bincount(out, minlength=3).tolist() == [1, 5, 3]
