@@ -870,36 +870,37 @@ __device__ inline void block_argmax_p(bool &cand, double &gain, i64 &id, int *sc
 
 // Grid barrier over nb <= 64 workgroups without a shared counter: workgroup
 // b publishes the barrier's sequence number in its own flag slot (one cache
-// line each), and the first wavefront of every workgroup polls all nb slots at
-// once.  Bounded wait; a workgroup that gives up raises bar[0] (abort), which
+// line each) with an agent-scope release, and the first wavefront of every
+// workgroup polls all nb slots at once with relaxed agent-scope loads, then
+// takes one agent-scope acquire (one L2 write-back per arrival and one
+// invalidate per departure, instead of an invalidate per poll: 2.2 us per
+// barrier at 64 workgroups against 4.3 us with system-scope polling).
+// Bounded wait; a workgroup that gives up raises bar[0] (abort), which
 // releases every other wait.  Returns false when the barrier was abandoned.
 __device__ bool grid_barrier(unsigned *bar, unsigned nb, unsigned seq) {
     __shared__ int s_ok;
     unsigned *flags = bar + 16;  // flag of workgroup b at flags[16 * b]
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        __atomic_store_n(&flags[16 * blockIdx.x], seq, __ATOMIC_RELEASE);
-    }
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&flags[16 * blockIdx.x], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x < kWave) {
         const int lane = threadIdx.x;
         unsigned spins = 0;
         int ok = 1;
         while (true) {
             bool done = true;
-            if ((unsigned)lane < nb) done = __atomic_load_n(&flags[16 * lane], __ATOMIC_ACQUIRE) >= seq;
+            if ((unsigned)lane < nb)
+                done = __hip_atomic_load(&flags[16 * lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seq;
             if (__ballot(!done) == 0) break;
-            if (__atomic_load_n(&bar[0], __ATOMIC_RELAXED) || ++spins > kSpinLimit) {
+            if (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > kSpinLimit) {
                 if (lane == 0) atomicOr(&bar[0], 1u);
                 ok = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        if (lane == 0) {
-            __threadfence();
-            s_ok = ok && !__atomic_load_n(&bar[0], __ATOMIC_RELAXED);
-        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (lane == 0) s_ok = ok && !__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     return s_ok != 0;
@@ -918,18 +919,29 @@ constexpr size_t kPHashBytes = (size_t)kPHashSlots * 16 + (size_t)kPBitmapWords 
 constexpr size_t kPSmemBytes = kPDenseBytes > kPHashBytes ? kPDenseBytes : kPHashBytes;
 
 // cperm: vertices grouped by color (cstart[c] .. cstart[c+1]).
+//
+// One grid barrier per color step: the labels, community degrees and sizes
+// are double-buffered ((C, a, n) and (C1, a1, n1), equal at launch).  Step k
+// reads buffer k % 2, which holds the state after step k-1, and brings the
+// other buffer -- two steps old -- up to date by applying there the moves of
+// step k-1 and of step k, so after the step's barrier it holds the state
+// after step k.  Decisions of a step therefore never see a move of their own
+// step (frozen state), as with a separate commit phase.  On exit the final
+// state is copied to (C, a, n).
 template <bool HASH>
 __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 *C, double *a,
                                                                i64 *n, const i64 *cperm,
                                                                const i64 *cstart, int ncol,
                                                                unsigned *bar, double *part,
-                                                               uint8_t *stale) {
+                                                               uint8_t *stale, i64 *C1, double *a1,
+                                                               i64 *n1) {
     __shared__ __align__(16) unsigned char smem[kPSmemBytes];
-    __shared__ int wsum[kPBlock / kWave];
     __shared__ double red[kPBlock / kWave];
     __shared__ int sc[kPBlock / kWave];
     __shared__ double sg[kPBlock / kWave];
     __shared__ i64 si[kPBlock / kWave];
+    __shared__ int s_cnt;     // communities listed for the current vertex (dense view)
+    __shared__ double s_sl;   // its self-loop weight
     // dense view
     double *acc = reinterpret_cast<double *>(smem);
     uint32_t *touched = reinterpret_cast<uint32_t *>(smem + kDenseMaxNV * 8);
@@ -943,6 +955,10 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
     const int nwords = (nv + 31) / 32;
     const unsigned nb = gridDim.x;
     unsigned seq = 0;
+    if (threadIdx.x == 0) {
+        s_cnt = 0;
+        s_sl = 0.0;
+    }
     if (HASH) {
         for (int i = threadIdx.x; i < kPHashSlots; i += kPBlock) {
             hkeys[i] = kEmpty;
@@ -953,15 +969,31 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
         for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
         for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
     }
-    __syncthreads();
-    bool alive = true;
+    for (i64 i = (i64)blockIdx.x * kPBlock + threadIdx.x; i < nv; i += (i64)nb * kPBlock) {
+        C1[i] = C[i];
+        a1[i] = a[i];
+        n1[i] = n[i];
+    }
+    bool alive = grid_barrier(bar, nb, ++seq);
+    int par = 0;            // buffer holding the state after the previous step
+    i64 plo = 0, ptot = 0;  // the previous step's class
     for (int c = 0; c < ncol && alive; c++) {
         const i64 lo = cstart[c], total = cstart[c + 1] - lo;
         if (total == 0) continue;
+        A.C = par ? C1 : C;
+        A.a = par ? a1 : a;
+        A.n = par ? n1 : n;
+        i64 *Cn = par ? C : C1;
+        double *an = par ? a : a1;
+        i64 *nn = par ? n : n1;
         for (i64 q = blockIdx.x; q < total; q += nb) {
             const i64 v = cperm[lo + q];
             const i64 beg = A.off[v], end = A.off[v + 1];
             const i64 s = A.C[v];
+            if (s < 0) {  // a vertex without entries (label -1) stays where it is
+                if (threadIdx.x == 0) A.tgt[v] = s;
+                continue;
+            }
             double sl = 0, e_s = 0;
             // HASH: LDS table of this vertex, or the dense HBM row when it is large
             const bool big = HASH && end - beg > kPHashMaxDeg;
@@ -977,7 +1009,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             for (i64 j = beg + threadIdx.x; j < end; j += kPBlock) {
                 const i64 t = A.tails[j];
                 const double ww = A.w[j];
-                if (t == v) sl += ww;
+                if (t == v) atomicAdd(&s_sl, ww);
                 const i64 cc = A.C[t];
                 if (HASH) {
                     if (big) {
@@ -987,13 +1019,15 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                         hash_insert(hkeys, hvals, mask, shift, cc, ww);
                     }
                 } else {
-                    atomicOr(&touched[cc >> 5], 1u << (cc & 31));
+                    // the first lane to touch a community lists it
+                    const uint32_t bit = 1u << (cc & 31);
+                    if (!(atomicOr(&touched[cc >> 5], bit) & bit)) list[atomicAdd(&s_cnt, 1)] = (uint16_t)cc;
                     atomicAdd(&acc[cc], ww);
                 }
             }
             if (HASH) __threadfence_block();
             __syncthreads();
-            sl = block_sum_p(sl, red);
+            sl = s_sl;
             const double kv = A.k[v];
             const double ax = A.a[s] - kv;
             bool cand = false;
@@ -1026,7 +1060,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 score_slots(hkeys, hvals, slots, threadIdx.x, kPBlock, A, s, kv, e_s - sl, ax, A.constant, cand,
                             gain, id, t1, t2);
             } else {
-                ntouched = compact_touched<kPBlock>(touched, nwords, list, wsum);
+                ntouched = s_cnt;
                 e_s = acc[s];
                 const double eix = e_s - sl;
                 const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
@@ -1052,6 +1086,11 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 }
                 A.tgt[v] = target;
                 if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+                if (target != s) {  // into the other buffer, read from the next step on
+                    move_vertex(target, s, kv, an, nn);
+                    Cn[v] = target;
+                    if (stale) stale[v] = 0;
+                }
             }
             __syncthreads();  // every read of the accumulator is done
             if (HASH && big) {
@@ -1071,27 +1110,36 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                     hvals[h] = 0.0;
                 }
             } else {
-                for (int k = threadIdx.x; k < ntouched; k += kPBlock) acc[list[k]] = 0.0;
-                for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+                for (int k = threadIdx.x; k < ntouched; k += kPBlock) {
+                    acc[list[k]] = 0.0;
+                    touched[list[k] >> 5] = 0;
+                }
+            }
+            if (threadIdx.x == 0) {
+                s_cnt = 0;
+                s_sl = 0.0;
             }
             __syncthreads();
         }
-        alive = grid_barrier(bar, nb, ++seq);
-        if (!alive) break;
-        // commit the moves of the vertices this workgroup swept
+        // the previous step's moves of this workgroup's vertices, into the other buffer
         if (threadIdx.x == 0) {
-            for (i64 q = blockIdx.x; q < total; q += nb) {
-                const i64 v = cperm[lo + q];
-                const i64 t = A.tgt[v], s = C[v];
+            for (i64 q = blockIdx.x; q < ptot; q += nb) {
+                const i64 v = cperm[plo + q];
+                const i64 t = A.tgt[v], s = Cn[v];
                 if (t != s) {
-                    move_vertex(t, s, A.k[v], a, n);
-                    C[v] = t;
-                    if (stale) stale[v] = 0;
+                    move_vertex(t, s, A.k[v], an, nn);
+                    Cn[v] = t;
                 }
             }
         }
         alive = grid_barrier(bar, nb, ++seq);
+        par ^= 1;
+        plo = lo;
+        ptot = total;
     }
+    A.C = par ? C1 : C;
+    A.a = par ? a1 : a;
+    A.n = par ? n1 : n;
     // modularity partials: internal weight over this workgroup's vertices, sum of a_c^2
     double exx = 0, a2 = 0;
     if (alive) {
@@ -1109,6 +1157,13 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
     if (threadIdx.x == 0) {
         part[blockIdx.x] = exx;
         part[nb + blockIdx.x] = a2;
+    }
+    if (alive && par) {  // the final state sits in the second buffer
+        for (i64 i = (i64)blockIdx.x * kPBlock + threadIdx.x; i < nv; i += (i64)nb * kPBlock) {
+            C[i] = C1[i];
+            a[i] = a1[i];
+            n[i] = n1[i];
+        }
     }
 }
 
@@ -2142,7 +2197,8 @@ int persistent_blocks() {
     return cached;
 }
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
-                              unsigned *bar, double *part, int nblocks, hipStream_t s, uint8_t *stale) {
+                              unsigned *bar, double *part, int nblocks, hipStream_t s, uint8_t *stale,
+                              i64 *C1, double *a1, i64 *n1) {
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || A.nv > kPersistMaxNV) return false;
     const bool hash = A.nv > kDenseMaxNV;
     // degrees above the LDS table's reach need the workgroup's dense HBM row
@@ -2157,7 +2213,10 @@ bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *st
     unsigned *b = bar;
     double *pt = part;
     uint8_t *sf = stale;
-    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt, &sf};
+    i64 *c1 = C1;
+    double *ca1 = a1;
+    i64 *cn1 = n1;
+    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt, &sf, &c1, &ca1, &cn1};
     const void *fn = hash ? (const void *)k_colored_iteration<true> : (const void *)k_colored_iteration<false>;
     hipError_t e = hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kPBlock), args, 0, s);
     if (e != hipSuccess) {

@@ -334,6 +334,9 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     DBuf<unsigned> bar(pblocks > 0 ? kBarrierWords : 0, s_);
     DBuf<double> ppart(pblocks > 0 ? 2 * pblocks : 0, s_);
     DBuf<i64> cperm(pblocks > 0 ? nv : 0, s_), cstart(pblocks > 0 ? ncol + 1 : 0, s_);
+    // second state buffers of the persistent sweep (one grid barrier per color step)
+    DBuf<i64> pC1(pblocks > 0 ? nv : 0, s_), pn1(pblocks > 0 ? nv : 0, s_);
+    DBuf<double> pa1(pblocks > 0 ? nv : 0, s_);
     if (pblocks > 0) launch_color_major(P.perm.p, colors, nv, ncol, cperm.p, cstart.p, s_);
     bool persistent = pblocks > 0;
 
@@ -428,7 +431,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
             launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
             Profiler::get().begin(K_PERSIST, s_);
             persistent = launch_colored_iteration(A, plist, pstart, ncol, bar.p, ppart.p, pblocks, s_,
-                                                  mode ? stale.p : nullptr);
+                                                  mode ? stale.p : nullptr, pC1.p, pa1.p, pn1.p);
             Profiler::get().end(K_PERSIST, 24.0 * g.nnz + 32.0 * nv, s_);
             if (persistent) {
                 launch_sum_partials(ppart.p, pblocks, res.p, s_);
