@@ -33,6 +33,17 @@ __device__ inline u64 jp_priority(i64 v) {
     return z ^ (z >> 31);
 }
 
+// Jones-Plassmann order: prio[] (larger degree first, then a hash of the id),
+// equal keys broken by the larger id, so the order is total
+__device__ inline bool jp_above(const u64 *prio, i64 u, u64 pv, i64 v) {
+    const u64 pu = prio[u];
+    return pu > pv || (pu == pv && u > v);
+}
+__device__ inline bool jp_below(const u64 *prio, i64 u, u64 pv, i64 v) {
+    const u64 pu = prio[u];
+    return pu < pv || (pu == pv && u < v);
+}
+
 __device__ inline i64 ld_relaxed(const i64 *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
 // Loads of tables that other lanes fill with atomics (performed in L2): agent
 // scope, so a line cached in the CU's L1 before the atomics cannot be served.
@@ -1336,10 +1347,10 @@ __global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cn
             if (e - b > kBinMaxDeg[6]) {
                 heavy_v = true;
             } else {
-                const u64 pv = jp_priority(v);
+                const u64 pv = L.prio[v];
                 for (i64 j = b; j < e; j++) {
                     const i64 u = tails[j];
-                    if (u != v && jp_priority(u) > pv) c++;
+                    if (u != v && jp_above(L.prio, u, pv, v)) c++;
                 }
                 cnt[v] = c;
             }
@@ -1358,11 +1369,11 @@ __global__ __launch_bounds__(kBlock) void k_jp_count_heavy(const i64 *off, const
     const i64 stride = (i64)gridDim.x * WPB;
     for (i64 q = (i64)blockIdx.x * WPB + threadIdx.x / kWave; q < nheavy; q += stride) {
         const i64 v = heavy[q];
-        const u64 pv = jp_priority(v);
+        const u64 pv = L.prio[v];
         int c = 0;
         for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
             const i64 u = tails[j];
-            if (u != v && jp_priority(u) > pv) c++;
+            if (u != v && jp_above(L.prio, u, pv, v)) c++;
         }
         for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
         if (lane == 0) cnt[v] = c;
@@ -1417,7 +1428,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
             colors[v] = color;
         }
         // release the lower-priority neighbours
-        const u64 pv = in ? jp_priority(v) : 0;
+        const u64 pv = in ? L.prio[v] : 0;
         const i64 deg = e - b;
         for (i64 j0 = 0; j0 < deg; j0 += kWave) {
             const i64 j = j0 + lane;
@@ -1425,7 +1436,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
             i64 u = 0;
             if (j < deg) {
                 u = tails[b + j];
-                if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+                if (u != v && jp_below(L.prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
             }
             frontier_push(ready, u, off, L);
         }
@@ -1443,7 +1454,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
     for (i64 q = blockIdx.x; q < nfront; q += gridDim.x) {
         const i64 v = front[q];
         const i64 b = off[v], e = off[v + 1];
-        const u64 pv = jp_priority(v);
+        const u64 pv = L.prio[v];
         if (threadIdx.x == 0) s_color = kMaxColors;
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) s_mark[i] = 0;
         __syncthreads();
@@ -1476,7 +1487,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
             i64 u = 0;
             if (j < deg) {
                 u = tails[b + j];
-                if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+                if (u != v && jp_below(L.prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
             }
             frontier_push(ready, u, off, L);
         }
@@ -1524,7 +1535,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, con
             colors[v] = color;
         }
     }
-    const u64 pv = jp_priority(v);
+    const u64 pv = L.prio[v];
     const i64 b = off[v], deg = off[v + 1] - b;
     const i64 step = (i64)gridDim.x * kBlock;
     for (i64 j0 = (i64)blockIdx.x * kBlock; j0 < deg; j0 += step) {
@@ -1533,7 +1544,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, con
         i64 u = 0;
         if (j < deg) {
             u = tails[b + j];
-            if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+            if (u != v && jp_below(L.prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
         }
         frontier_push(ready, u, off, L);
     }
@@ -1604,7 +1615,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                                                              int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB,
                                                              i64 *lA, i64 *lB, i64 *ring, unsigned *bar,
                                                              int32_t *err, i64 tail_at, int do_count,
-                                                             i64 colored0) {
+                                                             i64 colored0, const u64 *prio) {
     constexpr int WPB = kJpPBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
     __shared__ i64 tail_list[kJpPBlock];
@@ -1627,10 +1638,10 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 if (e - b > kBinMaxDeg[6]) {
                     heavy = true;
                 } else {
-                    const u64 pv = jp_priority(v);
+                    const u64 pv = prio[v];
                     for (i64 j = b; j < e; j++) {
                         const i64 u = tails[j];
-                        if (u != v && jp_priority(u) > pv) c++;
+                        if (u != v && jp_above(prio, u, pv, v)) c++;
                     }
                     cnt[v] = c;
                 }
@@ -1643,11 +1654,11 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             const i64 nheavy = __atomic_load_n(&ring[7], __ATOMIC_RELAXED);
             for (i64 q = (i64)blockIdx.x * WPB + w; q < nheavy; q += (i64)nb * WPB) {
                 const i64 v = lB[q];
-                const u64 pv = jp_priority(v);
+                const u64 pv = prio[v];
                 int c = 0;
                 for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
                     const i64 u = tails[j];
-                    if (u != v && jp_priority(u) > pv) c++;
+                    if (u != v && jp_above(prio, u, pv, v)) c++;
                 }
                 for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
                 if (lane == 0) cnt[v] = c;
@@ -1704,7 +1715,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 }
                 colors[v] = color;
             }
-            const u64 pv = in ? jp_priority(v) : 0;
+            const u64 pv = in ? prio[v] : 0;
             const i64 deg = e - b;
             for (i64 j0 = 0; j0 < deg; j0 += kWave) {
                 const i64 j = j0 + lane;
@@ -1712,7 +1723,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 i64 u = 0;
                 if (j < deg) {
                     u = tails[b + j];
-                    if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+                    if (u != v && jp_below(prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
                 }
                 const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
                 wave_append(ready && !big, u, ns_, cs);
@@ -1747,7 +1758,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 }
                 colors[v] = color;
             }
-            const u64 pv = jp_priority(v);
+            const u64 pv = prio[v];
             const i64 deg = e - b;
             for (i64 j0 = 0; j0 < deg; j0 += kJpPBlock) {
                 const i64 j = j0 + threadIdx.x;
@@ -1755,7 +1766,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 i64 u = 0;
                 if (j < deg) {
                     u = tails[b + j];
-                    if (u != v && jp_priority(u) < pv) ready = atomicSub(&cnt[u], 1) == 1;
+                    if (u != v && jp_below(prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
                 }
                 const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
                 wave_append(ready && !big, u, ns_, cs);
@@ -1823,6 +1834,13 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
         nl_ = t;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ring[6] = r;
+}
+
+__global__ void k_jp_prio(const i64 *off, i64 nv, u64 *prio) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
+        const u64 d = (u64)min(off[v + 1] - off[v], (i64(1) << 23) - 1);
+        prio[v] = (d << 40) | (jp_priority(v) >> 24);
+    }
 }
 
 __global__ void k_jp_finish(const int32_t *colors, i64 nv, int32_t *maxc) {
@@ -2344,12 +2362,13 @@ int jp_persistent_blocks() {
 }
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, int nblocks, hipStream_t s) {
+                          i64 colored0, const u64 *prio, int nblocks, hipStream_t s) {
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0) return false;
     const i64 *off = g.off, *tails = g.tails;
     i64 nv = g.nv, ta = tail_at, c0 = colored0;
     int dc = count ? 1 : 0;
-    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0};
+    const u64 *pr = prio;
+    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0, &pr};
     hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_persistent, dim3(nblocks), dim3(kJpPBlock),
                                               args, 0, s);
     if (e != hipSuccess) {
@@ -2357,6 +2376,11 @@ bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 
         return false;
     }
     return true;
+}
+void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_jp_prio<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.nv, prio);
+    GLV_LAUNCH_CHECK();
 }
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s) {
     if (nv <= 0) return;

@@ -134,7 +134,11 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
 struct JpLists {
     i64 *small, *large, *giant;
     i64 *counts;
+    const u64 *prio = nullptr;  // priority of every vertex (launch_jp_prio)
 };
+// Jones-Plassmann priorities: (min(degree, 2^23 - 1) << 40) | (splitmix64(v) >> 24),
+// equal keys broken by the larger id
+void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s);
 // count[v] = uncoloured higher-priority neighbours; count-0 vertices form the
 // first frontier.  heavy: nv scratch, heavy_count: one zeroed word.
 void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *heavy, i64 *heavy_count,
@@ -158,7 +162,7 @@ constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many unco
 int jp_persistent_blocks();
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, int nblocks, hipStream_t s);
+                          i64 colored0, const u64 *prio, int nblocks, hipStream_t s);
 // frontiers below this many vertices finish in one cooperative launch
 constexpr i64 kJpPersistFrontier = 16384;
 void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);

@@ -149,6 +149,8 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     DBuf<i64> ring(8, s_);
     DBuf<unsigned> bar(kBarrierWords, s_);
     DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
+    DBuf<u64> prio(nv, s_);
+    launch_jp_prio(g, prio.p, s_);
     launch_fill_i32(err.p, 1, 0, s_);
     launch_fill_i32(maxc.p, 1, -1, s_);
     // the rounds run in one cooperative launch once the frontier no longer fills the GPU
@@ -159,7 +161,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
         const bool ok = pnb > 0 && launch_jp_persistent(g, colors, cnt.p, sA, sB, lA, lB, ring.p, bar.p, err.p,
-                                                        kJpTail, count, colored0, pnb, s_);
+                                                        kJpTail, count, colored0, prio.p, pnb, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
         unsigned abort_flag = 0;
         i64 r = 0;
@@ -182,7 +184,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     if (!done) {
         DBuf<uint32_t> gmark;
         launch_fill_i64(counts.p, 8, 0, s_);
-        JpLists cur{sA.p, lA.p, gA.p, counts.p}, next{sB.p, lB.p, gB.p, counts.p + 4};
+        JpLists cur{sA.p, lA.p, gA.p, counts.p, prio.p}, next{sB.p, lB.p, gB.p, counts.p + 4, prio.p};
         Profiler::get().begin(K_COLOR, s_);
         launch_jp_count(g, cnt.p, cur, sB.p, counts.p + 3, s_);  // heavy list in the next round's buffer
         Profiler::get().end(K_COLOR, 0.0, s_);

@@ -5,8 +5,9 @@
 
 /* coloring.h:8 -- valid distance-1 coloring (self-loops ignored); returns the
    largest color used (the caller adds 1).  Colors come from Jones-Plassmann
-   rounds with hash priorities, so they differ from the reference's speculative
-   coloring; only validity is part of the contract. */
+   rounds, larger degree first and then a hash of the id, so they differ from
+   the reference's speculative coloring; only validity is part of the
+   contract. */
 int algoDistanceOneVertexColoringOpt(graph *G, int *vtxColor, int nThreads, double *totTime);
 
 /* coloring.h:12 -- coloringMultiHashMaxMin.cpp:60-226: incomplete coloring by

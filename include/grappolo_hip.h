@@ -103,8 +103,10 @@ int glv_runMultiPhaseBasic(int64_t nv, const int64_t *edgeListPtrs, const glv_ed
 
 /* algoDistanceOneVertexColoringOpt, DefineStructure/coloring.h:8 and
  * Coloring/coloringDistanceOne.cpp:50-233.  *maxColor = largest color used.
- * Jones-Plassmann rounds with hash priorities: a valid coloring, not the
- * reference's colors. */
+ * Jones-Plassmann rounds: a vertex takes the smallest color free among its
+ * coloured neighbours once every neighbour of higher priority is coloured;
+ * priority = (min(degree, 2^23-1) << 40) | (splitmix64(v) >> 24), equal
+ * keys to the larger id.  A valid coloring, not the reference's colors. */
 int glv_algoDistanceOneVertexColoringOpt(int64_t nv, const int64_t *edgeListPtrs,
                                          const glv_edge *edgeList, int32_t *vtxColor,
                                          int32_t *maxColor, double *totTime);

@@ -208,10 +208,24 @@ uint64_t orc_jp_priority(i64 v) {
     return z ^ (z >> 31);
 }
 
+/* The engine's coloring priority: larger degree first (capped at 2^23 - 1),
+ * then the top 40 bits of orc_jp_priority(v); equal keys go to the larger id. */
+static uint64_t jp_key(const i64 *off, i64 v) {
+    i64 d = off[v + 1] - off[v];
+    if (d > (((i64)1 << 23) - 1)) d = ((i64)1 << 23) - 1;
+    return ((uint64_t)d << 40) | (orc_jp_priority(v) >> 24);
+}
+
+static int jp_beats(const i64 *off, i64 u, i64 v) {
+    const uint64_t pu = jp_key(off, u), pv = jp_key(off, v);
+    return pu > pv || (pu == pv && u > v);
+}
+
 /* Restatement of the engine's coloring (grappolo_amd/csrc/engine/
  * kernels.hip, k_jp_* and k_color_tail): Jones-Plassmann rounds -- every
- * uncoloured vertex whose priority beats all uncoloured neighbours (self-loops
- * ignored) takes the smallest color absent from its coloured neighbours;
+ * uncoloured vertex whose priority (jp_key / jp_beats: degree, hash, id)
+ * beats all uncoloured neighbours (self-loops ignored) takes the smallest
+ * color absent from its coloured neighbours;
  * winners of one round are pairwise non-adjacent, so the order inside a round
  * does not matter.  Once at most ORC_JP_TAIL vertices remain uncoloured, the
  * rest is colored first-fit one vertex at a time in increasing id (the
@@ -236,12 +250,11 @@ int orc_color_jp(i64 nv, const i64 *off, const i64 *tails, int32_t *colors, int3
         i64 nwin = 0, nrest = 0;
         for (i64 q = 0; q < nwork; q++) {
             i64 v = work[q];
-            uint64_t pv = orc_jp_priority(v);
             int ok = 1;
             for (i64 j = off[v]; j < off[v + 1]; j++) {
                 i64 u = tails[j];
                 if (u == v || colors[u] >= 0) continue;
-                if (orc_jp_priority(u) > pv) {
+                if (jp_beats(off, u, v)) {
                     ok = 0;
                     break;
                 }

@@ -45,14 +45,18 @@ def test_coloring_valid_and_equal_to_restated_jp(name):
     np.testing.assert_array_equal(colors, ref)
 
 
-@pytest.mark.parametrize("which", ["rmat19", "cliques"])
+@pytest.mark.parametrize("which", ["rmat19", "planted", "cliques"])
 def test_coloring_rounds_paths_equal_restated_jp(which):
-    """Above 2^18 vertices the rounds are launched from the host, below they
-    run inside one cooperative launch; both equal the restated coloring, on
-    a skewed graph (long priority chains through hubs) and a dense one."""
+    """Above 2^18 vertices the rounds are launched from the host and handed to
+    one cooperative launch once the frontier is small; below, they run inside
+    one cooperative launch.  Both equal the restated coloring, on a skewed
+    graph (long priority chains through hubs), a planted one and dense
+    near-cliques."""
     from graphs import cliques
-    from grappolo_amd.graph import gen_rmat
-    g = gen_rmat(19, seed=5) if which == "rmat19" else cliques()
+    from grappolo_amd.graph import gen_planted, gen_rmat
+    g = {"rmat19": lambda: gen_rmat(19, seed=5),
+         "planted": lambda: gen_planted(300000, 64, 12, 0.3, seed=3),
+         "cliques": cliques}[which]()
     colors, mc = engine.algoDistanceOneVertexColoringOpt(g)
     ref, rmc, rounds = oracle.color_jp(g)
     assert oracle.color_conflicts(g, colors) == 0

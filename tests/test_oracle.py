@@ -267,16 +267,56 @@ def test_colorings_are_first_fit(name):
             assert all(c in have for c in range(colors[v])), v
 
 
-def test_jp_round_winners_are_local_maxima():
-    """On a graph where the rounds finish before the tail takes over, a vertex
-    whose priority beats all its neighbours is colored 0."""
-    g = catalog()["planted_unit"]
+def _jp_python(g, tail=1024):
+    """Jones-Plassmann written out in Python, independently of the C
+    restatement: in each round every uncoloured vertex whose key (degree
+    capped at 2^23-1, top 40 bits of the id hash, id) beats all uncoloured
+    neighbours takes the smallest colour its coloured neighbours leave free;
+    once at most `tail` vertices remain, they are coloured first-fit in
+    increasing id."""
+    nv = g.numVertices
+    deg = np.diff(g.edgeListPtrs)
+    key = [(min(int(deg[v]), (1 << 23) - 1), int(oracle.jp_priority(v)) >> 24, v) for v in range(nv)]
+    nbr = [[int(u) for u in g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]] if u != v] for v in range(nv)]
+    col = [-1] * nv
+
+    def first_fit(v):
+        used = {col[u] for u in nbr[v] if col[u] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        return c
+
+    work = list(range(nv))
+    rounds = 0
+    while work:
+        win = [v for v in work if all(col[u] >= 0 or key[u] < key[v] for u in nbr[v])]
+        for v in win:
+            col[v] = first_fit(v)
+        work = [v for v in work if col[v] < 0]
+        rounds += 1
+        if 0 < len(work) <= tail:
+            break
+    for v in range(nv):
+        if col[v] < 0:
+            col[v] = first_fit(v)
+    return np.array(col, np.int32), rounds
+
+
+@pytest.mark.parametrize("name", ["karate", "planted_unit", "hubs", "rmat12"])
+def test_jp_matches_python_rounds(name):
+    """The C restatement of the engine's coloring equals the rounds written
+    out in Python, and a vertex whose key beats all its neighbours gets 0."""
+    g = catalog()[name]
     colors, mc, rounds = oracle.color_jp(g)
-    pr = np.array([oracle.jp_priority(v) for v in range(g.numVertices)], dtype=np.uint64)
+    pc, prounds = _jp_python(g)
+    np.testing.assert_array_equal(colors, pc)
+    assert rounds == prounds and mc == pc.max()
+    deg = np.diff(g.edgeListPtrs)
+    key = [(min(int(deg[v]), (1 << 23) - 1), int(oracle.jp_priority(v)) >> 24, v) for v in range(g.numVertices)]
     for v in range(g.numVertices):
-        nb = g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]]
-        nb = nb[nb != v]
-        if len(nb) and np.all(pr[nb] < pr[v]):
+        nb = [u for u in g.tails[g.edgeListPtrs[v]:g.edgeListPtrs[v + 1]].tolist() if u != v]
+        if nb and all(key[u] < key[v] for u in nb):
             assert colors[v] == 0
 
 
