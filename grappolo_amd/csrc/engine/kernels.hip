@@ -1398,13 +1398,28 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
         const bool in = q < nfront;
         const i64 v = in ? front[q] : 0;
         const i64 b = in ? off[v] : 0, e = in ? off[v + 1] : 0;
+        const u64 pv = in ? L.prio[v] : 0;
+        const i64 deg = e - b;
         for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
         __syncthreads();
-        for (i64 j = b + lane; j < e; j += kWave) {
-            const i64 u = tails[j];
-            if (u == v) continue;
-            const int c = colors[u];
-            if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+        // one pass: colours of the higher-priority neighbours (all coloured),
+        // release of the lower-priority ones
+        for (i64 j0 = 0; j0 < deg; j0 += kWave) {
+            const i64 j = j0 + lane;
+            bool ready = false;
+            i64 u = 0;
+            if (j < deg) {
+                u = tails[b + j];
+                if (u != v) {
+                    if (jp_below(L.prio, u, pv, v)) {
+                        ready = atomicSub(&cnt[u], 1) == 1;
+                    } else {
+                        const int c = colors[u];
+                        if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+                    }
+                }
+            }
+            frontier_push(ready, u, off, L);
         }
         __syncthreads();
         int cand = kMaxColors;
@@ -1427,19 +1442,6 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
             }
             colors[v] = color;
         }
-        // release the lower-priority neighbours
-        const u64 pv = in ? L.prio[v] : 0;
-        const i64 deg = e - b;
-        for (i64 j0 = 0; j0 < deg; j0 += kWave) {
-            const i64 j = j0 + lane;
-            bool ready = false;
-            i64 u = 0;
-            if (j < deg) {
-                u = tails[b + j];
-                if (u != v && jp_below(L.prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
-            }
-            frontier_push(ready, u, off, L);
-        }
         __syncthreads();
     }
 }
@@ -1458,11 +1460,26 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
         if (threadIdx.x == 0) s_color = kMaxColors;
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) s_mark[i] = 0;
         __syncthreads();
-        for (i64 j = b + threadIdx.x; j < e; j += kBlock) {
-            const i64 u = tails[j];
-            if (u == v) continue;
-            const int c = colors[u];
-            if (c >= 0) atomicOr(&s_mark[c >> 5], 1u << (c & 31));
+        // one pass: colours of the higher-priority neighbours, release of the
+        // lower ones; whole wavefronts iterate together for the appends
+        const i64 deg = e - b;
+        const i64 trips = (deg + kBlock - 1) / kBlock;
+        for (i64 t = 0; t < trips; t++) {
+            const i64 j = t * kBlock + threadIdx.x;
+            bool ready = false;
+            i64 u = 0;
+            if (j < deg) {
+                u = tails[b + j];
+                if (u != v) {
+                    if (jp_below(L.prio, u, pv, v)) {
+                        ready = atomicSub(&cnt[u], 1) == 1;
+                    } else {
+                        const int c = colors[u];
+                        if (c >= 0) atomicOr(&s_mark[c >> 5], 1u << (c & 31));
+                    }
+                }
+            }
+            frontier_push(ready, u, off, L);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
@@ -1477,19 +1494,6 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                 color = 0;
             }
             colors[v] = color;
-        }
-        // release lower neighbours; whole wavefronts iterate together for the appends
-        const i64 deg = e - b;
-        const i64 trips = (deg + kBlock - 1) / kBlock;
-        for (i64 t = 0; t < trips; t++) {
-            const i64 j = t * kBlock + threadIdx.x;
-            bool ready = false;
-            i64 u = 0;
-            if (j < deg) {
-                u = tails[b + j];
-                if (u != v && jp_below(L.prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
-            }
-            frontier_push(ready, u, off, L);
         }
         __syncthreads();
     }
@@ -1686,13 +1690,29 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             const bool in = qq < nsmall;
             const i64 v = in ? fs[qq] : 0;
             const i64 b = in ? off[v] : 0, e = in ? off[v + 1] : 0;
+            const u64 pv = in ? prio[v] : 0;
+            const i64 deg = e - b;
             for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
             __syncthreads();
-            for (i64 j = b + lane; j < e; j += kWave) {
-                const i64 u = tails[j];
-                if (u == v) continue;
-                const int c = colors[u];
-                if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+            // one pass: colours of the higher-priority neighbours, release of the lower ones
+            for (i64 j0 = 0; j0 < deg; j0 += kWave) {
+                const i64 j = j0 + lane;
+                bool ready = false;
+                i64 u = 0;
+                if (j < deg) {
+                    u = tails[b + j];
+                    if (u != v) {
+                        if (jp_below(prio, u, pv, v)) {
+                            ready = atomicSub(&cnt[u], 1) == 1;
+                        } else {
+                            const int c = colors[u];
+                            if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+                        }
+                    }
+                }
+                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
+                wave_append(ready && !big, u, ns_, cs);
+                wave_append(big, u, nl_, cl);
             }
             __syncthreads();
             int cand = kMaxColors;
@@ -1715,34 +1735,37 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 }
                 colors[v] = color;
             }
-            const u64 pv = in ? prio[v] : 0;
-            const i64 deg = e - b;
-            for (i64 j0 = 0; j0 < deg; j0 += kWave) {
-                const i64 j = j0 + lane;
-                bool ready = false;
-                i64 u = 0;
-                if (j < deg) {
-                    u = tails[b + j];
-                    if (u != v && jp_below(prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
-                }
-                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
-                wave_append(ready && !big, u, ns_, cs);
-                wave_append(big, u, nl_, cl);
-            }
             __syncthreads();
         }
         // large frontier vertices: a workgroup each
         for (i64 qq = blockIdx.x; qq < nlarge; qq += nb) {
             const i64 v = fl[qq];
             const i64 b = off[v], e = off[v + 1];
+            const u64 pv = prio[v];
+            const i64 deg = e - b;
             for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
             if (threadIdx.x == 0) s_color = kMaxColors;
             __syncthreads();
-            for (i64 j = b + threadIdx.x; j < e; j += kJpPBlock) {
-                const i64 u = tails[j];
-                if (u == v) continue;
-                const int c = colors[u];
-                if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
+            // one pass: colours of the higher-priority neighbours (all coloured),
+            // release of the lower-priority ones (next round's frontier)
+            for (i64 j0 = 0; j0 < deg; j0 += kJpPBlock) {
+                const i64 j = j0 + threadIdx.x;
+                bool ready = false;
+                i64 u = 0;
+                if (j < deg) {
+                    u = tails[b + j];
+                    if (u != v) {
+                        if (jp_below(prio, u, pv, v)) {
+                            ready = atomicSub(&cnt[u], 1) == 1;
+                        } else {
+                            const int c = colors[u];
+                            if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
+                        }
+                    }
+                }
+                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
+                wave_append(ready && !big, u, ns_, cs);
+                wave_append(big, u, nl_, cl);
             }
             __syncthreads();
             if (threadIdx.x < kMaxColors / 32) {
@@ -1757,20 +1780,6 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                     color = 0;
                 }
                 colors[v] = color;
-            }
-            const u64 pv = prio[v];
-            const i64 deg = e - b;
-            for (i64 j0 = 0; j0 < deg; j0 += kJpPBlock) {
-                const i64 j = j0 + threadIdx.x;
-                bool ready = false;
-                i64 u = 0;
-                if (j < deg) {
-                    u = tails[b + j];
-                    if (u != v && jp_below(prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
-                }
-                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
-                wave_append(ready && !big, u, ns_, cs);
-                wave_append(big, u, nl_, cl);
             }
             __syncthreads();
         }
