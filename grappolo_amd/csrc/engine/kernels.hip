@@ -1331,6 +1331,110 @@ __device__ inline void frontier_push(bool pred, i64 u, const i64 *off, const JpL
     wave_append(giant, u, L.giant, &L.counts[2]);
 }
 
+// One wavefront colours up to eight frontier entries front[lo .. lo+n): a
+// vertex of at most kJpLightDeg entries takes a group of kJpGroup lanes (its
+// first-fit colour is at most its entry count, so a 128-colour mask per lane,
+// OR-ed over the group, decides it); heavier ones (up to 256 entries) take
+// the whole wave one after another, with the wave's LDS bitmap mk.  One pass
+// over a vertex's entries marks the colours of its higher-priority
+// neighbours (all coloured) and releases the lower-priority ones; frontier
+// vertices are pairwise non-adjacent, so every colour read is final.
+// push(ready, u) appends a released vertex (called by the whole wave).
+constexpr int kJpGroup = 8;
+constexpr i64 kJpLightDeg = 64;
+template <typename Push>
+__device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *cnt, const u64 *prio,
+                                const i64 *front, i64 lo, i64 n, uint32_t *mk, int32_t *err, Push push) {
+    const int lane = threadIdx.x % kWave, g = lane / kJpGroup, gl = lane % kJpGroup;
+    const bool in = g < n;
+    const i64 v = in ? front[lo + g] : -1;
+    const i64 b = in ? off[v] : 0, deg = in ? off[v + 1] - b : 0;
+    const bool light = in && deg <= kJpLightDeg;
+    const u64 pv = in ? prio[v] : 0;
+    u64 m0 = 0, m1 = 0;
+    i64 itmax = light ? (deg + kJpGroup - 1) / kJpGroup : 0;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const i64 x = __shfl_xor(itmax, o);
+        itmax = x > itmax ? x : itmax;
+    }
+    for (i64 it = 0; it < itmax; it++) {
+        const i64 j = it * kJpGroup + gl;
+        bool ready = false;
+        i64 u = 0;
+        if (light && j < deg) {
+            u = tails[b + j];
+            if (u != v) {
+                if (jp_below(prio, u, pv, v)) {
+                    ready = atomicSub(&cnt[u], 1) == 1;
+                } else {
+                    const int c = colors[u];
+                    if (c >= 0 && c < 64) m0 |= 1ull << c;
+                    else if (c >= 64 && c < 128) m1 |= 1ull << (c - 64);
+                }
+            }
+        }
+        push(ready, u);
+    }
+#pragma unroll
+    for (int o = 1; o < kJpGroup; o <<= 1) {
+        m0 |= __shfl_xor(m0, o);
+        m1 |= __shfl_xor(m1, o);
+    }
+    if (light && gl == 0) colors[v] = ~m0 ? __ffsll((long long)~m0) - 1 : 64 + __ffsll((long long)~m1) - 1;
+    u64 hm = __ballot(in && !light && gl == 0);
+    while (hm) {
+        const int src = __ffsll((long long)hm) - 1;
+        hm &= hm - 1;
+        const i64 hv = __shfl(v, src), hb = __shfl(b, src), hd = __shfl(deg, src);
+        const u64 hp = __shfl(pv, src);
+        for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        for (i64 j0 = 0; j0 < hd; j0 += kWave) {
+            const i64 j = j0 + lane;
+            bool ready = false;
+            i64 u = 0;
+            if (j < hd) {
+                u = tails[hb + j];
+                if (u != hv) {
+                    if (jp_below(prio, u, hp, hv)) {
+                        ready = atomicSub(&cnt[u], 1) == 1;
+                    } else {
+                        const int c = colors[u];
+                        if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+                    }
+                }
+            }
+            push(ready, u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        int cand = kMaxColors;
+        for (int i = lane; i < kMaxColors / 32; i += kWave) {
+            const uint32_t m = mk[i];
+            if (m != 0xFFFFFFFFu) {
+                const int c = i * 32 + __ffs((int)~m) - 1;
+                cand = c < cand ? c : cand;
+            }
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const int other = __shfl_xor(cand, o);
+            cand = other < cand ? other : cand;
+        }
+        if (lane == 0) {
+            int color = cand;
+            if (color >= kMaxColors) {
+                atomicOr(err, 1);
+                color = 0;
+            }
+            colors[hv] = color;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // count[v] of vertices up to 256 entries, one thread each; heavier ones are
 // listed in heavy[] (heavy_count) for k_jp_count_heavy.
 __global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cnt, JpLists L, i64 *heavy,
@@ -1388,61 +1492,14 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
                                                              const i64 *front, i64 nfront, JpLists L,
                                                              int32_t *err) {
     constexpr int WPB = kBlock / kWave;
+    constexpr i64 E = kWave / kJpGroup;  // entries per wavefront
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    uint32_t *mk = mark[w];
-    const i64 stride = (i64)gridDim.x * WPB;
-    const i64 trips = (nfront + stride - 1) / stride;
-    for (i64 t = 0; t < trips; t++) {
-        const i64 q = (i64)blockIdx.x * WPB + w + t * stride;
-        const bool in = q < nfront;
-        const i64 v = in ? front[q] : 0;
-        const i64 b = in ? off[v] : 0, e = in ? off[v + 1] : 0;
-        const u64 pv = in ? L.prio[v] : 0;
-        const i64 deg = e - b;
-        for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
-        __syncthreads();
-        // one pass: colours of the higher-priority neighbours (all coloured),
-        // release of the lower-priority ones
-        for (i64 j0 = 0; j0 < deg; j0 += kWave) {
-            const i64 j = j0 + lane;
-            bool ready = false;
-            i64 u = 0;
-            if (j < deg) {
-                u = tails[b + j];
-                if (u != v) {
-                    if (jp_below(L.prio, u, pv, v)) {
-                        ready = atomicSub(&cnt[u], 1) == 1;
-                    } else {
-                        const int c = colors[u];
-                        if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
-                    }
-                }
-            }
-            frontier_push(ready, u, off, L);
-        }
-        __syncthreads();
-        int cand = kMaxColors;
-        for (int i = lane; i < kMaxColors / 32; i += kWave) {
-            const uint32_t m = mk[i];
-            if (m != 0xFFFFFFFFu) {
-                const int c = i * 32 + __ffs((int)~m) - 1;
-                cand = c < cand ? c : cand;
-            }
-        }
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const int other = __shfl_xor(cand, o);
-            cand = other < cand ? other : cand;
-        }
-        if (in && lane == 0) {
-            int color = cand;
-            if (color >= kMaxColors) {
-                atomicOr(err, 1);
-                color = 0;
-            }
-            colors[v] = color;
-        }
-        __syncthreads();
+    const int w = threadIdx.x / kWave;
+    const i64 nw = (i64)gridDim.x * WPB, wid = (i64)blockIdx.x * WPB + w;
+    auto push = [&](bool ready, i64 u) { frontier_push(ready, u, off, L); };
+    for (i64 lo = wid * E; lo < nfront; lo += nw * E) {
+        const i64 n = nfront - lo < E ? nfront - lo : E;
+        jp_wave8(off, tails, colors, cnt, L.prio, front, lo, n, mark[w], err, push);
     }
 }
 
@@ -1611,15 +1668,19 @@ __global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const
 // the first frontier); otherwise continue from the frontier in sA/lA (counts
 // ring[0], ring[1]) after colored0 vertices were coloured.  When at most
 // tail_at vertices remain, workgroup 0 colours them by sequential first-fit
-// in increasing id (the host-driven path's tail).  ring: 8 words, [2q],
-// [2q+1] the small and large frontier counts of rounds r = q (mod 3), [6]
-// receives the rounds run here, [7] counts heavy vertices while counting.
+// in increasing id (the host-driven path's tail).  A frontier of more than
+// max_front vertices ends the launch before its round (the host's full-GPU
+// rounds take it; after r rounds the frontier is in sA/lA for even r, sB/lB
+// for odd r).  ring: 10 words, [2q], [2q+1] the small and large frontier
+// counts of rounds r = q (mod 3), [6] receives the rounds run here, [7]
+// counts heavy vertices while counting and then receives 1 when the launch
+// ended on a large frontier, [8] the coloured vertices at the end.
 constexpr int kJpPBlock = 1024;
 __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, const i64 *tails, i64 nv,
                                                              int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB,
                                                              i64 *lA, i64 *lB, i64 *ring, unsigned *bar,
                                                              int32_t *err, i64 tail_at, int do_count,
-                                                             i64 colored0, const u64 *prio) {
+                                                             i64 colored0, const u64 *prio, i64 max_front) {
     constexpr int WPB = kJpPBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
     __shared__ i64 tail_list[kJpPBlock];
@@ -1673,70 +1734,37 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
     }
     i64 colored = colored0;
     int r = 0;
+    bool exited = false;
     i64 *fs = sA, *fl = lA, *ns_ = sB, *nl_ = lB;
     while (alive) {
         const int q = r % 3, nq = (r + 1) % 3, zq = (r + 2) % 3;
-        const i64 nsmall = __atomic_load_n(&ring[2 * q], __ATOMIC_RELAXED);
-        const i64 nlarge = __atomic_load_n(&ring[2 * q + 1], __ATOMIC_RELAXED);
+        const i64 nsmall = __hip_atomic_load(&ring[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const i64 nlarge = __hip_atomic_load(&ring[2 * q + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (nsmall + nlarge == 0) break;
+        if (nsmall + nlarge > max_front) {  // a frontier that needs the whole GPU: back to the host
+            exited = true;
+            break;
+        }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             __atomic_store_n(&ring[2 * zq], (i64)0, __ATOMIC_RELAXED);
             __atomic_store_n(&ring[2 * zq + 1], (i64)0, __ATOMIC_RELAXED);
         }
         i64 *cs = &ring[2 * nq], *cl = &ring[2 * nq + 1];
-        // small frontier vertices: a wavefront each
-        for (i64 base = (i64)blockIdx.x * WPB; base < nsmall; base += (i64)nb * WPB) {
-            const i64 qq = base + w;
-            const bool in = qq < nsmall;
-            const i64 v = in ? fs[qq] : 0;
-            const i64 b = in ? off[v] : 0, e = in ? off[v + 1] : 0;
-            const u64 pv = in ? prio[v] : 0;
-            const i64 deg = e - b;
-            for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
-            __syncthreads();
-            // one pass: colours of the higher-priority neighbours, release of the lower ones
-            for (i64 j0 = 0; j0 < deg; j0 += kWave) {
-                const i64 j = j0 + lane;
-                bool ready = false;
-                i64 u = 0;
-                if (j < deg) {
-                    u = tails[b + j];
-                    if (u != v) {
-                        if (jp_below(prio, u, pv, v)) {
-                            ready = atomicSub(&cnt[u], 1) == 1;
-                        } else {
-                            const int c = colors[u];
-                            if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
-                        }
-                    }
-                }
+        // small frontier vertices: eight per wavefront
+        {
+            constexpr i64 E = kWave / kJpGroup;
+            const i64 nw = (i64)nb * WPB, wid = (i64)blockIdx.x * WPB + w;
+            auto push = [&](bool ready, i64 u) {
                 const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
                 wave_append(ready && !big, u, ns_, cs);
                 wave_append(big, u, nl_, cl);
+            };
+            for (i64 lo = wid * E; lo < nsmall; lo += nw * E) {
+                const i64 n = nsmall - lo < E ? nsmall - lo : E;
+                jp_wave8(off, tails, colors, cnt, prio, fs, lo, n, mk, err, push);
             }
-            __syncthreads();
-            int cand = kMaxColors;
-            for (int i = lane; i < kMaxColors / 32; i += kWave) {
-                const uint32_t m = mk[i];
-                if (m != 0xFFFFFFFFu) {
-                    const int c = i * 32 + __ffs((int)~m) - 1;
-                    cand = c < cand ? c : cand;
-                }
-            }
-            for (int o = kWave / 2; o > 0; o >>= 1) {
-                const int other = __shfl_xor(cand, o);
-                cand = other < cand ? other : cand;
-            }
-            if (in && lane == 0) {
-                int color = cand;
-                if (color >= kMaxColors) {
-                    atomicOr(err, 1);
-                    color = 0;
-                }
-                colors[v] = color;
-            }
-            __syncthreads();
         }
+        __syncthreads();  // mark[0] is the large loop's and wave 0's
         // large frontier vertices: a workgroup each
         for (i64 qq = blockIdx.x; qq < nlarge; qq += nb) {
             const i64 v = fl[qq];
@@ -1842,7 +1870,11 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
         fl = nl_;
         nl_ = t;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ring[6] = r;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ring[6] = r;
+        ring[7] = exited ? 1 : 0;
+        ring[8] = colored;
+    }
 }
 
 __global__ void k_jp_prio(const i64 *off, i64 nv, u64 *prio) {
@@ -2327,8 +2359,8 @@ void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *hea
 void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const JpLists &cur,
                         const i64 n[3], const JpLists &next, uint32_t *gmark, int32_t *err, hipStream_t s) {
     if (n[0] > 0) {
-        const i64 wpb = kBlock / kWave;
-        const unsigned blocks = (unsigned)std::max<i64>(1, std::min<i64>((n[0] + wpb - 1) / wpb, 16384));
+        const i64 per = (kBlock / kWave) * (kWave / kJpGroup);  // entries per workgroup
+        const unsigned blocks = (unsigned)std::max<i64>(1, std::min<i64>((n[0] + per - 1) / per, 16384));
         k_jp_frontier_wave<<<blocks, kBlock, 0, s>>>(g.off, g.tails, colors, cnt, cur.small, n[0], next, err);
         GLV_LAUNCH_CHECK();
     }
@@ -2371,13 +2403,13 @@ int jp_persistent_blocks() {
 }
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, const u64 *prio, int nblocks, hipStream_t s) {
+                          i64 colored0, const u64 *prio, i64 max_front, int nblocks, hipStream_t s) {
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0) return false;
     const i64 *off = g.off, *tails = g.tails;
-    i64 nv = g.nv, ta = tail_at, c0 = colored0;
+    i64 nv = g.nv, ta = tail_at, c0 = colored0, mf = max_front;
     int dc = count ? 1 : 0;
     const u64 *pr = prio;
-    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0, &pr};
+    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0, &pr, &mf};
     hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_persistent, dim3(nblocks), dim3(kJpPBlock),
                                               args, 0, s);
     if (e != hipSuccess) {

@@ -162,9 +162,11 @@ constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many unco
 int jp_persistent_blocks();
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, const u64 *prio, int nblocks, hipStream_t s);
+                          i64 colored0, const u64 *prio, i64 max_front, int nblocks, hipStream_t s);
 // frontiers below this many vertices finish in one cooperative launch
 constexpr i64 kJpPersistFrontier = 16384;
+// a cooperative launch hands back to full-GPU rounds once a frontier exceeds this
+constexpr i64 kJpPersistExit = 32768;
 void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);
 
 // --- renumber / compose --------------------------------------------------

@@ -146,64 +146,79 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     launch_fill_i32(colors, nv, -1, s_);
     const int pnb = (int)std::min<i64>({(i64)jp_persistent_blocks(), kPersistMaxBlocks,
                                          std::max<i64>(4, nv / 2048)});
-    DBuf<i64> ring(8, s_);
+    DBuf<i64> ring(10, s_);
     DBuf<unsigned> bar(kBarrierWords, s_);
     DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
     DBuf<u64> prio(nv, s_);
     launch_jp_prio(g, prio.p, s_);
     launch_fill_i32(err.p, 1, 0, s_);
     launch_fill_i32(maxc.p, 1, -1, s_);
-    // the rounds run in one cooperative launch once the frontier no longer fills the GPU
-    auto persistent = [&](i64 *sA, i64 *sB, i64 *lA, i64 *lB, const i64 n0[2], bool count, i64 colored0,
-                          int *rounds) -> bool {
-        i64 h[8] = {n0[0], n0[1], 0, 0, 0, 0, 0, 0};
+    int round = 0;
+    i64 colored = 0;
+    i64 hc[3] = {0, 0, 0};  // sizes of the current frontier's small, large and giant lists
+    DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
+    JpLists cur{sA.p, lA.p, gA.p, counts.p, prio.p}, next{sB.p, lB.p, gB.p, counts.p + 4, prio.p};
+    // Rounds in one cooperative launch while the frontier does not fill the
+    // GPU.  count: start from scratch; else continue from the frontier in
+    // cur (hc, no giant list).  Returns 0 when the launch was refused, 1 when
+    // the coloring is complete, 2 when a large frontier handed it back (cur,
+    // hc, colored and round then describe the state).
+    auto persistent = [&](bool count) -> int {
+        i64 h[10] = {count ? 0 : hc[0], count ? 0 : hc[1], 0, 0, 0, 0, 0, 0, 0, 0};
         GLV_HIP(hipMemcpyAsync(ring.p, h, sizeof(h), hipMemcpyHostToDevice, s_));
         launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
-        const bool ok = pnb > 0 && launch_jp_persistent(g, colors, cnt.p, sA, sB, lA, lB, ring.p, bar.p, err.p,
-                                                        kJpTail, count, colored0, prio.p, pnb, s_);
+        const bool ok = pnb > 0 && launch_jp_persistent(g, colors, cnt.p, cur.small, next.small, cur.large,
+                                                        next.large, ring.p, bar.p, err.p, kJpTail, count,
+                                                        colored, prio.p, kJpPersistExit, pnb, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
         unsigned abort_flag = 0;
-        i64 r = 0;
-        GLV_HIP(hipMemcpyAsync(&r, ring.p + 6, sizeof(i64), hipMemcpyDeviceToHost, s_));
+        GLV_HIP(hipMemcpyAsync(h, ring.p, sizeof(h), hipMemcpyDeviceToHost, s_));
         GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
         GLV_HIP(hipStreamSynchronize(s_));  // `h` is pageable host memory
-        if (!ok) return false;
+        if (!ok) return 0;
         if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
-        *rounds += (int)r;
-        return true;
+        const int r = (int)h[6];
+        round += r;
+        if (h[7] == 0) return 1;
+        colored = h[8];
+        hc[0] = h[2 * (r % 3)];
+        hc[1] = h[2 * (r % 3) + 1];
+        hc[2] = 0;
+        if (r % 2 == 1) std::swap(cur, next);  // the frontier sits in the second pair of lists
+        return 2;
     };
-    int round = 0;
-    DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
-    bool done = false;
+    bool done = false, have_frontier = false;
     if (nv <= kPersistMaxNV) {
-        const i64 z[2] = {0, 0};
-        done = persistent(sA.p, sB.p, lA.p, lB.p, z, true, 0, &round);
-        if (!done) launch_fill_i32(colors, nv, -1, s_);  // launch refused: host-driven rounds below
+        const int st = persistent(true);
+        done = st == 1;
+        have_frontier = st == 2;
+        if (st == 0) launch_fill_i32(colors, nv, -1, s_);  // launch refused: host-driven rounds below
     }
     if (!done) {
         DBuf<uint32_t> gmark;
-        launch_fill_i64(counts.p, 8, 0, s_);
-        JpLists cur{sA.p, lA.p, gA.p, counts.p, prio.p}, next{sB.p, lB.p, gB.p, counts.p + 4, prio.p};
-        Profiler::get().begin(K_COLOR, s_);
-        launch_jp_count(g, cnt.p, cur, sB.p, counts.p + 3, s_);  // heavy list in the next round's buffer
-        Profiler::get().end(K_COLOR, 0.0, s_);
-        i64 hc[3];
-        GLV_HIP(hipMemcpyAsync(hc, counts.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
-        GLV_HIP(hipStreamSynchronize(s_));
-        i64 colored = 0;
+        if (!have_frontier) {
+            launch_fill_i64(counts.p, 8, 0, s_);
+            Profiler::get().begin(K_COLOR, s_);
+            launch_jp_count(g, cnt.p, cur, next.small, cur.counts + 3, s_);  // heavy list in the next round's buffer
+            Profiler::get().end(K_COLOR, 0.0, s_);
+            GLV_HIP(hipMemcpyAsync(hc, cur.counts, sizeof(hc), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipStreamSynchronize(s_));
+        }
         while (hc[0] + hc[1] + hc[2] > 0) {
             if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
             const i64 won = hc[0] + hc[1] + hc[2];
             if (won < kJpPersistFrontier && colored > 0) {
-                // hand the remaining rounds to one cooperative launch: giant
+                // hand the following rounds to one cooperative launch: giant
                 // frontier vertices join the large list
                 if (hc[2] > 0)
                     GLV_HIP(hipMemcpyAsync(cur.large + hc[1], cur.giant, hc[2] * sizeof(i64),
                                            hipMemcpyDeviceToDevice, s_));
                 hc[1] += hc[2];
                 hc[2] = 0;
-                if (persistent(cur.small, next.small, cur.large, next.large, hc, false, colored, &round)) break;
+                const int st = persistent(false);
+                if (st == 1) break;
+                if (st == 2) continue;
             }
             if (hc[2] > 0 && (i64)gmark.n < hc[2] * (kMaxColors / 32)) gmark.alloc(hc[2] * (kMaxColors / 32), s_);
             launch_fill_i64(next.counts, 4, 0, s_);
