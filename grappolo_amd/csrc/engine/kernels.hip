@@ -923,6 +923,7 @@ __device__ bool grid_barrier(unsigned *bar, unsigned nb, unsigned seq) {
 // kPHashMaxDeg plus a touched-community bitmap of the whole graph for larger
 // vertices, which accumulate into the workgroup's dense HBM row (A.dacc).
 constexpr int kPHashSlots = 4096;
+constexpr int kGatherUnroll = 4;
 constexpr i64 kPHashMaxDeg = kPHashSlots / 2;
 constexpr int kPBitmapWords = (int)(kPersistMaxNV / 32);
 constexpr size_t kPDenseBytes = kDenseMaxNV * 8 + kDenseMaxNV / 32 * 4 + kDenseListMax * 2;
@@ -953,6 +954,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
     __shared__ i64 si[kPBlock / kWave];
     __shared__ int s_cnt;     // communities listed for the current vertex (dense view)
     __shared__ double s_sl;   // its self-loop weight
+    __shared__ i64 s_nid;     // dense view: size of the chosen community
     // dense view
     double *acc = reinterpret_cast<double *>(smem);
     uint32_t *touched = reinterpret_cast<uint32_t *>(smem + kDenseMaxNV * 8);
@@ -988,23 +990,61 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
     bool alive = grid_barrier(bar, nb, ++seq);
     int par = 0;            // buffer holding the state after the previous step
     i64 plo = 0, ptot = 0;  // the previous step's class
+    // Read-ahead of the workgroup's first vertex of the next step, issued along
+    // with the loads of the current step so that a step starts from the labels
+    // of its neighbours: id, entries, label (a vertex's label changes only in
+    // its own step, so either buffer holds it) and degree, and per thread the
+    // first kPBlock entries.
+    int ra_c = -1;
+    i64 ra_v = -1, ra_beg = 0, ra_end = 0, ra_s = -1, ra_t = 0;
+    double ra_k = 0, ra_w = 0;
+    // thread 0: the move made for its first vertex of the previous step
+    i64 mv_v = -1, mv_t = -1, mv_s = -1;
+    double mv_k = 0;
     for (int c = 0; c < ncol && alive; c++) {
-        const i64 lo = cstart[c], total = cstart[c + 1] - lo;
+        const i64 lo = cstart[c], hi = cstart[c + 1], total = hi - lo;
         if (total == 0) continue;
+        const i64 hi2 = c + 1 < ncol ? cstart[c + 2] : hi;
         A.C = par ? C1 : C;
         A.a = par ? a1 : a;
         A.n = par ? n1 : n;
         i64 *Cn = par ? C : C1;
         double *an = par ? a : a1;
         i64 *nn = par ? n : n1;
+        // the previous step's moves of this workgroup's vertices, into the other buffer
+        if (threadIdx.x == 0) {
+            if (mv_v >= 0 && mv_t != mv_s) {
+                move_vertex(mv_t, mv_s, mv_k, an, nn);
+                Cn[mv_v] = mv_t;
+            }
+            for (i64 q = blockIdx.x + nb; q < ptot; q += nb) {
+                const i64 v = cperm[plo + q];
+                const i64 t = A.tgt[v], s = Cn[v];
+                if (t != s) {
+                    move_vertex(t, s, A.k[v], an, nn);
+                    Cn[v] = t;
+                }
+            }
+        }
+        mv_v = -1;
+        // read-ahead, first load: the next step's vertex of this workgroup
+        const i64 nx_v = (i64)blockIdx.x < hi2 - hi ? cperm[hi + blockIdx.x] : -1;
+        i64 nx_beg = 0, nx_end = 0, nx_s = -1;
+        double nx_k = 0;
+        bool nx_done = false;
         for (i64 q = blockIdx.x; q < total; q += nb) {
-            const i64 v = cperm[lo + q];
-            const i64 beg = A.off[v], end = A.off[v + 1];
-            const i64 s = A.C[v];
+            const bool first = q == (i64)blockIdx.x;
+            const bool ra = first && ra_c == c;
+            const i64 v = ra ? ra_v : cperm[lo + q];
+            const i64 beg = ra ? ra_beg : A.off[v], end = ra ? ra_end : A.off[v + 1];
+            const i64 s = ra ? ra_s : A.C[v];
             if (s < 0) {  // a vertex without entries (label -1) stays where it is
                 if (threadIdx.x == 0) A.tgt[v] = s;
                 continue;
             }
+            const double as = A.a[s];
+            i64 ns = 0;
+            if (threadIdx.x == 0) ns = A.n[s];
             double sl = 0, e_s = 0;
             // HASH: LDS table of this vertex, or the dense HBM row when it is large
             const bool big = HASH && end - beg > kPHashMaxDeg;
@@ -1017,33 +1057,57 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 shift = 64 - lg;
             }
             const u64 mask = (u64)slots - 1;
-            for (i64 j = beg + threadIdx.x; j < end; j += kPBlock) {
-                const i64 t = A.tails[j];
-                const double ww = A.w[j];
-                if (t == v) atomicAdd(&s_sl, ww);
-                const i64 cc = A.C[t];
-                if (HASH) {
-                    if (big) {
-                        atomicOr(&hbits[cc >> 5], 1u << (cc & 31));
-                        atomicAdd(&dacc[cc], ww);
+            // kGatherUnroll passes of the workgroup at a time: their entry loads,
+            // then their label loads, then the accumulation, so that a heavy
+            // vertex waits for two loads per kGatherUnroll passes
+            for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
+                i64 tt[kGatherUnroll], cl[kGatherUnroll];
+                double wt[kGatherUnroll];
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    const i64 j = j0 + (i64)u * kPBlock;
+                    const bool pre = ra && u == 0 && j0 == beg + threadIdx.x;
+                    tt[u] = j < end ? (pre ? ra_t : A.tails[j]) : -1;
+                    wt[u] = j < end ? (pre ? ra_w : A.w[j]) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) cl[u] = tt[u] >= 0 ? A.C[tt[u]] : 0;
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    if (tt[u] < 0) continue;
+                    const i64 cc = cl[u];
+                    const double ww = wt[u];
+                    if (tt[u] == v) atomicAdd(&s_sl, ww);
+                    if (HASH) {
+                        if (big) {
+                            atomicOr(&hbits[cc >> 5], 1u << (cc & 31));
+                            atomicAdd(&dacc[cc], ww);
+                        } else {
+                            hash_insert(hkeys, hvals, mask, shift, cc, ww);
+                        }
                     } else {
-                        hash_insert(hkeys, hvals, mask, shift, cc, ww);
+                        // the first lane to touch a community lists it
+                        const uint32_t bit = 1u << (cc & 31);
+                        if (!(atomicOr(&touched[cc >> 5], bit) & bit)) list[atomicAdd(&s_cnt, 1)] = (uint16_t)cc;
+                        atomicAdd(&acc[cc], ww);
                     }
-                } else {
-                    // the first lane to touch a community lists it
-                    const uint32_t bit = 1u << (cc & 31);
-                    if (!(atomicOr(&touched[cc >> 5], bit) & bit)) list[atomicAdd(&s_cnt, 1)] = (uint16_t)cc;
-                    atomicAdd(&acc[cc], ww);
                 }
             }
             if (HASH) __threadfence_block();
             __syncthreads();
+            if (first && nx_v >= 0) {  // read-ahead, second load
+                nx_beg = A.off[nx_v];
+                nx_end = A.off[nx_v + 1];
+                nx_s = C[nx_v];
+                nx_k = A.k[nx_v];
+            }
             sl = s_sl;
-            const double kv = A.k[v];
-            const double ax = A.a[s] - kv;
+            const double kv = ra ? ra_k : A.k[v];
+            const double ax = as - kv;
             bool cand = false;
             double gain = 0;
             i64 id = INT64_MAX;
+            i64 nid = 0;  // dense view: size of the community `id`
             float t1 = -INFINITY, t2 = -INFINITY;
             int ntouched = 0;
             if (HASH && big) {
@@ -1075,25 +1139,59 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 e_s = acc[s];
                 const double eix = e_s - sl;
                 const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
-                for (int k = threadIdx.x; k < ntouched; k += kPBlock) {
-                    const i64 y = list[k];
-                    if (y == s) continue;
-                    const double g = gain_of(acc[y], eix, kv, A.a[y], ax, A.constant);
-                    if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
-                    if (better(g > 0, g, y, cand, gain, id)) {
-                        cand = true;
-                        gain = g;
-                        id = y;
+                for (int k0 = threadIdx.x; k0 < ntouched; k0 += kGatherUnroll * kPBlock) {
+                    i64 yy[kGatherUnroll], nyy[kGatherUnroll];
+                    double ay[kGatherUnroll];
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const int k = k0 + u * kPBlock;
+                        yy[u] = k < ntouched ? (i64)list[k] : -1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        ay[u] = yy[u] >= 0 ? A.a[yy[u]] : 0.0;
+                        nyy[u] = yy[u] >= 0 ? A.n[yy[u]] : 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const i64 y = yy[u];
+                        if (y < 0 || y == s) continue;
+                        const double g = gain_of(acc[y], eix, kv, ay[u], ax, A.constant);
+                        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+                        if (better(g > 0, g, y, cand, gain, id)) {
+                            cand = true;
+                            gain = g;
+                            id = y;
+                            nid = nyy[u];
+                        }
                     }
                 }
             }
+            const bool my_cand = cand;
+            const i64 my_id = id;
             block_argmax_p(cand, gain, id, sc, sg, si);
             if (A.feat) block_top2<kPBlock>(t1, t2);
+            if (!HASH) {  // the size of the winner, from the lane that scored it
+                if (cand && my_cand && my_id == id) s_nid = nid;
+                __syncthreads();
+            }
+            if (first && nx_v >= 0) {  // read-ahead, last load
+                const i64 j = nx_beg + threadIdx.x;
+                ra_t = j < nx_end ? A.tails[j] : 0;
+                ra_w = j < nx_end ? A.w[j] : 0.0;
+                ra_c = c + 1;
+                ra_v = nx_v;
+                ra_beg = nx_beg;
+                ra_end = nx_end;
+                ra_s = nx_s;
+                ra_k = nx_k;
+                nx_done = true;
+            }
             if (threadIdx.x == 0) {
                 i64 target = s;
                 if (cand) {
                     target = id;
-                    if (A.n[target] == 1 && A.n[s] == 1 && target > s) target = s;
+                    if (ns == 1 && target > s && (HASH ? A.n[target] : s_nid) == 1) target = s;
                 }
                 A.tgt[v] = target;
                 if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
@@ -1101,6 +1199,12 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                     move_vertex(target, s, kv, an, nn);
                     Cn[v] = target;
                     if (stale) stale[v] = 0;
+                }
+                if (first) {
+                    mv_v = v;
+                    mv_t = target;
+                    mv_s = s;
+                    mv_k = kv;
                 }
             }
             __syncthreads();  // every read of the accumulator is done
@@ -1132,15 +1236,18 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             }
             __syncthreads();
         }
-        // the previous step's moves of this workgroup's vertices, into the other buffer
-        if (threadIdx.x == 0) {
-            for (i64 q = blockIdx.x; q < ptot; q += nb) {
-                const i64 v = cperm[plo + q];
-                const i64 t = A.tgt[v], s = Cn[v];
-                if (t != s) {
-                    move_vertex(t, s, A.k[v], an, nn);
-                    Cn[v] = t;
-                }
+        if (!nx_done) {  // no vertex of this step took the read-ahead along
+            ra_c = -1;
+            if (nx_v >= 0) {
+                ra_beg = A.off[nx_v];
+                ra_end = A.off[nx_v + 1];
+                ra_s = C[nx_v];
+                ra_k = A.k[nx_v];
+                const i64 j = ra_beg + threadIdx.x;
+                ra_t = j < ra_end ? A.tails[j] : 0;
+                ra_w = j < ra_end ? A.w[j] : 0.0;
+                ra_v = nx_v;
+                ra_c = c + 1;
             }
         }
         alive = grid_barrier(bar, nb, ++seq);
@@ -1316,6 +1423,51 @@ __device__ inline void wave_append(bool pred, i64 v, i64 *list, i64 *count) {
     if (pred) list[base + __popcll(m & lt)] = v;
 }
 
+// Workgroup-local staging of list appends.  Device-scope atomics on one
+// counter serialise (about 12 ns each on MI355X, whatever the caller), so a
+// round that appends from every wavefront is bound by the counter; staged
+// appends take an LDS atomic per wavefront and stage_flush moves the staged
+// entries with one device atomic per workgroup.  Entries that do not fit go
+// straight to the list.  The order of a list is not significant to its users.
+template <int CAP>
+struct Stage {
+    i64 buf[CAP];
+    int n;
+    i64 base;
+};
+template <int CAP>
+__device__ inline void stage_init(Stage<CAP> &S) {
+    if (threadIdx.x == 0) S.n = 0;
+}
+template <int CAP>
+__device__ inline void stage_append(bool pred, i64 v, Stage<CAP> &S, i64 *list, i64 *count) {
+    const u64 m = __ballot(pred);
+    if (!m) return;
+    const int lane = threadIdx.x % kWave;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&S.n, __popcll(m));
+    base = __shfl(base, leader);
+    const u64 lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    const int at = base + __popcll(m & lt);
+    const bool fits = at < CAP;
+    if (pred && fits) S.buf[at] = v;
+    wave_append(pred && !fits, v, list, count);
+}
+// Called by every thread of the workgroup once its appends are done; leaves
+// the stage empty.
+template <int CAP>
+__device__ inline void stage_flush(Stage<CAP> &S, i64 *list, i64 *count) {
+    __syncthreads();
+    const int n = S.n < CAP ? S.n : CAP;
+    if (threadIdx.x == 0 && n > 0) S.base = (i64)atomicAdd((unsigned long long *)count, (unsigned long long)n);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) list[S.base + i] = S.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) S.n = 0;
+    __syncthreads();
+}
+
 // Jones-Plassmann as a topological sweep of the priority DAG: count[v] is the
 // number of uncoloured neighbours with a higher priority; a vertex enters the
 // frontier of round r when its count reaches zero, which is exactly when round
@@ -1330,6 +1482,17 @@ __device__ inline void frontier_push(bool pred, i64 u, const i64 *off, const JpL
     wave_append(big && !giant, u, L.large, &L.counts[1]);
     wave_append(giant, u, L.giant, &L.counts[2]);
 }
+// The same with the small list staged in S (the other two are rare).
+template <int CAP>
+__device__ inline void frontier_push(bool pred, i64 u, const i64 *off, const JpLists &L, Stage<CAP> &S) {
+    const i64 d = pred ? off[u + 1] - off[u] : 0;
+    const bool big = pred && d > kBinMaxDeg[6];
+    const bool giant = big && d > kGiantDeg;
+    stage_append(pred && !big, u, S, L.small, &L.counts[0]);
+    wave_append(big && !giant, u, L.large, &L.counts[1]);
+    wave_append(giant, u, L.giant, &L.counts[2]);
+}
+constexpr int kStageCap = 2048;
 
 // One wavefront colours up to eight frontier entries front[lo .. lo+n): a
 // vertex of at most kJpLightDeg entries takes a group of kJpGroup lanes (its
@@ -1437,12 +1600,21 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
 
 // count[v] of vertices up to 256 entries, one thread each; heavier ones are
 // listed in heavy[] (heavy_count) for k_jp_count_heavy.
-__global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cnt, JpLists L, i64 *heavy,
-                           i64 *heavy_count) {
+__global__ __launch_bounds__(kBlock) void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cnt,
+                                                     JpLists L, i64 *heavy, i64 *heavy_count) {
+    __shared__ Stage<kStageCap> S;
+    stage_init(S);
+    __syncthreads();
     const i64 stride = (i64)gridDim.x * blockDim.x;
     const i64 first = blockIdx.x * (i64)blockDim.x + threadIdx.x;
     const i64 trips = (nv + stride - 1) / stride;
     for (i64 t = 0; t < trips; t++) {
+        if (t > 0) {  // room for this trip's appends (a workgroup-uniform decision)
+            __syncthreads();
+            const int staged = S.n;
+            __syncthreads();
+            if (staged > kStageCap - kBlock) stage_flush(S, L.small, &L.counts[0]);
+        }
         const i64 v = first + t * stride;
         int c = 0;
         bool heavy_v = false;
@@ -1460,8 +1632,9 @@ __global__ void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cn
             }
         }
         wave_append(heavy_v, v, heavy, heavy_count);
-        frontier_push(v < nv && !heavy_v && c == 0, v, off, L);
+        frontier_push(v < nv && !heavy_v && c == 0, v, off, L, S);
     }
+    stage_flush(S, L.small, &L.counts[0]);
 }
 // count[v] of the heavy vertices, one wavefront each
 __global__ __launch_bounds__(kBlock) void k_jp_count_heavy(const i64 *off, const i64 *tails, int32_t *cnt,
@@ -1494,13 +1667,17 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
     constexpr int WPB = kBlock / kWave;
     constexpr i64 E = kWave / kJpGroup;  // entries per wavefront
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
+    __shared__ Stage<kStageCap> S;
+    stage_init(S);
+    __syncthreads();
     const int w = threadIdx.x / kWave;
     const i64 nw = (i64)gridDim.x * WPB, wid = (i64)blockIdx.x * WPB + w;
-    auto push = [&](bool ready, i64 u) { frontier_push(ready, u, off, L); };
+    auto push = [&](bool ready, i64 u) { frontier_push(ready, u, off, L, S); };
     for (i64 lo = wid * E; lo < nfront; lo += nw * E) {
         const i64 n = nfront - lo < E ? nfront - lo : E;
         jp_wave8(off, tails, colors, cnt, L.prio, front, lo, n, mark[w], err, push);
     }
+    stage_flush(S, L.small, &L.counts[0]);
 }
 
 // Frontier vertices of degree > 256, one workgroup each.
@@ -1510,6 +1687,8 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                                                               int32_t *err) {
     __shared__ uint32_t s_mark[kMaxColors / 32];
     __shared__ int s_color;
+    __shared__ Stage<kStageCap> S;
+    stage_init(S);
     for (i64 q = blockIdx.x; q < nfront; q += gridDim.x) {
         const i64 v = front[q];
         const i64 b = off[v], e = off[v + 1];
@@ -1536,7 +1715,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                     }
                 }
             }
-            frontier_push(ready, u, off, L);
+            frontier_push(ready, u, off, L, S);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
@@ -1553,7 +1732,9 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
             colors[v] = color;
         }
         __syncthreads();
+        if (S.n > kStageCap / 2) stage_flush(S, L.small, &L.counts[0]);  // S.n is settled here
     }
+    stage_flush(S, L.small, &L.counts[0]);
 }
 
 // Giant frontier vertices, kGiantBlocks workgroups each (blockIdx.y): mark
@@ -1577,6 +1758,9 @@ __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, con
                                                              const i64 *front, const uint32_t *gmark,
                                                              JpLists L, int32_t *err) {
     __shared__ int s_color;
+    __shared__ Stage<kStageCap> S;
+    stage_init(S);
+    __syncthreads();
     const i64 v = front[blockIdx.y];
     if (blockIdx.x == 0) {
         const uint32_t *mk = gmark + (i64)blockIdx.y * (kMaxColors / 32);
@@ -1607,8 +1791,9 @@ __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, con
             u = tails[b + j];
             if (u != v && jp_below(L.prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
         }
-        frontier_push(ready, u, off, L);
+        frontier_push(ready, u, off, L, S);
     }
+    stage_flush(S, L.small, &L.counts[0]);
 }
 
 // Vertices still uncoloured, in increasing id (flags + scan + scatter).
@@ -1686,6 +1871,9 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
     __shared__ i64 tail_list[kJpPBlock];
     __shared__ int wsum[kJpPBlock / kWave];
     __shared__ int s_color;
+    __shared__ Stage<kStageCap> S;  // appends to the next small frontier
+    stage_init(S);
+    __syncthreads();
     const unsigned nb = gridDim.x;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     uint32_t *mk = mark[w];
@@ -1695,6 +1883,12 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
         // counts: one thread per vertex up to 256 entries; heavier ones are listed
         // in lB (ring[7]) and counted by a wavefront each
         for (i64 base = (i64)blockIdx.x * kJpPBlock; base < nv; base += (i64)nb * kJpPBlock) {
+            if (base != (i64)blockIdx.x * kJpPBlock) {  // room for this trip's appends
+                __syncthreads();
+                const int staged = S.n;
+                __syncthreads();
+                if (staged > kStageCap - kJpPBlock) stage_flush(S, sA, &ring[0]);
+            }
             const i64 v = base + threadIdx.x;
             int c = 0;
             bool heavy = false;
@@ -1712,8 +1906,9 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 }
             }
             wave_append(heavy, v, lB, &ring[7]);
-            wave_append(v < nv && !heavy && c == 0, v, sA, &ring[0]);
+            stage_append(v < nv && !heavy && c == 0, v, S, sA, &ring[0]);
         }
+        stage_flush(S, sA, &ring[0]);
         alive = grid_barrier(bar, nb, ++seq);
         if (alive) {
             const i64 nheavy = __atomic_load_n(&ring[7], __ATOMIC_RELAXED);
@@ -1756,7 +1951,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             const i64 nw = (i64)nb * WPB, wid = (i64)blockIdx.x * WPB + w;
             auto push = [&](bool ready, i64 u) {
                 const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
-                wave_append(ready && !big, u, ns_, cs);
+                stage_append(ready && !big, u, S, ns_, cs);
                 wave_append(big, u, nl_, cl);
             };
             for (i64 lo = wid * E; lo < nsmall; lo += nw * E) {
@@ -1792,7 +1987,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                     }
                 }
                 const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
-                wave_append(ready && !big, u, ns_, cs);
+                stage_append(ready && !big, u, S, ns_, cs);
                 wave_append(big, u, nl_, cl);
             }
             __syncthreads();
@@ -1810,7 +2005,9 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 colors[v] = color;
             }
             __syncthreads();
+            if (S.n > kStageCap / 2) stage_flush(S, ns_, cs);  // S.n is settled here
         }
+        stage_flush(S, ns_, cs);
         alive = grid_barrier(bar, nb, ++seq);
         colored += nsmall + nlarge;
         r++;
@@ -2350,7 +2547,7 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
 void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *heavy, i64 *heavy_count,
                      hipStream_t s) {
     if (g.nv <= 0) return;
-    k_jp_count<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.tails, g.nv, cnt, L, heavy,
+    k_jp_count<<<grid_for(g.nv, kBlock, 2048), kBlock, 0, s>>>(g.off, g.tails, g.nv, cnt, L, heavy,
                                                                heavy_count);
     GLV_LAUNCH_CHECK();
     k_jp_count_heavy<<<4096, kBlock, 0, s>>>(g.off, g.tails, cnt, heavy, heavy_count, L);
@@ -2359,7 +2556,9 @@ void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *hea
 void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const JpLists &cur,
                         const i64 n[3], const JpLists &next, uint32_t *gmark, int32_t *err, hipStream_t s) {
     if (n[0] > 0) {
-        const i64 per = (kBlock / kWave) * (kWave / kJpGroup);  // entries per workgroup
+        // entries per workgroup: four passes of its wavefronts, so that the
+        // staged appends cost one device atomic per 128 frontier entries
+        const i64 per = 4 * (kBlock / kWave) * (kWave / kJpGroup);
         const unsigned blocks = (unsigned)std::max<i64>(1, std::min<i64>((n[0] + per - 1) / per, 16384));
         k_jp_frontier_wave<<<blocks, kBlock, 0, s>>>(g.off, g.tails, colors, cnt, cur.small, n[0], next, err);
         GLV_LAUNCH_CHECK();
