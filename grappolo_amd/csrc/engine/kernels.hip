@@ -924,6 +924,7 @@ __device__ bool grid_barrier(unsigned *bar, unsigned nb, unsigned seq) {
 // vertices, which accumulate into the workgroup's dense HBM row (A.dacc).
 constexpr int kPHashSlots = 4096;
 constexpr int kGatherUnroll = 4;
+constexpr int kSlabMaxNV = 2 * (int)kDenseMaxNV;
 constexpr i64 kPHashMaxDeg = kPHashSlots / 2;
 constexpr int kPBitmapWords = (int)(kPersistMaxNV / 32);
 constexpr size_t kPDenseBytes = kDenseMaxNV * 8 + kDenseMaxNV / 32 * 4 + kDenseListMax * 2;
@@ -1046,8 +1047,15 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             i64 ns = 0;
             if (threadIdx.x == 0) ns = A.n[s];
             double sl = 0, e_s = 0;
-            // HASH: LDS table of this vertex, or the dense HBM row when it is large
+            // HASH: LDS table of this vertex; when it is large, the dense view
+            // one slab of kDenseMaxNV communities at a time (graphs of at most
+            // kSlabMaxNV vertices, the slab of its own community first), else
+            // the dense HBM row
             const bool big = HASH && end - beg > kPHashMaxDeg;
+            const bool slab = big && nv <= kSlabMaxNV;
+            const bool dense = !HASH || slab;
+            const int nsl = slab ? (nv + (int)kDenseMaxNV - 1) / (int)kDenseMaxNV : 1;
+            const int s_slab = slab ? (int)(s / kDenseMaxNV) : 0;
             i64 slots = 64;
             int shift = 58;
             if (HASH && !big) {
@@ -1057,113 +1065,153 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 shift = 64 - lg;
             }
             const u64 mask = (u64)slots - 1;
-            // kGatherUnroll passes of the workgroup at a time: their entry loads,
-            // then their label loads, then the accumulation, so that a heavy
-            // vertex waits for two loads per kGatherUnroll passes
-            for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
-                i64 tt[kGatherUnroll], cl[kGatherUnroll];
-                double wt[kGatherUnroll];
-#pragma unroll
-                for (int u = 0; u < kGatherUnroll; u++) {
-                    const i64 j = j0 + (i64)u * kPBlock;
-                    const bool pre = ra && u == 0 && j0 == beg + threadIdx.x;
-                    tt[u] = j < end ? (pre ? ra_t : A.tails[j]) : -1;
-                    wt[u] = j < end ? (pre ? ra_w : A.w[j]) : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < kGatherUnroll; u++) cl[u] = tt[u] >= 0 ? A.C[tt[u]] : 0;
-#pragma unroll
-                for (int u = 0; u < kGatherUnroll; u++) {
-                    if (tt[u] < 0) continue;
-                    const i64 cc = cl[u];
-                    const double ww = wt[u];
-                    if (tt[u] == v) atomicAdd(&s_sl, ww);
-                    if (HASH) {
-                        if (big) {
-                            atomicOr(&hbits[cc >> 5], 1u << (cc & 31));
-                            atomicAdd(&dacc[cc], ww);
-                        } else {
-                            hash_insert(hkeys, hvals, mask, shift, cc, ww);
-                        }
-                    } else {
-                        // the first lane to touch a community lists it
-                        const uint32_t bit = 1u << (cc & 31);
-                        if (!(atomicOr(&touched[cc >> 5], bit) & bit)) list[atomicAdd(&s_cnt, 1)] = (uint16_t)cc;
-                        atomicAdd(&acc[cc], ww);
-                    }
-                }
+            if (slab) {  // the dense view replaces the (empty) hash view
+                for (int i = threadIdx.x; i < (int)kDenseMaxNV; i += kPBlock) acc[i] = 0.0;
+                for (int i = threadIdx.x; i < (int)(kDenseMaxNV / 32); i += kPBlock) touched[i] = 0;
+                __syncthreads();
             }
-            if (HASH) __threadfence_block();
-            __syncthreads();
-            if (first && nx_v >= 0) {  // read-ahead, second load
-                nx_beg = A.off[nx_v];
-                nx_end = A.off[nx_v + 1];
-                nx_s = C[nx_v];
-                nx_k = A.k[nx_v];
-            }
-            sl = s_sl;
             const double kv = ra ? ra_k : A.k[v];
             const double ax = as - kv;
+            const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
             bool cand = false;
             double gain = 0;
             i64 id = INT64_MAX;
             i64 nid = 0;  // dense view: size of the community `id`
             float t1 = -INFINITY, t2 = -INFINITY;
             int ntouched = 0;
-            if (HASH && big) {
-                e_s = ld_agent(&dacc[s]);
-                const double eix = e_s - sl;
-                const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
-                for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
-                    uint32_t m = hbits[wi];
-                    while (m) {
-                        const int b = __ffs((int)m) - 1;
-                        m &= m - 1;
-                        const i64 y = (i64)wi * 32 + b;
-                        if (y == s) continue;
-                        const double g = gain_of(ld_agent(&dacc[y]), eix, kv, A.a[y], ax, A.constant);
-                        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
-                        if (better(g > 0, g, y, cand, gain, id)) {
-                            cand = true;
-                            gain = g;
-                            id = y;
+            double eix = 0;
+            for (int pi = 0; pi < nsl; pi++) {
+                const i64 base = slab ? (i64)((s_slab + pi) % nsl) * kDenseMaxNV : 0;
+                // kGatherUnroll passes of the workgroup at a time: their entry loads,
+                // then their label loads, then the accumulation, so that a heavy
+                // vertex waits for two loads per kGatherUnroll passes
+                for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
+                    i64 tt[kGatherUnroll], cl[kGatherUnroll];
+                    double wt[kGatherUnroll];
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const i64 j = j0 + (i64)u * kPBlock;
+                        const bool pre = ra && pi == 0 && u == 0 && j0 == beg + threadIdx.x;
+                        tt[u] = j < end ? (pre ? ra_t : A.tails[j]) : -1;
+                        wt[u] = j < end ? (pre ? ra_w : A.w[j]) : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) cl[u] = tt[u] >= 0 ? A.C[tt[u]] : 0;
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const i64 cc = cl[u] - base;
+                        const bool valid = tt[u] >= 0;
+                        const double ww = wt[u];
+                        if (valid && pi == 0 && tt[u] == v) atomicAdd(&s_sl, ww);
+                        if (!dense) {
+                            if (!valid) continue;
+                            if (big) {
+                                atomicOr(&hbits[cc >> 5], 1u << (cc & 31));
+                                atomicAdd(&dacc[cc], ww);
+                            } else {
+                                hash_insert(hkeys, hvals, mask, shift, cc, ww);
+                            }
+                        } else {
+                            // the first lane to touch a community of the slab lists
+                            // it (one LDS counter update per wavefront)
+                            const bool in = valid && (!slab || (cc >= 0 && cc < kDenseMaxNV));
+                            bool fresh = false;
+                            if (in) {
+                                const uint32_t bit = 1u << (cc & 31);
+                                fresh = !(atomicOr(&touched[cc >> 5], bit) & bit);
+                            }
+                            const u64 fm = __ballot(fresh);
+                            if (fm) {
+                                const int lane = threadIdx.x % kWave;
+                                const int leader = __ffsll((long long)fm) - 1;
+                                int at = 0;
+                                if (lane == leader) at = atomicAdd(&s_cnt, __popcll(fm));
+                                at = __shfl(at, leader);
+                                const u64 lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+                                if (fresh) list[at + __popcll(fm & lt)] = (uint16_t)cc;
+                            }
+                            if (in) atomicAdd(&acc[cc], ww);
                         }
                     }
                 }
-            } else if (HASH) {
-                e_s = hash_lookup(hkeys, hvals, mask, shift, s);
-                score_slots(hkeys, hvals, slots, threadIdx.x, kPBlock, A, s, kv, e_s - sl, ax, A.constant, cand,
-                            gain, id, t1, t2);
-            } else {
-                ntouched = s_cnt;
-                e_s = acc[s];
-                const double eix = e_s - sl;
-                const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
-                for (int k0 = threadIdx.x; k0 < ntouched; k0 += kGatherUnroll * kPBlock) {
-                    i64 yy[kGatherUnroll], nyy[kGatherUnroll];
-                    double ay[kGatherUnroll];
-#pragma unroll
-                    for (int u = 0; u < kGatherUnroll; u++) {
-                        const int k = k0 + u * kPBlock;
-                        yy[u] = k < ntouched ? (i64)list[k] : -1;
+                if (HASH) __threadfence_block();
+                __syncthreads();
+                if (pi == 0) {
+                    if (first && nx_v >= 0) {  // read-ahead, second load
+                        nx_beg = A.off[nx_v];
+                        nx_end = A.off[nx_v + 1];
+                        nx_s = C[nx_v];
+                        nx_k = A.k[nx_v];
                     }
-#pragma unroll
-                    for (int u = 0; u < kGatherUnroll; u++) {
-                        ay[u] = yy[u] >= 0 ? A.a[yy[u]] : 0.0;
-                        nyy[u] = yy[u] >= 0 ? A.n[yy[u]] : 0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < kGatherUnroll; u++) {
-                        const i64 y = yy[u];
-                        if (y < 0 || y == s) continue;
-                        const double g = gain_of(acc[y], eix, kv, ay[u], ax, A.constant);
-                        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
-                        if (better(g > 0, g, y, cand, gain, id)) {
-                            cand = true;
-                            gain = g;
-                            id = y;
-                            nid = nyy[u];
+                    sl = s_sl;
+                }
+                if (HASH && !dense) {
+                    if (big) {
+                        e_s = ld_agent(&dacc[s]);
+                        eix = e_s - sl;
+                        for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
+                            uint32_t m = hbits[wi];
+                            while (m) {
+                                const int b = __ffs((int)m) - 1;
+                                m &= m - 1;
+                                const i64 y = (i64)wi * 32 + b;
+                                if (y == s) continue;
+                                const double g = gain_of(ld_agent(&dacc[y]), eix, kv, A.a[y], ax, A.constant);
+                                if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+                                if (better(g > 0, g, y, cand, gain, id)) {
+                                    cand = true;
+                                    gain = g;
+                                    id = y;
+                                }
+                            }
                         }
+                    } else {
+                        e_s = hash_lookup(hkeys, hvals, mask, shift, s);
+                        score_slots(hkeys, hvals, slots, threadIdx.x, kPBlock, A, s, kv, e_s - sl, ax,
+                                    A.constant, cand, gain, id, t1, t2);
+                    }
+                } else {
+                    ntouched = s_cnt;
+                    if (pi == 0) {
+                        e_s = acc[s - base];
+                        eix = e_s - sl;
+                    }
+                    for (int k0 = threadIdx.x; k0 < ntouched; k0 += kGatherUnroll * kPBlock) {
+                        i64 yy[kGatherUnroll], nyy[kGatherUnroll];
+                        double ay[kGatherUnroll];
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            const int k = k0 + u * kPBlock;
+                            yy[u] = k < ntouched ? (i64)list[k] : -1;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            ay[u] = yy[u] >= 0 ? A.a[base + yy[u]] : 0.0;
+                            nyy[u] = yy[u] >= 0 ? A.n[base + yy[u]] : 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            if (yy[u] < 0) continue;
+                            const i64 y = base + yy[u];
+                            if (y == s) continue;
+                            const double g = gain_of(acc[yy[u]], eix, kv, ay[u], ax, A.constant);
+                            if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+                            if (better(g > 0, g, y, cand, gain, id)) {
+                                cand = true;
+                                gain = g;
+                                id = y;
+                                nid = nyy[u];
+                            }
+                        }
+                    }
+                    if (pi + 1 < nsl) {  // the next slab starts from a clear view
+                        __syncthreads();
+                        for (int k = threadIdx.x; k < ntouched; k += kPBlock) {
+                            acc[list[k]] = 0.0;
+                            touched[list[k] >> 5] = 0;
+                        }
+                        if (threadIdx.x == 0) s_cnt = 0;
+                        __syncthreads();
                     }
                 }
             }
@@ -1208,7 +1256,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                 }
             }
             __syncthreads();  // every read of the accumulator is done
-            if (HASH && big) {
+            if (HASH && big && !slab) {
                 for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) {
                     uint32_t m = hbits[wi];
                     while (m) {
@@ -1219,6 +1267,12 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                     hbits[wi] = 0;
                 }
                 __threadfence_block();
+            } else if (slab) {  // back to the empty hash view
+                for (int h = threadIdx.x; h < kPHashSlots; h += kPBlock) {
+                    hkeys[h] = kEmpty;
+                    hvals[h] = 0.0;
+                }
+                for (int wi = threadIdx.x; wi < nwords; wi += kPBlock) hbits[wi] = 0;
             } else if (HASH) {
                 for (i64 h = threadIdx.x; h < slots; h += kPBlock) {
                     hkeys[h] = kEmpty;

@@ -72,7 +72,8 @@ def cliques(seed=13, nv=20000, size=200, inter=8, hub_degrees=(5000, 4500), weig
     """Dense near-cliques (every coloring needs ~size colors, so classes hold
     ~nv/size vertices) joined by random inter-clique edges, plus hubs: a graph
     above the dense-accumulator limit that the persistent sweep serves with its
-    LDS hash (and, past kPHashMaxDeg, the HBM table)."""
+    LDS hash (and, past kPHashMaxDeg, the dense view in slabs or, above two
+    slabs of vertices, the HBM table)."""
     rng = np.random.default_rng(seed)
     blocks = nv // size
     pairs = []

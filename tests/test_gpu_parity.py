@@ -176,13 +176,16 @@ def test_plain_phase(name, nomap):
         assert abs(q - qo) < 1e-3
 
 
+@pytest.mark.parametrize("nv,size", [(20000, 200), (40000, 300)])
 @pytest.mark.parametrize("weighted", [False, True])
-def test_colored_phase_persistent_hash(weighted):
-    """A graph above the dense-accumulator limit with small color classes: the
-    persistent sweep with per-workgroup hash tables (LDS, and HBM for the two
-    hubs) against the oracle's frozen color steps."""
+def test_colored_phase_persistent_hash(weighted, nv, size):
+    """Graphs above the dense-accumulator limit with small color classes: the
+    persistent sweep with per-workgroup LDS hash tables, and for the two hubs
+    the dense LDS view one slab of communities at a time (20,000 vertices: two
+    slabs) or the HBM table (40,000 vertices), against the oracle's frozen
+    color steps."""
     from graphs import cliques
-    g = cliques(weighted=weighted)
+    g = cliques(weighted=weighted, nv=nv, size=size)
     colors, mc = oracle.color_greedy(g)
     assert g.numVertices > 15360 and g.numVertices / (mc + 1) <= 256
     assert g.degrees().max() > 4096
