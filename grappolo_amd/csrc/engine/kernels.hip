@@ -1434,14 +1434,44 @@ __global__ __launch_bounds__(kBlock) void k_mod_pass(const i64 *off, const i64 *
                                                   const i64 *C, const i64 *verts, i64 count,
                                                   double *part) {
     constexpr int VPB = kBlock / G;
+    constexpr int U = 4;  // vertices of a group in flight together
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    const i64 stride = (i64)gridDim.x * VPB;
     double acc = 0;
-    for (i64 q = (i64)blockIdx.x * VPB + grp; q < count; q += (i64)gridDim.x * VPB) {
-        const i64 v = verts[q];
-        const i64 s = C[v];
-        if (s < 0) continue;
-        for (i64 j = off[v] + lane; j < off[v + 1]; j += G)
-            if (C[tails[j]] == s) acc += w[j];
+    for (i64 q0 = (i64)blockIdx.x * VPB + grp; q0 < count; q0 += U * stride) {
+        i64 vv[U], ss[U], bb[U], ee[U], tt[U];
+        double ww[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const i64 q = q0 + u * stride;
+            vv[u] = q < count ? verts[q] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ss[u] = vv[u] >= 0 ? C[vv[u]] : -1;
+            bb[u] = vv[u] >= 0 ? off[vv[u]] : 0;
+            ee[u] = vv[u] >= 0 ? off[vv[u] + 1] : 0;
+        }
+        // the first pass of every vertex, loads first
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const i64 j = bb[u] + lane;
+            const bool in = ss[u] >= 0 && j < ee[u];
+            tt[u] = in ? tails[j] : -1;
+            ww[u] = in ? w[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const i64 ct = tt[u] >= 0 ? C[tt[u]] : -2;
+            if (ct == ss[u]) acc += ww[u];
+        }
+        // vertices of more than G entries
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (ss[u] < 0) continue;
+            for (i64 j = bb[u] + lane + G; j < ee[u]; j += G)
+                if (C[tails[j]] == ss[u]) acc += w[j];
+        }
     }
     acc = block_sum(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -1558,6 +1588,7 @@ constexpr int kStageCap = 2048;
 // vertices are pairwise non-adjacent, so every colour read is final.
 // push(ready, u) appends a released vertex (called by the whole wave).
 constexpr int kJpGroup = 8;
+constexpr int kJpUnroll = 4;
 constexpr i64 kJpLightDeg = 64;
 template <typename Push>
 __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *cnt, const u64 *prio,
@@ -1575,23 +1606,34 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
         const i64 x = __shfl_xor(itmax, o);
         itmax = x > itmax ? x : itmax;
     }
-    for (i64 it = 0; it < itmax; it++) {
-        const i64 j = it * kJpGroup + gl;
-        bool ready = false;
-        i64 u = 0;
-        if (light && j < deg) {
-            u = tails[b + j];
-            if (u != v) {
-                if (jp_below(prio, u, pv, v)) {
-                    ready = atomicSub(&cnt[u], 1) == 1;
-                } else {
-                    const int c = colors[u];
-                    if (c >= 0 && c < 64) m0 |= 1ull << c;
-                    else if (c >= 64 && c < 128) m1 |= 1ull << (c - 64);
-                }
+    // kJpUnroll iterations at a time: entry loads, then priority loads, then
+    // the count updates and colour loads, then the appends
+    for (i64 it0 = 0; it0 < itmax; it0 += kJpUnroll) {
+        i64 uu[kJpUnroll];
+        u64 pu[kJpUnroll];
+        bool rd[kJpUnroll];
+#pragma unroll
+        for (int k = 0; k < kJpUnroll; k++) {
+            const i64 j = (it0 + k) * kJpGroup + gl;
+            uu[k] = light && j < deg ? tails[b + j] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != v ? prio[uu[k]] : 0;
+#pragma unroll
+        for (int k = 0; k < kJpUnroll; k++) {
+            rd[k] = false;
+            const i64 u = uu[k];
+            if (u < 0 || u == v) continue;
+            if (pu[k] < pv || (pu[k] == pv && u < v)) {
+                rd[k] = atomicSub(&cnt[u], 1) == 1;
+            } else {
+                const int c = colors[u];
+                if (c >= 0 && c < 64) m0 |= 1ull << c;
+                else if (c >= 64 && c < 128) m1 |= 1ull << (c - 64);
             }
         }
-        push(ready, u);
+#pragma unroll
+        for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k]);
     }
 #pragma unroll
     for (int o = 1; o < kJpGroup; o <<= 1) {
@@ -1608,22 +1650,31 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
         for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        for (i64 j0 = 0; j0 < hd; j0 += kWave) {
-            const i64 j = j0 + lane;
-            bool ready = false;
-            i64 u = 0;
-            if (j < hd) {
-                u = tails[hb + j];
-                if (u != hv) {
-                    if (jp_below(prio, u, hp, hv)) {
-                        ready = atomicSub(&cnt[u], 1) == 1;
-                    } else {
-                        const int c = colors[u];
-                        if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
-                    }
+        for (i64 j0 = 0; j0 < hd; j0 += kJpUnroll * kWave) {
+            i64 uu[kJpUnroll];
+            u64 pu[kJpUnroll];
+            bool rd[kJpUnroll];
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) {
+                const i64 j = j0 + k * kWave + lane;
+                uu[k] = j < hd ? tails[hb + j] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != hv ? prio[uu[k]] : 0;
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) {
+                rd[k] = false;
+                const i64 u = uu[k];
+                if (u < 0 || u == hv) continue;
+                if (pu[k] < hp || (pu[k] == hp && u < hv)) {
+                    rd[k] = atomicSub(&cnt[u], 1) == 1;
+                } else {
+                    const int c = colors[u];
+                    if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
                 }
             }
-            push(ready, u);
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k]);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -1753,23 +1804,31 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
         // one pass: colours of the higher-priority neighbours, release of the
         // lower ones; whole wavefronts iterate together for the appends
         const i64 deg = e - b;
-        const i64 trips = (deg + kBlock - 1) / kBlock;
-        for (i64 t = 0; t < trips; t++) {
-            const i64 j = t * kBlock + threadIdx.x;
-            bool ready = false;
-            i64 u = 0;
-            if (j < deg) {
-                u = tails[b + j];
-                if (u != v) {
-                    if (jp_below(L.prio, u, pv, v)) {
-                        ready = atomicSub(&cnt[u], 1) == 1;
-                    } else {
-                        const int c = colors[u];
-                        if (c >= 0) atomicOr(&s_mark[c >> 5], 1u << (c & 31));
-                    }
+        for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kBlock) {
+            i64 uu[kJpUnroll];
+            u64 pu[kJpUnroll];
+            bool rd[kJpUnroll];
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) {
+                const i64 j = j0 + k * kBlock + threadIdx.x;
+                uu[k] = j < deg ? tails[b + j] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != v ? L.prio[uu[k]] : 0;
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) {
+                rd[k] = false;
+                const i64 u = uu[k];
+                if (u < 0 || u == v) continue;
+                if (pu[k] < pv || (pu[k] == pv && u < v)) {
+                    rd[k] = atomicSub(&cnt[u], 1) == 1;
+                } else {
+                    const int c = colors[u];
+                    if (c >= 0) atomicOr(&s_mark[c >> 5], 1u << (c & 31));
                 }
             }
-            frontier_push(ready, u, off, L, S);
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) frontier_push(rd[k], uu[k], off, L, S);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
@@ -2025,24 +2084,38 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             __syncthreads();
             // one pass: colours of the higher-priority neighbours (all coloured),
             // release of the lower-priority ones (next round's frontier)
-            for (i64 j0 = 0; j0 < deg; j0 += kJpPBlock) {
-                const i64 j = j0 + threadIdx.x;
-                bool ready = false;
-                i64 u = 0;
-                if (j < deg) {
-                    u = tails[b + j];
-                    if (u != v) {
-                        if (jp_below(prio, u, pv, v)) {
-                            ready = atomicSub(&cnt[u], 1) == 1;
-                        } else {
-                            const int c = colors[u];
-                            if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
-                        }
+            for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kJpPBlock) {
+                i64 uu[kJpUnroll];
+                u64 pu[kJpUnroll];
+                bool rd[kJpUnroll];
+#pragma unroll
+                for (int k = 0; k < kJpUnroll; k++) {
+                    const i64 j = j0 + k * kJpPBlock + threadIdx.x;
+                    uu[k] = j < deg ? tails[b + j] : -1;
+                }
+#pragma unroll
+                for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != v ? prio[uu[k]] : 0;
+#pragma unroll
+                for (int k = 0; k < kJpUnroll; k++) {
+                    rd[k] = false;
+                    const i64 u = uu[k];
+                    if (u < 0 || u == v) continue;
+                    if (pu[k] < pv || (pu[k] == pv && u < v)) {
+                        rd[k] = atomicSub(&cnt[u], 1) == 1;
+                    } else {
+                        const int c = colors[u];
+                        if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
                     }
                 }
-                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
-                stage_append(ready && !big, u, S, ns_, cs);
-                wave_append(big, u, nl_, cl);
+                bool bg[kJpUnroll];
+#pragma unroll
+                for (int k = 0; k < kJpUnroll; k++)
+                    bg[k] = rd[k] && off[uu[k] + 1] - off[uu[k]] > kBinMaxDeg[6];
+#pragma unroll
+                for (int k = 0; k < kJpUnroll; k++) {
+                    stage_append(rd[k] && !bg[k], uu[k], S, ns_, cs);
+                    wave_append(bg[k], uu[k], nl_, cl);
+                }
             }
             __syncthreads();
             if (threadIdx.x < kMaxColors / 32) {
