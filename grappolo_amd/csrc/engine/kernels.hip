@@ -34,14 +34,16 @@ __device__ inline u64 jp_priority(i64 v) {
 }
 
 // Jones-Plassmann order: prio[] (larger degree first, then a hash of the id),
-// equal keys broken by the larger id, so the order is total
-__device__ inline bool jp_above(const u64 *prio, i64 u, u64 pv, i64 v) {
-    const u64 pu = prio[u];
-    return pu > pv || (pu == pv && u > v);
-}
-__device__ inline bool jp_below(const u64 *prio, i64 u, u64 pv, i64 v) {
-    const u64 pu = prio[u];
-    return pu < pv || (pu == pv && u < v);
+// equal keys broken by the larger id, so the order is total.  The count's
+// byte for entry (v, u) compares u's priority pu with v's pv; the degree
+// class of u comes from the degree field of its priority.
+__device__ inline uint8_t jp_code(u64 pu, i64 u, u64 pv, i64 v) {
+    if (u == v) return kJpSelf;
+    const u64 du = pu >> 40;
+    uint8_t c = (pu < pv || (pu == pv && u < v)) ? kJpRelease : 0;
+    if (du > (u64)kBinMaxDeg[6]) c |= kJpLarge;
+    if (du > (u64)kGiantDeg) c |= kJpGiant;
+    return c;
 }
 
 __device__ inline i64 ld_relaxed(const i64 *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
@@ -1566,12 +1568,12 @@ __device__ inline void frontier_push(bool pred, i64 u, const i64 *off, const JpL
     wave_append(big && !giant, u, L.large, &L.counts[1]);
     wave_append(giant, u, L.giant, &L.counts[2]);
 }
-// The same with the small list staged in S (the other two are rare).
+// The same with the small list staged in S (the other two are rare) and the
+// degree class of u from its entry's code.
 template <int CAP>
-__device__ inline void frontier_push(bool pred, i64 u, const i64 *off, const JpLists &L, Stage<CAP> &S) {
-    const i64 d = pred ? off[u + 1] - off[u] : 0;
-    const bool big = pred && d > kBinMaxDeg[6];
-    const bool giant = big && d > kGiantDeg;
+__device__ inline void frontier_push(bool pred, i64 u, uint8_t code, const JpLists &L, Stage<CAP> &S) {
+    const bool big = pred && (code & kJpLarge);
+    const bool giant = pred && (code & kJpGiant);
     stage_append(pred && !big, u, S, L.small, &L.counts[0]);
     wave_append(big && !giant, u, L.large, &L.counts[1]);
     wave_append(giant, u, L.giant, &L.counts[2]);
@@ -1591,14 +1593,14 @@ constexpr int kJpGroup = 8;
 constexpr int kJpUnroll = 4;
 constexpr i64 kJpLightDeg = 64;
 template <typename Push>
-__device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *cnt, const u64 *prio,
-                                const i64 *front, i64 lo, i64 n, uint32_t *mk, int32_t *err, Push push) {
+__device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *cnt,
+                                const uint8_t *code, const i64 *front, i64 lo, i64 n, uint32_t *mk, int32_t *err,
+                                Push push) {
     const int lane = threadIdx.x % kWave, g = lane / kJpGroup, gl = lane % kJpGroup;
     const bool in = g < n;
     const i64 v = in ? front[lo + g] : -1;
     const i64 b = in ? off[v] : 0, deg = in ? off[v + 1] - b : 0;
     const bool light = in && deg <= kJpLightDeg;
-    const u64 pv = in ? prio[v] : 0;
     u64 m0 = 0, m1 = 0;
     i64 itmax = light ? (deg + kJpGroup - 1) / kJpGroup : 0;
 #pragma unroll
@@ -1606,25 +1608,25 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
         const i64 x = __shfl_xor(itmax, o);
         itmax = x > itmax ? x : itmax;
     }
-    // kJpUnroll iterations at a time: entry loads, then priority loads, then
-    // the count updates and colour loads, then the appends
+    // kJpUnroll iterations at a time: entry and code loads, then the count
+    // updates and colour loads, then the appends
     for (i64 it0 = 0; it0 < itmax; it0 += kJpUnroll) {
         i64 uu[kJpUnroll];
-        u64 pu[kJpUnroll];
+        uint8_t cd[kJpUnroll];
         bool rd[kJpUnroll];
 #pragma unroll
         for (int k = 0; k < kJpUnroll; k++) {
             const i64 j = (it0 + k) * kJpGroup + gl;
-            uu[k] = light && j < deg ? tails[b + j] : -1;
+            const bool e = light && j < deg;
+            uu[k] = e ? tails[b + j] : -1;
+            cd[k] = e ? code[b + j] : kJpSelf;
         }
-#pragma unroll
-        for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != v ? prio[uu[k]] : 0;
 #pragma unroll
         for (int k = 0; k < kJpUnroll; k++) {
             rd[k] = false;
             const i64 u = uu[k];
-            if (u < 0 || u == v) continue;
-            if (pu[k] < pv || (pu[k] == pv && u < v)) {
+            if (cd[k] & kJpSelf) continue;
+            if (cd[k] & kJpRelease) {
                 rd[k] = atomicSub(&cnt[u], 1) == 1;
             } else {
                 const int c = colors[u];
@@ -1633,7 +1635,7 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
             }
         }
 #pragma unroll
-        for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k]);
+        for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k], cd[k]);
     }
 #pragma unroll
     for (int o = 1; o < kJpGroup; o <<= 1) {
@@ -1646,27 +1648,25 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
         const int src = __ffsll((long long)hm) - 1;
         hm &= hm - 1;
         const i64 hv = __shfl(v, src), hb = __shfl(b, src), hd = __shfl(deg, src);
-        const u64 hp = __shfl(pv, src);
         for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         for (i64 j0 = 0; j0 < hd; j0 += kJpUnroll * kWave) {
             i64 uu[kJpUnroll];
-            u64 pu[kJpUnroll];
+            uint8_t cd[kJpUnroll];
             bool rd[kJpUnroll];
 #pragma unroll
             for (int k = 0; k < kJpUnroll; k++) {
                 const i64 j = j0 + k * kWave + lane;
                 uu[k] = j < hd ? tails[hb + j] : -1;
+                cd[k] = j < hd ? code[hb + j] : kJpSelf;
             }
-#pragma unroll
-            for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != hv ? prio[uu[k]] : 0;
 #pragma unroll
             for (int k = 0; k < kJpUnroll; k++) {
                 rd[k] = false;
                 const i64 u = uu[k];
-                if (u < 0 || u == hv) continue;
-                if (pu[k] < hp || (pu[k] == hp && u < hv)) {
+                if (cd[k] & kJpSelf) continue;
+                if (cd[k] & kJpRelease) {
                     rd[k] = atomicSub(&cnt[u], 1) == 1;
                 } else {
                     const int c = colors[u];
@@ -1674,7 +1674,7 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
                 }
             }
 #pragma unroll
-            for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k]);
+            for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k], cd[k]);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -1731,13 +1731,15 @@ __global__ __launch_bounds__(kBlock) void k_jp_count(const i64 *off, const i64 *
                 const u64 pv = L.prio[v];
                 for (i64 j = b; j < e; j++) {
                     const i64 u = tails[j];
-                    if (u != v && jp_above(L.prio, u, pv, v)) c++;
+                    const uint8_t cd = jp_code(L.prio[u], u, pv, v);
+                    L.code[j] = cd;
+                    if (!(cd & (kJpRelease | kJpSelf))) c++;
                 }
                 cnt[v] = c;
             }
         }
         wave_append(heavy_v, v, heavy, heavy_count);
-        frontier_push(v < nv && !heavy_v && c == 0, v, off, L, S);
+        stage_append(v < nv && !heavy_v && c == 0, v, S, L.small, &L.counts[0]);  // at most 256 entries
     }
     stage_flush(S, L.small, &L.counts[0]);
 }
@@ -1755,7 +1757,9 @@ __global__ __launch_bounds__(kBlock) void k_jp_count_heavy(const i64 *off, const
         int c = 0;
         for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
             const i64 u = tails[j];
-            if (u != v && jp_above(L.prio, u, pv, v)) c++;
+            const uint8_t cd = jp_code(L.prio[u], u, pv, v);
+            L.code[j] = cd;
+            if (!(cd & (kJpRelease | kJpSelf))) c++;
         }
         for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
         if (lane == 0) cnt[v] = c;
@@ -1777,10 +1781,10 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
     __syncthreads();
     const int w = threadIdx.x / kWave;
     const i64 nw = (i64)gridDim.x * WPB, wid = (i64)blockIdx.x * WPB + w;
-    auto push = [&](bool ready, i64 u) { frontier_push(ready, u, off, L, S); };
+    auto push = [&](bool ready, i64 u, uint8_t cd) { frontier_push(ready, u, cd, L, S); };
     for (i64 lo = wid * E; lo < nfront; lo += nw * E) {
         const i64 n = nfront - lo < E ? nfront - lo : E;
-        jp_wave8(off, tails, colors, cnt, L.prio, front, lo, n, mark[w], err, push);
+        jp_wave8(off, tails, colors, cnt, L.code, front, lo, n, mark[w], err, push);
     }
     stage_flush(S, L.small, &L.counts[0]);
 }
@@ -1797,7 +1801,6 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
     for (i64 q = blockIdx.x; q < nfront; q += gridDim.x) {
         const i64 v = front[q];
         const i64 b = off[v], e = off[v + 1];
-        const u64 pv = L.prio[v];
         if (threadIdx.x == 0) s_color = kMaxColors;
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) s_mark[i] = 0;
         __syncthreads();
@@ -1806,21 +1809,20 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
         const i64 deg = e - b;
         for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kBlock) {
             i64 uu[kJpUnroll];
-            u64 pu[kJpUnroll];
+            uint8_t cd[kJpUnroll];
             bool rd[kJpUnroll];
 #pragma unroll
             for (int k = 0; k < kJpUnroll; k++) {
                 const i64 j = j0 + k * kBlock + threadIdx.x;
                 uu[k] = j < deg ? tails[b + j] : -1;
+                cd[k] = j < deg ? L.code[b + j] : kJpSelf;
             }
-#pragma unroll
-            for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != v ? L.prio[uu[k]] : 0;
 #pragma unroll
             for (int k = 0; k < kJpUnroll; k++) {
                 rd[k] = false;
                 const i64 u = uu[k];
-                if (u < 0 || u == v) continue;
-                if (pu[k] < pv || (pu[k] == pv && u < v)) {
+                if (cd[k] & kJpSelf) continue;
+                if (cd[k] & kJpRelease) {
                     rd[k] = atomicSub(&cnt[u], 1) == 1;
                 } else {
                     const int c = colors[u];
@@ -1828,7 +1830,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                 }
             }
 #pragma unroll
-            for (int k = 0; k < kJpUnroll; k++) frontier_push(rd[k], uu[k], off, L, S);
+            for (int k = 0; k < kJpUnroll; k++) frontier_push(rd[k], uu[k], cd[k], L, S);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
@@ -1893,18 +1895,19 @@ __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, con
             colors[v] = color;
         }
     }
-    const u64 pv = L.prio[v];
     const i64 b = off[v], deg = off[v + 1] - b;
     const i64 step = (i64)gridDim.x * kBlock;
     for (i64 j0 = (i64)blockIdx.x * kBlock; j0 < deg; j0 += step) {
         const i64 j = j0 + threadIdx.x;
         bool ready = false;
         i64 u = 0;
+        uint8_t cd = kJpSelf;
         if (j < deg) {
             u = tails[b + j];
-            if (u != v && jp_below(L.prio, u, pv, v)) ready = atomicSub(&cnt[u], 1) == 1;
+            cd = L.code[b + j];
+            if (cd & kJpRelease) ready = atomicSub(&cnt[u], 1) == 1;
         }
-        frontier_push(ready, u, off, L, S);
+        frontier_push(ready, u, cd, L, S);
     }
     stage_flush(S, L.small, &L.counts[0]);
 }
@@ -1978,7 +1981,8 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                                                              int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB,
                                                              i64 *lA, i64 *lB, i64 *ring, unsigned *bar,
                                                              int32_t *err, i64 tail_at, int do_count,
-                                                             i64 colored0, const u64 *prio, i64 max_front) {
+                                                             i64 colored0, const u64 *prio, uint8_t *code,
+                                                             i64 max_front) {
     constexpr int WPB = kJpPBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
     __shared__ i64 tail_list[kJpPBlock];
@@ -2013,7 +2017,9 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                     const u64 pv = prio[v];
                     for (i64 j = b; j < e; j++) {
                         const i64 u = tails[j];
-                        if (u != v && jp_above(prio, u, pv, v)) c++;
+                        const uint8_t cd = jp_code(prio[u], u, pv, v);
+                        code[j] = cd;
+                        if (!(cd & (kJpRelease | kJpSelf))) c++;
                     }
                     cnt[v] = c;
                 }
@@ -2031,7 +2037,9 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 int c = 0;
                 for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
                     const i64 u = tails[j];
-                    if (u != v && jp_above(prio, u, pv, v)) c++;
+                    const uint8_t cd = jp_code(prio[u], u, pv, v);
+                    code[j] = cd;
+                    if (!(cd & (kJpRelease | kJpSelf))) c++;
                 }
                 for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
                 if (lane == 0) cnt[v] = c;
@@ -2062,14 +2070,14 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
         {
             constexpr i64 E = kWave / kJpGroup;
             const i64 nw = (i64)nb * WPB, wid = (i64)blockIdx.x * WPB + w;
-            auto push = [&](bool ready, i64 u) {
-                const bool big = ready && off[u + 1] - off[u] > kBinMaxDeg[6];
+            auto push = [&](bool ready, i64 u, uint8_t cd) {
+                const bool big = ready && (cd & kJpLarge);
                 stage_append(ready && !big, u, S, ns_, cs);
                 wave_append(big, u, nl_, cl);
             };
             for (i64 lo = wid * E; lo < nsmall; lo += nw * E) {
                 const i64 n = nsmall - lo < E ? nsmall - lo : E;
-                jp_wave8(off, tails, colors, cnt, prio, fs, lo, n, mk, err, push);
+                jp_wave8(off, tails, colors, cnt, code, fs, lo, n, mk, err, push);
             }
         }
         __syncthreads();  // mark[0] is the large loop's and wave 0's
@@ -2077,7 +2085,6 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
         for (i64 qq = blockIdx.x; qq < nlarge; qq += nb) {
             const i64 v = fl[qq];
             const i64 b = off[v], e = off[v + 1];
-            const u64 pv = prio[v];
             const i64 deg = e - b;
             for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
             if (threadIdx.x == 0) s_color = kMaxColors;
@@ -2086,35 +2093,31 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             // release of the lower-priority ones (next round's frontier)
             for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kJpPBlock) {
                 i64 uu[kJpUnroll];
-                u64 pu[kJpUnroll];
+                uint8_t cd[kJpUnroll];
                 bool rd[kJpUnroll];
 #pragma unroll
                 for (int k = 0; k < kJpUnroll; k++) {
                     const i64 j = j0 + k * kJpPBlock + threadIdx.x;
                     uu[k] = j < deg ? tails[b + j] : -1;
+                    cd[k] = j < deg ? code[b + j] : kJpSelf;
                 }
-#pragma unroll
-                for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 && uu[k] != v ? prio[uu[k]] : 0;
 #pragma unroll
                 for (int k = 0; k < kJpUnroll; k++) {
                     rd[k] = false;
                     const i64 u = uu[k];
-                    if (u < 0 || u == v) continue;
-                    if (pu[k] < pv || (pu[k] == pv && u < v)) {
+                    if (cd[k] & kJpSelf) continue;
+                    if (cd[k] & kJpRelease) {
                         rd[k] = atomicSub(&cnt[u], 1) == 1;
                     } else {
                         const int c = colors[u];
                         if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
                     }
                 }
-                bool bg[kJpUnroll];
-#pragma unroll
-                for (int k = 0; k < kJpUnroll; k++)
-                    bg[k] = rd[k] && off[uu[k] + 1] - off[uu[k]] > kBinMaxDeg[6];
 #pragma unroll
                 for (int k = 0; k < kJpUnroll; k++) {
-                    stage_append(rd[k] && !bg[k], uu[k], S, ns_, cs);
-                    wave_append(bg[k], uu[k], nl_, cl);
+                    const bool bg = rd[k] && (cd[k] & kJpLarge);
+                    stage_append(rd[k] && !bg, uu[k], S, ns_, cs);
+                    wave_append(bg, uu[k], nl_, cl);
                 }
             }
             __syncthreads();
@@ -2729,13 +2732,16 @@ int jp_persistent_blocks() {
 }
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, const u64 *prio, i64 max_front, int nblocks, hipStream_t s) {
+                          i64 colored0, const u64 *prio, uint8_t *code, i64 max_front, int nblocks,
+                          hipStream_t s) {
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0) return false;
     const i64 *off = g.off, *tails = g.tails;
     i64 nv = g.nv, ta = tail_at, c0 = colored0, mf = max_front;
     int dc = count ? 1 : 0;
     const u64 *pr = prio;
-    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0, &pr, &mf};
+    uint8_t *cd = code;
+    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0, &pr,
+                    &cd, &mf};
     hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_persistent, dim3(nblocks), dim3(kJpPBlock),
                                               args, 0, s);
     if (e != hipSuccess) {

@@ -135,7 +135,13 @@ struct JpLists {
     i64 *small, *large, *giant;
     i64 *counts;
     const u64 *prio = nullptr;  // priority of every vertex (launch_jp_prio)
+    // one byte per adjacency entry j (row v, neighbour u = tails[j]), written
+    // by the count: kJpRelease when u has the lower priority (v releases u),
+    // kJpLarge / kJpGiant when u has more than 256 / kGiantDeg entries,
+    // kJpSelf for u == v; no release bit: u's colour counts for v
+    uint8_t *code = nullptr;
 };
+constexpr uint8_t kJpRelease = 1, kJpLarge = 2, kJpGiant = 4, kJpSelf = 8;
 // Jones-Plassmann priorities: (min(degree, 2^23 - 1) << 40) | (splitmix64(v) >> 24),
 // equal keys broken by the larger id
 void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s);
@@ -162,7 +168,8 @@ constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many unco
 int jp_persistent_blocks();
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, const u64 *prio, i64 max_front, int nblocks, hipStream_t s);
+                          i64 colored0, const u64 *prio, uint8_t *code, i64 max_front, int nblocks,
+                          hipStream_t s);
 // frontiers below this many vertices finish in one cooperative launch
 constexpr i64 kJpPersistFrontier = 16384;
 // a cooperative launch hands back to full-GPU rounds once a frontier exceeds this

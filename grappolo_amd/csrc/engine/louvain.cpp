@@ -150,6 +150,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     DBuf<unsigned> bar(kBarrierWords, s_);
     DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
     DBuf<u64> prio(nv, s_);
+    DBuf<uint8_t> code(std::max<i64>(g.nnz, 1), s_);
     launch_jp_prio(g, prio.p, s_);
     launch_fill_i32(err.p, 1, 0, s_);
     launch_fill_i32(maxc.p, 1, -1, s_);
@@ -157,7 +158,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     i64 colored = 0;
     i64 hc[3] = {0, 0, 0};  // sizes of the current frontier's small, large and giant lists
     DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
-    JpLists cur{sA.p, lA.p, gA.p, counts.p, prio.p}, next{sB.p, lB.p, gB.p, counts.p + 4, prio.p};
+    JpLists cur{sA.p, lA.p, gA.p, counts.p, prio.p, code.p}, next{sB.p, lB.p, gB.p, counts.p + 4, prio.p, code.p};
     // Rounds in one cooperative launch while the frontier does not fill the
     // GPU.  count: start from scratch; else continue from the frontier in
     // cur (hc, no giant list).  Returns 0 when the launch was refused, 1 when
@@ -170,7 +171,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         Profiler::get().begin(K_COLOR, s_);
         const bool ok = pnb > 0 && launch_jp_persistent(g, colors, cnt.p, cur.small, next.small, cur.large,
                                                         next.large, ring.p, bar.p, err.p, kJpTail, count,
-                                                        colored, prio.p, kJpPersistExit, pnb, s_);
+                                                        colored, prio.p, code.p, kJpPersistExit, pnb, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
         unsigned abort_flag = 0;
         GLV_HIP(hipMemcpyAsync(h, ring.p, sizeof(h), hipMemcpyDeviceToHost, s_));
