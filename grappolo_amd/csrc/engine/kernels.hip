@@ -42,6 +42,7 @@ __device__ inline uint8_t jp_code(u64 pu, i64 u, u64 pv, i64 v) {
     const u64 du = pu >> 40;
     uint8_t c = (pu < pv || (pu == pv && u < v)) ? kJpRelease : 0;
     if (du > (u64)kBinMaxDeg[6]) c |= kJpLarge;
+    if (du > (u64)kJpMidDeg) c |= kJpHuge;
     if (du > (u64)kGiantDeg) c |= kJpGiant;
     return c;
 }
@@ -2204,6 +2205,325 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
     }
 }
 
+// The colour of vertex v from its forbidden-colour bitmap, by the G lanes of
+// a group (lane gl reads words gl, gl + G, ... below `words`): the first
+// clear bit, kMaxColors when there is none.
+template <int G>
+__device__ inline int jp_first_free(const uint32_t *fb, int gl, int words) {
+    int cand = kMaxColors;
+    for (int i = gl; i < words; i += G) {
+        const uint32_t m = fb[i];
+        if (m != 0xFFFFFFFFu) {
+            const int c = i * 32 + __ffs((int)~m) - 1;
+            cand = c < cand ? c : cand;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const int x = __shfl_xor(cand, o);
+        cand = x < cand ? x : cand;
+    }
+    return cand;
+}
+
+// Release of entry (cd, u) by a vertex coloured c: bit c into u's bitmap and
+// one off u's count; true when u becomes ready.
+__device__ inline bool jp_push_release(uint8_t cd, i64 u, int c, int32_t *cnt, uint32_t *forbid) {
+    if (!(cd & kJpRelease)) return false;
+    atomicOr(&forbid[u * (kMaxColors / 32) + (c >> 5)], 1u << (c & 31));
+    return atomicSub(&cnt[u], 1) == 1;
+}
+
+// One wavefront colours up to eight frontier vertices of at most kJpMidDeg
+// entries, front[lo .. lo+n): those of at most kJpLightDeg entries take a
+// group of kJpGroup lanes each (a first-fit colour of at most 64, found in the
+// first eight bitmap words), heavier ones the whole wave one after another
+// (at most 1024 entries: a colour below 2048, in the first 64 words).
+template <typename Push>
+__device__ inline void jp_push_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *cnt,
+                                     const uint8_t *code, uint32_t *forbid, const i64 *front, i64 lo, i64 n,
+                                     Push push) {
+    constexpr int FW = kMaxColors / 32;
+    const int lane = threadIdx.x % kWave, g = lane / kJpGroup, gl = lane % kJpGroup;
+    const bool in = g < n;
+    const i64 v = in ? front[lo + g] : -1;
+    const i64 b = in ? off[v] : 0, deg = in ? off[v + 1] - b : 0;
+    const bool light = in && deg <= kJpLightDeg;
+    const int c = jp_first_free<kJpGroup>(forbid + (light ? v : 0) * FW, gl, light ? kJpGroup : 0);
+    if (light && gl == 0) colors[v] = c;
+    i64 itmax = light ? (deg + kJpGroup - 1) / kJpGroup : 0;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const i64 x = __shfl_xor(itmax, o);
+        itmax = x > itmax ? x : itmax;
+    }
+    for (i64 it0 = 0; it0 < itmax; it0 += kJpUnroll) {
+        i64 uu[kJpUnroll];
+        uint8_t cd[kJpUnroll];
+        bool rd[kJpUnroll];
+#pragma unroll
+        for (int k = 0; k < kJpUnroll; k++) {
+            const i64 j = (it0 + k) * kJpGroup + gl;
+            const bool e = light && j < deg;
+            uu[k] = e ? tails[b + j] : -1;
+            cd[k] = e ? code[b + j] : kJpSelf;
+        }
+#pragma unroll
+        for (int k = 0; k < kJpUnroll; k++) rd[k] = jp_push_release(cd[k], uu[k], c, cnt, forbid);
+#pragma unroll
+        for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k], cd[k]);
+    }
+    u64 hm = __ballot(in && !light && gl == 0);
+    while (hm) {
+        const int src = __ffsll((long long)hm) - 1;
+        hm &= hm - 1;
+        const i64 hv = __shfl(v, src), hb = __shfl(b, src), hd = __shfl(deg, src);
+        const int hc = jp_first_free<kWave>(forbid + hv * FW, lane, kWave);
+        if (lane == 0) colors[hv] = hc;
+        for (i64 j0 = 0; j0 < hd; j0 += kJpUnroll * kWave) {
+            i64 uu[kJpUnroll];
+            uint8_t cd[kJpUnroll];
+            bool rd[kJpUnroll];
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) {
+                const i64 j = j0 + k * kWave + lane;
+                uu[k] = j < hd ? tails[hb + j] : -1;
+                cd[k] = j < hd ? code[hb + j] : kJpSelf;
+            }
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) rd[k] = jp_push_release(cd[k], uu[k], hc, cnt, forbid);
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k], cd[k]);
+        }
+    }
+}
+
+// Jones-Plassmann rounds with pushed colours (graphs of at most kJpPushMaxNV
+// vertices).  The count is the counted rounds' (entry codes, counts, first
+// frontier); frontier vertices of at most kJpMidDeg entries are coloured by
+// jp_push_wave8, heavier ones are cut into chunks of kJpPBlock entries that
+// are dealt round-robin to all workgroups (each chunk reads the vertex's
+// bitmap for its colour; chunk 0 writes it).  The tail (at most tail_at
+// uncoloured vertices) is the counted rounds' sequential first-fit.  ring as
+// k_jp_persistent's ([6] rounds, [7] heavy vertices while counting, [8]
+// coloured vertices at the end).
+constexpr int kJpChunkBatch = kJpPBlock;  // large frontier vertices per prefix batch
+__global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64 *tails, i64 nv, int32_t *colors,
+                                                       int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
+                                                       i64 *ring, unsigned *bar, int32_t *err, i64 tail_at,
+                                                       const u64 *prio, uint8_t *code, uint32_t *forbid) {
+    constexpr int WPB = kJpPBlock / kWave;
+    constexpr int FW = kMaxColors / 32;
+    __shared__ uint32_t mark[kMaxColors / 32];
+    __shared__ i64 tail_list[kJpPBlock];
+    __shared__ i64 bv[kJpChunkBatch];          // large frontier vertices of the batch
+    __shared__ i64 bstart[kJpChunkBatch + 1];  // their first chunk (prefix of chunk counts)
+    __shared__ int wsum[kJpPBlock / kWave];
+    __shared__ int s_color;
+    __shared__ Stage<kStageCap> S;  // appends to the next small frontier
+    stage_init(S);
+    __syncthreads();
+    const unsigned nb = gridDim.x;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    unsigned seq = 0;
+    bool alive = true;
+    // counts: one thread per vertex up to 256 entries; heavier ones are listed
+    // in lB (ring[7]) and counted by a wavefront each
+    for (i64 base = (i64)blockIdx.x * kJpPBlock; base < nv; base += (i64)nb * kJpPBlock) {
+        if (base != (i64)blockIdx.x * kJpPBlock) {  // room for this trip's appends
+            __syncthreads();
+            const int staged = S.n;
+            __syncthreads();
+            if (staged > kStageCap - kJpPBlock) stage_flush(S, sA, &ring[0]);
+        }
+        const i64 v = base + threadIdx.x;
+        int c = 0;
+        bool heavy = false;
+        if (v < nv) {
+            const i64 b = off[v], e = off[v + 1];
+            if (e - b > kBinMaxDeg[6]) {
+                heavy = true;
+            } else {
+                const u64 pv = prio[v];
+                for (i64 j = b; j < e; j++) {
+                    const i64 u = tails[j];
+                    const uint8_t cd = jp_code(prio[u], u, pv, v);
+                    code[j] = cd;
+                    if (!(cd & (kJpRelease | kJpSelf))) c++;
+                }
+                cnt[v] = c;
+            }
+        }
+        wave_append(heavy, v, lB, &ring[7]);
+        stage_append(v < nv && !heavy && c == 0, v, S, sA, &ring[0]);
+    }
+    stage_flush(S, sA, &ring[0]);
+    alive = grid_barrier(bar, nb, ++seq);
+    if (alive) {
+        const i64 nheavy = __atomic_load_n(&ring[7], __ATOMIC_RELAXED);
+        for (i64 q = (i64)blockIdx.x * WPB + w; q < nheavy; q += (i64)nb * WPB) {
+            const i64 v = lB[q];
+            const u64 pv = prio[v];
+            const i64 deg = off[v + 1] - off[v];
+            int c = 0;
+            for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
+                const i64 u = tails[j];
+                const uint8_t cd = jp_code(prio[u], u, pv, v);
+                code[j] = cd;
+                if (!(cd & (kJpRelease | kJpSelf))) c++;
+            }
+            for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
+            if (lane == 0) cnt[v] = c;
+            wave_append(lane == 0 && c == 0 && deg <= kJpMidDeg, v, sA, &ring[0]);
+            wave_append(lane == 0 && c == 0 && deg > kJpMidDeg, v, lA, &ring[1]);
+        }
+        alive = grid_barrier(bar, nb, ++seq);
+    }
+    i64 colored = 0;
+    int r = 0;
+    i64 *fs = sA, *fl = lA, *ns_ = sB, *nl_ = lB;
+    while (alive) {
+        const int q = r % 3, nq = (r + 1) % 3, zq = (r + 2) % 3;
+        const i64 nsmall = __hip_atomic_load(&ring[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const i64 nlarge = __hip_atomic_load(&ring[2 * q + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (nsmall + nlarge == 0) break;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            __atomic_store_n(&ring[2 * zq], (i64)0, __ATOMIC_RELAXED);
+            __atomic_store_n(&ring[2 * zq + 1], (i64)0, __ATOMIC_RELAXED);
+        }
+        i64 *cs = &ring[2 * nq], *cl = &ring[2 * nq + 1];
+        auto push = [&](bool ready, i64 u, uint8_t cd) {
+            const bool big = ready && (cd & kJpHuge);
+            stage_append(ready && !big, u, S, ns_, cs);
+            wave_append(big, u, nl_, cl);
+        };
+        // frontier vertices of at most kJpMidDeg entries: eight per wavefront
+        {
+            constexpr i64 E = kWave / kJpGroup;
+            const i64 nw = (i64)nb * WPB, wid = (i64)blockIdx.x * WPB + w;
+            for (i64 lo = wid * E; lo < nsmall; lo += nw * E) {
+                const i64 n = nsmall - lo < E ? nsmall - lo : E;
+                jp_push_wave8(off, tails, colors, cnt, code, forbid, fs, lo, n, push);
+            }
+        }
+        // heavier ones: chunks of kJpPBlock entries, chunk t of the batch to
+        // workgroup t mod nb (every workgroup walks the batches together)
+        i64 tdone = 0;  // chunks of the earlier batches
+        for (i64 b0 = 0; b0 < nlarge; b0 += kJpChunkBatch) {
+            const int nbv = (int)(nlarge - b0 < kJpChunkBatch ? nlarge - b0 : kJpChunkBatch);
+            int nch = 0;
+            if (threadIdx.x < nbv) {
+                const i64 v = fl[b0 + threadIdx.x];
+                bv[threadIdx.x] = v;
+                nch = (int)((off[v + 1] - off[v] + kJpPBlock - 1) / kJpPBlock);
+            }
+            int total = 0;
+            const int before = block_exscan<kJpPBlock>(nch, wsum, &total);
+            if (threadIdx.x < nbv) bstart[threadIdx.x] = before;
+            if (threadIdx.x == 0) bstart[nbv] = total;
+            __syncthreads();
+            i64 t0 = ((i64)blockIdx.x - tdone) % (i64)nb;
+            if (t0 < 0) t0 += nb;
+            for (i64 t = t0; t < total; t += nb) {
+                // the batch vertex holding chunk t: last bstart[i] <= t
+                int lo_i = 0, hi_i = nbv - 1;
+                while (lo_i < hi_i) {
+                    const int mid = (lo_i + hi_i + 1) / 2;
+                    if (bstart[mid] <= t) lo_i = mid;
+                    else hi_i = mid - 1;
+                }
+                const i64 v = bv[lo_i];
+                const i64 k = t - bstart[lo_i];
+                const i64 b = off[v], deg = off[v + 1] - b;
+                const i64 j = k * kJpPBlock + threadIdx.x;
+                const i64 u = j < deg ? tails[b + j] : -1;
+                const uint8_t cd = j < deg ? code[b + j] : kJpSelf;
+                if (threadIdx.x == 0) s_color = kMaxColors;
+                __syncthreads();
+                if (threadIdx.x < FW) {
+                    const uint32_t m = forbid[v * FW + threadIdx.x];
+                    if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
+                }
+                __syncthreads();
+                int c = s_color;
+                if (c >= kMaxColors) {
+                    if (k == 0 && threadIdx.x == 0) atomicOr(err, 1);
+                    c = 0;
+                }
+                if (k == 0 && threadIdx.x == 0) colors[v] = c;
+                const bool rd = jp_push_release(cd, u, c, cnt, forbid);
+                push(rd, u, cd);
+                __syncthreads();  // s_color is read by every thread before the next chunk sets it
+            }
+            tdone += total;
+            __syncthreads();  // bv / bstart of this batch are read
+        }
+        stage_flush(S, ns_, cs);
+        alive = grid_barrier(bar, nb, ++seq);
+        colored += nsmall + nlarge;
+        r++;
+        const i64 rest = nv - colored;
+        if (rest == 0) break;
+        if (rest <= tail_at) {
+            if (blockIdx.x != 0) break;
+            // workgroup 0: the uncoloured vertices in increasing id, then first-fit one by one
+            int n = 0;
+            for (i64 base = 0; base < nv; base += kJpPBlock) {
+                const i64 v = base + threadIdx.x;
+                const bool un = v < nv && __atomic_load_n(&colors[v], __ATOMIC_RELAXED) < 0;
+                int total = 0;
+                const int at = n + block_exscan<kJpPBlock>(un ? 1 : 0, wsum, &total);
+                if (un && at < kJpPBlock) tail_list[at] = v;
+                n += total;
+            }
+            __syncthreads();
+            if (n > kJpPBlock) {
+                if (threadIdx.x == 0) atomicOr(err, 2);
+                break;
+            }
+            for (int qq = 0; qq < n; qq++) {
+                const i64 v = tail_list[qq];
+                for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[i] = 0;
+                if (threadIdx.x == 0) s_color = kMaxColors;
+                __syncthreads();
+                for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += kJpPBlock) {
+                    const i64 u = tails[j];
+                    if (u == v) continue;
+                    const int32_t su = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
+                    if (su < 0) continue;
+                    atomicOr(&mark[su >> 5], 1u << (su & 31));
+                }
+                __syncthreads();
+                if (threadIdx.x < kMaxColors / 32) {
+                    const uint32_t m = mark[threadIdx.x];
+                    if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    int color = s_color;
+                    if (color >= kMaxColors) {
+                        atomicOr(err, 1);
+                        color = 0;
+                    }
+                    __atomic_store_n(&colors[v], (int32_t)color, __ATOMIC_RELAXED);
+                }
+                __syncthreads();
+            }
+            break;
+        }
+        i64 *tp = fs;
+        fs = ns_;
+        ns_ = tp;
+        tp = fl;
+        fl = nl_;
+        nl_ = tp;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ring[6] = r;
+        ring[8] = colored;
+    }
+}
+
 __global__ void k_jp_prio(const i64 *off, i64 nv, u64 *prio) {
     for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
         const u64 d = (u64)min(off[v + 1] - off[v], (i64(1) << 23) - 1);
@@ -2744,6 +3064,34 @@ bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 
                     &cd, &mf};
     hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_persistent, dim3(nblocks), dim3(kJpPBlock),
                                               args, 0, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+int jp_push_blocks() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0, cus = 0, per = 0;
+        GLV_HIP(hipGetDevice(&dev));
+        GLV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_jp_push, kJpPBlock, 0));
+        cached = per > 0 ? cus * per : 0;
+    }
+    return cached;
+}
+bool launch_jp_push(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
+                    i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, const u64 *prio, uint8_t *code,
+                    uint32_t *forbid, int nblocks, hipStream_t s) {
+    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0 || g.nv > kJpPushMaxNV) return false;
+    const i64 *off = g.off, *tails = g.tails;
+    i64 nv = g.nv, ta = tail_at;
+    const u64 *pr = prio;
+    uint8_t *cd = code;
+    uint32_t *fb = forbid;
+    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &pr, &cd, &fb};
+    hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_push, dim3(nblocks), dim3(kJpPBlock), args, 0, s);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return false;

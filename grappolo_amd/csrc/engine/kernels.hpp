@@ -137,11 +137,13 @@ struct JpLists {
     const u64 *prio = nullptr;  // priority of every vertex (launch_jp_prio)
     // one byte per adjacency entry j (row v, neighbour u = tails[j]), written
     // by the count: kJpRelease when u has the lower priority (v releases u),
-    // kJpLarge / kJpGiant when u has more than 256 / kGiantDeg entries,
-    // kJpSelf for u == v; no release bit: u's colour counts for v
+    // kJpLarge / kJpHuge / kJpGiant when u has more than 256 / kJpMidDeg /
+    // kGiantDeg entries, kJpSelf for u == v; no release bit: u's colour counts
+    // for v
     uint8_t *code = nullptr;
 };
-constexpr uint8_t kJpRelease = 1, kJpLarge = 2, kJpGiant = 4, kJpSelf = 8;
+constexpr uint8_t kJpRelease = 1, kJpLarge = 2, kJpGiant = 4, kJpSelf = 8, kJpHuge = 16;
+constexpr i64 kJpMidDeg = 1024;
 // Jones-Plassmann priorities: (min(degree, 2^23 - 1) << 40) | (splitmix64(v) >> 24),
 // equal keys broken by the larger id
 void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s);
@@ -170,6 +172,20 @@ bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
                           i64 colored0, const u64 *prio, uint8_t *code, i64 max_front, int nblocks,
                           hipStream_t s);
+// Jones-Plassmann rounds of a graph of at most kJpPushMaxNV vertices in one
+// cooperative launch that pushes colours: a vertex coloured c sets bit c of
+// the forbidden-colour bitmap (forbid[u * kMaxColors/32 ..], zero on entry,
+// nv * kMaxColors/32 words) of every lower-priority neighbour u as it
+// releases it, so a vertex takes its colour from its own bitmap when its
+// count reaches zero and the entries of a heavy vertex can be shared out over
+// all workgroups.  Same colouring as the counted rounds; no hand-back to the
+// host.  ring: 10 zeroed words, [6] receives the rounds; false when the
+// launch is refused.
+constexpr i64 kJpPushMaxNV = i64(1) << 17;
+int jp_push_blocks();
+bool launch_jp_push(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
+                    i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, const u64 *prio, uint8_t *code,
+                    uint32_t *forbid, int nblocks, hipStream_t s);
 // frontiers below this many vertices finish in one cooperative launch
 constexpr i64 kJpPersistFrontier = 16384;
 // a cooperative launch hands back to full-GPU rounds once a frontier exceeds this

@@ -190,7 +190,33 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         return 2;
     };
     bool done = false, have_frontier = false;
-    if (nv <= kPersistMaxNV) {
+    if (nv <= kJpPushMaxNV) {
+        // small graphs: every round in one launch with pushed colours, the
+        // entries of heavy vertices shared out over all workgroups
+        const int nb = (int)std::min<i64>({(i64)jp_push_blocks(), kPersistMaxBlocks,
+                                           std::max<i64>(8, g.nnz / 65536)});
+        DBuf<uint32_t> forbid(nv * (kMaxColors / 32), s_);
+        launch_fill_i32((int32_t *)forbid.p, nv * (kMaxColors / 32), 0, s_);
+        launch_fill_i64(ring.p, 10, 0, s_);
+        launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
+        Profiler::get().begin(K_COLOR, s_);
+        const bool ok = nb > 0 && launch_jp_push(g, colors, cnt.p, sA.p, sB.p, lA.p, lB.p, ring.p, bar.p, err.p,
+                                                 kJpTail, prio.p, code.p, forbid.p, nb, s_);
+        Profiler::get().end(K_COLOR, 0.0, s_);
+        if (ok) {
+            i64 h[10];
+            unsigned abort_flag = 0;
+            GLV_HIP(hipMemcpyAsync(h, ring.p, sizeof(h), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
+            GLV_HIP(hipStreamSynchronize(s_));
+            if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
+            round = (int)h[6];
+            done = true;
+        } else {
+            launch_fill_i32(colors, nv, -1, s_);
+        }
+    }
+    if (!done && nv <= kPersistMaxNV) {
         const int st = persistent(true);
         done = st == 1;
         have_frontier = st == 2;

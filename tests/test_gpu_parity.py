@@ -45,17 +45,21 @@ def test_coloring_valid_and_equal_to_restated_jp(name):
     np.testing.assert_array_equal(colors, ref)
 
 
-@pytest.mark.parametrize("which", ["rmat19", "planted", "cliques"])
+@pytest.mark.parametrize("which", ["rmat19", "planted", "planted_200k", "planted_100k", "cliques"])
 def test_coloring_rounds_paths_equal_restated_jp(which):
     """Above 2^18 vertices the rounds are launched from the host and handed to
-    one cooperative launch once the frontier is small; below, they run inside
-    one cooperative launch.  Both equal the restated coloring, on a skewed
-    graph (long priority chains through hubs), a planted one and dense
-    near-cliques."""
+    one cooperative launch once the frontier is small; from 2^17 to 2^18 they
+    run inside one cooperative launch; up to 2^17 inside one launch that
+    pushes colours into per-vertex bitmaps (light, mid-degree and chunked
+    heavy vertices).  All equal the restated coloring, on a skewed graph (long
+    priority chains through hubs), planted ones and dense near-cliques with
+    hubs."""
     from graphs import cliques
     from grappolo_amd.graph import gen_planted, gen_rmat
     g = {"rmat19": lambda: gen_rmat(19, seed=5),
          "planted": lambda: gen_planted(300000, 64, 12, 0.3, seed=3),
+         "planted_200k": lambda: gen_planted(200000, 64, 12, 0.3, seed=4),
+         "planted_100k": lambda: gen_planted(100000, 64, 12, 0.3, seed=5),
          "cliques": cliques}[which]()
     colors, mc = engine.algoDistanceOneVertexColoringOpt(g)
     ref, rmc, rounds = oracle.color_jp(g)
