@@ -2302,12 +2302,13 @@ __device__ inline void jp_push_wave8(const i64 *off, const i64 *tails, int32_t *
 // vertices).  The count is the counted rounds' (entry codes, counts, first
 // frontier); frontier vertices of at most kJpMidDeg entries are coloured by
 // jp_push_wave8, heavier ones are cut into chunks of kJpPBlock entries that
-// are dealt round-robin to all workgroups (each chunk reads the vertex's
-// bitmap for its colour; chunk 0 writes it).  The tail (at most tail_at
+// are dealt out to all workgroups (each chunk reads the vertex's bitmap for
+// its colour; chunk 0 writes it).  The tail (at most tail_at
 // uncoloured vertices) is the counted rounds' sequential first-fit.  ring as
 // k_jp_persistent's ([6] rounds, [7] heavy vertices while counting, [8]
 // coloured vertices at the end).
-constexpr int kJpChunkBatch = kJpPBlock;  // large frontier vertices per prefix batch
+constexpr int kJpChunkBatch = kJpPBlock;  // large frontier vertices per batch
+constexpr i64 kJpStageMin = 4096;
 __global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64 *tails, i64 nv, int32_t *colors,
                                                        int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
                                                        i64 *ring, unsigned *bar, int32_t *err, i64 tail_at,
@@ -2316,10 +2317,12 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64
     constexpr int FW = kMaxColors / 32;
     __shared__ uint32_t mark[kMaxColors / 32];
     __shared__ i64 tail_list[kJpPBlock];
-    __shared__ i64 bv[kJpChunkBatch];          // large frontier vertices of the batch
-    __shared__ i64 bstart[kJpChunkBatch + 1];  // their first chunk (prefix of chunk counts)
+    __shared__ i64 bv[kJpChunkBatch];  // large frontier vertices of the batch: ids,
+    __shared__ i64 bb[kJpChunkBatch];  // first entries
+    __shared__ i64 bd[kJpChunkBatch];  // and entry counts
     __shared__ int wsum[kJpPBlock / kWave];
     __shared__ int s_color;
+    __shared__ int s_cw[kMaxColors / 32 / kWave];
     __shared__ Stage<kStageCap> S;  // appends to the next small frontier
     stage_init(S);
     __syncthreads();
@@ -2392,9 +2395,15 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64
             __atomic_store_n(&ring[2 * zq + 1], (i64)0, __ATOMIC_RELAXED);
         }
         i64 *cs = &ring[2 * nq], *cl = &ring[2 * nq + 1];
+        // the first large frontier vertices, read along with the counts
+        const i64 ahead = (i64)threadIdx.x < nv ? fl[threadIdx.x] : 0;
+        // appends staged in LDS only when the frontier is large (the flush
+        // costs workgroup barriers; a few appends do not contend)
+        const bool stg = nsmall + nlarge > kJpStageMin;
         auto push = [&](bool ready, i64 u, uint8_t cd) {
             const bool big = ready && (cd & kJpHuge);
-            stage_append(ready && !big, u, S, ns_, cs);
+            if (stg) stage_append(ready && !big, u, S, ns_, cs);
+            else wave_append(ready && !big, u, ns_, cs);
             wave_append(big, u, nl_, cl);
         };
         // frontier vertices of at most kJpMidDeg entries: eight per wavefront
@@ -2406,59 +2415,57 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64
                 jp_push_wave8(off, tails, colors, cnt, code, forbid, fs, lo, n, push);
             }
         }
-        // heavier ones: chunks of kJpPBlock entries, chunk t of the batch to
-        // workgroup t mod nb (every workgroup walks the batches together)
-        i64 tdone = 0;  // chunks of the earlier batches
+        // heavier ones: chunks of kJpPBlock entries, chunk k of the i-th vertex
+        // of the list to workgroup (k + 13 i) mod nb (every workgroup walks
+        // the list in batches of kJpChunkBatch vertices)
         for (i64 b0 = 0; b0 < nlarge; b0 += kJpChunkBatch) {
             const int nbv = (int)(nlarge - b0 < kJpChunkBatch ? nlarge - b0 : kJpChunkBatch);
-            int nch = 0;
             if (threadIdx.x < nbv) {
-                const i64 v = fl[b0 + threadIdx.x];
+                const i64 v = b0 == 0 ? ahead : fl[b0 + threadIdx.x];
+                const i64 beg = off[v];
                 bv[threadIdx.x] = v;
-                nch = (int)((off[v + 1] - off[v] + kJpPBlock - 1) / kJpPBlock);
+                bb[threadIdx.x] = beg;
+                bd[threadIdx.x] = off[v + 1] - beg;
             }
-            int total = 0;
-            const int before = block_exscan<kJpPBlock>(nch, wsum, &total);
-            if (threadIdx.x < nbv) bstart[threadIdx.x] = before;
-            if (threadIdx.x == 0) bstart[nbv] = total;
             __syncthreads();
-            i64 t0 = ((i64)blockIdx.x - tdone) % (i64)nb;
-            if (t0 < 0) t0 += nb;
-            for (i64 t = t0; t < total; t += nb) {
-                // the batch vertex holding chunk t: last bstart[i] <= t
-                int lo_i = 0, hi_i = nbv - 1;
-                while (lo_i < hi_i) {
-                    const int mid = (lo_i + hi_i + 1) / 2;
-                    if (bstart[mid] <= t) lo_i = mid;
-                    else hi_i = mid - 1;
+            for (int i = 0; i < nbv; i++) {
+                const i64 v = bv[i], b = bb[i], deg = bd[i];
+                const i64 nch = (deg + kJpPBlock - 1) / kJpPBlock;
+                i64 k = ((i64)blockIdx.x - ((b0 + i) * 13) % (i64)nb) % (i64)nb;
+                if (k < 0) k += nb;
+                for (; k < nch; k += nb) {
+                    const i64 j = k * kJpPBlock + threadIdx.x;
+                    const i64 u = j < deg ? tails[b + j] : -1;
+                    const uint8_t cd = j < deg ? code[b + j] : kJpSelf;
+                    // the colour: the first two wavefronts read the bitmap's words
+                    int cw = kMaxColors;
+                    if (threadIdx.x < FW) {
+                        const uint32_t m = forbid[v * FW + threadIdx.x];
+                        if (m != 0xFFFFFFFFu) cw = (int)threadIdx.x * 32 + __ffs((int)~m) - 1;
+                    }
+#pragma unroll
+                    for (int o = kWave / 2; o > 0; o >>= 1) {
+                        const int x = __shfl_xor(cw, o);
+                        cw = x < cw ? x : cw;
+                    }
+                    if (lane == 0 && w < FW / kWave) s_cw[w] = cw;
+                    __syncthreads();
+                    int c = kMaxColors;
+#pragma unroll
+                    for (int q2 = 0; q2 < FW / kWave; q2++) c = s_cw[q2] < c ? s_cw[q2] : c;
+                    if (c >= kMaxColors) {
+                        if (k == 0 && threadIdx.x == 0) atomicOr(err, 1);
+                        c = 0;
+                    }
+                    if (k == 0 && threadIdx.x == 0) colors[v] = c;
+                    const bool rd = jp_push_release(cd, u, c, cnt, forbid);
+                    push(rd, u, cd);
+                    __syncthreads();  // s_cw is read by every thread before the next chunk sets it
                 }
-                const i64 v = bv[lo_i];
-                const i64 k = t - bstart[lo_i];
-                const i64 b = off[v], deg = off[v + 1] - b;
-                const i64 j = k * kJpPBlock + threadIdx.x;
-                const i64 u = j < deg ? tails[b + j] : -1;
-                const uint8_t cd = j < deg ? code[b + j] : kJpSelf;
-                if (threadIdx.x == 0) s_color = kMaxColors;
-                __syncthreads();
-                if (threadIdx.x < FW) {
-                    const uint32_t m = forbid[v * FW + threadIdx.x];
-                    if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
-                }
-                __syncthreads();
-                int c = s_color;
-                if (c >= kMaxColors) {
-                    if (k == 0 && threadIdx.x == 0) atomicOr(err, 1);
-                    c = 0;
-                }
-                if (k == 0 && threadIdx.x == 0) colors[v] = c;
-                const bool rd = jp_push_release(cd, u, c, cnt, forbid);
-                push(rd, u, cd);
-                __syncthreads();  // s_color is read by every thread before the next chunk sets it
             }
-            tdone += total;
-            __syncthreads();  // bv / bstart of this batch are read
+            __syncthreads();  // bv, bb, bd of this batch are read
         }
-        stage_flush(S, ns_, cs);
+        if (stg) stage_flush(S, ns_, cs);
         alive = grid_barrier(bar, nb, ++seq);
         colored += nsmall + nlarge;
         r++;
