@@ -1923,42 +1923,86 @@ __global__ void k_scatter_flagged(const i64 *flags, const i64 *pos, i64 nv, i64 
         if (flags[v]) list[pos[v]] = v;
 }
 
-// Sequential first-fit of list[0..n) in order, by one workgroup: the
-// reference's one-thread tentative pass over the vertices JP left.
 constexpr int kTailBlock = 1024;
-__global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const i64 *tails,
-                                                           int32_t *state, const i64 *list, i64 n,
-                                                           int32_t *err) {
-    __shared__ uint32_t mark[kMaxColors / 32];
-    __shared__ int s_color;
-    for (i64 q = 0; q < n; q++) {
-        const i64 v = list[q];
-        for (int i = threadIdx.x; i < kMaxColors / 32; i += kTailBlock) mark[i] = 0;
-        if (threadIdx.x == 0) s_color = kMaxColors;
-        __syncthreads();
-        for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += kTailBlock) {
-            const i64 u = tails[j];
-            if (u == v) continue;
-            const int32_t su = __atomic_load_n(&state[u], __ATOMIC_RELAXED);
-            if (su < 0) continue;
-            atomicOr(&mark[su >> 5], 1u << (su & 31));
+
+// The tail in three steps.  tidx[v] = position of v in the tail list.
+__global__ void k_tail_index(const i64 *list, i64 n, i64 nv, int32_t *tidx) {
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < n; q += (i64)gridDim.x * blockDim.x)
+        if (list[q] >= 0 && list[q] < nv) tidx[list[q]] = (int32_t)q;
+}
+// One workgroup per tail vertex q: the colours of its coloured neighbours
+// into base[q] (fixed from here on), its earlier tail neighbours (position
+// below q) into the bit row adj[q].
+__global__ __launch_bounds__(kBlock) void k_tail_prep(const i64 *off, const i64 *tails, i64 nv,
+                                                      const int32_t *colors, const i64 *list, const int32_t *tidx,
+                                                      uint32_t *base, u64 *adj) {
+    constexpr int W = (int)(kJpTail / 64);
+    const i64 q = blockIdx.x;
+    const i64 v = list[q];
+    if (v < 0 || v >= nv) return;  // the list holds exactly the uncoloured vertices
+    for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += kBlock) {
+        const i64 u = tails[j];
+        if (u == v) continue;
+        const int32_t cu = colors[u];
+        if (cu >= 0) {
+            atomicOr(&base[q * (kMaxColors / 32) + (cu >> 5)], 1u << (cu & 31));
+        } else {
+            const int32_t t = tidx[u];
+            if (t >= 0 && t < q) atomicOr((unsigned long long *)&adj[q * W + (t >> 6)], 1ull << (t & 63));
         }
-        __syncthreads();
-        if (threadIdx.x < kMaxColors / 32) {
-            const uint32_t m = mark[threadIdx.x];
-            if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int color = s_color;
-            if (color >= kMaxColors) {
-                atomicOr(err, 1);
-                color = 0;
-            }
-            __atomic_store_n(&state[v], (int32_t)color, __ATOMIC_RELAXED);
-        }
-        __syncthreads();
     }
+}
+// Sequential first-fit over the n <= kJpTail tail vertices in list order, in
+// one workgroup with the bit rows in LDS: thread t holds the colour of tail
+// vertex t once it is chosen; vertex i's forbidden colours are base[i] plus
+// the colours of the set bits of row i.
+constexpr int kTailBatch = 8;  // base rows fetched together
+__global__ __launch_bounds__(kTailBlock) void k_tail_seq(const i64 *list, i64 n, i64 nv, const uint32_t *base,
+                                                         const u64 *adj, int32_t *colors, int32_t *err) {
+    constexpr int W = (int)(kJpTail / 64);
+    constexpr int FW = kMaxColors / 32;
+    __shared__ u64 sadj[kJpTail * W];
+    __shared__ uint32_t forb[kTailBatch][FW];
+    __shared__ int s_cw[FW / kWave];
+    const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
+    for (i64 x = tid; x < n * W; x += kTailBlock) sadj[x] = adj[x];
+    int mine = 0;  // colour of tail vertex tid
+    for (int i0 = 0; i0 < n; i0 += kTailBatch) {
+        __syncthreads();  // the previous batch is done with forb
+        for (int x = tid; x < kTailBatch * FW; x += kTailBlock) {
+            const int r = i0 + x / FW;
+            forb[x / FW][x % FW] = r < n ? base[(i64)r * FW + x % FW] : 0u;
+        }
+        __syncthreads();
+        const int iend = (int)(i0 + kTailBatch < n ? i0 + kTailBatch : n);
+        for (int i = i0; i < iend; i++) {
+            const int r = i - i0;
+            if (tid < i && ((sadj[i * W + (tid >> 6)] >> (tid & 63)) & 1ull))
+                atomicOr(&forb[r][mine >> 5], 1u << (mine & 31));
+            __syncthreads();
+            int cw = kMaxColors;
+            if (tid < FW) {
+                const uint32_t m = forb[r][tid];
+                if (m != 0xFFFFFFFFu) cw = tid * 32 + __ffs((int)~m) - 1;
+            }
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const int x = __shfl_xor(cw, o);
+                cw = x < cw ? x : cw;
+            }
+            if (lane == 0 && w < FW / kWave) s_cw[w] = cw;
+            __syncthreads();
+            int c = kMaxColors;
+#pragma unroll
+            for (int q2 = 0; q2 < FW / kWave; q2++) c = s_cw[q2] < c ? s_cw[q2] : c;
+            if (c >= kMaxColors) {
+                if (tid == 0) atomicOr(err, 1);
+                c = 0;
+            }
+            if (tid == i) mine = c;
+        }
+    }
+    if (tid < n && list[tid] >= 0 && list[tid] < nv) colors[list[tid]] = mine;
 }
 
 // Jones-Plassmann rounds in one cooperative launch: the same topological
@@ -1968,9 +2012,9 @@ __global__ __launch_bounds__(kTailBlock) void k_color_tail(const i64 *off, const
 // host round trip.  Frontier vertices of degree <= 256 take a wavefront each,
 // heavier ones a workgroup each.  do_count: start from scratch (counts and
 // the first frontier); otherwise continue from the frontier in sA/lA (counts
-// ring[0], ring[1]) after colored0 vertices were coloured.  When at most
-// tail_at vertices remain, workgroup 0 colours them by sequential first-fit
-// in increasing id (the host-driven path's tail).  A frontier of more than
+// ring[0], ring[1]) after colored0 vertices were coloured.  The launch ends
+// when at most tail_at vertices remain (the host colours them by sequential
+// first-fit in increasing id, launch_color_tail).  A frontier of more than
 // max_front vertices ends the launch before its round (the host's full-GPU
 // rounds take it; after r rounds the frontier is in sA/lA for even r, sB/lB
 // for odd r).  ring: 10 words, [2q], [2q+1] the small and large frontier
@@ -1986,8 +2030,6 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                                                              i64 max_front) {
     constexpr int WPB = kJpPBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
-    __shared__ i64 tail_list[kJpPBlock];
-    __shared__ int wsum[kJpPBlock / kWave];
     __shared__ int s_color;
     __shared__ Stage<kStageCap> S;  // appends to the next small frontier
     stage_init(S);
@@ -2144,53 +2186,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
         r++;
         const i64 rest = nv - colored;
         if (rest == 0) break;
-        if (rest <= tail_at) {
-            if (blockIdx.x != 0) break;
-            // workgroup 0: the uncoloured vertices in increasing id, then first-fit one by one
-            int n = 0;
-            for (i64 base = 0; base < nv; base += kJpPBlock) {
-                const i64 v = base + threadIdx.x;
-                const bool un = v < nv && __atomic_load_n(&colors[v], __ATOMIC_RELAXED) < 0;
-                int total = 0;
-                const int at = n + block_exscan<kJpPBlock>(un ? 1 : 0, wsum, &total);
-                if (un && at < kJpPBlock) tail_list[at] = v;
-                n += total;
-            }
-            __syncthreads();
-            if (n > kJpPBlock) {
-                if (threadIdx.x == 0) atomicOr(err, 2);
-                break;
-            }
-            for (int qq = 0; qq < n; qq++) {
-                const i64 v = tail_list[qq];
-                for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
-                if (threadIdx.x == 0) s_color = kMaxColors;
-                __syncthreads();
-                for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += kJpPBlock) {
-                    const i64 u = tails[j];
-                    if (u == v) continue;
-                    const int32_t su = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
-                    if (su < 0) continue;
-                    atomicOr(&mark[0][su >> 5], 1u << (su & 31));
-                }
-                __syncthreads();
-                if (threadIdx.x < kMaxColors / 32) {
-                    const uint32_t m = mark[0][threadIdx.x];
-                    if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    int color = s_color;
-                    if (color >= kMaxColors) {
-                        atomicOr(err, 1);
-                        color = 0;
-                    }
-                    __atomic_store_n(&colors[v], (int32_t)color, __ATOMIC_RELAXED);
-                }
-                __syncthreads();
-            }
-            break;
-        }
+        if (rest <= tail_at) break;  // the host colours the tail
         i64 *t = fs;
         fs = ns_;
         ns_ = t;
@@ -2303,8 +2299,8 @@ __device__ inline void jp_push_wave8(const i64 *off, const i64 *tails, int32_t *
 // frontier); frontier vertices of at most kJpMidDeg entries are coloured by
 // jp_push_wave8, heavier ones are cut into chunks of kJpPBlock entries that
 // are dealt out to all workgroups (each chunk reads the vertex's bitmap for
-// its colour; chunk 0 writes it).  The tail (at most tail_at
-// uncoloured vertices) is the counted rounds' sequential first-fit.  ring as
+// its colour; chunk 0 writes it).  The launch ends when at most tail_at
+// vertices remain (launch_color_tail colours them).  ring as
 // k_jp_persistent's ([6] rounds, [7] heavy vertices while counting, [8]
 // coloured vertices at the end).
 constexpr int kJpChunkBatch = kJpPBlock;  // large frontier vertices per batch
@@ -2315,13 +2311,9 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64
                                                        const u64 *prio, uint8_t *code, uint32_t *forbid) {
     constexpr int WPB = kJpPBlock / kWave;
     constexpr int FW = kMaxColors / 32;
-    __shared__ uint32_t mark[kMaxColors / 32];
-    __shared__ i64 tail_list[kJpPBlock];
     __shared__ i64 bv[kJpChunkBatch];  // large frontier vertices of the batch: ids,
     __shared__ i64 bb[kJpChunkBatch];  // first entries
     __shared__ i64 bd[kJpChunkBatch];  // and entry counts
-    __shared__ int wsum[kJpPBlock / kWave];
-    __shared__ int s_color;
     __shared__ int s_cw[kMaxColors / 32 / kWave];
     __shared__ Stage<kStageCap> S;  // appends to the next small frontier
     stage_init(S);
@@ -2471,53 +2463,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64
         r++;
         const i64 rest = nv - colored;
         if (rest == 0) break;
-        if (rest <= tail_at) {
-            if (blockIdx.x != 0) break;
-            // workgroup 0: the uncoloured vertices in increasing id, then first-fit one by one
-            int n = 0;
-            for (i64 base = 0; base < nv; base += kJpPBlock) {
-                const i64 v = base + threadIdx.x;
-                const bool un = v < nv && __atomic_load_n(&colors[v], __ATOMIC_RELAXED) < 0;
-                int total = 0;
-                const int at = n + block_exscan<kJpPBlock>(un ? 1 : 0, wsum, &total);
-                if (un && at < kJpPBlock) tail_list[at] = v;
-                n += total;
-            }
-            __syncthreads();
-            if (n > kJpPBlock) {
-                if (threadIdx.x == 0) atomicOr(err, 2);
-                break;
-            }
-            for (int qq = 0; qq < n; qq++) {
-                const i64 v = tail_list[qq];
-                for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[i] = 0;
-                if (threadIdx.x == 0) s_color = kMaxColors;
-                __syncthreads();
-                for (i64 j = off[v] + threadIdx.x; j < off[v + 1]; j += kJpPBlock) {
-                    const i64 u = tails[j];
-                    if (u == v) continue;
-                    const int32_t su = __atomic_load_n(&colors[u], __ATOMIC_RELAXED);
-                    if (su < 0) continue;
-                    atomicOr(&mark[su >> 5], 1u << (su & 31));
-                }
-                __syncthreads();
-                if (threadIdx.x < kMaxColors / 32) {
-                    const uint32_t m = mark[threadIdx.x];
-                    if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    int color = s_color;
-                    if (color >= kMaxColors) {
-                        atomicOr(err, 1);
-                        color = 0;
-                    }
-                    __atomic_store_n(&colors[v], (int32_t)color, __ATOMIC_RELAXED);
-                }
-                __syncthreads();
-            }
-            break;
-        }
+        if (rest <= tail_at) break;  // the host colours the tail
         i64 *tp = fs;
         fs = ns_;
         ns_ = tp;
@@ -3035,15 +2981,25 @@ void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const 
     }
 }
 void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
-                       int round, int32_t *err, hipStream_t s) {
+                       int32_t *tidx, uint32_t *base, u64 *adj, int32_t *err, hipStream_t s) {
     if (n <= 0) return;
+    if (n > kJpTail) {  // callers pass at most kJpTail vertices
+        launch_fill_i32(err, 1, 2, s);
+        return;
+    }
     k_flag_uncolored<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(state, g.nv, flags);
     GLV_LAUNCH_CHECK();
     scan_exclusive_i64(flags, pos, g.nv, s);
     k_scatter_flagged<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(flags, pos, g.nv, list);
     GLV_LAUNCH_CHECK();
-    (void)round;
-    k_color_tail<<<1, kTailBlock, 0, s>>>(g.off, g.tails, state, list, n, err);
+    launch_fill_i32(tidx, g.nv, -1, s);
+    launch_fill_i32((int32_t *)base, n * (kMaxColors / 32), 0, s);
+    launch_fill_i64((i64 *)adj, n * (kJpTail / 64), 0, s);
+    k_tail_index<<<grid_for(n, kBlock, 64), kBlock, 0, s>>>(list, n, g.nv, tidx);
+    GLV_LAUNCH_CHECK();
+    k_tail_prep<<<(unsigned)n, kBlock, 0, s>>>(g.off, g.tails, g.nv, state, list, tidx, base, adj);
+    GLV_LAUNCH_CHECK();
+    k_tail_seq<<<1, kTailBlock, 0, s>>>(list, n, g.nv, base, adj, state, err);
     GLV_LAUNCH_CHECK();
 }
 int jp_persistent_blocks() {

@@ -158,8 +158,12 @@ void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const 
                         hipStream_t s);
 void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
 // first-fit, one vertex at a time in increasing id, of the n uncoloured vertices
+// (the n vertices are listed, then the colours of their coloured neighbours
+// and their earlier tail neighbours are gathered in parallel, then one
+// workgroup picks the colours in order).  flags, pos, list: nv each; tidx: nv;
+// base: n * kMaxColors/32 words; adj: n * kJpTail/64 words; n <= kJpTail.
 void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
-                       int round, int32_t *err, hipStream_t s);
+                       int32_t *tidx, uint32_t *base, u64 *adj, int32_t *err, hipStream_t s);
 constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many uncoloured vertices
 // Jones-Plassmann rounds in one cooperative launch (same result as the
 // host-driven rounds): from scratch (count) or continuing from the frontier

@@ -181,8 +181,8 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
         const int r = (int)h[6];
         round += r;
-        if (h[7] == 0) return 1;
         colored = h[8];
+        if (h[7] == 0) return 1;
         hc[0] = h[2 * (r % 3)];
         hc[1] = h[2 * (r % 3) + 1];
         hc[2] = 0;
@@ -211,6 +211,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
             GLV_HIP(hipStreamSynchronize(s_));
             if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
             round = (int)h[6];
+            colored = h[8];
             done = true;
         } else {
             launch_fill_i32(colors, nv, -1, s_);
@@ -258,14 +259,17 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
             std::swap(cur, next);
             round++;
             const i64 rest = nv - colored;
-            if (rest > 0 && rest <= kJpTail) {
-                // the last vertices one at a time (scratch: lists not holding a frontier)
-                Profiler::get().begin(K_COLOR_TAIL, s_);
-                launch_color_tail(g, colors, next.small, next.large, next.giant, rest, round, err.p, s_);
-                Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
-                break;
-            }
+            if (rest > 0 && rest <= kJpTail) break;
         }
+    }
+    if (colored < nv) {
+        // the last vertices one at a time (scratch: lists not holding a frontier)
+        const i64 rest = nv - colored;
+        DBuf<uint32_t> base(rest * (kMaxColors / 32), s_);
+        DBuf<u64> adj(rest * (kJpTail / 64), s_);
+        Profiler::get().begin(K_COLOR_TAIL, s_);
+        launch_color_tail(g, colors, next.small, next.large, next.giant, rest, cnt.p, base.p, adj.p, err.p, s_);
+        Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
     }
     launch_jp_finish(colors, nv, maxc.p, s_);
     const int32_t e = fetch(err.p);
