@@ -1,0 +1,92 @@
+// Device block cache behind DBuf (declared in common.hpp).
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+
+namespace glv {
+namespace {
+
+struct Key {
+    int dev;
+    hipStream_t s;
+    size_t cap;
+    bool operator<(const Key &o) const { return std::tie(dev, s, cap) < std::tie(o.dev, o.s, o.cap); }
+};
+std::mutex g_blocks_mu;
+std::map<Key, std::vector<void *>> g_free;
+size_t g_cached = 0;                          // bytes on the free lists
+constexpr size_t kCacheLimit = size_t(32) << 30;  // beyond this, released blocks go back to the pool
+
+size_t size_class(size_t bytes) {
+    if (bytes <= 256) return 256;
+    size_t top = 1;
+    while (top < bytes) top <<= 1;                 // power of two >= bytes
+    const size_t step = std::max<size_t>(256, top >> 4);  // 1/8 of the power of two below
+    return (bytes + step - 1) / step * step;
+}
+
+int current_dev() {
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+}
+
+}  // namespace
+
+void *block_get(size_t bytes, hipStream_t s, size_t *cap) {
+    const size_t c = size_class(bytes);
+    const Key k{current_dev(), s, c};
+    {
+        std::lock_guard<std::mutex> lk(g_blocks_mu);
+        auto it = g_free.find(k);
+        if (it != g_free.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            g_cached -= c;
+            *cap = c;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMallocAsync(&p, c, s);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        block_trim();
+        e = hipMallocAsync(&p, c, s);
+    }
+    GLV_HIP(e);
+    *cap = c;
+    return p;
+}
+
+void block_put(void *p, size_t cap, hipStream_t s) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_blocks_mu);
+    if (g_cached + cap > kCacheLimit) {
+        (void)hipFreeAsync(p, s);
+        return;
+    }
+    g_free[Key{current_dev(), s, cap}].push_back(p);
+    g_cached += cap;
+}
+
+void block_trim() {
+    std::lock_guard<std::mutex> lk(g_blocks_mu);
+    int dev = current_dev();
+    for (auto it = g_free.begin(); it != g_free.end();) {
+        if (it->first.dev == dev) {
+            for (void *p : it->second) (void)hipFreeAsync(p, it->first.s);
+            g_cached -= it->first.cap * it->second.size();
+            (void)hipStreamSynchronize(it->first.s);
+            it = g_free.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+}  // namespace glv

@@ -34,6 +34,7 @@ thread_local std::string g_last_error;
 struct DeviceCtx {
     hipStream_t stream = nullptr;
     bool ready = false;
+    bool pool = false;
 };
 std::mutex g_mu;
 DeviceCtx g_ctx[64];
@@ -44,21 +45,26 @@ int current_device() {
     return d;
 }
 
-// The engine's stream on the current device; the device's default memory
-// pool keeps freed blocks (release threshold = max) so that repeated calls
-// reuse memory instead of going back to the driver.
+// The engine's stream on the current device (the caller's when given).  On
+// first use of a device its default memory pool is set to keep freed blocks
+// (release threshold = max), whichever stream the calls come on, so that
+// stream-ordered allocations are reused across synchronisations instead of
+// going back to the driver.
 hipStream_t engine_stream(void *user) {
-    if (user) return (hipStream_t)user;
     int d = current_device();
     std::lock_guard<std::mutex> lk(g_mu);
     DeviceCtx &c = g_ctx[d & 63];
-    if (!c.ready) {
-        GLV_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    if (!c.pool) {
         hipMemPool_t pool;
         if (hipDeviceGetDefaultMemPool(&pool, d) == hipSuccess) {
             uint64_t thr = UINT64_MAX;
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
         }
+        c.pool = true;
+    }
+    if (user) return (hipStream_t)user;
+    if (!c.ready) {
+        GLV_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         c.ready = true;
     }
     return c.stream;

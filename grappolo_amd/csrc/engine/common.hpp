@@ -106,13 +106,26 @@ struct Error {
                         std::string(#call) + ": " + hipGetErrorString(e_));                   \
     } while (0)
 
-// Stream-ordered device buffer (memory comes from the device's default pool,
-// which keeps freed blocks cached, so per-phase scratch does not hit the
-// driver allocator after warm-up).
+// Stream-ordered reuse of device blocks.  A released block goes back to the
+// free list of its size class, device and stream and serves the next request
+// of that class there (stream order makes the reuse safe without a
+// synchronisation); blocks come from the device's default pool the first
+// time.  Size classes round up to 1/8 of a power of two (at least 256 bytes).
+// block_get returns the block and its class size in *cap; block_put takes
+// both back.  block_trim frees the cached blocks (after an out-of-memory).
+void *block_get(size_t bytes, hipStream_t s, size_t *cap);
+void block_put(void *p, size_t cap, hipStream_t s);
+void block_trim();
+
+// Stream-ordered device buffer on the block cache, so per-phase scratch
+// reaches neither the driver nor the pool's bookkeeping after warm-up.  A
+// pointer taken out of a DBuf (p set to null) leaves the cache for good and
+// is freed with hipFreeAsync.
 template <class T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    size_t cap = 0;
     hipStream_t s = nullptr;
     DBuf() = default;
     DBuf(size_t count, hipStream_t stream) { alloc(count, stream); }
@@ -120,25 +133,32 @@ struct DBuf {
         release();
         s = stream;
         n = count;
-        if (count) GLV_HIP(hipMallocAsync((void **)&p, count * sizeof(T), stream));
+        if (count) p = static_cast<T *>(block_get(count * sizeof(T), stream, &cap));
     }
     void release() {
-        if (p) (void)hipFreeAsync(p, s);
+        if (p) block_put(p, cap, s);
         p = nullptr;
         n = 0;
+        cap = 0;
     }
     ~DBuf() { release(); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap), s(o.s) {
+        o.p = nullptr;
+        o.n = 0;
+        o.cap = 0;
+    }
     DBuf &operator=(DBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             n = o.n;
+            cap = o.cap;
             s = o.s;
             o.p = nullptr;
             o.n = 0;
+            o.cap = 0;
         }
         return *this;
     }
