@@ -222,7 +222,7 @@ static int jp_beats(const i64 *off, i64 u, i64 v) {
 }
 
 /* Restatement of the engine's coloring (grappolo_amd/csrc/engine/
- * kernels.hip, k_jp_* and k_color_tail): Jones-Plassmann rounds -- every
+ * kernels.hip, k_jp_* and k_tail_*): Jones-Plassmann rounds -- every
  * uncoloured vertex whose priority (jp_key / jp_beats: degree, hash, id)
  * beats all uncoloured neighbours (self-loops ignored) takes the smallest
  * color absent from its coloured neighbours;
