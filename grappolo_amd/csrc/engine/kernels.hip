@@ -1342,6 +1342,333 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
     }
 }
 
+// Dense persistent sweep (graphs of at most kDenseMaxNV vertices) with the
+// color steps dealt alternately to the two halves of the grid (workgroups of
+// even and of odd index).  While one half decides step k, the other gathers
+// the neighbour labels of its vertex of step k+1 from the state before step
+// k, setting aside the entries whose neighbour belongs to step k's class
+// (the only labels step k can change); after the barrier it reads those few
+// labels again and adds them, so its accumulator holds exactly the entries a
+// gather after the barrier would have added (the sum order of LDS atomics is
+// unordered either way).  State, buffers and the modularity partials as in
+// k_colored_iteration.  colors: the class of every vertex.
+constexpr int kSpecFix = 256;  // set-aside entries per vertex; more: gather after the barrier
+__global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A, i64 *C, double *a, i64 *n,
+                                                                    const i64 *cperm, const i64 *cstart,
+                                                                    int ncol, unsigned *bar, double *part,
+                                                                    uint8_t *stale, i64 *C1, double *a1,
+                                                                    i64 *n1, const int32_t *colors) {
+    __shared__ __align__(16) unsigned char smem[kPDenseBytes];
+    __shared__ double red[kPBlock / kWave];
+    __shared__ int sc[kPBlock / kWave];
+    __shared__ double sg[kPBlock / kWave];
+    __shared__ i64 si[kPBlock / kWave];
+    __shared__ int s_cnt;                // communities listed for the gathered vertex
+    __shared__ double s_sl;              // its self-loop weight
+    __shared__ i64 s_nid;                // size of the chosen community
+    __shared__ int s_fn;                 // set-aside entries (above kSpecFix: too many)
+    __shared__ uint16_t s_ft[kSpecFix];  // their neighbours
+    __shared__ double s_fw[kSpecFix];    // and weights
+    double *acc = reinterpret_cast<double *>(smem);
+    uint32_t *touched = reinterpret_cast<uint32_t *>(smem + kDenseMaxNV * 8);
+    uint16_t *list = reinterpret_cast<uint16_t *>(smem + kDenseMaxNV * 8 + kDenseMaxNV / 32 * 4);
+    const int nv = (int)A.nv;
+    const int nwords = (nv + 31) / 32;
+    const unsigned nb = gridDim.x;
+    const unsigned half = blockIdx.x & 1u, nbh = nb / 2, wid = blockIdx.x >> 1;
+    const int lane = threadIdx.x % kWave;
+    const u64 lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    unsigned seq = 0;
+    if (threadIdx.x == 0) {
+        s_cnt = 0;
+        s_sl = 0.0;
+        s_fn = 0;
+    }
+    for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
+    for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+    for (i64 i = (i64)blockIdx.x * kPBlock + threadIdx.x; i < nv; i += (i64)nb * kPBlock) {
+        C1[i] = C[i];
+        a1[i] = a[i];
+        n1[i] = n[i];
+    }
+    // community cc gets weight ww; the first lane to touch it lists it (one LDS
+    // counter update per wavefront); called by whole wavefronts
+    auto insert = [&](bool valid, i64 cc, double ww) {
+        bool fresh = false;
+        if (valid) {
+            const uint32_t bit = 1u << (cc & 31);
+            fresh = !(atomicOr(&touched[cc >> 5], bit) & bit);
+        }
+        const u64 fm = __ballot(fresh);
+        if (fm) {
+            const int leader = __ffsll((long long)fm) - 1;
+            int at = 0;
+            if (lane == leader) at = atomicAdd(&s_cnt, __popcll(fm));
+            at = __shfl(at, leader);
+            if (fresh) list[at + __popcll(fm & lt)] = (uint16_t)cc;
+        }
+        if (valid) atomicAdd(&acc[cc], ww);
+    };
+    // empties the dense view after a vertex (ntouched listed communities)
+    auto clear = [&](int ntouched) {
+        for (int k = threadIdx.x; k < ntouched; k += kPBlock) {
+            acc[list[k]] = 0.0;
+            touched[list[k] >> 5] = 0;
+        }
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            s_sl = 0.0;
+            s_fn = 0;
+        }
+    };
+    bool alive = grid_barrier(bar, nb, ++seq);
+    int par = 0;            // buffer holding the state after the previous step; also the deciding half
+    i64 plo = 0, ptot = 0;  // the previous step's class
+    // the vertex gathered ahead by this workgroup: its class, id, entries, label, degree
+    int sp_c = -1;
+    i64 sp_v = -1, sp_beg = 0, sp_end = 0, sp_s = -1;
+    double sp_k = 0;
+    // thread 0: the move made for this workgroup's first vertex of the previous step
+    i64 mv_v = -1, mv_t = -1, mv_s = -1;
+    double mv_k = 0;
+    for (int c = 0; c < ncol && alive; c++) {
+        const i64 lo = cstart[c], hi = cstart[c + 1], total = hi - lo;
+        if (total == 0) continue;
+        A.C = par ? C1 : C;
+        A.a = par ? a1 : a;
+        A.n = par ? n1 : n;
+        i64 *Cn = par ? C : C1;
+        double *an = par ? a : a1;
+        i64 *nn = par ? n : n1;
+        const bool deciding = half == (unsigned)par;
+        if (!deciding) {
+            // this workgroup decided the previous step: its moves, into the other buffer
+            if (threadIdx.x == 0) {
+                if (mv_v >= 0 && mv_t != mv_s) {
+                    move_vertex(mv_t, mv_s, mv_k, an, nn);
+                    Cn[mv_v] = mv_t;
+                }
+                for (i64 q = (i64)wid + nbh; q < ptot; q += nbh) {
+                    const i64 v = cperm[plo + q];
+                    const i64 t = A.tgt[v], s = Cn[v];
+                    if (t != s) {
+                        move_vertex(t, s, A.k[v], an, nn);
+                        Cn[v] = t;
+                    }
+                }
+            }
+            mv_v = -1;
+            // gather ahead: this workgroup's first vertex of the next non-empty class
+            int c2 = c + 1;
+            i64 lo2 = hi, hi2 = hi;
+            while (c2 < ncol) {
+                hi2 = cstart[c2 + 1];
+                if (hi2 > lo2) break;
+                lo2 = hi2;
+                c2++;
+            }
+            sp_c = -1;
+            if (c2 < ncol && (i64)wid < hi2 - lo2) {
+                const i64 v = cperm[lo2 + wid];
+                const i64 beg = A.off[v], end = A.off[v + 1];
+                const i64 s = A.C[v];  // changes only in v's own step
+                if (s >= 0) {
+                    for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
+                        i64 tt[kGatherUnroll], cl[kGatherUnroll];
+                        double wt[kGatherUnroll];
+                        int32_t co[kGatherUnroll];
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            const i64 j = j0 + (i64)u * kPBlock;
+                            tt[u] = j < end ? A.tails[j] : -1;
+                            wt[u] = j < end ? A.w[j] : 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            cl[u] = tt[u] >= 0 ? A.C[tt[u]] : 0;
+                            co[u] = tt[u] >= 0 ? colors[tt[u]] : -1;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            const bool valid = tt[u] >= 0;
+                            if (valid && tt[u] == v) atomicAdd(&s_sl, wt[u]);
+                            const bool aside = valid && co[u] == c && tt[u] != v;
+                            if (aside) {
+                                const int at = atomicAdd(&s_fn, 1);
+                                if (at < kSpecFix) {
+                                    s_ft[at] = (uint16_t)tt[u];
+                                    s_fw[at] = wt[u];
+                                }
+                            }
+                            insert(valid && !aside, cl[u], wt[u]);
+                        }
+                    }
+                    __syncthreads();
+                    if (s_fn > kSpecFix) {  // too many set aside: gather again after the barrier
+                        clear(s_cnt);
+                    } else {
+                        sp_c = c2;
+                        sp_v = v;
+                        sp_beg = beg;
+                        sp_end = end;
+                        sp_s = s;
+                        sp_k = A.k[v];
+                    }
+                    __syncthreads();
+                }
+            }
+        } else {
+            for (i64 q = wid; q < total; q += nbh) {
+                const bool first = q == (i64)wid;
+                const bool sp = first && sp_c == c;
+                const i64 v = sp ? sp_v : cperm[lo + q];
+                const i64 beg = sp ? sp_beg : A.off[v], end = sp ? sp_end : A.off[v + 1];
+                const i64 s = sp ? sp_s : A.C[v];
+                if (s < 0) {  // a vertex without entries (label -1) stays where it is
+                    if (threadIdx.x == 0) A.tgt[v] = s;
+                    continue;
+                }
+                const double as = A.a[s];
+                i64 ns = 0;
+                if (threadIdx.x == 0) ns = A.n[s];
+                if (sp) {
+                    // the set-aside entries with the labels after the previous step
+                    const int nf = s_fn;
+                    for (int i0 = 0; i0 < nf; i0 += kPBlock) {
+                        const int i = i0 + threadIdx.x;
+                        const bool valid = i < nf;
+                        const i64 t = valid ? (i64)s_ft[i] : 0;
+                        const double ww = valid ? s_fw[i] : 0.0;
+                        const i64 cc = valid ? A.C[t] : 0;
+                        insert(valid, cc, ww);
+                    }
+                } else {
+                    for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
+                        i64 tt[kGatherUnroll], cl[kGatherUnroll];
+                        double wt[kGatherUnroll];
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            const i64 j = j0 + (i64)u * kPBlock;
+                            tt[u] = j < end ? A.tails[j] : -1;
+                            wt[u] = j < end ? A.w[j] : 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) cl[u] = tt[u] >= 0 ? A.C[tt[u]] : 0;
+#pragma unroll
+                        for (int u = 0; u < kGatherUnroll; u++) {
+                            const bool valid = tt[u] >= 0;
+                            if (valid && tt[u] == v) atomicAdd(&s_sl, wt[u]);
+                            insert(valid, cl[u], wt[u]);
+                        }
+                    }
+                }
+                __syncthreads();
+                const double sl = s_sl;
+                const double kv = sp ? sp_k : A.k[v];
+                const double ax = as - kv;
+                const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
+                const int ntouched = s_cnt;
+                const double e_s = acc[s];
+                const double eix = e_s - sl;
+                bool cand = false;
+                double gain = 0;
+                i64 id = INT64_MAX;
+                i64 nid = 0;
+                float t1 = -INFINITY, t2 = -INFINITY;
+                for (int k0 = threadIdx.x; k0 < ntouched; k0 += kGatherUnroll * kPBlock) {
+                    i64 yy[kGatherUnroll], nyy[kGatherUnroll];
+                    double ay[kGatherUnroll];
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const int k = k0 + u * kPBlock;
+                        yy[u] = k < ntouched ? (i64)list[k] : -1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        ay[u] = yy[u] >= 0 ? A.a[yy[u]] : 0.0;
+                        nyy[u] = yy[u] >= 0 ? A.n[yy[u]] : 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const i64 y = yy[u];
+                        if (y < 0 || y == s) continue;
+                        const double g = gain_of(acc[y], eix, kv, ay[u], ax, A.constant);
+                        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+                        if (better(g > 0, g, y, cand, gain, id)) {
+                            cand = true;
+                            gain = g;
+                            id = y;
+                            nid = nyy[u];
+                        }
+                    }
+                }
+                const bool my_cand = cand;
+                const i64 my_id = id;
+                block_argmax_p(cand, gain, id, sc, sg, si);
+                if (A.feat) block_top2<kPBlock>(t1, t2);
+                if (cand && my_cand && my_id == id) s_nid = nid;  // the lane that scored the winner
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    i64 target = s;
+                    if (cand) {
+                        target = id;
+                        if (ns == 1 && target > s && s_nid == 1) target = s;
+                    }
+                    A.tgt[v] = target;
+                    if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+                    if (target != s) {  // into the other buffer, read from the next step on
+                        move_vertex(target, s, kv, an, nn);
+                        Cn[v] = target;
+                        if (stale) stale[v] = 0;
+                    }
+                    if (first) {
+                        mv_v = v;
+                        mv_t = target;
+                        mv_s = s;
+                        mv_k = kv;
+                    }
+                }
+                __syncthreads();  // every read of the accumulator is done
+                clear(ntouched);
+                __syncthreads();
+            }
+            sp_c = -1;
+        }
+        alive = grid_barrier(bar, nb, ++seq);
+        par ^= 1;
+        plo = lo;
+        ptot = total;
+    }
+    A.C = par ? C1 : C;
+    A.a = par ? a1 : a;
+    A.n = par ? n1 : n;
+    // modularity partials: internal weight over this workgroup's vertices, sum of a_c^2
+    double exx = 0, a2 = 0;
+    if (alive) {
+        for (i64 v = blockIdx.x; v < nv; v += nb) {
+            const i64 s = A.C[v];
+            if (s < 0) continue;
+            for (i64 j = A.off[v] + threadIdx.x; j < A.off[v + 1]; j += kPBlock)
+                if (A.C[A.tails[j]] == s) exx += A.w[j];
+        }
+        for (i64 i = (i64)blockIdx.x * kPBlock + threadIdx.x; i < nv; i += (i64)nb * kPBlock)
+            a2 += A.a[i] * A.a[i];
+    }
+    exx = block_sum_p(exx, red);
+    a2 = block_sum_p(a2, red);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = exx;
+        part[nb + blockIdx.x] = a2;
+    }
+    if (alive && par) {  // the final state sits in the second buffer
+        for (i64 i = (i64)blockIdx.x * kPBlock + threadIdx.x; i < nv; i += (i64)nb * kPBlock) {
+            C[i] = C1[i];
+            a[i] = a1[i];
+            n[i] = n1[i];
+        }
+    }
+}
+
 __global__ void k_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out) {
     i64 m = 0;
     for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < count; q += (i64)gridDim.x * blockDim.x) {
@@ -2851,13 +3178,15 @@ int persistent_blocks() {
         GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_colored_iteration<false>, kPBlock, 0));
         GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_h, k_colored_iteration<true>, kPBlock, 0));
         per = std::min(per, per_h);
+        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_h, k_colored_iteration_spec, kPBlock, 0));
+        per = std::min(per, per_h);
         cached = per > 0 ? cus * per : 0;
     }
     return cached;
 }
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
                               unsigned *bar, double *part, int nblocks, hipStream_t s, uint8_t *stale,
-                              i64 *C1, double *a1, i64 *n1) {
+                              i64 *C1, double *a1, i64 *n1, const int32_t *colors) {
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || A.nv > kPersistMaxNV) return false;
     const bool hash = A.nv > kDenseMaxNV;
     // degrees above the LDS table's reach need the workgroup's dense HBM row
@@ -2875,8 +3204,11 @@ bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *st
     i64 *c1 = C1;
     double *ca1 = a1;
     i64 *cn1 = n1;
-    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt, &sf, &c1, &ca1, &cn1};
-    const void *fn = hash ? (const void *)k_colored_iteration<true> : (const void *)k_colored_iteration<false>;
+    const int32_t *co = colors;
+    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt, &sf, &c1, &ca1, &cn1, &co};
+    const bool spec = !hash && colors && nblocks >= 2 && nblocks % 2 == 0;
+    const void *fn = spec ? (const void *)k_colored_iteration_spec
+                          : hash ? (const void *)k_colored_iteration<true> : (const void *)k_colored_iteration<false>;
     hipError_t e = hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kPBlock), args, 0, s);
     if (e != hipSuccess) {
         (void)hipGetLastError();

@@ -345,8 +345,12 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         const i64 per_class = (nv + std::max(nonempty, 1) - 1) / std::max(nonempty, 1);
         if (nv <= kDenseMaxNV || per_class <= kPersistMaxPerColor) {
             pblocks = persistent_blocks();
+            // dense graphs: the grid's two halves take the color steps in turn
+            // (an even count, twice the class size per half)
+            const i64 want = nv <= kDenseMaxNV ? 4 * per_class : 2 * per_class;
             if (pblocks > 0)
-                pblocks = (int)std::min<i64>({(i64)pblocks, kPersistMaxBlocks, std::max<i64>(4, 2 * per_class)});
+                pblocks = (int)std::min<i64>({(i64)pblocks, kPersistMaxBlocks, std::max<i64>(4, want)});
+            if (nv <= kDenseMaxNV) pblocks &= ~1;
         }
     }
     DBuf<i64> gkeys;
@@ -479,7 +483,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
             launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
             Profiler::get().begin(K_PERSIST, s_);
             persistent = launch_colored_iteration(A, plist, pstart, ncol, bar.p, ppart.p, pblocks, s_,
-                                                  mode ? stale.p : nullptr, pC1.p, pa1.p, pn1.p);
+                                                  mode ? stale.p : nullptr, pC1.p, pa1.p, pn1.p, colors);
             Profiler::get().end(K_PERSIST, 24.0 * g.nnz + 32.0 * nv, s_);
             if (persistent) {
                 launch_sum_partials(ppart.p, pblocks, res.p, s_);
