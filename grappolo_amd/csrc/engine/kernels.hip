@@ -385,7 +385,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
         if (A.feat) t1 = (float)(gain / (2.0 * kv));
         cand = gain > 0;  // max() starts from maxGain = 0 and needs a strict increase
     }
+    const i64 my_c = c;
+    const bool my_first = lane < deg && first;
     group_argmax<G>(cand, gain, c);
+    // weight to the winner, from the lane that summed it (exactly one lane)
+    double e_t = 0;
+    if (A.dein) {
+        e_t = cand && my_first && my_c == c ? e_mine : 0.0;
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) e_t += __shfl_xor(e_t, o, G);
+    }
     if (A.feat) top2_group<G>(t1, t2);
     if (valid && lane == 0) {
         i64 target = s;
@@ -398,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
             const double kv = A.k[v];
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * cst : 0.0;
+                A.dein[v] = target != s ? 2.0 * (e_t - (e_s - sl)) : 0.0;
         }
         if (A.eii_v) A.eii_v[v] = e_s;
     }
@@ -515,7 +524,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
             const double kv = A.k[v];
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * cst : 0.0;
+                A.dein[v] = target != s
+                                ? 2.0 * (hash_lookup(keys, vals, SLOTS - 1, SHIFT, target) - (e_s - sl))
+                                : 0.0;
         }
         if (A.eii_v) A.eii_v[v] = e_s;
     }
@@ -597,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
             A.tgt[v] = target;
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
             if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - (A.a[s] - kv)) * cst : 0.0;
+                A.dein[v] = target != s ? 2.0 * (hash_lookup(keys, vals, mask, shift, target) - (e_s - sl)) : 0.0;
             if (A.eii_v) A.eii_v[v] = e_s;
         }
         part += e_s;
@@ -669,11 +680,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
             }
             A.tgt[v] = target;
             if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
-            if (A.dein)
-                A.dein[v] = target != s ? gain + 2 * kv * (A.a[target] - ax) * cst : 0.0;
+            if (A.dein) A.dein[v] = target != s ? 2.0 * (acc[target] - eix) : 0.0;
             if (A.eii_v) A.eii_v[v] = e_s;
         }
         part += e_s;
+        __syncthreads();  // thread 0's read of acc[target] is done
         for (int k = threadIdx.x; k < ntouched; k += kBlock) acc[list[k]] = 0.0;
         for (int i = threadIdx.x; i < nwords; i += kBlock) touched[i] = 0;
         __syncthreads();
@@ -1687,7 +1698,7 @@ __global__ void k_isolated_targets(const i64 *verts, i64 count, i64 *tgt) {
 
 // Commit of one color step: community updates of the movers, then labels.
 __global__ void k_apply_colored(const i64 *perm, Slices S, const i64 *tgt, i64 *C, const double *k,
-                                double *a, i64 *n, uint8_t *stale, const double *dein, double *dsum) {
+                                double *a, i64 *n, uint8_t *stale, const double *dein, double *dpart) {
     i64 total = 0;
     for (int b = 0; b < kNumBins; b++) total += S.hi[b] - S.lo[b];
     double d = 0;
@@ -1707,10 +1718,17 @@ __global__ void k_apply_colored(const i64 *perm, Slices S, const i64 *tgt, i64 *
             if (dein) d += dein[v];
         }
     }
-    if (dsum) {
+    if (dpart) {
         d = block_sum(d);
-        if (threadIdx.x == 0 && d != 0.0) atomicAdd(dsum, d);
+        if (threadIdx.x == 0) dpart[blockIdx.x] = d;
     }
+}
+// *dsum += the n partials, in a fixed order (one workgroup)
+__global__ __launch_bounds__(kBlock) void k_add_partials(const double *part, i64 n, double *dsum) {
+    double x = 0;
+    for (i64 i = threadIdx.x; i < n; i += kBlock) x += part[i];
+    x = block_sum(x);
+    if (threadIdx.x == 0) *dsum += x;
 }
 // cInfo += cUpdate of a Jacobi iteration (parallelLouvainMethod.cpp:223-236, :275-279).
 __global__ void k_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a,
@@ -3223,13 +3241,17 @@ void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t 
 }
 void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 *C, const double *k,
                           double *a, i64 *n, hipStream_t s, uint8_t *stale, const double *dein,
-                          double *dsum) {
+                          double *dsum, double *dpart) {
     i64 total = 0;
     for (int b = 0; b < kNumBins; b++) total += S.hi[b] - S.lo[b];
     if (total <= 0) return;
-    k_apply_colored<<<grid_for(total, kBlock, 8192), kBlock, 0, s>>>(perm, S, tgt, C, k, a, n, stale, dein,
-                                                                     dsum);
+    const unsigned blocks = grid_for(total, kBlock, kApplyMaxBlocks);
+    k_apply_colored<<<blocks, kBlock, 0, s>>>(perm, S, tgt, C, k, a, n, stale, dein, dsum ? dpart : nullptr);
     GLV_LAUNCH_CHECK();
+    if (dsum) {
+        k_add_partials<<<1, kBlock, 0, s>>>(dpart, blocks, dsum);
+        GLV_LAUNCH_CHECK();
+    }
 }
 void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a, i64 *n,
                         hipStream_t s) {

@@ -103,10 +103,11 @@ bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *st
                               i64 *C1, double *a1, i64 *n1, const int32_t *colors);
 void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s);
 // stale (nullable): reset to 0 for every mover; dsum (nullable): += sum of
-// dein over the movers
+// dein over the movers, in a fixed order (dpart: kApplyMaxBlocks scratch)
+constexpr i64 kApplyMaxBlocks = 8192;
 void launch_apply_colored(const i64 *perm, const Slices &S, const i64 *tgt, i64 *C, const double *k,
                           double *a, i64 *n, hipStream_t s, uint8_t *stale = nullptr,
-                          const double *dein = nullptr, double *dsum = nullptr);
+                          const double *dein = nullptr, double *dsum = nullptr, double *dpart = nullptr);
 void launch_apply_plain(i64 nv, const i64 *tgt, const i64 *curr, const double *k, double *a, i64 *n,
                         hipStream_t s);
 void launch_init_comm_ass_opt(const DevGraph &g, const double *k, const double *a, const i64 *n,

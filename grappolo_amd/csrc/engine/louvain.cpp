@@ -194,7 +194,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         // small graphs: every round in one launch with pushed colours, the
         // entries of heavy vertices shared out over all workgroups
         const int nb = (int)std::min<i64>({(i64)jp_push_blocks(), kPersistMaxBlocks,
-                                           std::max<i64>(8, g.nnz / 65536)});
+                                           std::max<i64>(8, g.nnz / 1048576)});
         DBuf<uint32_t> forbid(nv * (kMaxColors / 32), s_);
         launch_fill_i32((int32_t *)forbid.p, nv * (kMaxColors / 32), 0, s_);
         launch_fill_i64(ring.p, 10, 0, s_);
@@ -322,7 +322,7 @@ static i64 mod_part_count(const PhaseSetup &P) {
 // With a learned active set (las), see LasParams.
 double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, double Lower,
                              double thresh, int *iters, std::vector<double> *trace, LasParams *las,
-                             int phase_index) {
+                             int phase_index, bool valid_colors) {
     const i64 nv = g.nv;
     PhaseSetup P;
     setup(g, colors, ncol, P);
@@ -402,7 +402,14 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     const i64 nkeys = (i64)kNumBins * ncol;
     DBuf<i64> start2(prune ? std::max<i64>(nkeys, ncol) + 1 : 0, s_);
     DBuf<i64> lcount(mode ? 2 : 0, s_);
-    DBuf<double> dein(prune ? nv : 0, s_), dsum(prune ? 1 : 0, s_);
+    // movers' internal-weight deltas: pruned sweeps, and the sweeps after the
+    // first with a valid coloring and no giant vertex (whose decision does not
+    // keep the winner's weight)
+    bool has_giant = false;
+    for (int c = 0; c < ncol; c++) has_giant = has_giant || P.hi(9, c) > P.lo(9, c);
+    const bool exact_incr = valid_colors && !has_giant && !prune;
+    const bool track = prune || exact_incr;
+    DBuf<double> dein(track ? nv : 0, s_), dsum(track ? 1 : 0, s_), dpart(track ? kApplyMaxBlocks : 0, s_);
     std::vector<i64> ptr2;
     LasView V;
     if (mode) {
@@ -476,8 +483,9 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
             las->full_sweeps++;
         }
         if (mode) launch_las_tick(stale.p, nv, s_);
-        const bool incremental = prune && !full && !persistent;
+        const bool incremental = ((prune && !full) || (exact_incr && numItrs > 1)) && !persistent;
         A.dein = incremental ? dein.p : nullptr;
+        if (incremental && !prune) launch_fill_f64(dsum.p, 1, 0.0, s_);
         bool done_iter = false;
         if (persistent) {
             launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
@@ -496,6 +504,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent sweep timed out");
                 const double c = P.constant;
                 currMod = h[0] * (double)c - h[1] * (double)c * (double)c;
+                exx = h[0];
                 done_iter = true;
             } else if (prune && !full) {
                 // cooperative launch refused: redo this pruned iteration as a full one
@@ -533,7 +542,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 }
                 Profiler::get().begin(K_APPLY, s_);
                 launch_apply_colored(plist, S, tgt.p, C, P.k.p, P.a.p, P.n.p, s_,
-                                     mode ? stale.p : nullptr, A.dein, incremental ? dsum.p : nullptr);
+                                     mode ? stale.p : nullptr, A.dein, incremental ? dsum.p : nullptr, dpart.p);
                 Profiler::get().end(K_APPLY, 24.0 * ncnt, s_);
             }
             if (incremental) {
@@ -848,6 +857,7 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
     DBuf<i64> Cbuf(NV, s_);
     double t0 = now();
     int nColors = color_for_run(g0, colors.p, p);
+    bool fresh_colors = true;  // colors were computed for the graph being swept
     r.t_coloring += now() - t0;
     r.colors_first = nColors;
 
@@ -868,7 +878,7 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
         const i64 swept0 = p.las ? p.las->swept_entries : 0;
         if (!nonColor) {
             currMod = colored_phase(*G, Cbuf.p, colors.p, nColors, currMod, p.C_threshold, &tmpItr,
-                                    &r.trace, p.las, phase - 1);
+                                    &r.trace, p.las, phase - 1, fresh_colors && p.coloring != 3);
             rec.colored = 1;
             rec.ncolors = nColors;
             r.edges_visited_mod += (i64)tmpItr * G->nnz;
@@ -908,9 +918,11 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
             r.t_building += rec.t_coarsen;
             launch_fill_i64(Cbuf.p, G->nv, -1, s_);
             phase++;
+            fresh_colors = false;
             if (G->nv > p.minGraphSize && !nonColor) {
                 t0 = now();
                 nColors = color_for_run(*G, colors.p, p);
+                fresh_colors = true;
                 rec.t_color = now() - t0;
                 r.t_coloring += rec.t_color;
             }

@@ -124,10 +124,14 @@ public:
     int color_for_run(const DevGraph &g, int32_t *colors, const RunParams &p);
     i64 count_conflicts(const DevGraph &g, const int32_t *colors);
 
-    // algoLouvainWithDistOneColoring: C (device, nv) receives the labels
+    // algoLouvainWithDistOneColoring: C (device, nv) receives the labels.
+    // valid_colors: the classes are independent sets of g (a fresh distance-1
+    // coloring), so a color step's moves change the internal weight by exactly
+    // the movers' 2 (e_vt - e_vs) and iterations after the first need no full
+    // modularity pass
     double colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, int ncol, double Lower,
                          double thresh, int *iters, std::vector<double> *trace = nullptr,
-                         LasParams *las = nullptr, int phase_index = 0);
+                         LasParams *las = nullptr, int phase_index = 0, bool valid_colors = false);
     // parallelLouvianMethod(NoMap): C receives pastCommAss
     double plain_phase(const DevGraph &g, i64 *C, double Lower, double thresh, bool nomap,
                        int *iters, std::vector<double> *trace = nullptr);
