@@ -21,47 +21,83 @@ __device__ inline u64 las_mix(u64 z) {
     return z ^ (z >> 31);
 }
 
-// B operand of layer 1 for vertex v (lanes of group 0: features 0-3, group 1: 4-7)
-__device__ inline mlp_s4 vertex_x(const LasView &S, i64 v, bool ok) {
-    const int g = (threadIdx.x % kWave) >> 4;
-    mlp_s4 x = {0, 0, 0, 0};
-    if (ok && g < 2) {
-        float f[4];
-        las_features4(S, v, g, f);
-#pragma unroll
-        for (int i = 0; i < 4; i++) x[i] = (short)bf16_bits(f[i]);
+// The network's input per vertex, for k_las_infer_x: the eight BF16 features
+// (X[2v]: features 0-3, X[2v+1]: 4-7; zero when v has no sweep record) and
+// flags (bit 0: v has a label, bit 1: a record, bit 2: giant degree).  One
+// thread per vertex, so the dependent loads of the features (the label, then
+// its community's size) overlap across many wavefronts.
+__global__ void k_las_xfeat(LasView S, uint2 *X, uint8_t *flags) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < S.nv; v += (i64)gridDim.x * blockDim.x) {
+        const bool has = S.C[v] >= 0;
+        const bool rec = has && S.feat[v].w != 0.0f;
+        const bool giant = has && S.off[v + 1] - S.off[v] > kGiantDeg;
+        uint2 x0 = make_uint2(0u, 0u), x1 = make_uint2(0u, 0u);
+        if (rec) {
+            float f[8];
+            las_features4(S, v, 0, f);
+            las_features4(S, v, 1, f + 4);
+            x0 = make_uint2((uint32_t)bf16_bits(f[0]) | ((uint32_t)bf16_bits(f[1]) << 16),
+                            (uint32_t)bf16_bits(f[2]) | ((uint32_t)bf16_bits(f[3]) << 16));
+            x1 = make_uint2((uint32_t)bf16_bits(f[4]) | ((uint32_t)bf16_bits(f[5]) << 16),
+                            (uint32_t)bf16_bits(f[6]) | ((uint32_t)bf16_bits(f[7]) << 16));
+        }
+        X[2 * v] = x0;
+        X[2 * v + 1] = x1;
+        flags[v] = (uint8_t)((has ? 1 : 0) | (rec ? 2 : 0) | (giant ? 4 : 0));
     }
-    return x;
 }
 
-__global__ __launch_bounds__(kLasBlock) void k_las_infer(LasView S, const float *P, float logit_tau,
-                                                         uint8_t *active, i64 *counts) {
+// Inference over the prepared inputs: a wavefront per tile of 16 vertices,
+// the next tile's inputs loaded while the current one goes through the three
+// MFMA layers.  A vertex is active when it has a label and either no record,
+// a giant degree or a logit at or above logit_tau.
+__global__ __launch_bounds__(kLasBlock) void k_las_infer_x(const uint2 *X, const uint8_t *flags, const i64 *off,
+                                                           i64 nv, const float *P, float logit_tau,
+                                                           uint8_t *active, i64 *counts) {
     __shared__ i64 s_cnt[2];
     if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     MlpW W;
     mlp_load(P, W);
-    const int lane = threadIdx.x % kWave, r = lane & 15;
+    const int lane = threadIdx.x % kWave, r = lane & 15, g = lane >> 4;
     const i64 wave = (i64)blockIdx.x * kLasWpb + threadIdx.x / kWave;
-    const i64 nwaves = (i64)gridDim.x * kLasWpb;
-    i64 na = 0, ne = 0;
-    for (i64 base = wave * 16; base < S.nv; base += nwaves * 16) {
+    const i64 step = (i64)gridDim.x * kLasWpb * 16;
+    auto load = [&](i64 base, mlp_s4 &x, uint8_t &f) {
         const i64 v = base + r;
-        const bool inb = v < S.nv;
-        const bool has = inb && S.C[v] >= 0;
-        const bool rec = has && S.feat[v].w != 0.0f;
+        x = mlp_s4{0, 0, 0, 0};
+        f = 0;
+        if (v < nv) {
+            if (g < 2) {
+                const uint2 u = X[2 * v + g];
+                x = mlp_s4{(short)(u.x & 0xFFFFu), (short)(u.x >> 16), (short)(u.y & 0xFFFFu), (short)(u.y >> 16)};
+            }
+            f = flags[v];
+        }
+    };
+    i64 na = 0, ne = 0;
+    i64 base = wave * 16;
+    mlp_s4 x;
+    uint8_t f;
+    load(base, x, f);
+    for (; base < nv; base += step) {
+        mlp_s4 xn;
+        uint8_t fn;
+        load(base + step, xn, fn);
         MlpAct A;
-        mlp_forward_tile(W, vertex_x(S, v, rec), A);
-        if (inb && lane < 16) {
+        mlp_forward_tile(W, x, A);
+        const i64 v = base + r;
+        if (v < nv && lane < 16) {
             // giant vertices always sweep: their HBM tables are laid out for the full class
-            const bool giant = has && S.off[v + 1] - S.off[v] > kGiantDeg;
+            const bool has = f & 1, rec = f & 2, giant = f & 4;
             const bool act = has && (!rec || giant || A.z >= logit_tau);
             active[v] = act ? 1 : 0;
             if (act) {
                 na++;
-                ne += S.off[v + 1] - S.off[v];
+                ne += off[v + 1] - off[v];
             }
         }
+        x = xn;
+        f = fn;
     }
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
@@ -155,9 +191,14 @@ __global__ void k_compact_starts(const i64 *start, i64 nkeys, const i64 *flags, 
 void launch_las_infer(const LasView &S, const float *P, float logit_tau, uint8_t *active, i64 *counts,
                       hipStream_t s) {
     if (S.nv <= 0) return;
+    // inputs first (one thread per vertex), then the network over them
+    DBuf<uint2> X(2 * S.nv, s);
+    DBuf<uint8_t> flags(S.nv, s);
+    k_las_xfeat<<<grid_for(S.nv, kBlock, 8192), kBlock, 0, s>>>(S, X.p, flags.p);
+    GLV_LAUNCH_CHECK();
     const i64 tiles = (S.nv + 15) / 16;
     const unsigned blocks = (unsigned)std::min<i64>((tiles + kLasWpb - 1) / kLasWpb, 4096);
-    k_las_infer<<<blocks, kLasBlock, 0, s>>>(S, P, logit_tau, active, counts);
+    k_las_infer_x<<<blocks, kLasBlock, 0, s>>>(X.p, flags.p, S.off, S.nv, P, logit_tau, active, counts);
     GLV_LAUNCH_CHECK();
 }
 
