@@ -394,7 +394,12 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
 
     // learned active set: recorded features, sweeps since the last move,
     // active flags and the compacted sweep lists
-    const int mode = las ? las->mode : 0;
+    // Pruning skips the phases of the dense persistent sweep: there a color
+    // step costs its heaviest vertex and a grid barrier, so sweeping fewer
+    // vertices saves nothing while the re-check costs an iteration; such a
+    // phase runs as the plain form (and counts as full sweeps).
+    const bool las_off = las && las->mode == 2 && nv <= kDenseMaxNV;
+    const int mode = las && !las_off ? las->mode : 0;
     const bool prune = mode == 2;
     DBuf<float4> feat(mode ? nv : 0, s_);
     DBuf<uint8_t> stale(mode ? nv : 0, s_), active(prune ? nv : 0, s_);
@@ -577,6 +582,11 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         prevMod = currMod;
     }
     if (mode) sync();
+    if (las_off) {
+        las->swept_vertices += (i64)numItrs * nonisolated;
+        las->swept_entries += (i64)numItrs * g.nnz;
+        las->full_sweeps += numItrs;
+    }
     if (iters) *iters = numItrs;
     return prevMod;  // :284
 }
