@@ -273,13 +273,42 @@ __global__ void k_slice_nnz(const i64 *start, const i64 *dscan, const i64 *degp,
 
 // sumVertexDegree: one thread per row sums in row order (the reference's
 // order) for rows up to 2048 entries; longer rows use a workgroup each.
-__global__ void k_degree_rows(const i64 *off, const double *w, i64 nv, double *k) {
+// k[v] = its weights summed in adjacency order (vertices of at most 2048
+// entries; the rest go to k_degree_big), one thread per row: for graphs of
+// high mean degree.
+__global__ void k_degree_rows_seq(const i64 *off, const double *w, i64 nv, double *k) {
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nv; i += (i64)gridDim.x * blockDim.x) {
-        i64 b = off[i], e = off[i + 1];
+        const i64 b = off[i], e = off[i + 1];
         if (e - b > kBinMaxDeg[7]) continue;
         double s = 0;
         for (i64 j = b; j < e; j++) s += w[j];
         k[i] = s;
+    }
+}
+// The same for graphs of low mean degree: a workgroup takes kBlock
+// consecutive rows, streams their entries through LDS in coalesced tiles and
+// each thread adds its row's part of every tile in order.
+constexpr int kDegTile = 2048;
+constexpr i64 kDegTiledMean = 64;  // mean degree up to which the tiled form is used
+__global__ __launch_bounds__(kBlock) void k_degree_rows(const i64 *off, const double *w, i64 nv, double *k) {
+    __shared__ double tile[kDegTile];
+    for (i64 r0 = (i64)blockIdx.x * kBlock; r0 < nv; r0 += (i64)gridDim.x * kBlock) {
+        const i64 r = r0 + threadIdx.x;
+        const i64 rb = r < nv ? off[r] : 0, re = r < nv ? off[r + 1] : 0;
+        const bool mine = r < nv && re - rb <= kBinMaxDeg[7];
+        const i64 e0 = off[r0], e1 = off[r0 + kBlock < nv ? r0 + kBlock : nv];
+        double acc = 0;
+        for (i64 t0 = e0; t0 < e1; t0 += kDegTile) {
+            const i64 t1 = t0 + kDegTile < e1 ? t0 + kDegTile : e1;
+            __syncthreads();  // the previous tile is read
+            for (i64 j = t0 + threadIdx.x; j < t1; j += kBlock) tile[j - t0] = w[j];
+            __syncthreads();
+            if (mine) {
+                const i64 lo = rb > t0 ? rb : t0, hi = re < t1 ? re : t1;
+                for (i64 j = lo; j < hi; j++) acc += tile[j - t0];
+            }
+        }
+        if (mine) k[r] = acc;
     }
 }
 __global__ void k_degree_big(const i64 *off, const double *w, const i64 *verts, i64 count, double *k) {
@@ -3086,7 +3115,10 @@ void launch_slices(const uint32_t *sorted_keys, const i64 *perm, const i64 *off,
 }
 void launch_degree(const DevGraph &g, const i64 *big, i64 nbig, double *k, hipStream_t s) {
     if (g.nv <= 0) return;
-    k_degree_rows<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.w, g.nv, k);
+    if (g.nnz <= kDegTiledMean * g.nv)
+        k_degree_rows<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.w, g.nv, k);
+    else
+        k_degree_rows_seq<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.w, g.nv, k);
     GLV_LAUNCH_CHECK();
     if (nbig > 0) {
         k_degree_big<<<(unsigned)std::min<i64>(nbig, 4096), kBlock, 0, s>>>(g.off, g.w, big, nbig, k);
