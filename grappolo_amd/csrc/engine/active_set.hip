@@ -23,14 +23,17 @@ __device__ inline u64 las_mix(u64 z) {
 
 // The network's input per vertex, for k_las_infer_x: the eight BF16 features
 // (X[2v]: features 0-3, X[2v+1]: 4-7; zero when v has no sweep record) and
-// flags (bit 0: v has a label, bit 1: a record, bit 2: giant degree).  One
-// thread per vertex, so the dependent loads of the features (the label, then
-// its community's size) overlap across many wavefronts.
-__global__ void k_las_xfeat(LasView S, uint2 *X, uint8_t *flags) {
+// flags (bit 0: v has a label, bit 1: a record, bit 2: giant degree) and
+// its entry count (capped at 2^31 - 1).  One thread per vertex, so the
+// dependent loads of the features (the label, then its community's size)
+// overlap across many wavefronts.
+__global__ void k_las_xfeat(LasView S, uint2 *X, uint8_t *flags, int32_t *deg) {
     for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < S.nv; v += (i64)gridDim.x * blockDim.x) {
+        const i64 d = S.off[v + 1] - S.off[v];
+        deg[v] = (int32_t)(d < INT32_MAX ? d : INT32_MAX);
         const bool has = S.C[v] >= 0;
         const bool rec = has && S.feat[v].w != 0.0f;
-        const bool giant = has && S.off[v + 1] - S.off[v] > kGiantDeg;
+        const bool giant = has && d > kGiantDeg;
         uint2 x0 = make_uint2(0u, 0u), x1 = make_uint2(0u, 0u);
         if (rec) {
             float f[8];
@@ -51,9 +54,9 @@ __global__ void k_las_xfeat(LasView S, uint2 *X, uint8_t *flags) {
 // the next tile's inputs loaded while the current one goes through the three
 // MFMA layers.  A vertex is active when it has a label and either no record,
 // a giant degree or a logit at or above logit_tau.
-__global__ __launch_bounds__(kLasBlock) void k_las_infer_x(const uint2 *X, const uint8_t *flags, const i64 *off,
-                                                           i64 nv, const float *P, float logit_tau,
-                                                           uint8_t *active, i64 *counts) {
+__global__ __launch_bounds__(kLasBlock) void k_las_infer_x(const uint2 *X, const uint8_t *flags,
+                                                           const int32_t *deg, i64 nv, const float *P,
+                                                           float logit_tau, uint8_t *active, i64 *counts) {
     __shared__ i64 s_cnt[2];
     if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -62,27 +65,31 @@ __global__ __launch_bounds__(kLasBlock) void k_las_infer_x(const uint2 *X, const
     const int lane = threadIdx.x % kWave, r = lane & 15, g = lane >> 4;
     const i64 wave = (i64)blockIdx.x * kLasWpb + threadIdx.x / kWave;
     const i64 step = (i64)gridDim.x * kLasWpb * 16;
-    auto load = [&](i64 base, mlp_s4 &x, uint8_t &f) {
+    auto load = [&](i64 base, mlp_s4 &x, uint8_t &f, int32_t &d) {
         const i64 v = base + r;
         x = mlp_s4{0, 0, 0, 0};
         f = 0;
+        d = 0;
         if (v < nv) {
             if (g < 2) {
                 const uint2 u = X[2 * v + g];
                 x = mlp_s4{(short)(u.x & 0xFFFFu), (short)(u.x >> 16), (short)(u.y & 0xFFFFu), (short)(u.y >> 16)};
             }
             f = flags[v];
+            d = deg[v];
         }
     };
     i64 na = 0, ne = 0;
     i64 base = wave * 16;
     mlp_s4 x;
     uint8_t f;
-    load(base, x, f);
+    int32_t d;
+    load(base, x, f, d);
     for (; base < nv; base += step) {
         mlp_s4 xn;
         uint8_t fn;
-        load(base + step, xn, fn);
+        int32_t dn;
+        load(base + step, xn, fn, dn);
         MlpAct A;
         mlp_forward_tile(W, x, A);
         const i64 v = base + r;
@@ -93,11 +100,12 @@ __global__ __launch_bounds__(kLasBlock) void k_las_infer_x(const uint2 *X, const
             active[v] = act ? 1 : 0;
             if (act) {
                 na++;
-                ne += off[v + 1] - off[v];
+                ne += d;
             }
         }
         x = xn;
         f = fn;
+        d = dn;
     }
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
@@ -194,11 +202,12 @@ void launch_las_infer(const LasView &S, const float *P, float logit_tau, uint8_t
     // inputs first (one thread per vertex), then the network over them
     DBuf<uint2> X(2 * S.nv, s);
     DBuf<uint8_t> flags(S.nv, s);
-    k_las_xfeat<<<grid_for(S.nv, kBlock, 8192), kBlock, 0, s>>>(S, X.p, flags.p);
+    DBuf<int32_t> deg(S.nv, s);
+    k_las_xfeat<<<grid_for(S.nv, kBlock, 8192), kBlock, 0, s>>>(S, X.p, flags.p, deg.p);
     GLV_LAUNCH_CHECK();
     const i64 tiles = (S.nv + 15) / 16;
     const unsigned blocks = (unsigned)std::min<i64>((tiles + kLasWpb - 1) / kLasWpb, 4096);
-    k_las_infer_x<<<blocks, kLasBlock, 0, s>>>(X.p, flags.p, S.off, S.nv, P, logit_tau, active, counts);
+    k_las_infer_x<<<blocks, kLasBlock, 0, s>>>(X.p, flags.p, deg.p, S.nv, P, logit_tau, active, counts);
     GLV_LAUNCH_CHECK();
 }
 
