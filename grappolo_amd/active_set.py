@@ -6,7 +6,8 @@ vertex: its features after one sweep and whether the next sweep moves it --
 and then predicts, at the start of every iteration after the first of a
 colored phase, which vertices will not move; those are not swept.  A phase
 ends only when an iteration over every vertex gains less than the threshold,
-so the unpruned method is both the fallback and the judge.  There is no
+so the unpruned method is both the fallback and the arbiter of the
+modularity reached.  There is no
 counterpart in the reference.
 
 Features of a vertex (8 BF16 values, computed on the device):

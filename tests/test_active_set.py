@@ -1,6 +1,6 @@
 """Learned active-set predictor: the MLP oracle pinned by calculus, the GPU
 kernels against the oracle, and the pruned Louvain run against the unpruned
-one (the fallback and judge).  The MLP has no counterpart in the reference."""
+one (the fallback and arbiter).  The MLP has no counterpart in the reference."""
 import numpy as np
 import pytest
 
