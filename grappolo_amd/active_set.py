@@ -7,8 +7,9 @@ and then predicts, at the start of every iteration after the first of a
 colored phase, which vertices will not move; those are not swept.  A phase
 ends only when an iteration over every vertex gains less than the threshold,
 so the unpruned method is both the fallback and the arbiter of the
-modularity reached.  There is no
-counterpart in the reference.
+modularity reached.  Phases of at most 15360 vertices (the dense persistent
+sweep, where a color step costs its heaviest vertex and a grid barrier
+whatever is pruned) run unpruned.  There is no counterpart in the reference.
 
 Features of a vertex (8 BF16 values, computed on the device):
   0 log2(1 + entries)/10, 1 weight to own community / degree,
