@@ -1,5 +1,6 @@
 // Device block cache behind DBuf (declared in common.hpp).
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -87,6 +88,34 @@ void block_trim() {
             ++it;
         }
     }
+}
+
+void Readback::run() {
+    if (items_.empty()) return;
+    thread_local char *stage = nullptr;  // pinned, grows; kept for the thread's life
+    thread_local size_t stage_cap = 0;
+    size_t total = 0;
+    for (const Item &it : items_) total += (it.bytes + 15) / 16 * 16;
+    if (total > stage_cap) {
+        if (stage) (void)hipHostFree(stage);
+        stage = nullptr;
+        stage_cap = 0;
+        const size_t want = std::max<size_t>(total, 4096);
+        GLV_HIP(hipHostMalloc((void **)&stage, want, hipHostMallocDefault));
+        stage_cap = want;
+    }
+    size_t off = 0;
+    for (const Item &it : items_) {
+        if (it.bytes) GLV_HIP(hipMemcpyAsync(stage + off, it.src, it.bytes, hipMemcpyDeviceToHost, s_));
+        off += (it.bytes + 15) / 16 * 16;
+    }
+    GLV_HIP(hipStreamSynchronize(s_));
+    off = 0;
+    for (const Item &it : items_) {
+        if (it.bytes) std::memcpy(it.dst, stage + off, it.bytes);
+        off += (it.bytes + 15) / 16 * 16;
+    }
+    items_.clear();
 }
 
 }  // namespace glv

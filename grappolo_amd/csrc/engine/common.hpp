@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "glv_status.h"
 
@@ -116,6 +117,24 @@ struct Error {
 void *block_get(size_t bytes, hipStream_t s, size_t *cap);
 void block_put(void *p, size_t cap, hipStream_t s);
 void block_trim();
+
+// Device-to-host reads gathered behind one synchronisation: the copies go
+// through a pinned staging buffer of the calling thread (into pageable
+// memory every copy would wait on its own), then to their destinations.
+class Readback {
+  public:
+    explicit Readback(hipStream_t s) : s_(s) {}
+    void add(void *dst, const void *src, size_t bytes) { items_.push_back(Item{dst, src, bytes}); }
+    void run();  // enqueue every copy, synchronise the stream, copy out
+  private:
+    struct Item {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    hipStream_t s_;
+    std::vector<Item> items_;
+};
 
 // Stream-ordered device buffer on the block cache, so per-phase scratch
 // reaches neither the driver nor the pool's bookkeeping after warm-up.  A

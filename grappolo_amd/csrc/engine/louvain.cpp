@@ -41,8 +41,9 @@ constexpr int kMaxPhaseIters = 100000;
 template <class T>
 T Engine::fetch(const T *dptr) {
     T h;
-    GLV_HIP(hipMemcpyAsync(&h, dptr, sizeof(T), hipMemcpyDeviceToHost, s_));
-    GLV_HIP(hipStreamSynchronize(s_));
+    Readback rb(s_);
+    rb.add(&h, dptr, sizeof(T));
+    rb.run();
     return h;
 }
 
@@ -68,12 +69,13 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
     P.ptr.assign(nkeys + 1, 0);
     P.pnnz.assign(nkeys, 0);
     int32_t e = 0;
-    GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s_));
+    Readback rb(s_);
+    rb.add(&e, err.p, sizeof(e));
     if (nv > 0) {
-        GLV_HIP(hipMemcpyAsync(P.ptr.data(), start, (nkeys + 1) * sizeof(i64), hipMemcpyDeviceToHost, s_));
-        GLV_HIP(hipMemcpyAsync(P.pnnz.data(), pnnz.p, nkeys * sizeof(i64), hipMemcpyDeviceToHost, s_));
+        rb.add(P.ptr.data(), start, (nkeys + 1) * sizeof(i64));
+        rb.add(P.pnnz.data(), pnnz.p, nkeys * sizeof(i64));
     }
-    GLV_HIP(hipStreamSynchronize(s_));
+    rb.run();
     if (e) fail(GLV_E_INVALID_ARG, "color values out of range [0, nColors)");
     // degrees and 1/(2m)
     const i64 big_lo = P.bin_lo(8), big_hi = P.bin_hi(9);
@@ -174,9 +176,10 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
                                                         colored, prio.p, code.p, kJpPersistExit, pnb, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
         unsigned abort_flag = 0;
-        GLV_HIP(hipMemcpyAsync(h, ring.p, sizeof(h), hipMemcpyDeviceToHost, s_));
-        GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
-        GLV_HIP(hipStreamSynchronize(s_));  // `h` is pageable host memory
+        Readback rb(s_);
+        rb.add(h, ring.p, sizeof(h));
+        rb.add(&abort_flag, bar.p, sizeof(unsigned));
+        rb.run();
         if (!ok) return 0;
         if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
         const int r = (int)h[6];
@@ -206,9 +209,10 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         if (ok) {
             i64 h[10];
             unsigned abort_flag = 0;
-            GLV_HIP(hipMemcpyAsync(h, ring.p, sizeof(h), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipStreamSynchronize(s_));
+            Readback rb(s_);
+            rb.add(h, ring.p, sizeof(h));
+            rb.add(&abort_flag, bar.p, sizeof(unsigned));
+            rb.run();
             if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
             round = (int)h[6];
             colored = h[8];
@@ -230,8 +234,9 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
             Profiler::get().begin(K_COLOR, s_);
             launch_jp_count(g, cnt.p, cur, next.small, cur.counts + 3, s_);  // heavy list in the next round's buffer
             Profiler::get().end(K_COLOR, 0.0, s_);
-            GLV_HIP(hipMemcpyAsync(hc, cur.counts, sizeof(hc), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipStreamSynchronize(s_));
+            Readback rb(s_);
+            rb.add(hc, cur.counts, sizeof(hc));
+            rb.run();
         }
         while (hc[0] + hc[1] + hc[2] > 0) {
             if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
@@ -253,8 +258,9 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
             Profiler::get().begin(K_COLOR, s_);
             launch_jp_frontier(g, colors, cnt.p, cur, hc, next, gmark.p, err.p, s_);
             Profiler::get().end(K_COLOR, 0.0, s_);
-            GLV_HIP(hipMemcpyAsync(hc, next.counts, sizeof(hc), hipMemcpyDeviceToHost, s_));
-            GLV_HIP(hipStreamSynchronize(s_));
+            Readback rb(s_);
+            rb.add(hc, next.counts, sizeof(hc));
+            rb.run();
             colored += won;
             std::swap(cur, next);
             round++;
@@ -304,8 +310,9 @@ double Engine::colored_modularity(const DevGraph &g, const PhaseSetup &P, const 
     launch_sumsq(P.a.p, g.nv, part + off, res + 1, s_);
     Profiler::get().end(K_MOD, 24.0 * g.nnz + 16.0 * g.nv, s_);
     double h[2];
-    GLV_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s_));
-    GLV_HIP(hipStreamSynchronize(s_));
+    Readback rb(s_);
+    rb.add(h, res, sizeof(h));
+    rb.run();
     if (exx) *exx = h[0];
     const double c = P.constant;
     return h[0] * (double)c - h[1] * (double)c * (double)c;
@@ -461,8 +468,9 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 launch_compact_active(cperm.p, nv, active.p, cstart.p, ncol, flags.p, fpos.p, list2.p,
                                       start2.p, s_);
                 i64 hc[2];
-                GLV_HIP(hipMemcpyAsync(hc, lcount.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipStreamSynchronize(s_));
+                Readback rb(s_);
+                rb.add(hc, lcount.p, sizeof(hc));
+                rb.run();
                 las->swept_vertices += hc[0];
                 las->swept_entries += hc[1];
                 pstart = start2.p;
@@ -471,10 +479,10 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                                       start2.p, s_);
                 ptr2.resize(nkeys + 1);
                 i64 hc[2];
-                GLV_HIP(hipMemcpyAsync(ptr2.data(), start2.p, (nkeys + 1) * sizeof(i64),
-                                       hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipMemcpyAsync(hc, lcount.p, sizeof(hc), hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipStreamSynchronize(s_));
+                Readback rb(s_);
+                rb.add(ptr2.data(), start2.p, (nkeys + 1) * sizeof(i64));
+                rb.add(hc, lcount.p, sizeof(hc));
+                rb.run();
                 las->swept_vertices += hc[0];
                 las->swept_entries += hc[1];
                 hptr = &ptr2;
@@ -503,9 +511,10 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 launch_sum_partials(ppart.p + pblocks, pblocks, res.p + 1, s_);
                 double h[2];
                 unsigned abort_flag = 0;
-                GLV_HIP(hipMemcpyAsync(h, res.p, sizeof(h), hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipMemcpyAsync(&abort_flag, bar.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipStreamSynchronize(s_));
+                Readback rb(s_);
+                rb.add(h, res.p, sizeof(h));
+                rb.add(&abort_flag, bar.p, sizeof(unsigned));
+                rb.run();
                 if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent sweep timed out");
                 const double c = P.constant;
                 currMod = h[0] * (double)c - h[1] * (double)c * (double)c;
@@ -555,9 +564,10 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 // internal weight changes by the sum of the movers' 2(e_yt - e_ys)
                 launch_sumsq(P.a.p, nv, part.p, res.p + 1, s_);
                 double h[2];
-                GLV_HIP(hipMemcpyAsync(&h[0], dsum.p, sizeof(double), hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipMemcpyAsync(&h[1], res.p + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
-                GLV_HIP(hipStreamSynchronize(s_));
+                Readback rb(s_);
+                rb.add(&h[0], dsum.p, sizeof(double));
+                rb.add(&h[1], res.p + 1, sizeof(double));
+                rb.run();
                 exx += h[0];
                 const double c = P.constant;
                 currMod = exx * (double)c - h[1] * (double)c * (double)c;
@@ -754,11 +764,12 @@ i64 Engine::renumber(i64 *C, i64 n) {
     launch_relabel(C, n, first.p, pos.p, s_);
     Profiler::get().end(K_RENUMBER, 16.0 * n, s_);
     i64 last[2];
-    GLV_HIP(hipMemcpyAsync(&last[0], pos.p + n - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
-    GLV_HIP(hipMemcpyAsync(&last[1], flags.p + n - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
     int32_t e = 0;
-    GLV_HIP(hipMemcpyAsync(&e, err.p, sizeof(e), hipMemcpyDeviceToHost, s_));
-    GLV_HIP(hipStreamSynchronize(s_));
+    Readback rb(s_);
+    rb.add(&last[0], pos.p + n - 1, sizeof(i64));
+    rb.add(&last[1], flags.p + n - 1, sizeof(i64));
+    rb.add(&e, err.p, sizeof(e));
+    rb.run();
     if (e) fail(GLV_E_INVALID_ARG, "cluster id >= number of vertices");
     return last[0] + last[1];
 }
@@ -793,9 +804,10 @@ void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out, bool
     i64 m = 0;
     if (nv > 0) {
         i64 h[2];
-        GLV_HIP(hipMemcpyAsync(&h[0], pos.p + nv - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
-        GLV_HIP(hipMemcpyAsync(&h[1], cnt.p + nv - 1, sizeof(i64), hipMemcpyDeviceToHost, s_));
-        GLV_HIP(hipStreamSynchronize(s_));
+        Readback rb(s_);
+        rb.add(&h[0], pos.p + nv - 1, sizeof(i64));
+        rb.add(&h[1], cnt.p + nv - 1, sizeof(i64));
+        rb.run();
         m = h[0] + h[1];
     }
     const i64 tot = m + (vf ? 0 : k);

@@ -307,3 +307,28 @@ def test_invalid_inputs_raise():
     C[0] = 10 ** 6
     with pytest.raises(engine.GlvError):
         engine.modularity(g, C)
+
+
+@pytest.mark.parametrize("name", ["lfr_w", "hubs_big", "hubs_giant"])
+def test_movers_modularity_update_equals_full_passes(name):
+    """A run's colored phase with a fresh coloring updates the internal weight
+    from the movers' 2 (e_vt - e_vs) after its first iteration; the stage
+    entry point (caller's colors, a full modularity pass every iteration)
+    gives the same number of iterations and the same modularity: exactly on
+    integer weights, to 1e-12 relative otherwise.  With giant vertices every
+    iteration takes the full pass."""
+    import torch
+    g = GRAPHS[name]
+    dg = engine.DeviceGraph(g)
+    colors = torch.empty(g.numVertices, dtype=torch.int32, device="cuda")
+    mc, _ = engine.color_dev(dg, colors)
+    C1 = torch.full((g.numVertices,), -1, dtype=torch.int64, device="cuda")
+    q1, it1 = engine.colored_phase_dev(dg, C1, colors, mc + 1, -1.0, 1e-6)
+    C2 = torch.full((g.numVertices,), -1, dtype=torch.int64, device="cuda")
+    st = engine.run_colored_dev(dg, C2, minGraphSize=100, threshold=1e-9, C_threshold=1e-6)
+    assert int(st["per_phase"]["phase_iters"][0]) == it1
+    q2 = float(st["per_phase"]["phase_mod"][0])
+    if unit_weights(g):
+        assert q2 == q1
+    else:
+        assert abs(q2 - q1) <= 1e-12 * abs(q1), (q1, q2)
