@@ -922,6 +922,47 @@ __device__ inline void block_argmax_p(bool &cand, double &gain, i64 &id, int *sc
     id = d;
 }
 
+// block_argmax_p that also carries `nid` (the size of the candidate community)
+__device__ inline void block_argmax_pn(bool &cand, double &gain, i64 &id, i64 &nid, int *sc, double *sg, i64 *si,
+                                       i64 *sn) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double og = __shfl_xor(gain, o);
+        const long long oid = __shfl_xor((long long)id, o);
+        const long long on = __shfl_xor((long long)nid, o);
+        const int ocd = __shfl_xor((int)cand, o);
+        if (better(ocd != 0, og, oid, cand, gain, id)) {
+            cand = true;
+            gain = og;
+            id = oid;
+            nid = on;
+        }
+    }
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    __syncthreads();
+    if (l == 0) {
+        sc[w] = cand;
+        sg[w] = gain;
+        si[w] = id;
+        sn[w] = nid;
+    }
+    __syncthreads();
+    bool c = sc[0] != 0;
+    double g = sg[0];
+    i64 d = si[0], nn = sn[0];
+    for (int q = 1; q < kPBlock / kWave; q++)
+        if (better(sc[q] != 0, sg[q], si[q], c, g, d)) {
+            c = true;
+            g = sg[q];
+            d = si[q];
+            nn = sn[q];
+        }
+    cand = c;
+    gain = g;
+    id = d;
+    nid = nn;
+}
+
 // Grid barrier over nb <= 64 workgroups without a shared counter: workgroup
 // b publishes the barrier's sequence number in its own flag slot (one cache
 // line each) with an agent-scope release, and the first wavefront of every
@@ -1403,9 +1444,9 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
     __shared__ int sc[kPBlock / kWave];
     __shared__ double sg[kPBlock / kWave];
     __shared__ i64 si[kPBlock / kWave];
+    __shared__ i64 sn[kPBlock / kWave];
     __shared__ int s_cnt;                // communities listed for the gathered vertex
     __shared__ double s_sl;              // its self-loop weight
-    __shared__ i64 s_nid;                // size of the chosen community
     __shared__ int s_fn;                 // set-aside entries (above kSpecFix: too many)
     __shared__ uint16_t s_ft[kSpecFix];  // their neighbours
     __shared__ double s_fw[kSpecFix];    // and weights
@@ -1642,17 +1683,13 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
                         }
                     }
                 }
-                const bool my_cand = cand;
-                const i64 my_id = id;
-                block_argmax_p(cand, gain, id, sc, sg, si);
+                block_argmax_pn(cand, gain, id, nid, sc, sg, si, sn);
                 if (A.feat) block_top2<kPBlock>(t1, t2);
-                if (cand && my_cand && my_id == id) s_nid = nid;  // the lane that scored the winner
-                __syncthreads();
                 if (threadIdx.x == 0) {
                     i64 target = s;
                     if (cand) {
                         target = id;
-                        if (ns == 1 && target > s && s_nid == 1) target = s;
+                        if (ns == 1 && target > s && nid == 1) target = s;
                     }
                     A.tgt[v] = target;
                     if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
