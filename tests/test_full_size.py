@@ -1,0 +1,136 @@
+"""The LFR-style workload at its full size (4,194,304 vertices, degree exponent
+2.5 on [10, 1000], community exponent 1.5 on [20, 5000], mixing 0.4, weights
+U[0.5, 2], seed 2: 103,084,214 entries), through what holds at any size:
+
+* the distance-1 coloring is proper, first-fit (every colour below a
+  vertex's is held by one of its neighbours) and, but for the at most 1024
+  vertices the sequential tail colours, the first free colour among its
+  higher-priority neighbours (larger degree, then the oracle's hash, then
+  the larger id);
+* the run's labels are community ids of its last phase (below the reported
+  cluster count; fewer may occur: after a colored phase that gains too
+  little, the reference runs the plain phase on the same graph and composes
+  C_orig through the earlier phase's cluster ids, runMultiPhaseColoring.cpp
+  :131-143,202-206);
+* coarsening preserves the total weight and the modularity: the first
+  phase's labels on the input graph have the modularity of the singletons
+  of the coarse graph, up to the summation order of FP64 weights (1e-12
+  relative);
+* the device's modularity of a labelling equals the oracle's (1e-12
+  relative), and the whole run reaches more than its first phase alone.
+
+The oracle's own coloring of this graph takes minutes on one core; its
+checksums are in golden/full_size_coloring.json (written by
+golden/make_full_size_coloring.py), and the device's coloring must hash to
+the same bytes.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+JP_TAIL = 1024  # vertices coloured by the sequential tail at most
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "full_size_coloring.json")
+
+
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def coloring_defects(g, colors):
+    """(vertices breaking first-fit over all neighbours, vertices breaking
+    first-fit over their higher-priority neighbours)."""
+    nv = g.numVertices
+    deg = np.diff(g.edgeListPtrs)
+    rows = np.repeat(np.arange(nv, dtype=np.int64), deg)
+    tails = g.tails
+    colors = np.asarray(colors, dtype=np.int64)
+    cu, cv = colors[tails], colors[rows]
+    h = np.fromiter((oracle.jp_priority(v) for v in range(nv)), dtype=np.uint64, count=nv)
+    key = (np.minimum(deg, (1 << 23) - 1).astype(np.uint64) << np.uint64(40)) | (h >> np.uint64(24))
+    ku, kv = key[tails], key[rows]
+    higher = (ku > kv) | ((ku == kv) & (tails > rows))
+    below = (cu < cv) & (tails != rows)
+
+    def short(sel):
+        pairs = np.unique(rows[sel] * 4096 + cu[sel])
+        held = np.bincount(pairs // 4096, minlength=nv)
+        return int(np.count_nonzero(held != colors))
+
+    return short(below), short(below & higher)
+
+
+@pytest.fixture(scope="module")
+def lfr4m():
+    import torch
+    from grappolo_amd import engine
+    from grappolo_amd.graph import gen_lfr
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    print("[full size] generating LFR 4,194,304 vertices, seed 2", flush=True)
+    g = gen_lfr(4194304, seed=2)
+    assert g.numVertices == 4194304 and g.nnz == 103084214
+    with open(GOLDEN) as f:
+        gold = json.load(f)
+    assert sha(g.edgeListPtrs, g.tails, g.weights) == gold["graph_sha256"]
+    return g, engine.DeviceGraph(g, "cuda")
+
+
+def test_full_size_coloring_properties(lfr4m):
+    import torch
+    from grappolo_amd import engine
+    g, dg = lfr4m
+    colors = torch.empty(g.numVertices, dtype=torch.int32, device="cuda")
+    mc, rounds = engine.color_dev(dg, colors)
+    assert engine.count_conflicts_dev(dg, colors) == 0
+    got = colors.cpu().numpy()
+    assert got.min() >= 0 and got.max() == mc
+    with open(GOLDEN) as f:
+        gold = json.load(f)
+    assert mc == gold["max_color"]
+    assert np.bincount(got, minlength=mc + 1).tolist() == gold["class_sizes"]
+    assert sha(got.astype(np.int32)) == gold["colors_int32_sha256"]
+    assert oracle.color_conflicts(g, got) == 0
+    grundy, jp = coloring_defects(g, got)
+    assert grundy == 0
+    assert jp <= JP_TAIL
+
+
+def test_full_size_run_invariants(lfr4m):
+    import torch
+    from grappolo_amd import engine
+    g, dg = lfr4m
+    C = torch.full((g.numVertices,), -1, dtype=torch.int64, device="cuda")
+    st = engine.run_colored_dev(dg, C, coloring=1, minGraphSize=1000, threshold=1e-9, C_threshold=1e-6)
+    torch.cuda.synchronize()
+    k = st["final_clusters"]
+    labels = C.cpu().numpy()
+    # every vertex has entries (degrees from 10), so every label is a community id
+    assert labels.min() >= 0 and labels.max() < k
+    assert np.unique(labels).size <= k
+    q_dev = engine.modularity_dev(dg, C)
+    q_ora = oracle.modularity(g, labels)
+    assert abs(q_dev - q_ora) <= 1e-12 * abs(q_ora)
+    # the first phase and its coarse graph
+    colors = torch.empty(g.numVertices, dtype=torch.int32, device="cuda")
+    mc, _ = engine.color_dev(dg, colors)
+    C1 = torch.full((g.numVertices,), -1, dtype=torch.int64, device="cuda")
+    engine.colored_phase_dev(dg, C1, colors, mc + 1, -1.0, 1e-6)
+    k1 = engine.renumber_dev(C1)
+    q_fine = engine.modularity_dev(dg, C1)
+    assert abs(q_fine - oracle.modularity(g, C1.cpu().numpy())) <= 1e-12 * abs(q_fine)
+    g2 = engine.coarsen_dev(dg, C1, k1)
+    assert g2.numVertices == k1
+    assert abs(g2.weights.sum() - g.weights.sum()) <= 1e-12 * g.weights.sum()
+    dg2 = engine.DeviceGraph(g2, "cuda")
+    q_coarse = engine.modularity_dev(dg2, torch.arange(k1, dtype=torch.int64, device="cuda"))
+    assert abs(q_coarse - q_fine) <= 1e-12 * abs(q_fine)
+    assert st["final_modularity"] > q_fine
