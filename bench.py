@@ -100,7 +100,7 @@ def cpu_baseline(spec):
 
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "lfr4m_colored_louvain_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
