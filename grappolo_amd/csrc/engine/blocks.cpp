@@ -64,10 +64,14 @@ void *block_get(size_t bytes, hipStream_t s, size_t *cap) {
     return p;
 }
 
+// Blocks are cached only under the engine's own streams, which live until
+// shutdown: a caller's stream may be destroyed while blocks wait on its free
+// list, so a block released on it goes back to the pool at once (the stream
+// is alive while the call that used it runs).
 void block_put(void *p, size_t cap, hipStream_t s) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(g_blocks_mu);
-    if (g_cached + cap > kCacheLimit) {
+    if (!is_engine_stream(s) || g_cached + cap > kCacheLimit) {
         (void)hipFreeAsync(p, s);
         return;
     }
@@ -75,14 +79,20 @@ void block_put(void *p, size_t cap, hipStream_t s) {
     g_cached += cap;
 }
 
-void block_trim() {
-    std::lock_guard<std::mutex> lk(g_blocks_mu);
-    int dev = current_dev();
+namespace {
+// frees the cached blocks of device dev (every device when dev < 0)
+void trim_locked(int dev) {
     for (auto it = g_free.begin(); it != g_free.end();) {
-        if (it->first.dev == dev) {
+        if (dev < 0 || it->first.dev == dev) {
+            int was = -1;
+            if (dev < 0) {
+                (void)hipGetDevice(&was);
+                (void)hipSetDevice(it->first.dev);
+            }
             for (void *p : it->second) (void)hipFreeAsync(p, it->first.s);
             g_cached -= it->first.cap * it->second.size();
             (void)hipStreamSynchronize(it->first.s);
+            if (was >= 0) (void)hipSetDevice(was);
             it = g_free.erase(it);
         } else {
             ++it;
@@ -90,19 +100,55 @@ void block_trim() {
     }
 }
 
+// pinned staging buffers of Readback, one per host thread; a thread's buffer
+// is valid while its generation equals g_stage_gen (shutdown frees them all)
+std::mutex g_stage_mu;
+std::vector<char *> g_stages;
+unsigned g_stage_gen = 1;
+}  // namespace
+
+void block_trim() {
+    std::lock_guard<std::mutex> lk(g_blocks_mu);
+    trim_locked(current_dev());
+}
+
+void blocks_shutdown() {
+    {
+        std::lock_guard<std::mutex> lk(g_blocks_mu);
+        trim_locked(-1);
+    }
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    for (char *p : g_stages) (void)hipHostFree(p);
+    g_stages.clear();
+    g_stage_gen++;
+}
+
 void Readback::run() {
     if (items_.empty()) return;
-    thread_local char *stage = nullptr;  // pinned, grows; kept for the thread's life
+    thread_local char *stage = nullptr;  // pinned, grows; kept until shutdown
     thread_local size_t stage_cap = 0;
+    thread_local unsigned stage_gen = 0;
     size_t total = 0;
     for (const Item &it : items_) total += (it.bytes + 15) / 16 * 16;
-    if (total > stage_cap) {
-        if (stage) (void)hipHostFree(stage);
-        stage = nullptr;
-        stage_cap = 0;
-        const size_t want = std::max<size_t>(total, 4096);
-        GLV_HIP(hipHostMalloc((void **)&stage, want, hipHostMallocDefault));
-        stage_cap = want;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        if (stage_gen != g_stage_gen) {  // freed by a shutdown
+            stage = nullptr;
+            stage_cap = 0;
+            stage_gen = g_stage_gen;
+        }
+        if (total > stage_cap) {
+            if (stage) {
+                (void)hipHostFree(stage);
+                g_stages.erase(std::find(g_stages.begin(), g_stages.end(), stage));
+            }
+            stage = nullptr;
+            stage_cap = 0;
+            const size_t want = std::max<size_t>(total, 4096);
+            GLV_HIP(hipHostMalloc((void **)&stage, want, hipHostMallocDefault));
+            g_stages.push_back(stage);
+            stage_cap = want;
+        }
     }
     size_t off = 0;
     for (const Item &it : items_) {

@@ -45,14 +45,25 @@ int current_device() {
     return d;
 }
 
+void shutdown_all();
+
 // The engine's stream on the current device (the caller's when given).  On
 // first use of a device its default memory pool is set to keep freed blocks
 // (release threshold = max), whichever stream the calls come on, so that
 // stream-ordered allocations are reused across synchronisations instead of
-// going back to the driver.
+// going back to the driver.  The engine's own stream is a blocking stream, so
+// a caller that passes no stream gets the order of the legacy default stream
+// (work queued there before the call completes before the engine reads its
+// inputs).  The first use registers the engine's shutdown to run at process
+// exit, before the HIP runtime's own teardown (registered earlier).
 hipStream_t engine_stream(void *user) {
     int d = current_device();
     std::lock_guard<std::mutex> lk(g_mu);
+    static bool registered = false;
+    if (!registered) {
+        std::atexit(shutdown_all);
+        registered = true;
+    }
     DeviceCtx &c = g_ctx[d & 63];
     if (!c.pool) {
         hipMemPool_t pool;
@@ -64,10 +75,30 @@ hipStream_t engine_stream(void *user) {
     }
     if (user) return (hipStream_t)user;
     if (!c.ready) {
-        GLV_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        GLV_HIP(hipStreamCreate(&c.stream));
         c.ready = true;
     }
     return c.stream;
+}
+
+// Releases what the engine holds: cached device blocks, pinned staging
+// buffers, profiler events, its streams.  Later calls recreate them.
+void shutdown_all() {
+    glv::blocks_shutdown();
+    glv::Profiler::get().shutdown();
+    std::lock_guard<std::mutex> lk(g_mu);
+    int was = -1;
+    (void)hipGetDevice(&was);
+    for (int d = 0; d < 64; d++) {
+        DeviceCtx &c = g_ctx[d];
+        if (!c.ready) continue;
+        (void)hipSetDevice(d);
+        (void)hipStreamSynchronize(c.stream);
+        (void)hipStreamDestroy(c.stream);
+        c.stream = nullptr;
+        c.ready = false;
+    }
+    if (was >= 0) (void)hipSetDevice(was);
 }
 
 template <class F>
@@ -242,8 +273,22 @@ void run_pipeline(bool colored, const DevGraph &g, int64_t *d_C_orig, const glv:
 
 }  // namespace
 
+namespace glv {
+bool is_engine_stream(hipStream_t s) {
+    if (s == nullptr) return false;
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (const DeviceCtx &c : g_ctx)
+        if (c.ready && c.stream == s) return true;
+    return false;
+}
+}  // namespace glv
+
 // ============================================================== C ABI
 extern "C" {
+
+int glv_shutdown(void) {
+    return guarded([&] { shutdown_all(); });
+}
 
 const char *glv_last_error(void) { return g_last_error.c_str(); }
 

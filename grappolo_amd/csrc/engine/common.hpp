@@ -117,6 +117,10 @@ struct Error {
 void *block_get(size_t bytes, hipStream_t s, size_t *cap);
 void block_put(void *p, size_t cap, hipStream_t s);
 void block_trim();
+// frees every cached block on every device and the pinned staging buffers
+void blocks_shutdown();
+// true for a stream the engine created (those live until shutdown)
+bool is_engine_stream(hipStream_t s);
 
 // Device-to-host reads gathered behind one synchronisation: the copies go
 // through a pinned staging buffer of the calling thread (into pageable

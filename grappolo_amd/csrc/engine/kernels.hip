@@ -1585,8 +1585,11 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
                         }
                     }
                     __syncthreads();
-                    if (s_fn > kSpecFix) {  // too many set aside: gather again after the barrier
-                        clear(s_cnt);
+                    // every wavefront reads the counts before thread 0 may reset them in clear()
+                    const int fn = s_fn, cn = s_cnt;
+                    __syncthreads();
+                    if (fn > kSpecFix) {  // too many set aside: gather again after the barrier
+                        clear(cn);
                     } else {
                         sp_c = c2;
                         sp_v = v;

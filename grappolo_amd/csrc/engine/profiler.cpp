@@ -13,6 +13,22 @@ void Profiler::enable(bool v) {
     on_ = v;
 }
 
+void Profiler::shutdown() {
+    on_ = false;
+    for (auto &r : pending_) {
+        (void)hipEventSynchronize(r.b);
+        free_.push_back(r.a);
+        free_.push_back(r.b);
+    }
+    pending_.clear();
+    for (auto &e : open_) {
+        if (e) free_.push_back(e);
+        e = nullptr;
+    }
+    for (hipEvent_t e : free_) (void)hipEventDestroy(e);
+    free_.clear();
+}
+
 void Profiler::reset() {
     collect();
     for (auto &s : stats_) s = KStat();

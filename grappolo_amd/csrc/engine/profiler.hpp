@@ -39,6 +39,8 @@ public:
     void end(int cls, double bytes, hipStream_t s);
     // waits for all recorded events and folds them into the totals
     void collect();
+    // turns timing off and destroys every event
+    void shutdown();
     KStat stat(int cls) const { return stats_[cls]; }
 
 private:

@@ -76,6 +76,13 @@ int glv_get_device_count(int *count);
 const char *glv_last_error(void);
 /* The engine's build: "gfx950", plus version text. */
 const char *glv_version(void);
+/* No counterpart in the reference (its routines free what they allocate
+ * before returning).  Releases what the engine keeps between calls: cached
+ * device blocks, pinned staging buffers, timing events and its own streams.
+ * Runs by itself at process exit (registered with atexit on the first call,
+ * so before the HIP runtime's teardown); a later call starts afresh.  Must not
+ * overlap another call. */
+int glv_shutdown(void);
 
 /* ---- whole-pipeline entry points (host arrays) --------------------------- */
 

@@ -1,12 +1,14 @@
 // ref_bench.cpp -- times the compiled reference's colored multi-phase Louvain
 // on host cores.  TEST INFRASTRUCTURE ONLY (bench.py's cpu_baseline leg).
 //
-// usage: ref_bench <graph.bin> <threads> [minGraphSize threshold C_threshold]
+// usage: ref_bench <graph.bin> <threads> [minGraphSize threshold C_threshold [driver <log>]]
 // The graph file is the reference's -f 9 binary CSR (InputsOutput/
 // writeBinary.cpp:2-29 layout).  The phase loop below is the control flow of
 // Coloring/runMultiPhaseColoring.cpp:104-207 (coloring == 1, map sweeps) and
 // calls the reference's own routines for every step, so that the edges visited
 // per phase can be counted; the reference driver itself exposes no counts.
+// With `driver <log>` the reference's runMultiPhaseColoring itself runs and
+// the values of its printed summary are reported.
 #include <fcntl.h>
 #include <omp.h>
 #include <unistd.h>
@@ -41,12 +43,65 @@ static graph *load(const char *path) {
     return G;
 }
 
+// Value printed after `label` in the reference's compact summary
+// (Coloring/runMultiPhaseColoring.cpp:209-224), or NaN.
+static double summary_value(const char *path, const char *label) {
+    FILE *f = fopen(path, "r");
+    if (!f) return 0.0 / 0.0;
+    char line[512];
+    double v = 0.0 / 0.0;
+    const size_t n = strlen(label);
+    while (fgets(line, sizeof line, f)) {
+        if (strncmp(line, label, n) != 0) continue;
+        const char *colon = strchr(line, ':');
+        if (colon) v = atof(colon + 1);
+    }
+    fclose(f);
+    return v;
+}
+
+// The reference's own driver entry point, unchanged: runMultiPhaseColoring
+// (Coloring/runMultiPhaseColoring.cpp:51-238) with coloring 1 and map sweeps,
+// as driverForGraphClustering.cpp:163 calls it; its printed summary is parsed.
+static int run_driver(graph *G, int nT, long minGraphSize, double threshold, double C_threshold,
+                      const char *log_path) {
+    long NV = G->numVertices;
+    long *C_orig = (long *)malloc(NV * sizeof(long));
+    for (long i = 0; i < NV; i++) C_orig[i] = -1;  // driverForGraphClustering.cpp:180-183
+    int saved = dup(1);
+    fflush(stdout);
+    int fd = open(log_path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    dup2(fd, 1);
+    close(fd);
+    double t0 = omp_get_wtime();
+    runMultiPhaseColoring(G, C_orig, 1, 16, 0, minGraphSize, threshold, C_threshold, nT, 1);
+    double t1 = omp_get_wtime();
+    fflush(stdout);
+    dup2(saved, 1);
+    close(saved);
+    long isolated = 0;
+    for (long i = 0; i < NV; i++) isolated += C_orig[i] < 0;
+    free(C_orig);
+    printf("{\"threads\": %d, \"seconds\": %.6f, \"total_time\": %.6f, \"final_modularity\": %.6f, "
+           "\"phases\": %.0f, \"iterations\": %.0f, \"clusters\": %.0f, \"unlabelled\": %ld}\n",
+           nT, t1 - t0, summary_value(log_path, "TOTAL TIME"),
+           summary_value(log_path, "Final modularity"), summary_value(log_path, "Total number of phases"),
+           summary_value(log_path, "Total number of iterations"),
+           summary_value(log_path, "Final number of clusters"), isolated);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s graph.bin threads [minGraphSize threshold C_threshold]\n", argv[0]);
+        fprintf(stderr,
+                "usage: %s graph.bin threads [minGraphSize threshold C_threshold [driver log]]\n"
+                "  driver: call the reference's runMultiPhaseColoring and parse its summary\n",
+                argv[0]);
         return 2;
     }
     graph *G = load(argv[1]);
+    if (argc > 7 && strcmp(argv[6], "driver") == 0)
+        return run_driver(G, atoi(argv[2]), atol(argv[3]), atof(argv[4]), atof(argv[5]), argv[7]);
     int nT = atoi(argv[2]);
     long minGraphSize = argc > 3 ? atol(argv[3]) : 1000;
     double threshold = argc > 4 ? atof(argv[4]) : 1e-9;

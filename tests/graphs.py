@@ -94,6 +94,26 @@ def cliques(seed=13, nv=20000, size=200, inter=8, hub_degrees=(5000, 4500), weig
     return Graph.from_pairs(nv, p, w)
 
 
+def bipartite_blocks(seed=19, sides=((350, 350, 0.9), (300, 330, 0.95)), extra=200, weighted=False):
+    """Dense near-complete bipartite blocks plus a few random edges: the
+    first-coloured side of a block takes one color and the other side the
+    next, so a vertex of the second side has about 300 neighbours in the
+    preceding color class (more than the persistent dense sweep sets aside
+    while it gathers ahead of a color step)."""
+    rng = np.random.default_rng(seed)
+    pairs, base = [], 0
+    for nx, ny, dens in sides:
+        xs, ys = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        keep = rng.random(xs.shape) < dens
+        pairs.append(np.stack([xs[keep] + base, ys[keep] + base + nx], 1))
+        base += nx + ny
+    u, v = rng.integers(0, base, extra), rng.integers(0, base, extra)
+    pairs.append(np.stack([u, v], 1)[u != v])
+    p = np.unique(np.sort(np.concatenate(pairs), 1), axis=0)
+    w = rng.uniform(0.5, 2.0, len(p)) if weighted else None
+    return Graph.from_pairs(base, p, w)
+
+
 def catalog():
     return {
         "karate": karate(),
@@ -107,6 +127,8 @@ def catalog():
         # degree > 16384: several workgroups per vertex (sweep, modularity, coloring)
         "hubs_giant": hubs(seed=9, nv=60000, hub_degrees=(20000, 31000, 17000, 5000)),
         "loops_isolated": with_loops_and_isolated(),
+        # vertices with ~300 neighbours in the preceding color class
+        "bipartite": bipartite_blocks(),
     }
 
 

@@ -332,3 +332,26 @@ def test_movers_modularity_update_equals_full_passes(name):
         assert q2 == q1
     else:
         assert abs(q2 - q1) <= 1e-12 * abs(q1), (q1, q2)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_colored_phase_set_aside_overflow(weighted):
+    """The persistent dense sweep gathers a vertex's neighbour labels ahead of
+    its color step and sets aside the entries whose neighbour belongs to the
+    step before (at most 256 per vertex); a vertex with more is gathered again
+    after the step.  On dense bipartite blocks the second-coloured side has
+    ~300 neighbours in the preceding class: the phase equals the oracle's
+    frozen color steps with the same (Jones-Plassmann) coloring."""
+    from graphs import bipartite_blocks
+    g = bipartite_blocks(weighted=weighted)
+    colors, mc, _ = oracle.color_jp(g)
+    rows = np.repeat(np.arange(g.numVertices), g.degrees())
+    prev = colors[g.tails] == colors[rows] - 1
+    assert (np.bincount(rows[prev], minlength=g.numVertices) > 256).sum() > 64
+    q, C, it = engine.algoLouvainWithDistOneColoring(g, colors.astype(np.int32), mc + 1, -1.0, 1e-6)
+    qo, Co, ito, _ = oracle.louvain_colored(g, colors, mc + 1, -1.0, 1e-6, frozen=True)
+    if not weighted:
+        assert it == ito and q == qo
+        np.testing.assert_array_equal(C, Co)
+    else:
+        assert abs(q - qo) < 1e-3
