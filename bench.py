@@ -3,17 +3,20 @@
 
 One step = one complete run to convergence (coloring, colored sweeps,
 modularity, renumbering, coarsening, the final plain phase) of the seeded
-LFR-style graph with 4,194,304 vertices (degrees power law 2.5 on [10, 1000],
-community sizes power law 1.5 on [20, 5000], mixing 0.4, weights U[0.5, 2],
-seed 2), resident in HBM before the timed region starts, with coloring 1,
-threshold 1e-9, C_threshold 1e-6, minGraphSize 1000.
+RMAT graph of scale 24 (16,777,216 vertices, edge factor 16, a/b/c/d
+0.57/0.19/0.19/0.05, duplicates and self-loops removed, symmetrised: about
+521M directed entries, unit weights, seed 3), resident in HBM before the timed
+region starts, with coloring 1, threshold 1e-9, C_threshold 1e-6,
+minGraphSize 1000.  Other workloads: --workload lfr4m / lfr8m / planted65k.
 
 value = edges visited per second over the K timed steps, where edges visited
 = sum over phases of iterations x directed entries of the phase's graph.
 The JSON line also carries the HBM roofline of the dominant kernel (HIP
-events around every launch, algorithmic bytes per launch) and the CPU
-baseline: the reference's own OpenMP code (oracle/_ref/ref_bench) on a
-bounded sample, or the C restatement when the reference build is absent.
+events around every launch, algorithmic bytes per launch), the atomic
+throughput of the moves, the CPU baseline (the reference's own OpenMP code,
+oracle/_ref/ref_bench, on a bounded sample of the workload, or the C
+restatement when the reference build is absent) and, when time allows, the
+learned active-set comparison (--learned runs it alone).
 """
 import argparse
 import json
@@ -27,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Louvain edges-visited/s + time-to-converge vs plain GPU at ΔQ≤1e-3"
+# the CPU baseline runs the reference on this RMAT scale (same parameters):
+# about 10-30 s on 16 host cores, where the full scale-24 graph takes tens of minutes
+RMAT_SAMPLE_SCALE = 20
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
@@ -55,11 +61,33 @@ def make_graph(spec, nv=None):
 
 
 def host_cores():
+    """Physical cores this process may run on (lscpu: distinct (socket, core)
+    pairs among the CPUs of the affinity mask, SMT siblings counted once),
+    bounded by OMP_NUM_THREADS when the environment sets it (the host's CPU
+    share, e.g. 16 cores per GPU)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        allowed = os.sched_getaffinity(0)
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        allowed = set(range(os.cpu_count() or 1))
+    n = len(allowed)
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                             check=True).stdout
+        cores = set()
+        for line in out.splitlines():
+            if not line or line.startswith("#"):
+                continue
+            cpu, core, sock = (int(x) if x else -1 for x in line.split(","))
+            if cpu in allowed:
+                cores.add((sock, core))
+        if cores:
+            n = len(cores)
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pass
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
 def cpu_baseline(spec):
@@ -69,8 +97,9 @@ def cpu_baseline(spec):
     sample_nv = {"lfr": 1048576, "planted": 65536, "rmat": None}[spec["kind"]]
     if spec["kind"] == "rmat":
         from grappolo_amd.graph import gen_rmat
-        g = gen_rmat(20, seed=spec["seed"])
-        sample = "RMAT scale 20 (same a/b/c/d, edge factor 16, seed 3)"
+        g = gen_rmat(RMAT_SAMPLE_SCALE, seed=spec["seed"])
+        sample = (f"RMAT scale {RMAT_SAMPLE_SCALE} (same a/b/c/d and edge factor 16, seed 3; "
+                  f"{g.numVertices:,} vertices, {g.nnz:,} entries)")
     else:
         g = make_graph(spec, sample_nv)
         sample = f"{spec['kind']} graph with {sample_nv:,} vertices, same parameters and seed"
@@ -98,9 +127,9 @@ def cpu_baseline(spec):
             "final_modularity": st["final_modularity"], "seconds": dt}
 
 
-def read_traffic(kernel):
+def read_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "lfr4m_colored_louvain_hbm_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"{workload}_colored_louvain_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -242,12 +271,12 @@ def run_batched(args):
     import numpy as np
     import torch
     from grappolo_amd import engine
-    from grappolo_amd.graph import gen_planted, union
+    from grappolo_amd.graph import gen_planted_many, union
 
     torch.cuda.set_device(0)
     t = time.perf_counter()
     specs = batch_specs(args.batch_graphs)
-    graphs = [gen_planted(nv, b, d, m, seed=sd) for nv, b, d, m, sd in specs]
+    graphs = gen_planted_many(specs)
     U, vb = union(graphs)
     t_gen = time.perf_counter() - t
     print(f"[bench] batch of {len(graphs)} graphs: {U} generated in {t_gen:.1f}s", file=sys.stderr,
@@ -313,7 +342,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lfr4m", choices=list(WORKLOADS))
+    ap.add_argument("--workload", default="rmat24", choices=list(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="do not time kernels with HIP events in the timed region")
@@ -395,11 +424,12 @@ def main():
 
     roofline = None
     if kern:
-        sweeps = {k: v for k, v in kern.items() if v[2] > 0 and k.startswith("sweep")}
-        dom = max(sweeps, key=lambda k: sweeps[k][0])
-        ms, by, n = sweeps[dom]
+        # the kernel class with the most device time among those with algorithmic bytes
+        timed = {k: v for k, v in kern.items() if v[2] > 0 and v[1] > 0}
+        dom = max(timed, key=lambda k: timed[k][0])
+        ms, by, n = timed[dom]
         achieved = by / (ms * 1e-3) / 1e9
-        traffic = read_traffic(dom)
+        traffic = read_traffic(dom, args.workload)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "kernel": dom, "launches": n,
@@ -434,6 +464,13 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
+    # atomics: each move updates the community degrees and sizes of the source
+    # and target communities (2 FP64 + 2 int64 atomics,
+    # parallelLouvainWithColoring.cpp:193-201)
+    moves = sum(r["moves"] for r in runs) / args.steps
+    out["atomics"] = {"moves_per_step": moves, "moves_per_s": moves / (elapsed / args.steps),
+                      "atomics_per_s": 4 * moves / (elapsed / args.steps),
+                      "counted": "moves of the whole-GPU colored sweeps (graphs above 262,144 vertices)"}
     print(json.dumps(out))
 
 

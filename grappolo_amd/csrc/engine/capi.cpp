@@ -207,6 +207,7 @@ void fill_stats(const glv::RunResult &r, glv_stats *st) {
     st->edges_visited = r.edges_visited;
     st->edges_visited_mod = r.edges_visited_mod;
     st->colors_first = r.colors_first;
+    st->moves = r.moves;
     st->n_phase = (int32_t)r.phase.size();
     for (int i = 0; i < (int)r.phase.size() && i < st->phase_cap; i++) {
         const auto &p = r.phase[i];

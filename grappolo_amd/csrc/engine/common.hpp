@@ -40,7 +40,7 @@ constexpr int kBlockHashSlots = 4096;
 // 160 KB of LDS of a gfx950 CU.
 constexpr i64 kDenseMaxNV = 15360;
 constexpr int kDenseBlocks = 1024;
-constexpr i64 kPersistMaxBlocks = 64;  // cooperative grid of the persistent sweep
+constexpr i64 kPersistMaxBlocks = 64;  // grid of the persistent sweep
 // The persistent sweep serves graphs of at most this many vertices whose
 // color classes hold at most kPersistMaxPerColor vertices on average (coarse
 // phases: many small classes, where per-color launches would be latency-bound).
@@ -90,6 +90,11 @@ __host__ __device__ inline int degree_bin(i64 d) {
     if (d <= kGiantDeg) return 8;
     return 9;
 }
+
+// Diagnostics on stderr when the environment sets GLV_TRACE (which launch
+// path each stage took, refused persistent launches); silent otherwise.
+bool trace_on();
+void trace(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 
 // Error carried from the HIP layer up to the C ABI.
 struct Error {

@@ -18,12 +18,11 @@
 
 #include "kernels.hpp"
 #include "mlp_device.hpp"
+#include "sweep_device.hpp"
 
 namespace glv {
 
 namespace {
-
-constexpr u64 kFib = 0x9E3779B97F4A7C15ULL;
 
 __device__ inline u64 jp_priority(i64 v) {
     // splitmix64 finaliser: a bijection, so priorities of distinct vertices differ
@@ -45,56 +44,6 @@ __device__ inline uint8_t jp_code(u64 pu, i64 u, u64 pv, i64 v) {
     if (du > (u64)kJpMidDeg) c |= kJpHuge;
     if (du > (u64)kGiantDeg) c |= kJpGiant;
     return c;
-}
-
-__device__ inline i64 ld_relaxed(const i64 *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
-// Loads of tables that other lanes fill with atomics (performed in L2): agent
-// scope, so a line cached in the CU's L1 before the atomics cannot be served.
-__device__ inline double ld_agent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline i64 ld_agent(const i64 *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Gain of moving a vertex of degree kv from community s (internal weight
-// eix excluding its self-loop, degree ax excluding the vertex) to community y
-// holding weight eiy towards it and degree ay: max() at
-// utilityClusteringFunctions.cpp:243, same operation order, no contraction.
-__device__ inline double gain_of(double eiy, double eix, double kv, double ay, double ax,
-                                 double constant) {
-    return 2 * (eiy - eix) - 2 * kv * (ay - ax) * constant;
-}
-
-// 1/(2m) of the graph vertex v belongs to (batched graphs carry one per graph)
-__device__ inline double vconst(const SweepArgs &A, i64 v) {
-    return A.gid ? A.gconst[A.gid[v]] : A.constant;
-}
-
-// "better" of two candidates: larger gain, then smaller community id.
-__device__ inline bool better(bool oc, double og, i64 oid, bool c, double g, i64 id) {
-    return oc && (!c || og > g || (og == g && oid < id));
-}
-
-template <int W>
-__device__ inline void group_argmax(bool &cand, double &gain, i64 &id) {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) {
-        double og = __shfl_xor(gain, o, W);
-        long long oid = __shfl_xor((long long)id, o, W);
-        int ocd = __shfl_xor((int)cand, o, W);
-        if (better(ocd != 0, og, oid, cand, gain, id)) {
-            cand = true;
-            gain = og;
-            id = oid;
-        }
-    }
-}
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
 }
 
 // Block-wide sum, result valid in every thread.  Deterministic order.
@@ -869,23 +818,15 @@ __global__ void k_giant_table_sizes(const i64 *off, const i64 *verts, i64 count,
     }
 }
 
-__device__ inline void move_vertex(i64 t, i64 s, double kv, double *a, i64 *n) {
-    atomicAdd(&a[t], kv);
-    atomicAdd(&a[s], -kv);
-    atomicAdd((unsigned long long *)&n[t], 1ULL);
-    atomicAdd((unsigned long long *)&n[s], ~0ULL);  // -1
-}
-
 // ---------------------------------------------------------- persistent sweep
 // A whole colored iteration of a small graph (at most kDenseMaxNV vertices) in
-// one cooperative launch: for every color, the workgroups share out the
+// one persistent launch: for every color, the workgroups share out the
 // vertices of the class (dense LDS accumulator per workgroup), meet at a grid
 // barrier, commit their own vertices' moves, and meet again.  The modularity
 // partials of the iteration follow the last color.  Grid barriers spin on a
 // generation counter with a bounded wait; a workgroup that gives up raises
 // `abort`, which releases every other wait, so every wave always exits.
 constexpr int kPBlock = 1024;
-constexpr unsigned kSpinLimit = 1u << 22;
 
 __device__ inline double block_sum_p(double v, double *scratch) {
     v = wave_sum(v);
@@ -961,44 +902,6 @@ __device__ inline void block_argmax_pn(bool &cand, double &gain, i64 &id, i64 &n
     gain = g;
     id = d;
     nid = nn;
-}
-
-// Grid barrier over nb <= 64 workgroups without a shared counter: workgroup
-// b publishes the barrier's sequence number in its own flag slot (one cache
-// line each) with an agent-scope release, and the first wavefront of every
-// workgroup polls all nb slots at once with relaxed agent-scope loads, then
-// takes one agent-scope acquire (one L2 write-back per arrival and one
-// invalidate per departure, instead of an invalidate per poll: 2.2 us per
-// barrier at 64 workgroups against 4.3 us with system-scope polling).
-// Bounded wait; a workgroup that gives up raises bar[0] (abort), which
-// releases every other wait.  Returns false when the barrier was abandoned.
-__device__ bool grid_barrier(unsigned *bar, unsigned nb, unsigned seq) {
-    __shared__ int s_ok;
-    unsigned *flags = bar + 16;  // flag of workgroup b at flags[16 * b]
-    __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&flags[16 * blockIdx.x], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    if (threadIdx.x < kWave) {
-        const int lane = threadIdx.x;
-        unsigned spins = 0;
-        int ok = 1;
-        while (true) {
-            bool done = true;
-            if ((unsigned)lane < nb)
-                done = __hip_atomic_load(&flags[16 * lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seq;
-            if (__ballot(!done) == 0) break;
-            if (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > kSpinLimit) {
-                if (lane == 0) atomicOr(&bar[0], 1u);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (lane == 0) s_ok = ok && !__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    return s_ok != 0;
 }
 
 // LDS of the persistent sweep: either the dense accumulator of graphs with at
@@ -2419,7 +2322,7 @@ __global__ __launch_bounds__(kTailBlock) void k_tail_seq(const i64 *list, i64 n,
     if (tid < n && list[tid] >= 0 && list[tid] < nv) colors[list[tid]] = mine;
 }
 
-// Jones-Plassmann rounds in one cooperative launch: the same topological
+// Jones-Plassmann rounds in one persistent launch: the same topological
 // sweep as the host-driven rounds (count of higher-priority uncoloured
 // neighbours, frontier = count zero, first-fit against the coloured higher
 // neighbours), with a grid barrier between rounds instead of a launch and a
@@ -3259,6 +3162,27 @@ void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hi
     k_max_degree<<<grid_for(count, kBlock, 4096), kBlock, 0, s>>>(off, verts, count, out);
     GLV_LAUNCH_CHECK();
 }
+// Launch of a kernel that synchronises its workgroups with its own flag
+// barriers (grid_barrier): an ordinary launch in stream order whose grid never
+// exceeds what the device holds resident at once (`resident`, from the
+// kernel's occupancy), so every workgroup runs concurrently and each barrier
+// completes; the barriers' bounded waits turn anything else into an error flag
+// instead of a hang.
+static bool launch_resident(const void *fn, int nblocks, int block, void **args, int resident, hipStream_t s,
+                            const char *what) {
+    if (nblocks <= 0 || nblocks > resident) {
+        trace("%s: %d workgroups exceed the %d resident ones", what, nblocks, resident);
+        return false;
+    }
+    hipError_t e = hipLaunchKernel(fn, dim3(nblocks), dim3(block), args, 0, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trace("%s (%d workgroups) refused: %s", what, nblocks, hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
 int persistent_blocks() {
     static int cached = -1;
     if (cached < 0) {
@@ -3299,12 +3223,7 @@ bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *st
     const bool spec = !hash && colors && nblocks >= 2 && nblocks % 2 == 0;
     const void *fn = spec ? (const void *)k_colored_iteration_spec
                           : hash ? (const void *)k_colored_iteration<true> : (const void *)k_colored_iteration<false>;
-    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kPBlock), args, 0, s);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    return true;
+    return launch_resident(fn, nblocks, kPBlock, args, persistent_blocks(), s, "persistent colored iteration");
 }
 void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s) {
     if (count <= 0) return;
@@ -3451,13 +3370,8 @@ bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 
     uint8_t *cd = code;
     void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0, &pr,
                     &cd, &mf};
-    hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_persistent, dim3(nblocks), dim3(kJpPBlock),
-                                              args, 0, s);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    return true;
+    return launch_resident((const void *)k_jp_persistent, nblocks, kJpPBlock, args, jp_persistent_blocks(), s,
+                           "persistent JP rounds");
 }
 int jp_push_blocks() {
     static int cached = -1;
@@ -3480,12 +3394,8 @@ bool launch_jp_push(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i
     uint8_t *cd = code;
     uint32_t *fb = forbid;
     void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &pr, &cd, &fb};
-    hipError_t e = hipLaunchCooperativeKernel((const void *)k_jp_push, dim3(nblocks), dim3(kJpPBlock), args, 0, s);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    return true;
+    return launch_resident((const void *)k_jp_push, nblocks, kJpPBlock, args, jp_push_blocks(), s,
+                           "pushed-colour JP rounds");
 }
 void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s) {
     if (g.nv <= 0) return;

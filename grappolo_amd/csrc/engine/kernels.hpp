@@ -88,9 +88,9 @@ void launch_sweep(int bin, bool plain, const SweepArgs &A, hipStream_t s);
 void launch_max_degree(const i64 *off, const i64 *verts, i64 count, i64 *out, hipStream_t s);
 constexpr int kGlobalHashBlocks = 256;
 // One colored iteration of a graph with at most kDenseMaxNV vertices in one
-// cooperative launch (all colors, commits, modularity partials: part[0..nb)
+// persistent launch (all colors, commits, modularity partials: part[0..nb)
 // internal weight, part[nb..2nb) sum of a^2).  bar: 3 zeroed words.  Returns
-// false when the cooperative launch is not possible (caller falls back).
+// false when the persistent launch is not possible (caller falls back).
 int persistent_blocks();
 // vertices grouped by color (stable: within a color in perm's order); cstart: ncol+1
 void launch_color_major(const i64 *perm, const int32_t *colors, i64 n, int ncol, i64 *cperm,
@@ -168,7 +168,7 @@ void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
 void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
                        int32_t *tidx, uint32_t *base, u64 *adj, int32_t *err, hipStream_t s);
 constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many uncoloured vertices
-// Jones-Plassmann rounds in one cooperative launch (same result as the
+// Jones-Plassmann rounds in one persistent launch (same result as the
 // host-driven rounds): from scratch (count) or continuing from the frontier
 // in sA (small, count ring[0]) and lA (large, ring[1]) after colored0
 // vertices; false when the launch is refused.  ring: 8 words (zero except the
@@ -180,7 +180,7 @@ bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 
                           i64 colored0, const u64 *prio, uint8_t *code, i64 max_front, int nblocks,
                           hipStream_t s);
 // Jones-Plassmann rounds of a graph of at most kJpPushMaxNV vertices in one
-// cooperative launch that pushes colours: a vertex coloured c sets bit c of
+// persistent launch that pushes colours: a vertex coloured c sets bit c of
 // the forbidden-colour bitmap (forbid[u * kMaxColors/32 ..], zero on entry,
 // nv * kMaxColors/32 words) of every lower-priority neighbour u as it
 // releases it, so a vertex takes its colour from its own bitmap when its
@@ -193,9 +193,9 @@ int jp_push_blocks();
 bool launch_jp_push(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
                     i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, const u64 *prio, uint8_t *code,
                     uint32_t *forbid, int nblocks, hipStream_t s);
-// frontiers below this many vertices finish in one cooperative launch
+// frontiers below this many vertices finish in one persistent launch
 constexpr i64 kJpPersistFrontier = 16384;
-// a cooperative launch hands back to full-GPU rounds once a frontier exceeds this
+// a persistent launch hands back to full-GPU rounds once a frontier exceeds this
 constexpr i64 kJpPersistExit = 32768;
 void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);
 

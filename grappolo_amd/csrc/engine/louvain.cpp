@@ -9,6 +9,7 @@
 #include "active_set.hpp"
 #include "kernels.hpp"
 #include "profiler.hpp"
+#include "wide.hpp"
 
 namespace glv {
 
@@ -77,6 +78,13 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
     }
     rb.run();
     if (e) fail(GLV_E_INVALID_ARG, "color values out of range [0, nColors)");
+    if (trace_on()) {
+        for (int b = 1; b < kNumBins; b++) {
+            i64 ents = 0;
+            for (int c = 0; c < ncol; c++) ents += P.pnnz[(size_t)b * ncol + c];
+            trace("  bin %d: %ld vertices, %ld entries", b, (long)(P.bin_hi(b) - P.bin_lo(b)), (long)ents);
+        }
+    }
     // degrees and 1/(2m)
     const i64 big_lo = P.bin_lo(8), big_hi = P.bin_hi(9);
     launch_degree(g, P.perm.p + big_lo, big_hi - big_lo, P.k.p, s_);
@@ -112,6 +120,12 @@ void Engine::setup(const DevGraph &g, const int32_t *colors, int ncol, PhaseSetu
                 tab.push_back(at);
             }
             most = std::max(most, at);
+        }
+        if (trace_on()) {
+            i64 mx = 0;
+            for (i64 x : hsz) mx = std::max(mx, x);
+            trace("  giants: %ld, largest table %ld slots, tables of one class at most %ld slots", (long)ng,
+                  (long)mx, (long)most);
         }
         P.gt_tab.alloc(tab.size(), s_);
         GLV_HIP(hipMemcpyAsync(P.gt_tab.p, tab.data(), tab.size() * sizeof(i64), hipMemcpyHostToDevice, s_));
@@ -161,7 +175,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     i64 hc[3] = {0, 0, 0};  // sizes of the current frontier's small, large and giant lists
     DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
     JpLists cur{sA.p, lA.p, gA.p, counts.p, prio.p, code.p}, next{sB.p, lB.p, gB.p, counts.p + 4, prio.p, code.p};
-    // Rounds in one cooperative launch while the frontier does not fill the
+    // Rounds in one persistent launch while the frontier does not fill the
     // GPU.  count: start from scratch; else continue from the frontier in
     // cur (hc, no giant list).  Returns 0 when the launch was refused, 1 when
     // the coloring is complete, 2 when a large frontier handed it back (cur,
@@ -242,7 +256,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
             if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
             const i64 won = hc[0] + hc[1] + hc[2];
             if (won < kJpPersistFrontier && colored > 0) {
-                // hand the following rounds to one cooperative launch: giant
+                // hand the following rounds to one persistent launch: giant
                 // frontier vertices join the large list
                 if (hc[2] > 0)
                     GLV_HIP(hipMemcpyAsync(cur.large + hc[1], cur.giant, hc[2] * sizeof(i64),
@@ -277,6 +291,8 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         launch_color_tail(g, colors, next.small, next.large, next.giant, rest, cnt.p, base.p, adj.p, err.p, s_);
         Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
     }
+    glv::trace("coloring nv %ld: %s, %d rounds, tail %ld", (long)nv,
+          nv <= kJpPushMaxNV && done && round >= 0 ? "rounds done" : "host rounds", round, (long)(nv - colored));
     launch_jp_finish(colors, nv, maxc.p, s_);
     const int32_t e = fetch(err.p);
     if (e & 1) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
@@ -342,7 +358,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     for (int c = 0; c < ncol; c++)
         for (int b = 1; b < kNumBins; b++)
             if (P.hi(b, c) > P.lo(b, c)) color_nonempty[c] = 1;
-    // coarse graphs (small, or many small color classes): one cooperative
+    // coarse graphs (small, or many small color classes): one persistent
     // launch per iteration (all colors + modularity), grid sized to the color
     // classes: every extra workgroup only lengthens the barriers
     int pblocks = 0;
@@ -360,6 +376,19 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
             if (nv <= kDenseMaxNV) pblocks &= ~1;
         }
     }
+    // large graphs: the whole-GPU persistent sweep (wide.hpp), plain form only
+    const bool las_mode = las && !(las->mode == 2 && nv <= kDenseMaxNV) && las->mode != 0;
+    int wblocks = 0;
+    {
+        int nonempty = 0;
+        for (int c = 0; c < ncol; c++) nonempty += color_nonempty[c];
+        // classes of more than kWideMaxPerColor vertices on average fill the GPU
+        // with per-color launches of the binned kernels
+        const bool small_classes = nonempty > 0 && nv / nonempty <= kWideMaxPerColor;
+        if (!las_mode && pblocks == 0 && nv > kPersistMaxNV && small_classes && wide_enabled())
+            wblocks = wide_blocks();
+    }
+    bool use_wide = wblocks > 0;
     DBuf<i64> gkeys;
     DBuf<double> gvals;
     i64 gslots = 0;
@@ -392,12 +421,53 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     }
     DBuf<unsigned> bar(pblocks > 0 ? kBarrierWords : 0, s_);
     DBuf<double> ppart(pblocks > 0 ? 2 * pblocks : 0, s_);
-    DBuf<i64> cperm(pblocks > 0 ? nv : 0, s_), cstart(pblocks > 0 ? ncol + 1 : 0, s_);
-    // second state buffers of the persistent sweep (one grid barrier per color step)
-    DBuf<i64> pC1(pblocks > 0 ? nv : 0, s_), pn1(pblocks > 0 ? nv : 0, s_);
-    DBuf<double> pa1(pblocks > 0 ? nv : 0, s_);
-    if (pblocks > 0) launch_color_major(P.perm.p, colors, nv, ncol, cperm.p, cstart.p, s_);
+    const bool dbl = pblocks > 0 || use_wide;
+    DBuf<i64> cperm(dbl ? nv : 0, s_), cstart(dbl ? ncol + 1 : 0, s_);
+    // second state buffers of the persistent sweeps (one grid barrier per color step)
+    DBuf<i64> pC1(dbl ? nv : 0, s_), pn1(dbl ? nv : 0, s_);
+    DBuf<double> pa1(dbl ? nv : 0, s_);
+    if (dbl) launch_color_major(P.perm.p, colors, nv, ncol, cperm.p, cstart.p, s_);
     bool persistent = pblocks > 0;
+    WidePlan WP;
+    WideArgs W{};
+    DBuf<unsigned> wbar(use_wide ? 16 + 16 * (size_t)wblocks : 0, s_);
+    DBuf<double> wpart(use_wide ? 2 * (size_t)wblocks : 0, s_);
+    DBuf<i64> wmoves(use_wide ? 1 : 0, s_);
+    if (use_wide) {
+        build_wide_plan(g, P, wblocks, WP, s_);
+        launch_fill_i64(wmoves.p, 1, 0, s_);
+        W.A = A;
+        W.C0 = C;
+        W.C1 = pC1.p;
+        W.a0 = P.a.p;
+        W.a1 = pa1.p;
+        W.n0 = P.n.p;
+        W.n1 = pn1.p;
+        W.perm = P.perm.p;
+        W.cperm = cperm.p;
+        W.cstart = cstart.p;
+        W.witems = WP.witems.p;
+        W.wstart = WP.wstart.p;
+        W.vitems = WP.vitems.p;
+        W.vstart = WP.vstart.p;
+        W.giants = WP.giants.p;
+        W.ngiants = WP.ngiants;
+        W.gkeys = WP.gkeys.p;
+        W.gvals = WP.gvals.p;
+        W.gstate = WP.gstate.p;
+        W.gsc = WP.gsc.p;
+        W.gcand = WP.gcand.p;
+        W.gcid = WP.gcid.p;
+        W.gslices = WP.gslices;
+        W.gout = WP.gout.p;
+        W.ncol = ncol;
+        W.bar = wbar.p;
+        W.part = wpart.p;
+        W.moves = wmoves.p;
+    }
+    glv::trace("colored phase nv %ld, %d colors: %s (%d workgroups)", (long)nv, ncol,
+               use_wide ? "wide persistent sweep" : persistent ? "persistent sweep" : "per-color launches",
+               use_wide ? wblocks : pblocks);
 
     // learned active set: recorded features, sweeps since the last move,
     // active flags and the compacted sweep lists
@@ -500,7 +570,37 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         A.dein = incremental ? dein.p : nullptr;
         if (incremental && !prune) launch_fill_f64(dsum.p, 1, 0.0, s_);
         bool done_iter = false;
-        if (persistent) {
+        if (use_wide) {
+            launch_fill_i32((int32_t *)wbar.p, (i64)wbar.n, 0, s_);
+            Profiler::get().begin(K_WIDE, s_);
+            use_wide = launch_wide_iteration(W, wblocks, s_);
+            Profiler::get().end(K_WIDE, 24.0 * g.nnz + 32.0 * nv, s_);
+            if (use_wide) {
+                launch_sum_partials(wpart.p, wblocks, res.p, s_);
+                launch_sum_partials(wpart.p + wblocks, wblocks, res.p + 1, s_);
+                double h[2];
+                unsigned abort_flag = 0;
+                Readback rb(s_);
+                rb.add(h, res.p, sizeof(h));
+                rb.add(&abort_flag, wbar.p, sizeof(unsigned));
+                rb.run();
+                if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the wide sweep timed out");
+                if (numItrs == 1 || !valid_colors) {
+                    // a full pass: the first iteration, or classes that may hold an edge
+                    currMod = colored_modularity(g, P, C, part.p, res.p, &exx);
+                } else {
+                    // moves of one color step are pairwise non-adjacent, so the
+                    // internal weight changes by the sum of the movers' 2(e_yt - e_ys)
+                    exx += h[0];
+                    const double c = P.constant;
+                    currMod = exx * (double)c - h[1] * (double)c * (double)c;
+                }
+                done_iter = true;
+            }
+            // launch refused: this phase continues with per-color launches
+        }
+        if (done_iter) {
+        } else if (persistent) {
             launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
             Profiler::get().begin(K_PERSIST, s_);
             persistent = launch_colored_iteration(A, plist, pstart, ncol, bar.p, ppart.p, pblocks, s_,
@@ -521,13 +621,13 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 exx = h[0];
                 done_iter = true;
             } else if (prune && !full) {
-                // cooperative launch refused: redo this pruned iteration as a full one
+                // persistent launch refused: redo this pruned iteration as a full one
                 full = true;
                 numItrs--;
                 las->pruned_sweeps--;
                 continue;
             }
-            // cooperative launch refused: this phase continues with per-color launches
+            // persistent launch refused: this phase continues with per-color launches
         }
         if (!done_iter) {
             const std::vector<i64> &ptr = *hptr;
@@ -591,6 +691,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         if (numItrs >= kMaxPhaseIters) break;
         prevMod = currMod;
     }
+    if (wmoves.n) moves_ += fetch(wmoves.p);
     if (mode) sync();
     if (las_off) {
         las->swept_vertices += (i64)numItrs * nonisolated;
@@ -964,6 +1065,7 @@ void Engine::run_colored(const DevGraph &g0, i64 *C_orig, const RunParams &p, Ru
     r.final_clusters = numClusters;
     r.final_modularity = prevMod;
     r.last_modularity = currMod;
+    r.moves = moves_;
     r.t_total = now() - t_start;
 }
 

@@ -29,6 +29,7 @@ struct RunResult {
     i64 edges_visited = 0;      // sum over phases of iterations * nnz(phase graph)
     i64 edges_visited_mod = 0;  // separate modularity passes (colored phases)
     int colors_first = 0;
+    i64 moves = 0;              // vertices moved by the colored sweeps of large graphs
     std::vector<PhaseRecord> phase;
     std::vector<double> trace;   // modularity of every iteration, all phases in order
 };
@@ -152,9 +153,13 @@ public:
 
     void sync();
 
+    // vertices moved by the whole-GPU sweeps since the engine was made
+    i64 moves() const { return moves_; }
+
 private:
     hipStream_t s_;
     BinConfig bins_;
+    i64 moves_ = 0;
     // small pinned-free scratch for scalars copied back to the host
     template <class T>
     T fetch(const T *dptr);

@@ -1,7 +1,26 @@
 // profiler.cpp -- see profiler.hpp.
 #include "profiler.hpp"
 
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+
 namespace glv {
+
+bool trace_on() {
+    static const bool on = std::getenv("GLV_TRACE") != nullptr;
+    return on;
+}
+
+void trace(const char *fmt, ...) {
+    if (!trace_on()) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::fputs("[glv] ", stderr);
+    std::vfprintf(stderr, fmt, ap);
+    std::fputc('\n', stderr);
+    va_end(ap);
+}
 
 Profiler &Profiler::get() {
     static Profiler p;
@@ -58,10 +77,17 @@ void Profiler::end(int cls, double bytes, hipStream_t s) {
     GLV_HIP(hipEventRecord(b, s));
     pending_.push_back(Rec{cls, bytes, open_[cls], b});
     open_[cls] = nullptr;
+    // long runs record hundreds of thousands of launches: fold the older half
+    // of the records once many are pending (those events completed long ago,
+    // so the wait is short) instead of holding every event until the end
+    if (pending_.size() >= kMaxPending) fold(pending_.size() / 2);
 }
 
-void Profiler::collect() {
-    for (auto &r : pending_) {
+void Profiler::collect() { fold(pending_.size()); }
+
+void Profiler::fold(size_t count) {
+    for (size_t i = 0; i < count; i++) {
+        Rec &r = pending_[i];
         GLV_HIP(hipEventSynchronize(r.b));
         float ms = 0;
         GLV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
@@ -71,7 +97,7 @@ void Profiler::collect() {
         free_.push_back(r.a);
         free_.push_back(r.b);
     }
-    pending_.clear();
+    pending_.erase(pending_.begin(), pending_.begin() + count);
 }
 
 }  // namespace glv

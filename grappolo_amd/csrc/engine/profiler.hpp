@@ -16,10 +16,11 @@ enum KClass {
     K_COLOR = 10,    // one Jones-Plassmann round
     K_COARSEN = 11,  // whole coarsening (sorts included)
     K_RENUMBER = 12,
-    K_PERSIST = 13,  // one cooperative launch per colored iteration of a small graph
+    K_PERSIST = 13,  // one persistent launch per colored iteration of a small graph
     K_COLOR_TAIL = 14,  // sequential first-fit of the vertices JP rounds left
     K_SWEEP_GIANT = 15,  // sweep of degree bin 9 (giant vertices, several workgroups each)
-    K_NUM = 16
+    K_WIDE = 16,         // one whole-GPU persistent launch per colored iteration of a large graph
+    K_NUM = 17
 };
 inline int sweep_class(int bin) { return bin <= 8 ? K_SWEEP_B1 + bin - 1 : K_SWEEP_GIANT; }
 
@@ -55,6 +56,9 @@ private:
     hipEvent_t open_[K_NUM] = {};
     KStat stats_[K_NUM];
     hipEvent_t take();
+    // folds the oldest `count` records into the totals
+    void fold(size_t count);
+    static constexpr size_t kMaxPending = 1 << 14;
 };
 
 }  // namespace glv

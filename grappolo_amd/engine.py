@@ -42,7 +42,8 @@ class Stats(ctypes.Structure):
                 ("n_trace", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("phase_nv", _I64P), ("phase_nnz", _I64P), ("phase_iters", _I32P),
                 ("phase_colors", _I32P), ("phase_colored", _I32P), ("phase_mod", _F64P),
-                ("phase_clusters", _I64P), ("trace_q", _F64P), ("phase_seconds", _F64P)]
+                ("phase_clusters", _I64P), ("trace_q", _F64P), ("phase_seconds", _F64P),
+                ("moves", ctypes.c_int64)]
 
 
 _SIG = {
@@ -128,7 +129,8 @@ _SIG = {
 KERNEL_CLASSES = ["sweep_bin1_deg4", "sweep_bin2_deg8", "sweep_bin3_deg16", "sweep_bin4_deg32",
                   "sweep_bin5_deg64", "sweep_bin6_wavehash", "sweep_bin7_blockhash",
                   "sweep_bin8_hbmhash", "modularity", "commit", "coloring_round", "coarsen",
-                  "renumber", "sweep_persistent_iteration", "coloring_tail", "sweep_bin9_giant"]
+                  "renumber", "sweep_persistent_iteration", "coloring_tail", "sweep_bin9_giant",
+                  "sweep_wide_iteration"]
 
 _fns = {}
 
@@ -180,6 +182,7 @@ def new_stats(phase_cap=256, trace_cap=4096):
 
 def stats_dict(st):
     d = {name: getattr(st, name) for name, _ in Stats._fields_[:18]}
+    d["moves"] = st.moves
     n = min(st.n_phase, st.phase_cap)
     d["per_phase"] = {k: v[:n].copy() for k, v in st._arrays.items() if k != "trace_q"}
     d["trace"] = st._arrays["trace_q"][:min(st.n_trace, st.trace_cap)].copy()

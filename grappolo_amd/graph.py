@@ -8,6 +8,7 @@ uploads.  ``edge_list()`` rebuilds the 24-byte AoS records when a caller needs
 the reference layout.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -144,6 +145,20 @@ def gen_planted(nv, nblocks, mean_degree, mu, seed, wlo=1.0, whi=1.0, return_blo
              blocks.ctypes.data_as(_I64P), ctypes.byref(csr)), "glv_gen_planted")
     g = _take(csr)
     return (g, blocks) if return_blocks else g
+
+
+def gen_planted_many(specs, workers=None):
+    """gen_planted over a list of (nv, nblocks, mean_degree, mu, seed), the
+    graphs generated concurrently one per host thread (the generator's output
+    does not depend on its thread count)."""
+    from concurrent.futures import ThreadPoolExecutor
+    workers = workers or min(16, os.cpu_count() or 1)
+    set_threads(1)
+    try:
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            return list(ex.map(lambda sp: gen_planted(*sp[:4], seed=sp[4]), specs))
+    finally:
+        set_threads(0)
 
 
 def gen_lfr(nv, seed, deg_exp=2.5, dmin=10, dmax=1000, comm_exp=1.5, cmin=20, cmax=5000, mu=0.4,

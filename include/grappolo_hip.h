@@ -68,6 +68,8 @@ typedef struct glv_stats {
     int64_t *phase_clusters;
     double *trace_q;          /* modularity after every iteration of every phase */
     double *phase_seconds;    /* wall time of each phase: sweeps, renumbering, coarsening, coloring */
+    int64_t moves;            /* vertices moved by the whole-GPU colored sweeps (graphs above
+                                 262,144 vertices; each move is 2 FP64 and 2 int64 atomics) */
 } glv_stats;
 
 /* ---- device and errors ------------------------------------------------ */

@@ -266,3 +266,53 @@ def test_trained_predictor_prunes_work_at_equal_modularity():
     # exceed the partition's own modularity by that phase's last change
     q = engine.modularity_dev(dg, C1)
     assert abs(q - st1["final_modularity"]) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_learned_workload_32x1M_training_8M_heldout():
+    """The learned active-set workload itself: the network trained on the
+    unpruned method's traces of 32 LFR graphs of 1,048,576 vertices (seeds
+    100-131, one row in eight sampled), its threshold chosen for a mover
+    recall of 0.98 on the trace of a held-out graph (seed 132), then both
+    forms run on the held-out LFR graph of 8,388,608 vertices (seed 200).
+    The pruned run must read fewer CSR entries than the unpruned one and end
+    no lower than 1e-3 below its final modularity."""
+    import time
+
+    import torch
+    from grappolo_amd import active_set as las
+    from grappolo_amd import engine
+    from grappolo_amd.graph import gen_lfr
+    t0 = time.perf_counter()
+    feats, labels, calib = [], [], None
+    for i in range(33):
+        g = gen_lfr(1048576, seed=100 + i)
+        dg = engine.DeviceGraph(g, "cuda")
+        C = torch.empty(g.numVertices, dtype=torch.int64, device="cuda")
+        f, l, _, _, _ = las.trace_dev(dg, C, rate=0.125, seed=1000 + i, cap=1 << 24)
+        if i < 32:
+            feats.append(f.clone())
+            labels.append(l.clone())
+        else:
+            calib = (f.clone(), l.clone())
+        del dg, C, f, l
+    print(f"traces in {time.perf_counter() - t0:.1f}s", flush=True)
+    feats, labels = torch.cat(feats), torch.cat(labels)
+    assert feats.shape[0] > 10_000_000
+    pred = las.ActiveSetPredictor(seed=1)
+    losses = pred.fit(feats, labels, steps=3000, batch=4096, lr=2e-3)
+    assert float(losses[-50:].mean()) < float(losses[:50].mean())
+    tau, kept = pred.calibrate(calib[0], calib[1], recall=0.98)
+    assert 0 < tau < 1 and kept < 0.8
+    g = gen_lfr(8388608, seed=200)
+    dg = engine.DeviceGraph(g, "cuda")
+    C0 = torch.empty(g.numVertices, dtype=torch.int64, device="cuda")
+    C1 = torch.empty_like(C0)
+    st0 = engine.run_colored_dev(dg, C0)
+    st1, ls = pred.run(dg, C1, max_rechecks=1)
+    print(f"plain Q {st0['final_modularity']:.6f} entries {st0['edges_visited']}, pruned Q "
+          f"{st1['final_modularity']:.6f} entries {ls['swept_entries']}", flush=True)
+    assert st1["final_modularity"] >= st0["final_modularity"] - 1e-3
+    assert ls["swept_entries"] < st0["edges_visited"]
+    q = engine.modularity_dev(dg, C1)
+    assert abs(q - st1["final_modularity"]) <= 1e-3

@@ -4,7 +4,7 @@ on integer weights), through the C ABI."""
 import numpy as np
 import pytest
 
-from grappolo_amd.graph import Graph, gen_planted, union
+from grappolo_amd.graph import Graph, gen_planted, gen_planted_many, union
 
 from graphs import hubs, karate, with_loops_and_isolated
 
@@ -117,3 +117,43 @@ def test_autotuned_bins_keep_results():
     assert st0["final_modularity"] == st1["final_modularity"]
     with pytest.raises(engine.GlvError):
         engine.set_bins(np.array([0, 8, 8, 16, 32, 64, 256, 2048, 16384, default[9]], np.int64))
+
+
+def test_batched_4096_graph_workload():
+    """The batched workload itself: the 4096 planted-partition graphs of
+    bench.batch_specs() (vertices log-uniform on [100, 50000], blocks
+    log-uniform on [4, 256], mean degree U[8, 32], mixing U[0.1, 0.5], unit
+    weights, seeds 1000 + g) in one batched runMultiPhaseBasic.  Every 128th
+    graph is run alone and must match bit for bit (labels, modularity,
+    phases, iterations); the smallest graphs of that sample are checked
+    against the oracle's runMultiPhaseBasic (BasicCommunitiesDetection/
+    runMultiPhaseBasic.cpp:52-170)."""
+    import time
+
+    import torch
+
+    import bench
+    import oracle
+    from grappolo_amd import engine
+    t = time.perf_counter()
+    specs = bench.batch_specs(4096)
+    graphs = gen_planted_many(specs)
+    U, vb = union(graphs)
+    print(f"batch workload: {U} in {time.perf_counter() - t:.1f}s", flush=True)
+    dg = engine.DeviceGraph(U, "cuda")
+    C = torch.empty(U.numVertices, dtype=torch.int64, device="cuda")
+    res = engine.run_basic_batch_dev(dg, vb, C)
+    Cb = C.cpu().numpy()
+    del dg
+    sample = list(range(0, len(graphs), 128))
+    for i in sample:
+        Ci, st = _standalone(graphs[i], 0)
+        np.testing.assert_array_equal(Cb[vb[i]:vb[i + 1]], Ci, err_msg=f"graph {i}")
+        assert res["final_modularity"][i] == st["final_modularity"], i
+        assert res["phases"][i] == st["phases"], i
+        assert res["total_iterations"][i] == st["total_iterations"], i
+    small = sorted(sample, key=lambda i: graphs[i].numVertices)[:6]
+    for i in small:
+        Co, so = oracle.run_basic(graphs[i], basicOpt=0)
+        np.testing.assert_array_equal(Cb[vb[i]:vb[i + 1]], Co, err_msg=f"graph {i} vs oracle")
+        assert res["final_modularity"][i] == so["final_modularity"], i

@@ -28,7 +28,9 @@ The RMAT scale-24 workload (16,777,216 vertices, edge factor 16, a/b/c/d
 0.57/0.19/0.19/0.05, symmetrised, unit weights, seed 3: hub vertices take the
 giant-vertex paths of the coloring and the sweep) is checked through its
 coloring's validity, its labels and the modularity of its run against the
-oracle's; the oracle's coloring of it would take about an hour on one core
+oracle's.  On both graphs the run's final modularity must reach the
+reference's own result (its runMultiPhaseColoring compiled from its sources
+and run here, golden/full_size_reference.json) within 1e-3; the oracle's coloring of it would take about an hour on one core
 (3x per scale: 5.5, 15.6, 45.8 s at scales 18, 19, 20).
 """
 import hashlib
@@ -44,6 +46,14 @@ pytestmark = pytest.mark.gpu
 
 JP_TAIL = 1024  # vertices coloured by the sequential tail at most
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "full_size_coloring.json")
+# the reference's own runMultiPhaseColoring on the same graphs (compiled from
+# its sources, -Ofast -fopenmp; written by golden/make_full_size_reference.py)
+REFERENCE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "full_size_reference.json")
+
+
+def reference(name):
+    with open(REFERENCE) as f:
+        return json.load(f)[name]
 
 
 def sha(*arrays):
@@ -141,6 +151,11 @@ def test_full_size_run_invariants(lfr4m):
     q_coarse = engine.modularity_dev(dg2, torch.arange(k1, dtype=torch.int64, device="cuda"))
     assert abs(q_coarse - q_fine) <= 1e-12 * abs(q_fine)
     assert st["final_modularity"] > q_fine
+    # the reference's printed final modularity on this graph
+    # (Coloring/runMultiPhaseColoring.cpp:216): within 1e-3 or above
+    ref = reference("lfr4m")
+    assert ref["graph_sha256"] == sha(g.edgeListPtrs, g.tails, g.weights)
+    assert st["final_modularity"] >= ref["reference"]["final_modularity"] - 1e-3
 
 
 def test_full_size_rmat24_coloring_and_run():
@@ -150,6 +165,8 @@ def test_full_size_rmat24_coloring_and_run():
     print("[full size] generating RMAT scale 24, seed 3", flush=True)
     g = gen_rmat(24, seed=3)
     assert g.numVertices == 1 << 24
+    ref = reference("rmat24")
+    assert ref["graph_sha256"] == sha(g.edgeListPtrs, g.tails, g.weights)
     dg = engine.DeviceGraph(g, "cuda")
     colors = torch.empty(g.numVertices, dtype=torch.int32, device="cuda")
     mc, _ = engine.color_dev(dg, colors)
@@ -167,5 +184,8 @@ def test_full_size_rmat24_coloring_and_run():
     q_dev = engine.modularity_dev(dg, C)
     q_ora = oracle.modularity(g, labels)
     assert abs(q_dev - q_ora) <= 1e-12 * abs(q_ora)
-    # a skewed graph with little community structure (0.0597 here), but positive
+    # a skewed graph with little community structure, but positive, and within
+    # 1e-3 of the reference's printed final modularity or above
+    # (Coloring/runMultiPhaseColoring.cpp:216)
     assert st["final_modularity"] > 0.0
+    assert st["final_modularity"] >= ref["reference"]["final_modularity"] - 1e-3
