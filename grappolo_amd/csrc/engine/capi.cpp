@@ -19,6 +19,7 @@
 #include "grappolo_hip.h"
 #include "kernels.hpp"
 #include "louvain.hpp"
+#include "noalloc.hpp"
 #include "mlp.hpp"
 #include "profiler.hpp"
 
@@ -649,6 +650,31 @@ int glv_modularity_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const 
         Engine E(s);
         double q = E.modularity(g, d_C);
         if (modularity) *modularity = q;
+    });
+}
+
+int glv_modularity_noalloc_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                               const double *d_weights, const int64_t *d_C, double *modularity,
+                               void *stream) {
+    return guarded([&] {
+        if (nv < 0 || nnz < 0) glv::fail(GLV_E_INVALID_ARG, "negative size");
+        if (nv > glv::noalloc_max_nv())
+            glv::fail(GLV_E_INVALID_ARG, "graph has more vertices than the no-allocation scratch holds (" +
+                                             std::to_string(glv::noalloc_max_nv()) + ")");
+        if (nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        if (!d_offsets || !d_tails || !d_weights || (nv > 0 && !d_C))
+            glv::fail(GLV_E_INVALID_ARG, "null device pointer");
+        hipStream_t s = engine_stream(stream);
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        const double q = glv::modularity_noalloc(g, d_C, s);
+        if (modularity) *modularity = q;
+    });
+}
+
+int glv_noalloc_max_vertices(int64_t *max_nv) {
+    return guarded([&] {
+        if (!max_nv) glv::fail(GLV_E_INVALID_ARG, "null argument");
+        *max_nv = glv::noalloc_max_nv();
     });
 }
 

@@ -195,6 +195,19 @@ int glv_color_step_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const 
 int glv_modularity_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
                        const double *d_weights, const int64_t *d_C, double *modularity,
                        void *stream);
+/* No counterpart in the reference.  glv_modularity_dev for callers that
+ * cannot allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc): same validation and result (to FP64
+ * rounding of the sums; exact on integer weights), but no allocation of any
+ * kind -- its scratch is a fixed device array of 64 MiB + 64 KiB per GPU, held
+ * from the library's first use on that GPU until the process ends.  Graphs of
+ * more than glv_noalloc_max_vertices() vertices (4,194,304) are rejected with
+ * GLV_E_INVALID_ARG.  Calls that overlap on one GPU share that array and race:
+ * the caller serialises them. */
+int glv_modularity_noalloc_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                               const double *d_weights, const int64_t *d_C, double *modularity,
+                               void *stream);
+int glv_noalloc_max_vertices(int64_t *max_nv);
 int glv_renumber_dev(int64_t n, int64_t *d_C, int64_t *numClusters, void *stream);
 /* Coarsened graph in engine-allocated device memory; release with
  * glv_dev_free on each of the three pointers. */
