@@ -433,16 +433,17 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     DBuf<unsigned> wbar(use_wide ? 16 + 16 * (size_t)wblocks : 0, s_);
     DBuf<double> wpart(use_wide ? 2 * (size_t)wblocks : 0, s_);
     DBuf<i64> wmoves(use_wide ? 1 : 0, s_);
+    DBuf<CommInfo> wm0(use_wide ? nv : 0, s_), wm1(use_wide ? nv : 0, s_);
     if (use_wide) {
         build_wide_plan(g, P, wblocks, WP, s_);
         launch_fill_i64(wmoves.p, 1, 0, s_);
         W.A = A;
         W.C0 = C;
         W.C1 = pC1.p;
-        W.a0 = P.a.p;
-        W.a1 = pa1.p;
-        W.n0 = P.n.p;
-        W.n1 = pn1.p;
+        W.a = P.a.p;
+        W.n = P.n.p;
+        W.m0 = wm0.p;
+        W.m1 = wm1.p;
         W.perm = P.perm.p;
         W.cperm = cperm.p;
         W.cstart = cstart.p;

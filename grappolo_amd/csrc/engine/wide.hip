@@ -184,12 +184,17 @@ __device__ inline void wg_best(Cand &c, Red &R) {
 // write the moves into the other buffer.
 struct Step {
     const i64 *C;
-    const double *a;
-    const i64 *n;
+    const CommInfo *m;
     i64 *Cq;
-    double *aq;
-    i64 *nq;
+    CommInfo *mq;
 };
+
+__device__ inline void move_comm(i64 t, i64 s, double kv, CommInfo *m) {
+    atomicAdd(&m[t].a, kv);
+    atomicAdd(&m[s].a, -kv);
+    atomicAdd((unsigned long long *)&m[t].n, 1ULL);
+    atomicAdd((unsigned long long *)&m[s].n, ~0ULL);  // -1
+}
 
 // Move decision of vertex v (one thread): the best candidate, or stay; the
 // swap rule of utilityClusteringFunctions.cpp:245-256 (two singletons move
@@ -203,7 +208,7 @@ __device__ inline void decide(const WideArgs &W, const Step &S, i64 v, i64 s, do
     }
     W.A.tgt[v] = target;
     if (target != s) {
-        move_vertex(target, s, kv, S.aq, S.nq);
+        move_comm(target, s, kv, S.mq);
         S.Cq[v] = target;
         dein += 2.0 * (c.e - eix);
         moved++;
@@ -262,8 +267,9 @@ __device__ inline void score_lds(const Step &S, const uint32_t *keys, const doub
 #pragma unroll
         for (int u = 0; u < kWUnroll; u++) {
             const bool y = yk[u] != kEmptyKey && yk[u] != s;
-            ay[u] = y ? S.a[yk[u]] : 0.0;
-            ny[u] = y ? S.n[yk[u]] : 0;
+            const CommInfo m = y ? S.m[yk[u]] : CommInfo{0.0, 0};
+            ay[u] = m.a;
+            ny[u] = m.n;
         }
 #pragma unroll
         for (int u = 0; u < kWUnroll; u++) {
@@ -284,8 +290,9 @@ __device__ void heavy_vertex(const WideArgs &W, const Step &S, i64 v, uint32_t *
     const double kv = W.A.k[v];
     const uint32_t T = (uint32_t)std::min<i64>(kWHeavySlots, (2 * (end - beg) + kWBlock - 1) / kWBlock * kWBlock);
     clear_table(keys, vals, T, threadIdx.x, kWBlock);
-    const double as = S.a[s];
-    const i64 ns = S.n[s];
+    const CommInfo ms = S.m[s];
+    const double as = ms.a;
+    const i64 ns = ms.n;
     __syncthreads();
     double sl = wg_gather(W, S, v, beg, end, keys, vals, T);
     sl = wg_sum(sl, R);
@@ -366,7 +373,8 @@ __device__ bool giant_score(const WideArgs &W, const Step &S, i64 g, int j, Red 
     const i64 v = G.v;
     const i64 s = S.C[v];
     const double kv = W.A.k[v];
-    const double as = S.a[s];
+    const CommInfo ms = S.m[s];
+    const double as = ms.a;
     if (threadIdx.x == 0) {
         int ok = 1;
         unsigned spins = 0;
@@ -402,8 +410,9 @@ __device__ bool giant_score(const WideArgs &W, const Step &S, i64 g, int j, Red 
 #pragma unroll
         for (int u = 0; u < kWUnroll; u++) {
             const bool y = yk[u] != kEmptyKey && yk[u] != (uint32_t)s;
-            ay[u] = y ? S.a[yk[u]] : 0.0;
-            ny[u] = y ? S.n[yk[u]] : 0;
+            const CommInfo m = y ? S.m[yk[u]] : CommInfo{0.0, 0};
+            ay[u] = m.a;
+            ny[u] = m.n;
             if (yk[u] != kEmptyKey) {  // empty between uses
                 const i64 i = i0 + (i64)u * kWBlock;
                 __hip_atomic_store(&gk[i], kEmptyKey, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -442,7 +451,7 @@ __device__ bool giant_score(const WideArgs &W, const Step &S, i64 g, int j, Red 
                 }
             }
             double dg = 0;
-            decide(W, S, v, s, kv, S.n[s], b, eix, dg, moved);
+            decide(W, S, v, s, kv, ms.n, b, eix, dg, moved);
             W.gout[g] = dg;
             // ready for the next iteration
             __hip_atomic_store(&W.gstate[3 * g], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -465,8 +474,9 @@ __device__ void mid_vertex(const WideArgs &W, const Step &S, i64 v, uint32_t *ke
     uint32_t T = 64;
     while ((i64)T < 2 * (end - beg)) T <<= 1;
     clear_table(keys, vals, T, lane, kWave);
-    const double as = S.a[s];
-    const i64 ns = S.n[s];
+    const CommInfo ms = S.m[s];
+    const double as = ms.a;
+    const i64 ns = ms.n;
     wave_lds_sync();
     double sl = 0;
     for (i64 j0 = beg + lane; j0 < end; j0 += (i64)kWUnroll * kWave) {
@@ -529,8 +539,9 @@ __device__ void light_vertices(const WideArgs &W, const Step &S, i64 pos, int co
         const i64 t = W.A.tails[beg + gl];
         ww = W.A.w[beg + gl];
         if (gl == 0) {
-            as = S.a[s];
-            ns = S.n[s];
+            const CommInfo ms = S.m[s];
+            as = ms.a;
+            ns = ms.n;
         }
         c = (int32_t)S.C[t];
         self = t == v;
@@ -558,10 +569,9 @@ __device__ void light_vertices(const WideArgs &W, const Step &S, i64 pos, int co
     as = __shfl(as, gb, kWave);
     Cand b;
     if (gl < deg && first && c != s) {
-        const double ac = S.a[c];
-        const i64 nc = S.n[c];
+        const CommInfo mc = S.m[c];
         // max() starts from maxGain = 0 and needs a strict increase
-        offer(b, gain_of(e_mine, e_s - sl, kv, ac, as - kv, W.A.constant), c, e_mine, nc);
+        offer(b, gain_of(e_mine, e_s - sl, kv, mc.a, as - kv, W.A.constant), c, e_mine, mc.n);
     }
     group_best<G>(b);
     if (valid && gl == 0) decide(W, S, v, s, kv, ns, b, e_s - sl, dein, moved);
@@ -586,8 +596,9 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
     // the second buffer starts equal to the first
     for (i64 i = gtid; i < nv; i += gstride) {
         W.C1[i] = W.C0[i];
-        W.a1[i] = W.a0[i];
-        W.n1[i] = W.n0[i];
+        const CommInfo m{W.a[i], W.n[i]};
+        W.m0[i] = m;
+        W.m1[i] = m;
     }
     unsigned seq = 0;
     bool alive = grid_barrier(W.bar, nb, ++seq);
@@ -603,11 +614,9 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
         if (hi == lo) continue;
         Step S;
         S.C = par ? W.C1 : W.C0;
-        S.a = par ? W.a1 : W.a0;
-        S.n = par ? W.n1 : W.n0;
+        S.m = par ? W.m1 : W.m0;
         S.Cq = par ? W.C0 : W.C1;
-        S.aq = par ? W.a0 : W.a1;
-        S.nq = par ? W.n0 : W.n1;
+        S.mq = par ? W.m0 : W.m1;
         // workgroup items
         const i64 wlo = W.wstart[(i64)c * nb + blockIdx.x], whi = W.wstart[(i64)c * nb + blockIdx.x + 1];
         for (i64 i = wlo; i < whi; i++) {
@@ -642,7 +651,7 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
             const i64 v = W.cperm[q];
             const i64 t = W.A.tgt[v], s = S.Cq[v];
             if (s >= 0 && t != s) {  // vertices without entries (label -1) never move
-                move_vertex(t, s, W.A.k[v], S.aq, S.nq);
+                move_comm(t, s, W.A.k[v], S.mq);
                 S.Cq[v] = t;
             }
         }
@@ -651,7 +660,7 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
         plo = lo;
         phi = hi;
     }
-    const double *af = par ? W.a1 : W.a0;
+    const CommInfo *mf = par ? W.m1 : W.m0;
     // modularity partials: the movers' change of the internal weight (giants
     // in giant order) and the sum of a^2
     double exx = 0, a2 = 0;
@@ -659,7 +668,12 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
         exx = dein;
         if (blockIdx.x == 0 && threadIdx.x == 0)
             for (i64 g = 0; g < W.ngiants; g++) exx += W.gout[g];
-        for (i64 i = gtid; i < nv; i += gstride) a2 += af[i] * af[i];
+        for (i64 i = gtid; i < nv; i += gstride) {
+            const CommInfo m = mf[i];
+            a2 += m.a * m.a;
+            W.a[i] = m.a;
+            W.n[i] = m.n;
+        }
     }
     exx = wg_sum(exx, R);
     a2 = wg_sum(a2, R);
@@ -674,13 +688,8 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
         W.part[nb + blockIdx.x] = a2;
         if (s_moved) atomicAdd((unsigned long long *)W.moves, s_moved);
     }
-    if (alive && par) {  // the final state sits in the second buffer
-        for (i64 i = gtid; i < nv; i += gstride) {
-            W.C0[i] = W.C1[i];
-            W.a0[i] = W.a1[i];
-            W.n0[i] = W.n1[i];
-        }
-    }
+    if (alive && par)  // the final labels sit in the second buffer
+        for (i64 i = gtid; i < nv; i += gstride) W.C0[i] = W.C1[i];
 }
 
 __global__ void k_wide_degrees(const i64 *perm, const i64 *off, i64 lo, i64 hi, i64 *vd) {

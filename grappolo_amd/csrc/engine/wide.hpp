@@ -64,11 +64,19 @@ struct WideGiant {
     int32_t nslices;
 };
 
+// Degree and size of a community side by side (one 16-byte load serves both,
+// as the reference's Comm record, DefineStructure/defs.h:66-70, holds them).
+struct CommInfo {
+    double a;
+    i64 n;
+};
+
 struct WideArgs {
-    SweepArgs A;                 // graph, k, constant, tgt (A.C/a/n: the first state buffer)
-    i64 *C0, *C1;                // labels, two buffers
-    double *a0, *a1;             // community degrees
-    i64 *n0, *n1;                // community sizes
+    SweepArgs A;                 // graph, k, constant, tgt
+    i64 *C0, *C1;                // labels, two buffers (C0: the phase's labels)
+    double *a;                   // community degrees and sizes of the phase, read at the
+    i64 *n;                      //   start of the iteration and written at its end
+    CommInfo *m0, *m1;           // community degrees and sizes, two buffers
     const i64 *perm;             // vertices grouped by (degree bin, color)
     const i64 *cperm;            // vertices grouped by color
     const i64 *cstart;           // ncol + 1
