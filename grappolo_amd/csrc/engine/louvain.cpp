@@ -434,8 +434,8 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     DBuf<double> wpart(use_wide ? 2 * (size_t)wblocks : 0, s_);
     DBuf<i64> wmoves(use_wide ? 1 : 0, s_);
     DBuf<CommInfo> wm0(use_wide ? nv : 0, s_), wm1(use_wide ? nv : 0, s_);
+    if (use_wide) use_wide = build_wide_plan(g, P, wblocks, WP, s_);
     if (use_wide) {
-        build_wide_plan(g, P, wblocks, WP, s_);
         launch_fill_i64(wmoves.p, 1, 0, s_);
         W.A = A;
         W.C0 = C;
@@ -453,13 +453,14 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         W.vstart = WP.vstart.p;
         W.giants = WP.giants.p;
         W.ngiants = WP.ngiants;
-        W.gkeys = WP.gkeys.p;
-        W.gvals = WP.gvals.p;
+        W.gpk = WP.gpk.p;
+        W.gpv = WP.gpv.p;
+        W.gcnt = WP.gcnt.p;
         W.gstate = WP.gstate.p;
         W.gsc = WP.gsc.p;
         W.gcand = WP.gcand.p;
         W.gcid = WP.gcid.p;
-        W.gslices = WP.gslices;
+        W.gparts = WP.gparts;
         W.gout = WP.gout.p;
         W.ncol = ncol;
         W.bar = wbar.p;
@@ -585,6 +586,8 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 rb.add(h, res.p, sizeof(h));
                 rb.add(&abort_flag, wbar.p, sizeof(unsigned));
                 rb.run();
+                if (abort_flag & 2)
+                    fail(GLV_E_UNSUPPORTED, "a partition of a giant vertex's communities outgrew the LDS table");
                 if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the wide sweep timed out");
                 if (numItrs == 1 || !valid_colors) {
                     // a full pass: the first iteration, or classes that may hold an edge

@@ -58,33 +58,32 @@ __device__ inline double lds_lookup(const uint32_t *keys, const double *vals, ui
     }
 }
 
-// giant tables in HBM: power-of-two sizes, Fibonacci hashing
-__device__ inline u64 gslot(uint32_t key, int lg) { return ((u64)key * kFib) >> (64 - lg); }
+// partition of a community id among P (a hash independent of slot_of's, so
+// the ids of one partition still spread over a table's slots)
+__device__ inline int part_of(uint32_t key, int P) {
+    uint32_t h = key * 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return (int)(((u64)h * (u64)P) >> 32);
+}
 
-__device__ inline void hbm_insert(uint32_t *keys, double *vals, int lg, u64 mask, uint32_t key, double w) {
-    u64 h = gslot(key, lg);
-    while (true) {
-        uint32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// lds_insert that gives up (false) once every slot was probed
+__device__ inline bool lds_insert_bounded(uint32_t *keys, double *vals, uint32_t T, uint32_t key, double w) {
+    uint32_t h = slot_of(key, T);
+    for (uint32_t probes = 0; probes < T; probes++) {
+        uint32_t k = keys[h];
         if (k == kEmptyKey) {
             k = atomicCAS(&keys[h], kEmptyKey, key);
             if (k == kEmptyKey) k = key;
         }
         if (k == key) {
             atomicAdd(&vals[h], w);
-            return;
+            return true;
         }
-        h = (h + 1) & mask;
+        h = h + 1 == T ? 0 : h + 1;
     }
-}
-
-__device__ inline double hbm_lookup(const uint32_t *keys, const double *vals, int lg, u64 mask, uint32_t key) {
-    u64 h = gslot(key, lg);
-    while (true) {
-        const uint32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k == key) return ld_agent(&vals[h]);
-        if (k == kEmptyKey) return 0.0;
-        h = (h + 1) & mask;
-    }
+    return false;
 }
 
 // LDS writes of a wavefront are visible to its other lanes after this
@@ -113,6 +112,29 @@ __device__ inline double wg_sum(double v, Red &R) {
     double r = 0;
     for (int q = 0; q < kWWaves; q++) r += R.d[q];
     return r;
+}
+
+// Exclusive prefix sum over the workgroup (one value per thread); *total
+// gets the sum.  Uses R.c.
+__device__ inline int wg_exscan(int x, Red &R, int *total) {
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    int incl = x;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    __syncthreads();
+    if (lane == kWave - 1) R.c[w] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int q = 0; q < kWWaves; q++) {
+        const int t = R.c[q];
+        if (q < w) before += t;
+        all += t;
+    }
+    *total = all;
+    return before + incl - x;
 }
 
 // The best move of a vertex so far: target community, its gain, the
@@ -305,76 +327,81 @@ __device__ void heavy_vertex(const WideArgs &W, const Step &S, i64 v, uint32_t *
     __syncthreads();  // the table is cleared by the next item
 }
 
-// Chunk j of giant g: its entries summed per community in LDS, then added to
-// the giant's HBM table; the workgroup adding the last chunk looks up the
-// weight to the giant's own community and marks the table ready.
-__device__ void giant_chunk(const WideArgs &W, const Step &S, i64 g, int j, uint32_t *keys, double *vals,
+// Chunk j of giant g: its entries summed per community in LDS, then written
+// out as (community, weight) pairs grouped by partition (bounds in gcnt), and
+// its part of the self-loop weight and of the weight to the giant's own
+// community added to gsc; the workgroup writing the last chunk marks the
+// pairs ready.  pc: kWMaxParts + 1 ints of LDS.
+__device__ void giant_chunk(const WideArgs &W, const Step &S, i64 g, int j, uint32_t *keys, double *vals, int *pc,
                             Red &R) {
     const WideGiant G = W.giants[g];
     const i64 v = G.v;
+    const i64 s = S.C[v];
+    const int P = G.nparts;
     const i64 beg0 = W.A.off[v], end0 = W.A.off[v + 1];
     const i64 beg = beg0 + (i64)j * kWChunk, end = std::min(beg + kWChunk, end0);
     const uint32_t T = (uint32_t)std::min<i64>(kWHeavySlots, (2 * (end - beg) + kWBlock - 1) / kWBlock * kWBlock);
     clear_table(keys, vals, T, threadIdx.x, kWBlock);
+    for (int q = threadIdx.x; q <= P; q += kWBlock) pc[q] = 0;
     __syncthreads();
     double sl = wg_gather(W, S, v, beg, end, keys, vals, T);
     sl = wg_sum(sl, R);
-    int lg = 0;
-    while (((i64)1 << lg) < G.slots) lg++;
-    const u64 mask = (u64)G.slots - 1;
-    uint32_t *gk = W.gkeys + G.tab;
-    double *gv = W.gvals + G.tab;
-    // every thread first claims the home slots of its kWUnroll keys at once
-    // (most land there at load factor <= 1/2), then probes on for the rest
-    for (uint32_t i0 = threadIdx.x; i0 < T; i0 += kWUnroll * kWBlock) {
-        uint32_t k[kWUnroll], old[kWUnroll];
-        double x[kWUnroll];
-        u64 h[kWUnroll];
-#pragma unroll
-        for (int u = 0; u < kWUnroll; u++) {
-            const uint32_t i = i0 + u * kWBlock;
-            k[u] = i < T ? keys[i] : kEmptyKey;
-            x[u] = i < T ? vals[i] : 0.0;
-            h[u] = gslot(k[u], lg);
-        }
-#pragma unroll
-        for (int u = 0; u < kWUnroll; u++) old[u] = k[u] != kEmptyKey ? atomicCAS(&gk[h[u]], kEmptyKey, k[u]) : kEmptyKey;
-#pragma unroll
-        for (int u = 0; u < kWUnroll; u++) {
-            if (k[u] == kEmptyKey) continue;
-            if (old[u] == kEmptyKey || old[u] == k[u])
-                atomicAdd(&gv[h[u]], x[u]);
-            else
-                hbm_insert(gk, gv, lg, mask, k[u], x[u]);
-        }
+    const double es = lds_lookup(keys, vals, T, (uint32_t)s);
+    for (uint32_t i = threadIdx.x; i < T; i += kWBlock) {
+        const uint32_t k = keys[i];
+        if (k != kEmptyKey) atomicAdd(&pc[part_of(k, P)], 1);
     }
-    if (threadIdx.x == 0 && sl != 0.0) atomicAdd(&W.gsc[2 * g], sl);
-    __threadfence();
+    __syncthreads();
+    int total = 0;
+    const int ex = wg_exscan(threadIdx.x < (unsigned)P ? pc[threadIdx.x] : 0, R, &total);
+    __syncthreads();  // every count was read
+    int32_t *cnt = W.gcnt + G.cnt + (i64)j * (P + 1);
+    if (threadIdx.x < (unsigned)P) {
+        pc[threadIdx.x] = ex;
+        cnt[threadIdx.x] = ex;
+    }
+    if (threadIdx.x == 0) cnt[P] = total;
+    __syncthreads();
+    uint32_t *pk = W.gpk + G.pairs + (i64)j * kWChunk;
+    double *pv = W.gpv + G.pairs + (i64)j * kWChunk;
+    for (uint32_t i = threadIdx.x; i < T; i += kWBlock) {
+        const uint32_t k = keys[i];
+        if (k == kEmptyKey) continue;
+        const int at = atomicAdd(&pc[part_of(k, P)], 1);
+        pk[at] = k;
+        pv[at] = vals[i];
+    }
+    // every wavefront's pairs are written before thread 0 releases them (one
+    // agent-scope fence for the workgroup)
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (sl != 0.0) atomicAdd(&W.gsc[2 * g], sl);
+        if (es != 0.0) atomicAdd(&W.gsc[2 * g + 1], es);
+        __threadfence();
         const int done = atomicAdd(&W.gstate[3 * g], 1) + 1;
         if (done == G.nchunks) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const double e_s = hbm_lookup(gk, gv, lg, mask, (uint32_t)S.C[v]);
-            __hip_atomic_store(&W.gsc[2 * g + 1], e_s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&W.gstate[3 * g + 1], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
 }
 
-// Slice j of giant g's table: waits for the table, scores and clears its
-// slots; the workgroup scoring the last slice merges the slices' candidates in
-// slice order and decides the giant's move.  Returns false when the wait gave
-// up (abort raised).
-__device__ bool giant_score(const WideArgs &W, const Step &S, i64 g, int j, Red &R, i64 &moved) {
+// Partition p of giant g: waits for the pairs, sums the partition's pairs of
+// every chunk in the LDS table and scores them; the workgroup scoring the
+// last partition merges the partitions' candidates in partition order and
+// decides the giant's move.  pc: 2 * kWMaxChunks + 1 ints of LDS.  Returns
+// false when the wait gave up or the partition outgrew the table (abort
+// raised: bit 1, or bit 2 for the table).
+__device__ bool giant_part(const WideArgs &W, const Step &S, i64 g, int p, uint32_t *keys, double *vals, int *pc,
+                           Red &R, i64 &moved) {
     __shared__ int s_ok;
     const WideGiant G = W.giants[g];
     const i64 v = G.v;
     const i64 s = S.C[v];
     const double kv = W.A.k[v];
     const CommInfo ms = S.m[s];
-    const double as = ms.a;
+    const int P = G.nparts, nc = G.nchunks;
     if (threadIdx.x == 0) {
         int ok = 1;
         unsigned spins = 0;
@@ -388,57 +415,79 @@ __device__ bool giant_score(const WideArgs &W, const Step &S, i64 g, int j, Red 
         }
         s_ok = ok;
     }
-    __syncthreads();
+    __syncthreads();  // thread 0's acquire covers the workgroup
     if (!s_ok) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const double sl = ld_agent(&W.gsc[2 * g]), e_s = ld_agent(&W.gsc[2 * g + 1]);
-    const double eix = e_s - sl, ax = as - kv;
-    uint32_t *gk = W.gkeys + G.tab;
-    double *gv = W.gvals + G.tab;
-    const i64 lo = (i64)j * kWScoreSlice, hi = std::min(lo + kWScoreSlice, G.slots);
-    Cand c;
-    for (i64 i0 = lo + threadIdx.x; i0 < hi; i0 += (i64)kWUnroll * kWBlock) {
-        uint32_t yk[kWUnroll];
-        double e[kWUnroll], ay[kWUnroll];
-        i64 ny[kWUnroll];
+    const double eix = e_s - sl, ax = ms.a - kv;
+    // this partition's segment of every chunk
+    int lo = 0, len = 0;
+    if (threadIdx.x < (unsigned)nc) {
+        const int32_t *cnt = W.gcnt + G.cnt + (i64)threadIdx.x * (P + 1);
+        lo = cnt[p];
+        len = cnt[p + 1] - lo;
+    }
+    int total = 0;
+    const int ex = wg_exscan(len, R, &total);
+    int *pre = pc, *seg = pc + kWMaxChunks + 1;
+    if (threadIdx.x < (unsigned)nc) {
+        pre[threadIdx.x] = ex;
+        seg[threadIdx.x] = lo;
+    }
+    if (threadIdx.x == 0) {
+        pre[nc] = total;
+        s_ok = 1;
+    }
+    const uint32_t T = (uint32_t)std::min<i64>(kWHeavySlots, ((i64)2 * total + kWBlock - 1) / kWBlock * kWBlock);
+    clear_table(keys, vals, std::max<uint32_t>(T, kWBlock), threadIdx.x, kWBlock);
+    __syncthreads();
+    const uint32_t *pk = W.gpk + G.pairs;
+    const double *pv = W.gpv + G.pairs;
+    for (int i0 = threadIdx.x; i0 < total; i0 += kWUnroll * kWBlock) {
+        uint32_t k[kWUnroll];
+        double x[kWUnroll];
 #pragma unroll
         for (int u = 0; u < kWUnroll; u++) {
-            const i64 i = i0 + (i64)u * kWBlock;
-            yk[u] = i < hi ? __hip_atomic_load(&gk[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kEmptyKey;
-            e[u] = yk[u] != kEmptyKey ? ld_agent(&gv[i]) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kWUnroll; u++) {
-            const bool y = yk[u] != kEmptyKey && yk[u] != (uint32_t)s;
-            const CommInfo m = y ? S.m[yk[u]] : CommInfo{0.0, 0};
-            ay[u] = m.a;
-            ny[u] = m.n;
-            if (yk[u] != kEmptyKey) {  // empty between uses
-                const i64 i = i0 + (i64)u * kWBlock;
-                __hip_atomic_store(&gk[i], kEmptyKey, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&gv[i], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int i = i0 + u * kWBlock;
+            k[u] = kEmptyKey;
+            x[u] = 0.0;
+            if (i < total) {
+                int a = 0, b = nc;  // the chunk c with pre[c] <= i < pre[c + 1]
+                while (b - a > 1) {
+                    const int m = (a + b) / 2;
+                    if (pre[m] <= i) a = m; else b = m;
+                }
+                const i64 at = (i64)a * kWChunk + seg[a] + (i - pre[a]);
+                k[u] = pk[at];
+                x[u] = pv[at];
             }
         }
 #pragma unroll
-        for (int u = 0; u < kWUnroll; u++) {
-            if (yk[u] == kEmptyKey || yk[u] == (uint32_t)s) continue;
-            offer(c, gain_of(e[u], eix, kv, ay[u], ax, W.A.constant), (i64)yk[u], e[u], ny[u]);
-        }
+        for (int u = 0; u < kWUnroll; u++)
+            if (k[u] != kEmptyKey && !lds_insert_bounded(keys, vals, std::max<uint32_t>(T, kWBlock), k[u], x[u]))
+                s_ok = 0;
     }
+    __syncthreads();
+    if (!s_ok) {
+        if (threadIdx.x == 0) atomicOr(&W.bar[0], 3u);
+        return false;
+    }
+    Cand c;
+    score_lds(S, keys, vals, std::max<uint32_t>(T, kWBlock), threadIdx.x, kWBlock, (uint32_t)s, kv, eix, ax,
+              W.A.constant, c);
     wg_best(c, R);
     if (threadIdx.x == 0) {
-        const i64 at = g * W.gslices + j;
+        const i64 at = g * W.gparts + p;
         W.gcid[at] = c.ok ? c.id : -1;
         W.gcand[3 * at] = c.gain;
         W.gcand[3 * at + 1] = c.e;
         W.gcand[3 * at + 2] = (double)c.n;
         __threadfence();
         const int done = atomicAdd(&W.gstate[3 * g + 2], 1) + 1;
-        if (done == G.nslices) {
+        if (done == P) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             Cand b;
-            for (int q = 0; q < G.nslices; q++) {
-                const i64 bq = g * W.gslices + q;
+            for (int q = 0; q < P; q++) {
+                const i64 bq = g * W.gparts + q;
                 const i64 iq = ld_agent(&W.gcid[bq]);
                 if (iq < 0) continue;
                 const double gq = ld_agent(&W.gcand[3 * bq]);
@@ -578,12 +627,10 @@ __device__ void light_vertices(const WideArgs &W, const Step &S, i64 pos, int co
     wave_lds_sync();  // the staging arrays are rewritten by the next item
 }
 
-// position of item round r for worker w of nw in the snake order
-__device__ inline i64 snake(i64 r, i64 w, i64 nw) { return r * nw + ((r & 1) ? nw - 1 - w : w); }
-
 __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
     __shared__ __align__(16) unsigned char smem[kWHeavyBytes];
     __shared__ Red R;
+    __shared__ int pc[2 * kWMaxChunks + 2];  // giant items: partition counts or chunk segments
     uint32_t *hkeys = reinterpret_cast<uint32_t *>(smem);
     double *hvals = reinterpret_cast<double *>(smem + (size_t)kWHeavySlots * 4);
     const unsigned nb = gridDim.x;
@@ -606,9 +653,6 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
     i64 moved = 0;
     int par = 0;
     i64 plo = 0, phi = 0;  // the previous step's class in cperm
-    // wavefront items in the snake order over the wavefronts
-    const i64 nw = (i64)nb * kWWaves;
-    const i64 gw = (i64)blockIdx.x * kWWaves + wave;
     for (int c = 0; c < W.ncol && alive; c++) {
         const i64 lo = W.cstart[c], hi = W.cstart[c + 1];
         if (hi == lo) continue;
@@ -624,19 +668,17 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
             if (it.kind == kWHeavy) {
                 heavy_vertex(W, S, it.x, hkeys, hvals, R, dein, moved);
             } else if (it.kind == kWGiantChunk) {
-                giant_chunk(W, S, it.x, it.y, hkeys, hvals, R);
-            } else if (!giant_score(W, S, it.x, it.y, R, moved)) {
+                giant_chunk(W, S, it.x, it.y, hkeys, hvals, pc, R);
+            } else if (!giant_part(W, S, it.x, it.y, hkeys, hvals, pc, R, moved)) {
                 alive = false;
                 break;
             }
         }
         if (!alive) break;
         // wavefront items
-        const i64 vlo = W.vstart[c], vn = W.vstart[c + 1] - vlo;
-        for (i64 r = 0; r * nw < vn; r++) {
-            const i64 i = snake(r, gw, nw);
-            if (i >= vn) continue;
-            const WideItem it = W.vitems[vlo + i];
+        const i64 vlo = W.vstart[(i64)c * nb + blockIdx.x], vhi = W.vstart[(i64)c * nb + blockIdx.x + 1];
+        for (i64 i = vlo + wave; i < vhi; i += kWWaves) {
+            const WideItem it = W.vitems[i];
             switch (it.kind) {
                 case kWMid: mid_vertex(W, S, W.perm[it.x], wkeys, wvals, lane, dein, moved); break;
                 case kWLight1: light_vertices<4>(W, S, it.x, it.y, wbase, lane, dein, moved); break;
@@ -700,13 +742,8 @@ __global__ void k_wide_degrees(const i64 *perm, const i64 *off, i64 lo, i64 hi, 
     }
 }
 
-__global__ void k_wide_clear_giants(uint32_t *gkeys, double *gvals, i64 slots, int32_t *gstate, double *gsc,
-                                    double *gout, i64 ng) {
+__global__ void k_wide_clear_giants(int32_t *gstate, double *gsc, double *gout, i64 ng) {
     const i64 stride = (i64)gridDim.x * blockDim.x;
-    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < slots; i += stride) {
-        gkeys[i] = kEmptyKey;
-        gvals[i] = 0.0;
-    }
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < ng; i += stride) {
         gstate[3 * i] = gstate[3 * i + 1] = gstate[3 * i + 2] = 0;
         gsc[2 * i] = gsc[2 * i + 1] = 0.0;
@@ -757,11 +794,9 @@ void launch_wide_degrees(const i64 *perm, const i64 *off, i64 lo, i64 hi, i64 *v
     GLV_LAUNCH_CHECK();
 }
 
-void launch_wide_clear_giants(uint32_t *gkeys, double *gvals, i64 slots, int32_t *gstate, double *gsc,
-                              double *gout, i64 ng, hipStream_t s) {
-    const i64 n = std::max(slots, ng);
-    if (n <= 0) return;
-    k_wide_clear_giants<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(gkeys, gvals, slots, gstate, gsc, gout, ng);
+void launch_wide_clear_giants(int32_t *gstate, double *gsc, double *gout, i64 ng, hipStream_t s) {
+    if (ng <= 0) return;
+    k_wide_clear_giants<<<grid_for(ng, kBlock, 8192), kBlock, 0, s>>>(gstate, gsc, gout, ng);
     GLV_LAUNCH_CHECK();
 }
 

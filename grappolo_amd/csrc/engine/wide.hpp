@@ -7,13 +7,16 @@
 // list of items fixed when the phase starts:
 //   workgroup items: a heavy vertex (257..kWHeavyMaxDeg entries, LDS table of
 //     the whole workgroup), a chunk of kWChunk entries of a giant vertex
-//     (summed in LDS, then added into the giant's HBM table) and a slice of a
-//     giant's table to score (the last slice scored decides the move);
+//     (summed per community in LDS, then written out as (community, weight)
+//     pairs grouped by a hash partition of the community ids) and a
+//     partition of a giant's communities (its pairs from every chunk summed in
+//     LDS and scored; the last partition scored decides the move);
 //   wavefront items: one mid vertex (65..256 entries, a wavefront's LDS table)
 //     or 64/G light vertices of degree bin b (G lanes each, the in-order sums
 //     of buildLocalMapCounter, Utility/utilityClusteringFunctions.cpp:172-194).
-// Items are dealt to workgroups and wavefronts in a fixed (snake) order, so a
-// run is reproducible.  Labels, community degrees and sizes are
+// Items are dealt to workgroups when the phase starts (longest first, each to
+// the least loaded workgroup; a workgroup's wavefronts take its wavefront
+// items in turn), so a run is reproducible.  Labels, community degrees and sizes are
 // double-buffered as in the small-graph persistent sweep: step k reads the
 // state after step k-1 and writes into the other buffer the moves of steps
 // k-1 and k.
@@ -28,8 +31,11 @@ namespace glv {
 constexpr int kWBlock = 1024;            // threads per workgroup
 constexpr int kWHeavySlots = 12288;      // LDS table of a workgroup: u32 keys + f64 values
 constexpr i64 kWHeavyMaxDeg = 8192;      // larger vertices are giants (several workgroups)
-constexpr i64 kWChunk = 8192;            // entries of a giant per accumulation item
-constexpr i64 kWScoreSlice = 8192;       // slots of a giant's table per scoring item
+constexpr i64 kWChunk = 4096;            // entries of a giant per accumulation item
+constexpr i64 kWPartKeys = 4096;         // communities per partition of a giant (expected)
+constexpr int kWMaxParts = 1024;         // partitions of a giant at most
+constexpr int kWMaxChunks = 1024;        // chunks of a giant at most (4,194,304 entries; a phase
+                                         // with a larger vertex takes the per-color launches)
 constexpr int kWMidSlots = 512;          // table of a wavefront (mid vertices: <= 256 entries)
 // Phases whose color classes hold more vertices than this on average take the
 // per-color launches instead (a class that large fills the GPU by itself).
@@ -37,7 +43,11 @@ constexpr i64 kWideMaxPerColor = 100000;
 
 // Relative costs of workgroup items (microseconds, measured order of
 // magnitude): a heavy vertex kWCostHeavy plus one per 1024 entries.
-constexpr double kWCostHeavy = 10.0, kWCostChunk = 16.0, kWCostSlice = 8.0;
+constexpr double kWCostHeavy = 10.0, kWCostChunk = 10.0, kWCostPart = 10.0;
+// wavefront items: a mid vertex, a batch of light vertices (a workgroup runs
+// kWWavesPerGroup of them at a time)
+constexpr double kWCostMid = 8.0, kWCostLight = 6.0;
+constexpr int kWWavesPerGroup = kWBlock / kWave;
 
 // Item kinds.  Workgroup items: x = vertex (heavy, y = its entries) or giant
 // index (y = chunk or slice).  Wavefront items: x = first position in perm,
@@ -45,7 +55,7 @@ constexpr double kWCostHeavy = 10.0, kWCostChunk = 16.0, kWCostSlice = 8.0;
 enum WideKind : int32_t {
     kWHeavy = 0,
     kWGiantChunk = 1,
-    kWGiantScore = 2,
+    kWGiantPart = 2,
     kWMid = 3,
     kWLight1 = 4,  // .. kWLight1 + 4: light vertices of degree bins 1..5
 };
@@ -54,14 +64,16 @@ struct WideItem {
     int32_t kind;
     int32_t y;
 };
-// A giant vertex of the phase: its HBM table (u32 keys, kEmptyKey when free;
-// f64 values, 0 when free) of `slots` slots at table offset `tab`.
+// A giant vertex of the phase: chunk j writes its pairs at pairs + j *
+// kWChunk (partition p's from cnt[j * (nparts + 1) + p] on, relative) and the
+// partition bounds at cnt + j * (nparts + 1) (offsets into the pair arrays and
+// the count array of its color class).
 struct WideGiant {
     i64 v;
-    i64 tab;
-    i64 slots;
+    i64 pairs;
+    i64 cnt;
     int32_t nchunks;
-    int32_t nslices;
+    int32_t nparts;
 };
 
 // Degree and size of a community side by side (one 16-byte load serves both,
@@ -82,17 +94,19 @@ struct WideArgs {
     const i64 *cstart;           // ncol + 1
     const WideItem *witems;      // workgroup items of color c for workgroup w at wstart[c*nb+w] .. wstart[c*nb+w+1]
     const i64 *wstart;
-    const WideItem *vitems;      // wavefront items, color c at vstart[c] .. vstart[c+1]
+    const WideItem *vitems;      // wavefront items of color c for workgroup w at vstart[c*nb+w] .. (its
+                                 //   wavefronts take them in turn)
     const i64 *vstart;
     const WideGiant *giants;
     i64 ngiants;
-    uint32_t *gkeys;             // giant tables (one class at a time)
-    double *gvals;
-    int32_t *gstate;             // per giant: chunks added, table ready, slices scored
+    uint32_t *gpk;               // giant pairs: community ids (one class at a time)
+    double *gpv;                 //   and summed weights
+    int32_t *gcnt;               // partition bounds of every chunk
+    int32_t *gstate;             // per giant: chunks written, pairs ready, partitions scored
     double *gsc;                 // per giant: self-loop weight, weight to its own community
-    double *gcand;               // per giant and slice: gain, weight to the candidate, its size
-    i64 *gcid;                   // per giant and slice: candidate (-1: none)
-    i64 gslices;                 // slices per giant in gcand/gcid (the largest nslices)
+    double *gcand;               // per giant and partition: gain, weight to the candidate, its size
+    i64 *gcid;                   // per giant and partition: candidate (-1: none)
+    i64 gparts;                  // partitions per giant in gcand/gcid (the largest nparts)
     double *gout;                // per giant: 2(e_t - e_s) of its move this iteration, 0 if none
     int ncol;
     unsigned *bar;               // 16 + 16 * grid words, zero at launch
@@ -107,17 +121,17 @@ struct WidePlan {
     DBuf<WideItem> witems, vitems;
     DBuf<i64> wstart, vstart;
     DBuf<WideGiant> giants;
-    DBuf<uint32_t> gkeys;
-    DBuf<double> gvals, gsc, gcand, gout;
-    DBuf<int32_t> gstate;
+    DBuf<uint32_t> gpk;
+    DBuf<double> gpv, gsc, gcand, gout;
+    DBuf<int32_t> gcnt, gstate;
     DBuf<i64> gcid;
-    i64 ngiants = 0, gslices = 1, gtable = 0;
+    i64 ngiants = 0, gparts = 1, gpairs = 0, gcounts = 0;
     i64 nwitems = 0, nvitems = 0;
 };
 // Builds the items of every color class of P (vertices of g grouped by
-// degree bin and color) for a grid of nb workgroups; the giant tables start
-// empty.
-void build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L, hipStream_t s);
+// degree bin and color) for a grid of nb workgroups; false (nothing built)
+// when a vertex has more entries than kWMaxChunks chunks hold.
+bool build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L, hipStream_t s);
 
 // false when the environment sets GLV_WIDE=0 (large graphs then take the
 // per-color launches; for comparisons)
@@ -128,7 +142,6 @@ int wide_blocks();
 bool launch_wide_iteration(const WideArgs &W, int nblocks, hipStream_t s);
 // (vertex, degree) of perm[lo .. hi) into vd[2 * (hi - lo)]
 void launch_wide_degrees(const i64 *perm, const i64 *off, i64 lo, i64 hi, i64 *vd, hipStream_t s);
-void launch_wide_clear_giants(uint32_t *gkeys, double *gvals, i64 slots, int32_t *gstate, double *gsc,
-                              double *gout, i64 ng, hipStream_t s);
+void launch_wide_clear_giants(int32_t *gstate, double *gsc, double *gout, i64 ng, hipStream_t s);
 
 }  // namespace glv

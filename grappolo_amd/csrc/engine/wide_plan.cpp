@@ -8,7 +8,7 @@
 
 namespace glv {
 
-void build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L, hipStream_t s) {
+bool build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L, hipStream_t s) {
     const int ncol = P.ncol;
     // vertex and degree of every vertex of bins 7..9 (heavy and giant vertices)
     const i64 hlo = P.bin_lo(7), hhi = P.bin_hi(kNumBins - 1);
@@ -21,36 +21,40 @@ void build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L
         rb.run();
     }
     std::vector<WideItem> wi, vi;
-    std::vector<i64> wstart((size_t)ncol * nb + 1, 0), vstart(ncol + 1, 0);
+    std::vector<i64> wstart((size_t)ncol * nb + 1, 0), vstart((size_t)ncol * nb + 1, 0);
     std::vector<WideItem> cls;        // the workgroup items of one class
     std::vector<double> cost;
     std::vector<std::vector<WideItem>> per(nb);
     std::vector<WideGiant> giants;
-    i64 gtable = 0, gslices = 1;
+    i64 gpairs = 0, gcounts = 0, gparts = 1;
     for (int c = 0; c < ncol; c++) {
         cls.clear();
-        vstart[c] = (i64)vi.size();
         // giants of the class: accumulation chunks first
         const size_t g0 = giants.size();
-        i64 tab = 0;
+        i64 pairs = 0, counts = 0;
         for (int b = 8; b < kNumBins; b++)
             for (i64 q = P.lo(b, c); q < P.hi(b, c); q++) {
                 const i64 v = vd[2 * (q - hlo)], d = vd[2 * (q - hlo) + 1];
                 if (d <= kWHeavyMaxDeg) continue;
                 WideGiant G;
                 G.v = v;
-                G.tab = tab;
-                G.slots = 1;
-                while (G.slots < 2 * d) G.slots <<= 1;
                 G.nchunks = (int32_t)((d + kWChunk - 1) / kWChunk);
-                G.nslices = (int32_t)((G.slots + kWScoreSlice - 1) / kWScoreSlice);
-                tab += G.slots;
-                gslices = std::max<i64>(gslices, G.nslices);
+                if (G.nchunks > kWMaxChunks) {
+                    trace("  wide plan: a vertex of %ld entries exceeds %ld", (long)d, (long)(kWMaxChunks * kWChunk));
+                    return false;
+                }
+                G.nparts = (int32_t)std::min<i64>(kWMaxParts, std::max<i64>(1, (d + kWPartKeys - 1) / kWPartKeys));
+                G.pairs = pairs;
+                G.cnt = counts;
+                pairs += (i64)G.nchunks * kWChunk;
+                counts += (i64)G.nchunks * (G.nparts + 1);
+                gparts = std::max<i64>(gparts, G.nparts);
                 const i64 gi = (i64)giants.size();
                 giants.push_back(G);
                 for (int32_t j = 0; j < G.nchunks; j++) cls.push_back(WideItem{gi, kWGiantChunk, j});
             }
-        gtable = std::max(gtable, tab);
+        gpairs = std::max(gpairs, pairs);
+        gcounts = std::max(gcounts, counts);
         // heavy vertices, the larger bin first
         for (int b = kNumBins - 1; b >= 7; b--)
             for (i64 q = P.lo(b, c); q < P.hi(b, c); q++) {
@@ -59,7 +63,7 @@ void build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L
             }
         // the giants' table slices, scored once every chunk is in
         for (size_t gi = g0; gi < giants.size(); gi++)
-            for (int32_t j = 0; j < giants[gi].nslices; j++) cls.push_back(WideItem{(i64)gi, kWGiantScore, j});
+            for (int32_t j = 0; j < giants[gi].nparts; j++) cls.push_back(WideItem{(i64)gi, kWGiantPart, j});
         // longest processing time first: each item, costliest first, to the
         // workgroup with the least work so far; then every workgroup's list in
         // kind order (chunks, heavy vertices, slices), so that a slice only
@@ -73,7 +77,7 @@ void build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L
         for (size_t i = 0; i < cls.size(); i++) order[i] = i;
         auto item_cost = [](const WideItem &it) {
             return it.kind == kWHeavy ? kWCostHeavy + it.y / 1024.0
-                                      : it.kind == kWGiantChunk ? kWCostChunk : kWCostSlice;
+                                      : it.kind == kWGiantChunk ? kWCostChunk : kWCostPart;
         };
         std::stable_sort(order.begin(), order.end(),
                          [&](size_t a, size_t b) { return item_cost(cls[a]) > item_cost(cls[b]); });
@@ -93,44 +97,62 @@ void build_wide_plan(const DevGraph &g, const PhaseSetup &P, int nb, WidePlan &L
             });
             wi.insert(wi.end(), per[w].begin(), per[w].end());
         }
-        // wavefront items: mid vertices, then light ones from the largest bin down
-        for (i64 q = P.lo(6, c); q < P.hi(6, c); q++) vi.push_back(WideItem{q, kWMid, 1});
+        // wavefront items (mid vertices, then light ones from the largest bin
+        // down), each to the workgroup whose work (its workgroup items plus its
+        // wavefront items spread over its wavefronts) is least so far
+        for (auto &q : per) q.clear();
+        auto give = [&](const WideItem &it, double cost) {
+            std::pop_heap(heap.begin(), heap.end());
+            auto &h = heap.back();
+            per[-h.second].push_back(it);
+            h.first -= cost / kWWavesPerGroup;
+            std::push_heap(heap.begin(), heap.end());
+        };
+        for (i64 q = P.lo(6, c); q < P.hi(6, c); q++) give(WideItem{q, kWMid, 1}, kWCostMid);
         for (int b = 5; b >= 1; b--) {
-            const int per = kWave / kBinGroup[b];
-            for (i64 q = P.lo(b, c); q < P.hi(b, c); q += per)
-                vi.push_back(WideItem{q, (int32_t)(kWLight1 + b - 1), (int32_t)std::min<i64>(per, P.hi(b, c) - q)});
+            const int per_item = kWave / kBinGroup[b];
+            for (i64 q = P.lo(b, c); q < P.hi(b, c); q += per_item)
+                give(WideItem{q, (int32_t)(kWLight1 + b - 1), (int32_t)std::min<i64>(per_item, P.hi(b, c) - q)},
+                     kWCostLight);
+        }
+        for (int w = 0; w < nb; w++) {
+            vstart[(size_t)c * nb + w] = (i64)vi.size();
+            vi.insert(vi.end(), per[w].begin(), per[w].end());
         }
     }
     wstart[(size_t)ncol * nb] = (i64)wi.size();
-    vstart[ncol] = (i64)vi.size();
+    vstart[(size_t)ncol * nb] = (i64)vi.size();
     L.nwitems = (i64)wi.size();
     L.nvitems = (i64)vi.size();
     L.ngiants = (i64)giants.size();
-    L.gslices = gslices;
-    L.gtable = gtable;
+    L.gparts = gparts;
+    L.gpairs = gpairs;
+    L.gcounts = gcounts;
     L.witems.alloc(std::max<size_t>(wi.size(), 1), s);
     L.vitems.alloc(std::max<size_t>(vi.size(), 1), s);
     L.wstart.alloc(wstart.size(), s);
-    L.vstart.alloc(ncol + 1, s);
+    L.vstart.alloc(vstart.size(), s);
     L.giants.alloc(std::max<size_t>(giants.size(), 1), s);
     const i64 ng = std::max<i64>(L.ngiants, 1);
-    L.gkeys.alloc(std::max<i64>(gtable, 1), s);
-    L.gvals.alloc(std::max<i64>(gtable, 1), s);
+    L.gpk.alloc(std::max<i64>(gpairs, 1), s);
+    L.gpv.alloc(std::max<i64>(gpairs, 1), s);
+    L.gcnt.alloc(std::max<i64>(gcounts, 1), s);
     L.gstate.alloc(3 * ng, s);
     L.gsc.alloc(2 * ng, s);
     L.gout.alloc(ng, s);
-    L.gcand.alloc(3 * ng * gslices, s);
-    L.gcid.alloc(ng * gslices, s);
+    L.gcand.alloc(3 * ng * gparts, s);
+    L.gcid.alloc(ng * gparts, s);
     if (!wi.empty()) GLV_HIP(hipMemcpyAsync(L.witems.p, wi.data(), wi.size() * sizeof(WideItem), hipMemcpyHostToDevice, s));
     if (!vi.empty()) GLV_HIP(hipMemcpyAsync(L.vitems.p, vi.data(), vi.size() * sizeof(WideItem), hipMemcpyHostToDevice, s));
     GLV_HIP(hipMemcpyAsync(L.wstart.p, wstart.data(), wstart.size() * sizeof(i64), hipMemcpyHostToDevice, s));
-    GLV_HIP(hipMemcpyAsync(L.vstart.p, vstart.data(), (ncol + 1) * sizeof(i64), hipMemcpyHostToDevice, s));
+    GLV_HIP(hipMemcpyAsync(L.vstart.p, vstart.data(), vstart.size() * sizeof(i64), hipMemcpyHostToDevice, s));
     if (!giants.empty())
         GLV_HIP(hipMemcpyAsync(L.giants.p, giants.data(), giants.size() * sizeof(WideGiant), hipMemcpyHostToDevice, s));
-    launch_wide_clear_giants(L.gkeys.p, L.gvals.p, std::max<i64>(gtable, 1), L.gstate.p, L.gsc.p, L.gout.p, ng, s);
+    launch_wide_clear_giants(L.gstate.p, L.gsc.p, L.gout.p, ng, s);
     GLV_HIP(hipStreamSynchronize(s));  // the host vectors are pageable
-    trace("  wide plan: %ld workgroup items, %ld wavefront items, %ld giants (table %ld slots, %ld slices)",
-          (long)L.nwitems, (long)L.nvitems, (long)L.ngiants, (long)gtable, (long)gslices);
+    trace("  wide plan: %ld workgroup items, %ld wavefront items, %ld giants (%ld pairs, %ld partitions at most)",
+          (long)L.nwitems, (long)L.nvitems, (long)L.ngiants, (long)gpairs, (long)gparts);
+    return true;
 }
 
 }  // namespace glv
