@@ -95,6 +95,7 @@ __device__ inline void wave_lds_sync() {
 
 // Reduction scratch of a workgroup (one slot per wavefront).
 struct Red {
+    double sl;  // self-loop weight of the vertex being gathered (one entry)
     double d[kWWaves];
     double e[kWWaves];
     i64 i[kWWaves];
@@ -177,11 +178,11 @@ __device__ inline void group_best(Cand &c) {
     }
 }
 
-// best over the workgroup (every thread gets it)
-__device__ inline void wg_best(Cand &c, Red &R) {
+// best over the workgroup, in thread 0 only (one barrier; R's slots are
+// rewritten only after the caller's next barrier)
+__device__ inline void wg_best0(Cand &c, Red &R) {
     group_best<kWave>(c);
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
-    __syncthreads();
     if (l == 0) {
         R.c[w] = c.ok;
         R.d[w] = c.gain;
@@ -190,6 +191,7 @@ __device__ inline void wg_best(Cand &c, Red &R) {
         R.n[w] = c.n;
     }
     __syncthreads();
+    if (threadIdx.x != 0) return;
     Cand b;
     for (int q = 0; q < kWWaves; q++)
         if (better(R.c[q] != 0, R.d[q], R.i[q], b.ok, b.gain, b.id)) {
@@ -246,11 +248,10 @@ __device__ inline void clear_table(uint32_t *keys, double *vals, uint32_t T, int
 }
 
 // Sums entries [beg, end) of vertex v per neighbouring community into the
-// workgroup's LDS table of T slots (empty); returns this thread's part of the
-// self-loop weight.  kWUnroll passes of loads are in flight at a time.
-__device__ inline double wg_gather(const WideArgs &W, const Step &S, i64 v, i64 beg, i64 end, uint32_t *keys,
-                                   double *vals, uint32_t T) {
-    double sl = 0;
+// workgroup's LDS table of T slots (empty) and its self-loop weight into
+// R.sl (zero); kWUnroll passes of loads are in flight at a time.
+__device__ inline void wg_gather(const WideArgs &W, const Step &S, i64 v, i64 beg, i64 end, uint32_t *keys,
+                                 double *vals, uint32_t T, Red &R) {
     for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += (i64)kWUnroll * kWBlock) {
         i64 tt[kWUnroll], cl[kWUnroll];
         double wt[kWUnroll];
@@ -265,12 +266,11 @@ __device__ inline double wg_gather(const WideArgs &W, const Step &S, i64 v, i64 
 #pragma unroll
         for (int u = 0; u < kWUnroll; u++) {
             if (tt[u] < 0) continue;
-            if (tt[u] == v) sl += wt[u];
+            if (tt[u] == v) atomicAdd(&R.sl, wt[u]);
             lds_insert(keys, vals, T, (uint32_t)cl[u], wt[u]);
         }
     }
     __syncthreads();
-    return sl;
 }
 
 // Scores slots [first, T) step `step` of an LDS table for vertex v: the
@@ -312,17 +312,18 @@ __device__ void heavy_vertex(const WideArgs &W, const Step &S, i64 v, uint32_t *
     const double kv = W.A.k[v];
     const uint32_t T = (uint32_t)std::min<i64>(kWHeavySlots, (2 * (end - beg) + kWBlock - 1) / kWBlock * kWBlock);
     clear_table(keys, vals, T, threadIdx.x, kWBlock);
+    if (threadIdx.x == 0) R.sl = 0.0;
     const CommInfo ms = S.m[s];
     const double as = ms.a;
     const i64 ns = ms.n;
     __syncthreads();
-    double sl = wg_gather(W, S, v, beg, end, keys, vals, T);
-    sl = wg_sum(sl, R);
+    wg_gather(W, S, v, beg, end, keys, vals, T, R);
+    const double sl = R.sl;
     const double e_s = lds_lookup(keys, vals, T, (uint32_t)s);
     const double eix = e_s - sl, ax = as - kv;
     Cand c;
     score_lds(S, keys, vals, T, threadIdx.x, kWBlock, (uint32_t)s, kv, eix, ax, W.A.constant, c);
-    wg_best(c, R);
+    wg_best0(c, R);
     if (threadIdx.x == 0) decide(W, S, v, s, kv, ns, c, eix, dein, moved);
     __syncthreads();  // the table is cleared by the next item
 }
@@ -343,9 +344,10 @@ __device__ void giant_chunk(const WideArgs &W, const Step &S, i64 g, int j, uint
     const uint32_t T = (uint32_t)std::min<i64>(kWHeavySlots, (2 * (end - beg) + kWBlock - 1) / kWBlock * kWBlock);
     clear_table(keys, vals, T, threadIdx.x, kWBlock);
     for (int q = threadIdx.x; q <= P; q += kWBlock) pc[q] = 0;
+    if (threadIdx.x == 0) R.sl = 0.0;
     __syncthreads();
-    double sl = wg_gather(W, S, v, beg, end, keys, vals, T);
-    sl = wg_sum(sl, R);
+    wg_gather(W, S, v, beg, end, keys, vals, T, R);
+    const double sl = R.sl;
     const double es = lds_lookup(keys, vals, T, (uint32_t)s);
     for (uint32_t i = threadIdx.x; i < T; i += kWBlock) {
         const uint32_t k = keys[i];
@@ -474,7 +476,7 @@ __device__ bool giant_part(const WideArgs &W, const Step &S, i64 g, int p, uint3
     Cand c;
     score_lds(S, keys, vals, std::max<uint32_t>(T, kWBlock), threadIdx.x, kWBlock, (uint32_t)s, kv, eix, ax,
               W.A.constant, c);
-    wg_best(c, R);
+    wg_best0(c, R);
     if (threadIdx.x == 0) {
         const i64 at = g * W.gparts + p;
         W.gcid[at] = c.ok ? c.id : -1;
