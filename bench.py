@@ -26,6 +26,7 @@ import sys
 import tempfile
 import time
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -143,11 +144,17 @@ def read_traffic(kernel, workload):
     return None
 
 
-def run_learned(args):
-    """Learned active-set pruning against the unpruned colored method: train
-    the MLP on traces of seeded 1M-vertex LFR graphs (seeds 100, 101, ...),
-    pick the threshold on the last graph's trace (held out from training),
-    then time both forms on the held-out LFR graph (seed 200)."""
+def learned_comparison(args, seeds, rechecks):
+    """Learned active-set pruning against the unpruned colored method: traces
+    of seeded 1M-vertex LFR graphs (seeds 100, 101, ...) train one network per
+    seed in `seeds`, each threshold is picked on the trace of the next graph
+    (held out from training), then the unpruned form and every network's
+    pruned form run on the held-out LFR graph (seed 200).  `rechecks`: the
+    stop policies timed (0: a phase ends only when a full re-check sweep gains
+    less than the threshold, the reference's test; 1: after one full
+    re-check).  Returns a dict with the median and minimum speed-up."""
+    import statistics
+
     import torch
     from grappolo_amd import active_set as las
     from grappolo_amd import engine
@@ -176,28 +183,19 @@ def run_learned(args):
         else:
             calib = (f.clone(), l.clone())
         del dg, C, f, l
-        print(f"[bench] trace {i}: {feats[-1].shape[0] if i < ntrain else calib[0].shape[0]} rows",
-              file=sys.stderr, flush=True)
     feats, labels = torch.cat(feats), torch.cat(labels)
-    pred = las.ActiveSetPredictor(seed=1)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    losses = pred.fit(feats, labels, steps=args.las_steps, batch=4096, lr=2e-3)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t
-    tau, kept = pred.calibrate(calib[0], calib[1], recall=args.las_recall)
-    print(f"[bench] trained {args.las_steps} steps in {t_train:.2f}s, loss {losses[:50].mean():.4f} -> "
-          f"{losses[-50:].mean():.4f}, tau {tau:.4f}, kept {kept:.3f}", file=sys.stderr, flush=True)
+    print(f"[bench] learned: {feats.shape[0]} trace rows from {ntrain} graphs in {t_gen + t_trace:.1f}s",
+          file=sys.stderr, flush=True)
 
     g = gen_lfr(args.las_eval_nv, seed=200)
     dg = engine.DeviceGraph(g, "cuda:0")
     C = torch.empty(g.numVertices, dtype=torch.int64, device="cuda:0")
 
-    def timed(fn):
-        for _ in range(args.warmup):
+    def timed(fn, warmup, steps):
+        for _ in range(warmup):
             fn()
         res, ts = [], []
-        for _ in range(args.steps):
+        for _ in range(steps):
             torch.cuda.synchronize()
             t = time.perf_counter()
             res.append(fn())
@@ -210,41 +208,64 @@ def run_learned(args):
         return [{"nv": int(pp["phase_nv"][i]), "iterations": int(pp["phase_iters"][i]),
                  "ms": round(float(pp["phase_seconds"][i]) * 1e3, 2)} for i in range(len(pp["phase_nv"]))]
 
-    st0, t_plain = timed(lambda: engine.run_colored_dev(dg, C, **params))
-    policies = []
-    for R in [int(x) for x in args.las_rechecks.split(",")]:
-        (st1, ls), t_las = timed(lambda: pred.run(dg, C, max_rechecks=R, **params))
-        q_las = engine.modularity_dev(dg, C)
-        policies.append(dict(max_rechecks=R, ms=t_las * 1e3, speedup=t_plain / t_las,
+    st0, t_plain = timed(lambda: engine.run_colored_dev(dg, C, **params), args.las_warmup, args.las_timed)
+    runs = []
+    for sd in seeds:
+        pred = las.ActiveSetPredictor(seed=sd)
+        t = time.perf_counter()
+        losses = pred.fit(feats, labels, steps=args.las_steps, batch=4096, lr=2e-3, seed=sd)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t
+        tau, kept = pred.calibrate(calib[0], calib[1], recall=args.las_recall)
+        for R in rechecks:
+            (st1, ls), t_las = timed(lambda: pred.run(dg, C, max_rechecks=R, **params), args.las_warmup,
+                                     args.las_timed)
+            q_las = engine.modularity_dev(dg, C)
+            runs.append(dict(seed=sd, max_rechecks=R, ms=t_las * 1e3, speedup=t_plain / t_las,
                              final_modularity=st1["final_modularity"], partition_modularity=q_las,
                              dQ=st1["final_modularity"] - st0["final_modularity"],
-                             edges_visited=st1["edges_visited"], per_phase=phases(st1), **ls))
-        print(f"[bench] rechecks {R}: {t_las * 1e3:.1f} ms vs plain {t_plain * 1e3:.1f} ms, "
-              f"dQ {policies[-1]['dQ']:.2e}", file=sys.stderr, flush=True)
-    best = policies[0]
-    t_las = best["ms"] / 1e3
-    out = {
-        "metric": "Louvain time-to-converge, learned-pruned vs plain GPU form (held-out LFR)",
-        "value": t_plain / t_las, "unit": "x", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "higher_is_better": True, "dtype": "f64 (MLP bf16/fp32)",
-        "data": "synthetic",
-        "config": {"workload": f"LFR-style {g.numVertices:,} vertices (seed 200), "
-                               f"trained on {ntrain} x {args.las_train_nv:,}-vertex LFR "
-                               f"(seeds 100..{99 + ntrain})", "nnz": g.nnz,
-                   "threshold": 1e-9, "C_threshold": 1e-6, "minGraphSize": 1000},
+                             edges_visited=st1["edges_visited"], tau=tau, calibration_kept=kept,
+                             train_s=t_train, loss_first=float(losses[:50].mean().item()),
+                             loss_last=float(losses[-50:].mean().item()), per_phase=phases(st1), **ls))
+            print(f"[bench] learned seed {sd} rechecks {R}: {t_las * 1e3:.1f} ms vs plain "
+                  f"{t_plain * 1e3:.1f} ms ({t_plain / t_las:.2f}x), dQ {runs[-1]['dQ']:.2e}",
+                  file=sys.stderr, flush=True)
+    summary = {}
+    for R in rechecks:
+        sp = [r["speedup"] for r in runs if r["max_rechecks"] == R]
+        dq = [r["dQ"] for r in runs if r["max_rechecks"] == R]
+        summary[f"rechecks_{R}"] = {"speedup_median": statistics.median(sp), "speedup_min": min(sp),
+                                    "speedup_max": max(sp), "dQ_min": min(dq), "dQ_max": max(dq)}
+    return {
+        "workload": f"LFR-style {g.numVertices:,} vertices (seed 200), trained on {ntrain} x "
+                    f"{args.las_train_nv:,}-vertex LFR (seeds 100..{99 + ntrain}), threshold on seed "
+                    f"{100 + ntrain}, recall {args.las_recall}",
+        "nnz": g.nnz, "network_seeds": list(seeds),
         "plain": {"ms": t_plain * 1e3, "final_modularity": st0["final_modularity"],
                   "edges_visited": st0["edges_visited"], "iterations": st0["total_iterations"],
-                  "edges_per_s": st0["edges_visited"] / t_plain, "per_phase": phases(st0)},
-        "pruned": best,
-        "policies": policies[1:],
-        "dQ": best["dQ"],
+                  "per_phase": phases(st0)},
+        "summary": summary, "runs": runs,
         "train": {"graphs": ntrain, "rows": int(feats.shape[0]),
                   "positive_rate": float(labels.double().mean().item()), "steps": args.las_steps,
-                  "batch": 4096, "train_s": t_train, "trace_s": t_trace, "generate_s": t_gen,
-                  "loss_first": float(losses[:50].mean().item()),
-                  "loss_last": float(losses[-50:].mean().item()), "tau": tau,
-                  "calibration_kept": kept, "recall_target": args.las_recall,
-                  "total_s": time.perf_counter() - t0},
+                  "batch": 4096, "trace_s": t_trace, "generate_s": t_gen},
+        "total_s": time.perf_counter() - t0,
+    }
+
+
+def run_learned(args):
+    """The learned comparison alone, as its own JSON line."""
+    seeds = [int(x) for x in args.las_seeds.split(",")]
+    rechecks = [int(x) for x in args.las_rechecks.split(",")]
+    res = learned_comparison(args, seeds, rechecks)
+    head = res["summary"][f"rechecks_{rechecks[0]}"]
+    out = {
+        "metric": "Louvain time-to-converge, learned-pruned vs plain GPU form (held-out LFR)",
+        "value": head["speedup_median"], "unit": "x", "n_gpus": 1, "steps": args.las_timed,
+        "warmup": args.las_warmup, "higher_is_better": True, "dtype": "f64 (MLP bf16/fp32)",
+        "data": "synthetic",
+        "config": {"workload": res["workload"], "nnz": res["nnz"], "threshold": 1e-9,
+                   "C_threshold": 1e-6, "minGraphSize": 1000},
+        **res,
     }
     print(json.dumps(out))
 
@@ -361,8 +382,17 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--batch-sample", type=int, default=128,
                     help="graphs timed with per-graph launches")
-    ap.add_argument("--las-rechecks", default="1",
-                    help="comma list of per-phase re-check caps to time (first one is reported)")
+    ap.add_argument("--las-rechecks", default="1,0",
+                    help="comma list of per-phase re-check caps to time (the first one heads the summary; "
+                         "0: phases end by the reference's threshold test only)")
+    ap.add_argument("--las-seeds", default="1,2,3,4,5", help="network seeds (one trained network each)")
+    ap.add_argument("--las-warmup", type=int, default=1)
+    ap.add_argument("--las-timed", type=int, default=3)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="leave the learned comparison out of the default line")
+    ap.add_argument("--learned-budget", type=float, default=240.0,
+                    help="the default line runs the learned comparison only if the run so far took "
+                         "less than this many seconds")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -464,6 +494,18 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
+    # the metric's second half: time to converge of the learned-pruned form
+    # against the plain GPU form (held-out LFR graph), when time allows
+    if not args.no_learned and time.perf_counter() - T_START < args.learned_budget:
+        try:
+            res = learned_comparison(args, [int(x) for x in args.las_seeds.split(",")][:3], [1, 0])
+            out["learned"] = {"workload": res["workload"], "plain_ms": res["plain"]["ms"],
+                              "summary": res["summary"], "network_seeds": res["network_seeds"],
+                              "pruned_ms": {f"seed{r['seed']}_rechecks{r['max_rechecks']}": round(r["ms"], 2)
+                                            for r in res["runs"]},
+                              "total_s": res["total_s"]}
+        except Exception as e:  # reported, not fatal: the headline line still prints
+            out["learned"] = {"error": repr(e)}
     # atomics: each move updates the community degrees and sizes of the source
     # and target communities (2 FP64 + 2 int64 atomics,
     # parallelLouvainWithColoring.cpp:193-201)
