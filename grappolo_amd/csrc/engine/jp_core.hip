@@ -1,0 +1,364 @@
+// jp_core.hip -- the core of the Jones-Plassmann coloring and the depth cut.
+//
+// The engine's coloring is Jones-Plassmann with degree-first priorities plus
+// a sequential tail (the restated algorithm: colour every vertex whose
+// priority beats all uncoloured neighbours with the smallest colour its
+// coloured neighbours do not use, round after round, until at most kJpTail
+// vertices remain; those take first-fit colours one at a time in increasing
+// id).  A vertex coloured in round r takes the smallest colour absent from
+// its higher-priority neighbours, all coloured in earlier rounds, and r is
+// its depth in the priority DAG (1 + the largest depth of its higher
+// neighbours).  So the colours of the rounds and the tail set follow from the
+// DAG alone, whatever order computes them, as long as a vertex is coloured
+// after its higher neighbours.
+//
+// On skewed graphs the DAG is thousands of rounds deep through the hubs, and
+// a round of the whole GPU costs a grid barrier plus a hub's full adjacency.
+// The core -- the vertices of degree dcore or more, a prefix of the priority
+// order since the degree leads the priority -- is coloured here by one
+// workgroup: its vertices' core neighbours are listed (16-bit core indices,
+// higher ones first), a vertex's state (uncoloured: its count of uncoloured
+// higher core neighbours; coloured: depth and colour) lives in LDS, and each
+// wavefront takes the next vertex of a ready queue as soon as its last higher
+// neighbour is coloured -- no round barriers.  The frontier rounds then
+// colour the rest, taking the core's colours and depths as given.  Finally
+// the depths decide the tail: the smallest round r with at most kJpTail
+// vertices deeper than r, those vertices uncoloured and handed to the
+// sequential first-fit.
+#include "kernels.hpp"
+
+namespace glv {
+namespace {
+
+#define GLV_LAUNCH_CHECK() GLV_HIP(hipGetLastError())
+
+constexpr int kCoreBlock = 1024;
+constexpr int kCoreWaves = kCoreBlock / kWave;
+constexpr int kColorWords = kMaxColors / 32;
+constexpr uint32_t kColoured = 0x80000000u;  // state word: coloured (depth << 12 | colour)
+constexpr uint32_t kQueueEmpty = 0xFFFFFFFFu;
+// polls of an unwritten queue slot before the core pass gives up (a DAG
+// whose counts do not match its lists, as an asymmetric adjacency gives)
+constexpr int kCoreSpinLimit = 1 << 22;
+
+static_assert(kMaxColors <= 4096, "colours take 12 bits of the core state");
+static_assert(kJpCoreMax < (1 << 16), "core indices are 16-bit");
+
+// degree bucket: degrees below 8 exactly, then 8 buckets per octave
+__device__ inline int deg_bucket(i64 d) {
+    if (d < 8) return (int)d;
+    const int e = 63 - __clzll((long long)d);
+    return 8 + (e - 3) * 8 + (int)((d >> (e - 3)) & 7);
+}
+
+__global__ __launch_bounds__(kBlock) void k_jp_degree_hist(const i64 *off, i64 nv, unsigned long long *hist) {
+    __shared__ unsigned s_h[kJpDegBuckets];
+    for (int i = threadIdx.x; i < kJpDegBuckets; i += kBlock) s_h[i] = 0;
+    __syncthreads();
+    for (i64 v = blockIdx.x * (i64)kBlock + threadIdx.x; v < nv; v += (i64)gridDim.x * kBlock)
+        atomicAdd(&s_h[deg_bucket(off[v + 1] - off[v])], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kJpDegBuckets; i += kBlock)
+        if (s_h[i]) atomicAdd(&hist[i], (unsigned long long)s_h[i]);
+}
+
+__global__ void k_jp_core_flags(const i64 *off, i64 nv, u64 dcore, i64 *flags) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        flags[v] = (u64)(off[v + 1] - off[v]) >= dcore ? 1 : 0;
+}
+__global__ void k_jp_core_scatter(const i64 *flags, const i64 *pos, i64 nv, i64 *hlist, int32_t *cnt) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        if (flags[v]) {
+            hlist[pos[v]] = v;
+            cnt[v] = (int32_t)pos[v];
+        }
+}
+__global__ void k_jp_core_sizes(const int32_t *core_cnt, i64 K, i64 *sizes) {
+    for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q <= K; q += (i64)gridDim.x * blockDim.x)
+        sizes[q] = q < K ? (i64)core_cnt[2 * q] + core_cnt[2 * q + 1] : 0;
+}
+
+// One workgroup per core vertex q: its core neighbours' indices (count[u] of
+// a core vertex u is its index) into ladj, the higher ones from loff[q], the
+// lower ones from loff[q] + core_cnt[2q]; the order inside each part does
+// not matter (colours, depths and counts only take sets and sums of them).
+constexpr int kFillBlock = 1024;
+constexpr int kFillUnroll = 4;
+__global__ __launch_bounds__(kFillBlock) void k_jp_core_fill(const i64 *off, const i64 *tails, const uint8_t *code,
+                                                             const int32_t *cnt, const i64 *hlist, i64 K,
+                                                             const i64 *loff, const int32_t *core_cnt,
+                                                             uint16_t *ladj) {
+    __shared__ int s_hi, s_lo;
+    const int lane = threadIdx.x % kWave;
+    const u64 below = (1ull << lane) - 1;
+    for (i64 q = blockIdx.x; q < K; q += gridDim.x) {
+        const i64 v = hlist[q];
+        const i64 b = off[v], deg = off[v + 1] - b;
+        const i64 hb = loff[q], lb = loff[q] + core_cnt[2 * q];
+        if (threadIdx.x == 0) {
+            s_hi = 0;
+            s_lo = 0;
+        }
+        __syncthreads();
+        for (i64 j0 = 0; j0 < deg; j0 += kFillUnroll * kFillBlock) {
+            uint8_t cd[kFillUnroll];
+#pragma unroll
+            for (int k = 0; k < kFillUnroll; k++) {
+                const i64 j = j0 + k * kFillBlock + threadIdx.x;
+                cd[k] = j < deg ? code[b + j] : kJpSelf;
+            }
+            int x[kFillUnroll];
+#pragma unroll
+            for (int k = 0; k < kFillUnroll; k++) {
+                const bool core = (cd[k] & (kJpCore | kJpSelf)) == kJpCore;
+                x[k] = core ? cnt[tails[b + j0 + k * kFillBlock + threadIdx.x]] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kFillUnroll; k++) {
+                const bool core = (cd[k] & (kJpCore | kJpSelf)) == kJpCore;
+                const bool lo = core && (cd[k] & kJpRelease), hi = core && !(cd[k] & kJpRelease);
+                const u64 bh = __ballot(hi), bl = __ballot(lo);
+                int ph = 0, pl = 0;
+                if (lane == 0) {
+                    if (bh) ph = atomicAdd(&s_hi, __popcll(bh));
+                    if (bl) pl = atomicAdd(&s_lo, __popcll(bl));
+                }
+                ph = __shfl(ph, 0);
+                pl = __shfl(pl, 0);
+                if (hi) ladj[hb + ph + __popcll(bh & below)] = (uint16_t)x[k];
+                if (lo) ladj[lb + pl + __popcll(bl & below)] = (uint16_t)x[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The core pass, one workgroup.  st[q]: while q is uncoloured, its count of
+// uncoloured higher core neighbours; then kColoured | depth << 12 | colour.
+// A vertex enters the queue when its count reaches zero (its higher
+// neighbours are all coloured: their states are final); a wavefront takes the
+// next queue slot, waits until it is written, colours the vertex from its
+// higher neighbours' states and then counts its lower neighbours down,
+// queueing those that reach zero.  Every vertex enters the queue exactly
+// once, so K slots are taken and every wait ends (slots are taken in order
+// and a queued vertex's releases never wait on a later slot).
+__global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K, const i64 *loff,
+                                                        const int32_t *core_cnt, const uint16_t *ladj,
+                                                        int32_t *colors, int32_t *dep, int32_t *err) {
+    __shared__ uint32_t st[kJpCoreMax];
+    __shared__ uint32_t queue[kJpCoreMax];
+    __shared__ uint32_t mk[kCoreWaves][kColorWords];
+    __shared__ int s_head, s_tail;
+    const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
+    const u64 below = (1ull << lane) - 1;
+    for (int q = tid; q < K; q += kCoreBlock) {
+        st[q] = (uint32_t)core_cnt[2 * q];
+        queue[q] = kQueueEmpty;
+    }
+    if (tid == 0) {
+        s_head = 0;
+        s_tail = 0;
+    }
+    __syncthreads();
+    auto enqueue = [&](bool ready, int x) {
+        const u64 bal = __ballot(ready);
+        if (!bal) return;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s_tail, __popcll(bal));
+        base = __shfl(base, 0);
+        if (ready) __hip_atomic_store(&queue[base + __popcll(bal & below)], (uint32_t)x, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    for (int q0 = 0; q0 < K; q0 += kCoreBlock) {
+        const int q = q0 + tid;
+        enqueue(q < K && core_cnt[2 * q] == 0, q);
+    }
+    __syncthreads();
+    uint32_t *m = mk[w];
+    while (true) {
+        int i = 0;
+        if (lane == 0) i = atomicAdd(&s_head, 1);
+        i = __shfl(i, 0);
+        if (i >= K) break;
+        uint32_t r = kQueueEmpty;
+        int spins = 0;
+        if (lane == 0) {
+            while ((r = __hip_atomic_load(&queue[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ==
+                       kQueueEmpty &&
+                   spins < kCoreSpinLimit) {
+                __builtin_amdgcn_s_sleep(1);
+                spins++;
+            }
+        }
+        r = __shfl(r, 0);
+        if (r == kQueueEmpty) {
+            if (lane == 0) atomicOr(err, 4);
+            break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const i64 b = loff[r], len = loff[r + 1] - b;
+        const int nhi = core_cnt[2 * r];
+        m[lane] = 0;
+        m[lane + kWave] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        int dm = 0;
+        for (i64 j0 = 0; j0 < nhi; j0 += 4 * kWave) {
+            int x[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const i64 j = j0 + k * kWave + lane;
+                x[k] = j < nhi ? (int)ladj[b + j] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (x[k] < 0) continue;
+                const uint32_t s = st[x[k]];
+                const int c = (int)(s & 0xFFFu), d = (int)((s >> 12) & 0x7FFFFu);
+                atomicOr(&m[c >> 5], 1u << (c & 31));
+                dm = d > dm ? d : dm;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        int cand = kMaxColors;
+        {
+            const uint32_t m0 = m[lane], m1 = m[lane + kWave];
+            if (m0 != 0xFFFFFFFFu) cand = lane * 32 + __ffs((int)~m0) - 1;
+            else if (m1 != 0xFFFFFFFFu) cand = (lane + kWave) * 32 + __ffs((int)~m1) - 1;
+        }
+        // colours 0..2047 sit in the first words of the lanes: the smallest
+        // candidate over the wave is the first free colour
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const int oc = __shfl_xor(cand, o);
+            cand = oc < cand ? oc : cand;
+            const int od = __shfl_xor(dm, o);
+            dm = od > dm ? od : dm;
+        }
+        if (lane == 0) {
+            int color = cand;
+            if (color >= kMaxColors) {
+                atomicOr(err, 1);
+                color = 0;
+            }
+            st[r] = kColoured | ((uint32_t)(dm + 1) << 12) | (uint32_t)color;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        // the lower core neighbours: one count down each, queued at zero
+        for (i64 j0 = nhi; j0 < len; j0 += 4 * kWave) {
+            int x[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const i64 j = j0 + k * kWave + lane;
+                x[k] = j < len ? (int)ladj[b + j] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool ready = x[k] >= 0 && atomicSub(&st[x[k]], 1u) == 1u;
+                enqueue(ready, x[k]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < K; q += kCoreBlock) {
+        const uint32_t s = st[q];
+        const i64 v = hlist[q];
+        if (s & kColoured) {
+            colors[v] = (int32_t)(s & 0xFFFu);
+            dep[v] = (int32_t)((s >> 12) & 0x7FFFFu);
+        } else {
+            atomicOr(err, 4);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_jp_depth_max(const int32_t *dep, i64 nv, int32_t *maxd) {
+    int m = 0;
+    for (i64 v = blockIdx.x * (i64)kBlock + threadIdx.x; v < nv; v += (i64)gridDim.x * kBlock)
+        m = dep[v] > m ? dep[v] : m;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const int x = __shfl_xor(m, o);
+        m = x > m ? x : m;
+    }
+    if (threadIdx.x % kWave == 0 && m > 0) atomicMax(maxd, m);
+}
+__global__ __launch_bounds__(kBlock) void k_jp_depth_hist(const int32_t *dep, i64 nv, int32_t top, unsigned *hist) {
+    __shared__ unsigned s_h[kJpDepthWindow];
+    for (int i = threadIdx.x; i < kJpDepthWindow; i += kBlock) s_h[i] = 0;
+    __syncthreads();
+    for (i64 v = blockIdx.x * (i64)kBlock + threadIdx.x; v < nv; v += (i64)gridDim.x * kBlock) {
+        const int i = top - dep[v];
+        if (i >= 0 && i < kJpDepthWindow) atomicAdd(&s_h[i], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kJpDepthWindow; i += kBlock)
+        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+__global__ void k_jp_depth_cut(const int32_t *dep, i64 nv, int32_t rounds, int32_t *colors) {
+    for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x)
+        if (dep[v] > rounds) colors[v] = -1;
+}
+
+}  // namespace
+
+i64 jp_bucket_lo(int b) {
+    if (b < 8) return b;
+    const int e = 3 + (b - 8) / 8, m = (b - 8) % 8;
+    return (i64)(8 + m) << (e - 3);
+}
+
+void launch_jp_degree_hist(const DevGraph &g, unsigned long long *hist, hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_jp_degree_hist<<<grid_for(g.nv, kBlock, 2048), kBlock, 0, s>>>(g.off, g.nv, hist);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_core_list(const DevGraph &g, u64 dcore, i64 *flags, i64 *pos, i64 *hlist, int32_t *cnt,
+                         hipStream_t s) {
+    if (g.nv <= 0) return;
+    k_jp_core_flags<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(g.off, g.nv, dcore, flags);
+    GLV_LAUNCH_CHECK();
+    scan_exclusive_i64(flags, pos, g.nv, s);
+    k_jp_core_scatter<<<grid_for(g.nv, kBlock, 8192), kBlock, 0, s>>>(flags, pos, g.nv, hlist, cnt);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_core_sizes(const int32_t *core_cnt, i64 K, i64 *sizes, hipStream_t s) {
+    k_jp_core_sizes<<<grid_for(K + 1, kBlock, 1024), kBlock, 0, s>>>(core_cnt, K, sizes);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_core_fill(const DevGraph &g, const uint8_t *code, const int32_t *cnt, const i64 *hlist, i64 K,
+                         const i64 *loff, const int32_t *core_cnt, uint16_t *ladj, hipStream_t s) {
+    if (K <= 0) return;
+    const unsigned blocks = (unsigned)std::min<i64>(K, 4096);
+    k_jp_core_fill<<<blocks, kFillBlock, 0, s>>>(g.off, g.tails, code, cnt, hlist, K, loff, core_cnt, ladj);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_core(const i64 *hlist, i64 K, const i64 *loff, const int32_t *core_cnt, const uint16_t *ladj,
+                    int32_t *colors, int32_t *dep, int32_t *err, hipStream_t s) {
+    if (K <= 0) return;
+    if (K > kJpCoreMax) {  // callers pass at most kJpCoreMax vertices
+        launch_fill_i32(err, 1, 4, s);
+        return;
+    }
+    k_jp_core<<<1, kCoreBlock, 0, s>>>(hlist, (int)K, loff, core_cnt, ladj, colors, dep, err);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_depth_max(const int32_t *dep, i64 nv, int32_t *maxd, hipStream_t s) {
+    if (nv <= 0) return;
+    k_jp_depth_max<<<grid_for(nv, kBlock, 2048), kBlock, 0, s>>>(dep, nv, maxd);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_depth_hist(const int32_t *dep, i64 nv, int32_t top, unsigned *hist, hipStream_t s) {
+    if (nv <= 0) return;
+    k_jp_depth_hist<<<grid_for(nv, kBlock, 1024), kBlock, 0, s>>>(dep, nv, top, hist);
+    GLV_LAUNCH_CHECK();
+}
+void launch_jp_depth_cut(const int32_t *dep, i64 nv, int32_t rounds, int32_t *colors, hipStream_t s) {
+    if (nv <= 0) return;
+    k_jp_depth_cut<<<grid_for(nv, kBlock, 8192), kBlock, 0, s>>>(dep, nv, rounds, colors);
+    GLV_LAUNCH_CHECK();
+}
+
+}  // namespace glv

@@ -35,14 +35,16 @@ __device__ inline u64 jp_priority(i64 v) {
 // Jones-Plassmann order: prio[] (larger degree first, then a hash of the id),
 // equal keys broken by the larger id, so the order is total.  The count's
 // byte for entry (v, u) compares u's priority pu with v's pv; the degree
-// class of u comes from the degree field of its priority.
-__device__ inline uint8_t jp_code(u64 pu, i64 u, u64 pv, i64 v) {
+// class of u, and whether u is a core vertex (degree dcore or more), come
+// from the degree field of its priority.
+__device__ inline uint8_t jp_code(u64 pu, i64 u, u64 pv, i64 v, u64 dcore) {
     if (u == v) return kJpSelf;
     const u64 du = pu >> 40;
     uint8_t c = (pu < pv || (pu == pv && u < v)) ? kJpRelease : 0;
     if (du > (u64)kBinMaxDeg[6]) c |= kJpLarge;
     if (du > (u64)kJpMidDeg) c |= kJpHuge;
     if (du > (u64)kGiantDeg) c |= kJpGiant;
+    if (du >= dcore) c |= kJpCore;
     return c;
 }
 
@@ -1903,7 +1905,7 @@ constexpr int kStageCap = 2048;
 // first-fit colour is at most its entry count, so a 128-colour mask per lane,
 // OR-ed over the group, decides it); heavier ones (up to 256 entries) take
 // the whole wave one after another, with the wave's LDS bitmap mk.  One pass
-// over a vertex's entries marks the colours of its higher-priority
+// over a vertex's entries marks the colours and depths of its higher-priority
 // neighbours (all coloured) and releases the lower-priority ones; frontier
 // vertices are pairwise non-adjacent, so every colour read is final.
 // push(ready, u) appends a released vertex (called by the whole wave).
@@ -1911,7 +1913,7 @@ constexpr int kJpGroup = 8;
 constexpr int kJpUnroll = 4;
 constexpr i64 kJpLightDeg = 64;
 template <typename Push>
-__device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *cnt,
+__device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *dep, int32_t *cnt,
                                 const uint8_t *code, const i64 *front, i64 lo, i64 n, uint32_t *mk, int32_t *err,
                                 Push push) {
     const int lane = threadIdx.x % kWave, g = lane / kJpGroup, gl = lane % kJpGroup;
@@ -1920,6 +1922,7 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
     const i64 b = in ? off[v] : 0, deg = in ? off[v + 1] - b : 0;
     const bool light = in && deg <= kJpLightDeg;
     u64 m0 = 0, m1 = 0;
+    int dm = 0;
     i64 itmax = light ? (deg + kJpGroup - 1) / kJpGroup : 0;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -1948,6 +1951,8 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
                 rd[k] = atomicSub(&cnt[u], 1) == 1;
             } else {
                 const int c = colors[u];
+                const int du = dep[u];
+                dm = du > dm ? du : dm;
                 if (c >= 0 && c < 64) m0 |= 1ull << c;
                 else if (c >= 64 && c < 128) m1 |= 1ull << (c - 64);
             }
@@ -1959,8 +1964,13 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
     for (int o = 1; o < kJpGroup; o <<= 1) {
         m0 |= __shfl_xor(m0, o);
         m1 |= __shfl_xor(m1, o);
+        const int x = __shfl_xor(dm, o);
+        dm = x > dm ? x : dm;
     }
-    if (light && gl == 0) colors[v] = ~m0 ? __ffsll((long long)~m0) - 1 : 64 + __ffsll((long long)~m1) - 1;
+    if (light && gl == 0) {
+        colors[v] = ~m0 ? __ffsll((long long)~m0) - 1 : 64 + __ffsll((long long)~m1) - 1;
+        dep[v] = dm + 1;
+    }
     u64 hm = __ballot(in && !light && gl == 0);
     while (hm) {
         const int src = __ffsll((long long)hm) - 1;
@@ -1969,6 +1979,7 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
         for (int i = lane; i < kMaxColors / 32; i += kWave) mk[i] = 0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        int hdm = 0;
         for (i64 j0 = 0; j0 < hd; j0 += kJpUnroll * kWave) {
             i64 uu[kJpUnroll];
             uint8_t cd[kJpUnroll];
@@ -1988,6 +1999,8 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
                     rd[k] = atomicSub(&cnt[u], 1) == 1;
                 } else {
                     const int c = colors[u];
+                    const int du = dep[u];
+                    hdm = du > hdm ? du : hdm;
                     if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
                 }
             }
@@ -2008,6 +2021,8 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
         for (int o = kWave / 2; o > 0; o >>= 1) {
             const int other = __shfl_xor(cand, o);
             cand = other < cand ? other : cand;
+            const int x = __shfl_xor(hdm, o);
+            hdm = x > hdm ? x : hdm;
         }
         if (lane == 0) {
             int color = cand;
@@ -2016,13 +2031,18 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
                 color = 0;
             }
             colors[hv] = color;
+            dep[hv] = hdm + 1;
         }
         __builtin_amdgcn_wave_barrier();
     }
 }
 
 // count[v] of vertices up to 256 entries, one thread each; heavier ones are
-// listed in heavy[] (heavy_count) for k_jp_count_heavy.
+// listed in heavy[] (heavy_count) for k_jp_count_heavy.  A core vertex v
+// (count[v] = its core index q on entry) gets its higher and lower core
+// neighbours counted into core_cnt[2q], [2q+1] instead, and never enters a
+// frontier: the core pass colours it.  A non-core vertex's higher core
+// neighbours are coloured before the rounds and do not count.
 __global__ __launch_bounds__(kBlock) void k_jp_count(const i64 *off, const i64 *tails, i64 nv, int32_t *cnt,
                                                      JpLists L, i64 *heavy, i64 *heavy_count) {
     __shared__ Stage<kStageCap> S;
@@ -2040,24 +2060,38 @@ __global__ __launch_bounds__(kBlock) void k_jp_count(const i64 *off, const i64 *
         }
         const i64 v = first + t * stride;
         int c = 0;
-        bool heavy_v = false;
+        bool heavy_v = false, core_v = false;
         if (v < nv) {
             const i64 b = off[v], e = off[v + 1];
+            const u64 pv = L.prio[v];
+            core_v = (pv >> 40) >= L.dcore;
             if (e - b > kBinMaxDeg[6]) {
                 heavy_v = true;
             } else {
-                const u64 pv = L.prio[v];
+                int hi = 0, lo = 0;
                 for (i64 j = b; j < e; j++) {
                     const i64 u = tails[j];
-                    const uint8_t cd = jp_code(L.prio[u], u, pv, v);
+                    const uint8_t cd = jp_code(L.prio[u], u, pv, v, L.dcore);
                     L.code[j] = cd;
-                    if (!(cd & (kJpRelease | kJpSelf))) c++;
+                    if (cd & kJpSelf) continue;
+                    if (cd & kJpCore) {
+                        if (cd & kJpRelease) lo++;
+                        else hi++;
+                    } else if (!(cd & kJpRelease)) {
+                        c++;
+                    }
                 }
-                cnt[v] = c;
+                if (core_v) {
+                    const int q = cnt[v];
+                    L.core_cnt[2 * q] = hi;
+                    L.core_cnt[2 * q + 1] = lo;
+                } else {
+                    cnt[v] = c;
+                }
             }
         }
         wave_append(heavy_v, v, heavy, heavy_count);
-        stage_append(v < nv && !heavy_v && c == 0, v, S, L.small, &L.counts[0]);  // at most 256 entries
+        stage_append(v < nv && !heavy_v && !core_v && c == 0, v, S, L.small, &L.counts[0]);  // at most 256 entries
     }
     stage_flush(S, L.small, &L.counts[0]);
 }
@@ -2072,16 +2106,35 @@ __global__ __launch_bounds__(kBlock) void k_jp_count_heavy(const i64 *off, const
     for (i64 q = (i64)blockIdx.x * WPB + threadIdx.x / kWave; q < nheavy; q += stride) {
         const i64 v = heavy[q];
         const u64 pv = L.prio[v];
-        int c = 0;
+        const bool core_v = (pv >> 40) >= L.dcore;
+        int c = 0, hi = 0, lo = 0;
         for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
             const i64 u = tails[j];
-            const uint8_t cd = jp_code(L.prio[u], u, pv, v);
+            const uint8_t cd = jp_code(L.prio[u], u, pv, v, L.dcore);
             L.code[j] = cd;
-            if (!(cd & (kJpRelease | kJpSelf))) c++;
+            if (cd & kJpSelf) continue;
+            if (cd & kJpCore) {
+                if (cd & kJpRelease) lo++;
+                else hi++;
+            } else if (!(cd & kJpRelease)) {
+                c++;
+            }
         }
-        for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
-        if (lane == 0) cnt[v] = c;
-        frontier_push(lane == 0 && c == 0, v, off, L);
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            c += __shfl_xor(c, o);
+            hi += __shfl_xor(hi, o);
+            lo += __shfl_xor(lo, o);
+        }
+        if (core_v) {
+            if (lane == 0) {
+                const int cq = cnt[v];
+                L.core_cnt[2 * cq] = hi;
+                L.core_cnt[2 * cq + 1] = lo;
+            }
+        } else {
+            if (lane == 0) cnt[v] = c;
+            frontier_push(lane == 0 && c == 0, v, off, L);
+        }
     }
 }
 
@@ -2102,7 +2155,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_wave(const i64 *off, con
     auto push = [&](bool ready, i64 u, uint8_t cd) { frontier_push(ready, u, cd, L, S); };
     for (i64 lo = wid * E; lo < nfront; lo += nw * E) {
         const i64 n = nfront - lo < E ? nfront - lo : E;
-        jp_wave8(off, tails, colors, cnt, L.code, front, lo, n, mark[w], err, push);
+        jp_wave8(off, tails, colors, L.dep, cnt, L.code, front, lo, n, mark[w], err, push);
     }
     stage_flush(S, L.small, &L.counts[0]);
 }
@@ -2113,18 +2166,23 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                                                               const i64 *front, i64 nfront, JpLists L,
                                                               int32_t *err) {
     __shared__ uint32_t s_mark[kMaxColors / 32];
-    __shared__ int s_color;
+    __shared__ int s_color, s_dep;
     __shared__ Stage<kStageCap> S;
     stage_init(S);
     for (i64 q = blockIdx.x; q < nfront; q += gridDim.x) {
         const i64 v = front[q];
         const i64 b = off[v], e = off[v + 1];
-        if (threadIdx.x == 0) s_color = kMaxColors;
+        if (threadIdx.x == 0) {
+            s_color = kMaxColors;
+            s_dep = 0;
+        }
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) s_mark[i] = 0;
         __syncthreads();
-        // one pass: colours of the higher-priority neighbours, release of the
-        // lower ones; whole wavefronts iterate together for the appends
+        // one pass: colours and depths of the higher-priority neighbours,
+        // release of the lower ones; whole wavefronts iterate together for
+        // the appends
         const i64 deg = e - b;
+        int dm = 0;
         for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kBlock) {
             i64 uu[kJpUnroll];
             uint8_t cd[kJpUnroll];
@@ -2144,12 +2202,19 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                     rd[k] = atomicSub(&cnt[u], 1) == 1;
                 } else {
                     const int c = colors[u];
+                    const int du = L.dep[u];
+                    dm = du > dm ? du : dm;
                     if (c >= 0) atomicOr(&s_mark[c >> 5], 1u << (c & 31));
                 }
             }
 #pragma unroll
             for (int k = 0; k < kJpUnroll; k++) frontier_push(rd[k], uu[k], cd[k], L, S);
         }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const int x = __shfl_xor(dm, o);
+            dm = x > dm ? x : dm;
+        }
+        if (threadIdx.x % kWave == 0 && dm > 0) atomicMax(&s_dep, dm);
         __syncthreads();
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
             const uint32_t m = s_mark[i];
@@ -2163,6 +2228,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
                 color = 0;
             }
             colors[v] = color;
+            L.dep[v] = s_dep + 1;
         }
         __syncthreads();
         if (S.n > kStageCap / 2) stage_flush(S, L.small, &L.counts[0]);  // S.n is settled here
@@ -2172,18 +2238,30 @@ __global__ __launch_bounds__(kBlock) void k_jp_frontier_block(const i64 *off, co
 
 // Giant frontier vertices, kGiantBlocks workgroups each (blockIdx.y): mark
 // the colors of the (coloured, higher) neighbours in the vertex's HBM bitmap
-// gmark[y][kMaxColors/32] (zero on entry) ...
+// gmark[y][0 .. kMaxColors/32) and their largest depth in gmark[y][kMaxColors/32]
+// (zero on entry) ...
+constexpr int kJpGiantMarkWords = kMaxColors / 32 + 1;
 __global__ __launch_bounds__(kBlock) void k_jp_giant_mark(const i64 *off, const i64 *tails,
-                                                          const int32_t *colors, const i64 *front,
-                                                          uint32_t *gmark) {
+                                                          const int32_t *colors, const int32_t *dep,
+                                                          const i64 *front, uint32_t *gmark) {
     const i64 v = front[blockIdx.y];
-    uint32_t *mk = gmark + (i64)blockIdx.y * (kMaxColors / 32);
+    uint32_t *mk = gmark + (i64)blockIdx.y * kJpGiantMarkWords;
+    int dm = 0;
     for (i64 j = off[v] + (i64)blockIdx.x * kBlock + threadIdx.x; j < off[v + 1]; j += (i64)gridDim.x * kBlock) {
         const i64 u = tails[j];
         if (u == v) continue;
         const int c = colors[u];
-        if (c >= 0) atomicOr(&mk[c >> 5], 1u << (c & 31));
+        if (c >= 0) {
+            atomicOr(&mk[c >> 5], 1u << (c & 31));
+            const int du = dep[u];
+            dm = du > dm ? du : dm;
+        }
     }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const int x = __shfl_xor(dm, o);
+        dm = x > dm ? x : dm;
+    }
+    if (threadIdx.x % kWave == 0 && dm > 0) atomicMax(&mk[kMaxColors / 32], (uint32_t)dm);
 }
 // ... then take the first free color and release the lower neighbours.
 __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, const i64 *tails,
@@ -2196,7 +2274,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, con
     __syncthreads();
     const i64 v = front[blockIdx.y];
     if (blockIdx.x == 0) {
-        const uint32_t *mk = gmark + (i64)blockIdx.y * (kMaxColors / 32);
+        const uint32_t *mk = gmark + (i64)blockIdx.y * kJpGiantMarkWords;
         if (threadIdx.x == 0) s_color = kMaxColors;
         __syncthreads();
         for (int i = threadIdx.x; i < kMaxColors / 32; i += kBlock) {
@@ -2211,6 +2289,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_giant_release(const i64 *off, con
                 color = 0;
             }
             colors[v] = color;
+            L.dep[v] = (int32_t)mk[kMaxColors / 32] + 1;
         }
     }
     const i64 b = off[v], deg = off[v + 1] - b;
@@ -2327,27 +2406,23 @@ __global__ __launch_bounds__(kTailBlock) void k_tail_seq(const i64 *list, i64 n,
 // neighbours, frontier = count zero, first-fit against the coloured higher
 // neighbours), with a grid barrier between rounds instead of a launch and a
 // host round trip.  Frontier vertices of degree <= 256 take a wavefront each,
-// heavier ones a workgroup each.  do_count: start from scratch (counts and
-// the first frontier); otherwise continue from the frontier in sA/lA (counts
-// ring[0], ring[1]) after colored0 vertices were coloured.  The launch ends
-// when at most tail_at vertices remain (the host colours them by sequential
-// first-fit in increasing id, launch_color_tail).  A frontier of more than
-// max_front vertices ends the launch before its round (the host's full-GPU
-// rounds take it; after r rounds the frontier is in sA/lA for even r, sB/lB
-// for odd r).  ring: 10 words, [2q], [2q+1] the small and large frontier
-// counts of rounds r = q (mod 3), [6] receives the rounds run here, [7]
-// counts heavy vertices while counting and then receives 1 when the launch
+// heavier ones a workgroup each.  Continues from the frontier in sA/lA
+// (counts ring[0], ring[1]) after colored0 vertices were coloured, until
+// every vertex is.  A frontier of more than max_front vertices ends the
+// launch before its round (the host's full-GPU rounds take it; after r
+// rounds the frontier is in sA/lA for even r, sB/lB for odd r).  ring: 10
+// words, [2q], [2q+1] the small and large frontier counts of rounds r = q
+// (mod 3), [6] receives the rounds run here, [7] receives 1 when the launch
 // ended on a large frontier, [8] the coloured vertices at the end.
 constexpr int kJpPBlock = 1024;
 __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, const i64 *tails, i64 nv,
                                                              int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB,
                                                              i64 *lA, i64 *lB, i64 *ring, unsigned *bar,
-                                                             int32_t *err, i64 tail_at, int do_count,
-                                                             i64 colored0, const u64 *prio, uint8_t *code,
-                                                             i64 max_front) {
+                                                             int32_t *err, i64 colored0, uint8_t *code,
+                                                             int32_t *dep, i64 max_front) {
     constexpr int WPB = kJpPBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
-    __shared__ int s_color;
+    __shared__ int s_color, s_dep;
     __shared__ Stage<kStageCap> S;  // appends to the next small frontier
     stage_init(S);
     __syncthreads();
@@ -2356,58 +2431,6 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
     uint32_t *mk = mark[w];
     unsigned seq = 0;
     bool alive = true;
-    if (do_count) {
-        // counts: one thread per vertex up to 256 entries; heavier ones are listed
-        // in lB (ring[7]) and counted by a wavefront each
-        for (i64 base = (i64)blockIdx.x * kJpPBlock; base < nv; base += (i64)nb * kJpPBlock) {
-            if (base != (i64)blockIdx.x * kJpPBlock) {  // room for this trip's appends
-                __syncthreads();
-                const int staged = S.n;
-                __syncthreads();
-                if (staged > kStageCap - kJpPBlock) stage_flush(S, sA, &ring[0]);
-            }
-            const i64 v = base + threadIdx.x;
-            int c = 0;
-            bool heavy = false;
-            if (v < nv) {
-                const i64 b = off[v], e = off[v + 1];
-                if (e - b > kBinMaxDeg[6]) {
-                    heavy = true;
-                } else {
-                    const u64 pv = prio[v];
-                    for (i64 j = b; j < e; j++) {
-                        const i64 u = tails[j];
-                        const uint8_t cd = jp_code(prio[u], u, pv, v);
-                        code[j] = cd;
-                        if (!(cd & (kJpRelease | kJpSelf))) c++;
-                    }
-                    cnt[v] = c;
-                }
-            }
-            wave_append(heavy, v, lB, &ring[7]);
-            stage_append(v < nv && !heavy && c == 0, v, S, sA, &ring[0]);
-        }
-        stage_flush(S, sA, &ring[0]);
-        alive = grid_barrier(bar, nb, ++seq);
-        if (alive) {
-            const i64 nheavy = __atomic_load_n(&ring[7], __ATOMIC_RELAXED);
-            for (i64 q = (i64)blockIdx.x * WPB + w; q < nheavy; q += (i64)nb * WPB) {
-                const i64 v = lB[q];
-                const u64 pv = prio[v];
-                int c = 0;
-                for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
-                    const i64 u = tails[j];
-                    const uint8_t cd = jp_code(prio[u], u, pv, v);
-                    code[j] = cd;
-                    if (!(cd & (kJpRelease | kJpSelf))) c++;
-                }
-                for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
-                if (lane == 0) cnt[v] = c;
-                wave_append(lane == 0 && c == 0, v, lA, &ring[1]);
-            }
-            alive = grid_barrier(bar, nb, ++seq);
-        }
-    }
     i64 colored = colored0;
     int r = 0;
     bool exited = false;
@@ -2437,7 +2460,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             };
             for (i64 lo = wid * E; lo < nsmall; lo += nw * E) {
                 const i64 n = nsmall - lo < E ? nsmall - lo : E;
-                jp_wave8(off, tails, colors, cnt, code, fs, lo, n, mk, err, push);
+                jp_wave8(off, tails, colors, dep, cnt, code, fs, lo, n, mk, err, push);
             }
         }
         __syncthreads();  // mark[0] is the large loop's and wave 0's
@@ -2447,10 +2470,15 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
             const i64 b = off[v], e = off[v + 1];
             const i64 deg = e - b;
             for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
-            if (threadIdx.x == 0) s_color = kMaxColors;
+            if (threadIdx.x == 0) {
+                s_color = kMaxColors;
+                s_dep = 0;
+            }
             __syncthreads();
-            // one pass: colours of the higher-priority neighbours (all coloured),
-            // release of the lower-priority ones (next round's frontier)
+            // one pass: colours and depths of the higher-priority neighbours
+            // (all coloured), release of the lower-priority ones (next round's
+            // frontier)
+            int dm = 0;
             for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kJpPBlock) {
                 i64 uu[kJpUnroll];
                 uint8_t cd[kJpUnroll];
@@ -2470,6 +2498,8 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                         rd[k] = atomicSub(&cnt[u], 1) == 1;
                     } else {
                         const int c = colors[u];
+                        const int du = dep[u];
+                        dm = du > dm ? du : dm;
                         if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
                     }
                 }
@@ -2480,6 +2510,11 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                     wave_append(bg, uu[k], nl_, cl);
                 }
             }
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const int x = __shfl_xor(dm, o);
+                dm = x > dm ? x : dm;
+            }
+            if (lane == 0 && dm > 0) atomicMax(&s_dep, dm);
             __syncthreads();
             if (threadIdx.x < kMaxColors / 32) {
                 const uint32_t m = mark[0][threadIdx.x];
@@ -2493,6 +2528,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                     color = 0;
                 }
                 colors[v] = color;
+                dep[v] = s_dep + 1;
             }
             __syncthreads();
             if (S.n > kStageCap / 2) stage_flush(S, ns_, cs);  // S.n is settled here
@@ -2501,9 +2537,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
         alive = grid_barrier(bar, nb, ++seq);
         colored += nsmall + nlarge;
         r++;
-        const i64 rest = nv - colored;
-        if (rest == 0) break;
-        if (rest <= tail_at) break;  // the host colours the tail
+        if (nv - colored <= 0) break;
         i64 *t = fs;
         fs = ns_;
         ns_ = t;
@@ -2514,282 +2548,6 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ring[6] = r;
         ring[7] = exited ? 1 : 0;
-        ring[8] = colored;
-    }
-}
-
-// The colour of vertex v from its forbidden-colour bitmap, by the G lanes of
-// a group (lane gl reads words gl, gl + G, ... below `words`): the first
-// clear bit, kMaxColors when there is none.
-template <int G>
-__device__ inline int jp_first_free(const uint32_t *fb, int gl, int words) {
-    int cand = kMaxColors;
-    for (int i = gl; i < words; i += G) {
-        const uint32_t m = fb[i];
-        if (m != 0xFFFFFFFFu) {
-            const int c = i * 32 + __ffs((int)~m) - 1;
-            cand = c < cand ? c : cand;
-        }
-    }
-#pragma unroll
-    for (int o = 1; o < G; o <<= 1) {
-        const int x = __shfl_xor(cand, o);
-        cand = x < cand ? x : cand;
-    }
-    return cand;
-}
-
-// Release of entry (cd, u) by a vertex coloured c: bit c into u's bitmap and
-// one off u's count; true when u becomes ready.
-__device__ inline bool jp_push_release(uint8_t cd, i64 u, int c, int32_t *cnt, uint32_t *forbid) {
-    if (!(cd & kJpRelease)) return false;
-    atomicOr(&forbid[u * (kMaxColors / 32) + (c >> 5)], 1u << (c & 31));
-    return atomicSub(&cnt[u], 1) == 1;
-}
-
-// One wavefront colours up to eight frontier vertices of at most kJpMidDeg
-// entries, front[lo .. lo+n): those of at most kJpLightDeg entries take a
-// group of kJpGroup lanes each (a first-fit colour of at most 64, found in the
-// first eight bitmap words), heavier ones the whole wave one after another
-// (at most 1024 entries: a colour below 2048, in the first 64 words).
-template <typename Push>
-__device__ inline void jp_push_wave8(const i64 *off, const i64 *tails, int32_t *colors, int32_t *cnt,
-                                     const uint8_t *code, uint32_t *forbid, const i64 *front, i64 lo, i64 n,
-                                     Push push) {
-    constexpr int FW = kMaxColors / 32;
-    const int lane = threadIdx.x % kWave, g = lane / kJpGroup, gl = lane % kJpGroup;
-    const bool in = g < n;
-    const i64 v = in ? front[lo + g] : -1;
-    const i64 b = in ? off[v] : 0, deg = in ? off[v + 1] - b : 0;
-    const bool light = in && deg <= kJpLightDeg;
-    const int c = jp_first_free<kJpGroup>(forbid + (light ? v : 0) * FW, gl, light ? kJpGroup : 0);
-    if (light && gl == 0) colors[v] = c;
-    i64 itmax = light ? (deg + kJpGroup - 1) / kJpGroup : 0;
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const i64 x = __shfl_xor(itmax, o);
-        itmax = x > itmax ? x : itmax;
-    }
-    for (i64 it0 = 0; it0 < itmax; it0 += kJpUnroll) {
-        i64 uu[kJpUnroll];
-        uint8_t cd[kJpUnroll];
-        bool rd[kJpUnroll];
-#pragma unroll
-        for (int k = 0; k < kJpUnroll; k++) {
-            const i64 j = (it0 + k) * kJpGroup + gl;
-            const bool e = light && j < deg;
-            uu[k] = e ? tails[b + j] : -1;
-            cd[k] = e ? code[b + j] : kJpSelf;
-        }
-#pragma unroll
-        for (int k = 0; k < kJpUnroll; k++) rd[k] = jp_push_release(cd[k], uu[k], c, cnt, forbid);
-#pragma unroll
-        for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k], cd[k]);
-    }
-    u64 hm = __ballot(in && !light && gl == 0);
-    while (hm) {
-        const int src = __ffsll((long long)hm) - 1;
-        hm &= hm - 1;
-        const i64 hv = __shfl(v, src), hb = __shfl(b, src), hd = __shfl(deg, src);
-        const int hc = jp_first_free<kWave>(forbid + hv * FW, lane, kWave);
-        if (lane == 0) colors[hv] = hc;
-        for (i64 j0 = 0; j0 < hd; j0 += kJpUnroll * kWave) {
-            i64 uu[kJpUnroll];
-            uint8_t cd[kJpUnroll];
-            bool rd[kJpUnroll];
-#pragma unroll
-            for (int k = 0; k < kJpUnroll; k++) {
-                const i64 j = j0 + k * kWave + lane;
-                uu[k] = j < hd ? tails[hb + j] : -1;
-                cd[k] = j < hd ? code[hb + j] : kJpSelf;
-            }
-#pragma unroll
-            for (int k = 0; k < kJpUnroll; k++) rd[k] = jp_push_release(cd[k], uu[k], hc, cnt, forbid);
-#pragma unroll
-            for (int k = 0; k < kJpUnroll; k++) push(rd[k], uu[k], cd[k]);
-        }
-    }
-}
-
-// Jones-Plassmann rounds with pushed colours (graphs of at most kJpPushMaxNV
-// vertices).  The count is the counted rounds' (entry codes, counts, first
-// frontier); frontier vertices of at most kJpMidDeg entries are coloured by
-// jp_push_wave8, heavier ones are cut into chunks of kJpPBlock entries that
-// are dealt out to all workgroups (each chunk reads the vertex's bitmap for
-// its colour; chunk 0 writes it).  The launch ends when at most tail_at
-// vertices remain (launch_color_tail colours them).  ring as
-// k_jp_persistent's ([6] rounds, [7] heavy vertices while counting, [8]
-// coloured vertices at the end).
-constexpr int kJpChunkBatch = kJpPBlock;  // large frontier vertices per batch
-constexpr i64 kJpStageMin = 4096;
-__global__ __launch_bounds__(kJpPBlock) void k_jp_push(const i64 *off, const i64 *tails, i64 nv, int32_t *colors,
-                                                       int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
-                                                       i64 *ring, unsigned *bar, int32_t *err, i64 tail_at,
-                                                       const u64 *prio, uint8_t *code, uint32_t *forbid) {
-    constexpr int WPB = kJpPBlock / kWave;
-    constexpr int FW = kMaxColors / 32;
-    __shared__ i64 bv[kJpChunkBatch];  // large frontier vertices of the batch: ids,
-    __shared__ i64 bb[kJpChunkBatch];  // first entries
-    __shared__ i64 bd[kJpChunkBatch];  // and entry counts
-    __shared__ int s_cw[kMaxColors / 32 / kWave];
-    __shared__ Stage<kStageCap> S;  // appends to the next small frontier
-    stage_init(S);
-    __syncthreads();
-    const unsigned nb = gridDim.x;
-    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    unsigned seq = 0;
-    bool alive = true;
-    // counts: one thread per vertex up to 256 entries; heavier ones are listed
-    // in lB (ring[7]) and counted by a wavefront each
-    for (i64 base = (i64)blockIdx.x * kJpPBlock; base < nv; base += (i64)nb * kJpPBlock) {
-        if (base != (i64)blockIdx.x * kJpPBlock) {  // room for this trip's appends
-            __syncthreads();
-            const int staged = S.n;
-            __syncthreads();
-            if (staged > kStageCap - kJpPBlock) stage_flush(S, sA, &ring[0]);
-        }
-        const i64 v = base + threadIdx.x;
-        int c = 0;
-        bool heavy = false;
-        if (v < nv) {
-            const i64 b = off[v], e = off[v + 1];
-            if (e - b > kBinMaxDeg[6]) {
-                heavy = true;
-            } else {
-                const u64 pv = prio[v];
-                for (i64 j = b; j < e; j++) {
-                    const i64 u = tails[j];
-                    const uint8_t cd = jp_code(prio[u], u, pv, v);
-                    code[j] = cd;
-                    if (!(cd & (kJpRelease | kJpSelf))) c++;
-                }
-                cnt[v] = c;
-            }
-        }
-        wave_append(heavy, v, lB, &ring[7]);
-        stage_append(v < nv && !heavy && c == 0, v, S, sA, &ring[0]);
-    }
-    stage_flush(S, sA, &ring[0]);
-    alive = grid_barrier(bar, nb, ++seq);
-    if (alive) {
-        const i64 nheavy = __atomic_load_n(&ring[7], __ATOMIC_RELAXED);
-        for (i64 q = (i64)blockIdx.x * WPB + w; q < nheavy; q += (i64)nb * WPB) {
-            const i64 v = lB[q];
-            const u64 pv = prio[v];
-            const i64 deg = off[v + 1] - off[v];
-            int c = 0;
-            for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) {
-                const i64 u = tails[j];
-                const uint8_t cd = jp_code(prio[u], u, pv, v);
-                code[j] = cd;
-                if (!(cd & (kJpRelease | kJpSelf))) c++;
-            }
-            for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
-            if (lane == 0) cnt[v] = c;
-            wave_append(lane == 0 && c == 0 && deg <= kJpMidDeg, v, sA, &ring[0]);
-            wave_append(lane == 0 && c == 0 && deg > kJpMidDeg, v, lA, &ring[1]);
-        }
-        alive = grid_barrier(bar, nb, ++seq);
-    }
-    i64 colored = 0;
-    int r = 0;
-    i64 *fs = sA, *fl = lA, *ns_ = sB, *nl_ = lB;
-    while (alive) {
-        const int q = r % 3, nq = (r + 1) % 3, zq = (r + 2) % 3;
-        const i64 nsmall = __hip_atomic_load(&ring[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const i64 nlarge = __hip_atomic_load(&ring[2 * q + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (nsmall + nlarge == 0) break;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            __atomic_store_n(&ring[2 * zq], (i64)0, __ATOMIC_RELAXED);
-            __atomic_store_n(&ring[2 * zq + 1], (i64)0, __ATOMIC_RELAXED);
-        }
-        i64 *cs = &ring[2 * nq], *cl = &ring[2 * nq + 1];
-        // the first large frontier vertices, read along with the counts
-        const i64 ahead = (i64)threadIdx.x < nv ? fl[threadIdx.x] : 0;
-        // appends staged in LDS only when the frontier is large (the flush
-        // costs workgroup barriers; a few appends do not contend)
-        const bool stg = nsmall + nlarge > kJpStageMin;
-        auto push = [&](bool ready, i64 u, uint8_t cd) {
-            const bool big = ready && (cd & kJpHuge);
-            if (stg) stage_append(ready && !big, u, S, ns_, cs);
-            else wave_append(ready && !big, u, ns_, cs);
-            wave_append(big, u, nl_, cl);
-        };
-        // frontier vertices of at most kJpMidDeg entries: eight per wavefront
-        {
-            constexpr i64 E = kWave / kJpGroup;
-            const i64 nw = (i64)nb * WPB, wid = (i64)blockIdx.x * WPB + w;
-            for (i64 lo = wid * E; lo < nsmall; lo += nw * E) {
-                const i64 n = nsmall - lo < E ? nsmall - lo : E;
-                jp_push_wave8(off, tails, colors, cnt, code, forbid, fs, lo, n, push);
-            }
-        }
-        // heavier ones: chunks of kJpPBlock entries, chunk k of the i-th vertex
-        // of the list to workgroup (k + 13 i) mod nb (every workgroup walks
-        // the list in batches of kJpChunkBatch vertices)
-        for (i64 b0 = 0; b0 < nlarge; b0 += kJpChunkBatch) {
-            const int nbv = (int)(nlarge - b0 < kJpChunkBatch ? nlarge - b0 : kJpChunkBatch);
-            if (threadIdx.x < nbv) {
-                const i64 v = b0 == 0 ? ahead : fl[b0 + threadIdx.x];
-                const i64 beg = off[v];
-                bv[threadIdx.x] = v;
-                bb[threadIdx.x] = beg;
-                bd[threadIdx.x] = off[v + 1] - beg;
-            }
-            __syncthreads();
-            for (int i = 0; i < nbv; i++) {
-                const i64 v = bv[i], b = bb[i], deg = bd[i];
-                const i64 nch = (deg + kJpPBlock - 1) / kJpPBlock;
-                i64 k = ((i64)blockIdx.x - ((b0 + i) * 13) % (i64)nb) % (i64)nb;
-                if (k < 0) k += nb;
-                for (; k < nch; k += nb) {
-                    const i64 j = k * kJpPBlock + threadIdx.x;
-                    const i64 u = j < deg ? tails[b + j] : -1;
-                    const uint8_t cd = j < deg ? code[b + j] : kJpSelf;
-                    // the colour: the first two wavefronts read the bitmap's words
-                    int cw = kMaxColors;
-                    if (threadIdx.x < FW) {
-                        const uint32_t m = forbid[v * FW + threadIdx.x];
-                        if (m != 0xFFFFFFFFu) cw = (int)threadIdx.x * 32 + __ffs((int)~m) - 1;
-                    }
-#pragma unroll
-                    for (int o = kWave / 2; o > 0; o >>= 1) {
-                        const int x = __shfl_xor(cw, o);
-                        cw = x < cw ? x : cw;
-                    }
-                    if (lane == 0 && w < FW / kWave) s_cw[w] = cw;
-                    __syncthreads();
-                    int c = kMaxColors;
-#pragma unroll
-                    for (int q2 = 0; q2 < FW / kWave; q2++) c = s_cw[q2] < c ? s_cw[q2] : c;
-                    if (c >= kMaxColors) {
-                        if (k == 0 && threadIdx.x == 0) atomicOr(err, 1);
-                        c = 0;
-                    }
-                    if (k == 0 && threadIdx.x == 0) colors[v] = c;
-                    const bool rd = jp_push_release(cd, u, c, cnt, forbid);
-                    push(rd, u, cd);
-                    __syncthreads();  // s_cw is read by every thread before the next chunk sets it
-                }
-            }
-            __syncthreads();  // bv, bb, bd of this batch are read
-        }
-        if (stg) stage_flush(S, ns_, cs);
-        alive = grid_barrier(bar, nb, ++seq);
-        colored += nsmall + nlarge;
-        r++;
-        const i64 rest = nv - colored;
-        if (rest == 0) break;
-        if (rest <= tail_at) break;  // the host colours the tail
-        i64 *tp = fs;
-        fs = ns_;
-        ns_ = tp;
-        tp = fl;
-        fl = nl_;
-        nl_ = tp;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ring[6] = r;
         ring[8] = colored;
     }
 }
@@ -3317,9 +3075,9 @@ void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const 
         GLV_LAUNCH_CHECK();
     }
     if (n[2] > 0) {
-        launch_fill_i32((int32_t *)gmark, n[2] * (kMaxColors / 32), 0, s);
+        launch_fill_i32((int32_t *)gmark, n[2] * kJpGiantMarkWords, 0, s);
         const dim3 grid(kGiantBlocks, (unsigned)n[2]);
-        k_jp_giant_mark<<<grid, kBlock, 0, s>>>(g.off, g.tails, colors, cur.giant, gmark);
+        k_jp_giant_mark<<<grid, kBlock, 0, s>>>(g.off, g.tails, colors, cur.dep, cur.giant, gmark);
         GLV_LAUNCH_CHECK();
         k_jp_giant_release<<<grid, kBlock, 0, s>>>(g.off, g.tails, colors, cnt, cur.giant, gmark, next, err);
         GLV_LAUNCH_CHECK();
@@ -3359,43 +3117,17 @@ int jp_persistent_blocks() {
     return cached;
 }
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
-                          i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, const u64 *prio, uint8_t *code, i64 max_front, int nblocks,
-                          hipStream_t s) {
+                          i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 colored0, const u64 *prio,
+                          uint8_t *code, int32_t *dep, i64 max_front, int nblocks, hipStream_t s) {
+    (void)prio;
     if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0) return false;
     const i64 *off = g.off, *tails = g.tails;
-    i64 nv = g.nv, ta = tail_at, c0 = colored0, mf = max_front;
-    int dc = count ? 1 : 0;
-    const u64 *pr = prio;
+    i64 nv = g.nv, c0 = colored0, mf = max_front;
     uint8_t *cd = code;
-    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &dc, &c0, &pr,
-                    &cd, &mf};
+    int32_t *dp = dep;
+    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &c0, &cd, &dp, &mf};
     return launch_resident((const void *)k_jp_persistent, nblocks, kJpPBlock, args, jp_persistent_blocks(), s,
                            "persistent JP rounds");
-}
-int jp_push_blocks() {
-    static int cached = -1;
-    if (cached < 0) {
-        int dev = 0, cus = 0, per = 0;
-        GLV_HIP(hipGetDevice(&dev));
-        GLV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        GLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_jp_push, kJpPBlock, 0));
-        cached = per > 0 ? cus * per : 0;
-    }
-    return cached;
-}
-bool launch_jp_push(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
-                    i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, const u64 *prio, uint8_t *code,
-                    uint32_t *forbid, int nblocks, hipStream_t s) {
-    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0 || g.nv > kJpPushMaxNV) return false;
-    const i64 *off = g.off, *tails = g.tails;
-    i64 nv = g.nv, ta = tail_at;
-    const u64 *pr = prio;
-    uint8_t *cd = code;
-    uint32_t *fb = forbid;
-    void *args[] = {&off, &tails, &nv, &colors, &cnt, &sA, &sB, &lA, &lB, &ring, &bar, &err, &ta, &pr, &cd, &fb};
-    return launch_resident((const void *)k_jp_push, nblocks, kJpPBlock, args, jp_push_blocks(), s,
-                           "pushed-colour JP rounds");
 }
 void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s) {
     if (g.nv <= 0) return;

@@ -131,9 +131,22 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
                      double *part, hipStream_t s);
 
 // --- coloring ------------------------------------------------------------
-// Frontier lists of the Jones-Plassmann rounds by degree: small (<= 256, a
-// wavefront each), large (<= kGiantDeg, a workgroup each), giant
-// (kGiantBlocks workgroups each); counts[0..2] their lengths.
+// Jones-Plassmann with degree-first priorities, computed as a topological
+// sweep of the priority DAG (a vertex is coloured once its higher-priority
+// neighbours are, with the smallest colour they do not use).  The round of a
+// vertex in the classic formulation is its depth in that DAG (1 + the largest
+// depth of its higher neighbours); the rounds end once at most kJpTail
+// vertices are deeper than the last round, and those are coloured first-fit
+// one at a time in increasing id.  The core -- the vertices of the largest
+// degrees, a prefix of the priority order -- is coloured first by one
+// workgroup from LDS (k_jp_core), the rest by frontier rounds that take the
+// core's colours as given; depths are tracked, so the tail is the same set
+// whatever order computed the colours.
+//
+// Frontier lists of the rounds by degree: small (<= 256, a wavefront each),
+// large (<= kGiantDeg, a workgroup each), giant (kGiantBlocks workgroups
+// each); counts[0..2] their lengths.
+constexpr u64 kJpNoCore = u64(1) << 24;  // above every priority's degree field
 struct JpLists {
     i64 *small, *large, *giant;
     i64 *counts;
@@ -141,21 +154,31 @@ struct JpLists {
     // one byte per adjacency entry j (row v, neighbour u = tails[j]), written
     // by the count: kJpRelease when u has the lower priority (v releases u),
     // kJpLarge / kJpHuge / kJpGiant when u has more than 256 / kJpMidDeg /
-    // kGiantDeg entries, kJpSelf for u == v; no release bit: u's colour counts
-    // for v
+    // kGiantDeg entries, kJpSelf for u == v, kJpCore when u is a core vertex;
+    // no release bit: u's colour counts for v
     uint8_t *code = nullptr;
+    int32_t *dep = nullptr;      // depth of every coloured vertex
+    u64 dcore = kJpNoCore;       // core: vertices of at least this degree (kJpNoCore: none)
+    int32_t *core_cnt = nullptr; // per core vertex q: [2q] higher, [2q+1] lower core neighbours
 };
-constexpr uint8_t kJpRelease = 1, kJpLarge = 2, kJpGiant = 4, kJpSelf = 8, kJpHuge = 16;
+constexpr uint8_t kJpRelease = 1, kJpLarge = 2, kJpGiant = 4, kJpSelf = 8, kJpHuge = 16, kJpCore = 32;
 constexpr i64 kJpMidDeg = 1024;
+// core vertices at most (one workgroup's LDS: state, counts and queue)
+constexpr i64 kJpCoreMax = 16384;
+// graphs of more than kJpCoreMax vertices take a core only when it starts at
+// this degree or above (hubs, whose priority chains make the rounds deep)
+constexpr i64 kJpCoreMinDeg = 1024;
 // Jones-Plassmann priorities: (min(degree, 2^23 - 1) << 40) | (splitmix64(v) >> 24),
 // equal keys broken by the larger id
 void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s);
-// count[v] = uncoloured higher-priority neighbours; count-0 vertices form the
-// first frontier.  heavy: nv scratch, heavy_count: one zeroed word.
+// count[v] = uncoloured non-core higher-priority neighbours of a non-core v;
+// count-0 vertices form the first frontier.  A core vertex v holds its core
+// index in count[v] on entry; its core neighbour counts go to L.core_cnt.
+// heavy: nv scratch, heavy_count: one zeroed word.
 void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *heavy, i64 *heavy_count,
                      hipStream_t s);
 // color one frontier (n[0..2] vertices in cur's lists) and build the next
-// one into next (whose counts are zero); gmark: n[2] * kMaxColors/32 words
+// one into next (whose counts are zero); gmark: n[2] * (kMaxColors/32 + 1) words
 void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const JpLists &cur,
                         const i64 n[3], const JpLists &next, uint32_t *gmark, int32_t *err,
                         hipStream_t s);
@@ -167,36 +190,47 @@ void launch_jp_finish(int32_t *state, i64 nv, int32_t *maxc, hipStream_t s);
 // base: n * kMaxColors/32 words; adj: n * kJpTail/64 words; n <= kJpTail.
 void launch_color_tail(const DevGraph &g, int32_t *state, i64 *flags, i64 *pos, i64 *list, i64 n,
                        int32_t *tidx, uint32_t *base, u64 *adj, int32_t *err, hipStream_t s);
-constexpr i64 kJpTail = 1024;   // switch to the tail at or below this many uncoloured vertices
+constexpr i64 kJpTail = 1024;   // the tail: at most this many vertices
 // Jones-Plassmann rounds in one persistent launch (same result as the
-// host-driven rounds): from scratch (count) or continuing from the frontier
-// in sA (small, count ring[0]) and lA (large, ring[1]) after colored0
-// vertices; false when the launch is refused.  ring: 8 words (zero except the
-// frontier counts), ring[6] receives the rounds; bar: kBarrierWords zeroed
-// words; err: 1 too many colors.  sA..lB: nv each.
+// host-driven rounds), continuing from the frontier in sA (small, count
+// ring[0]) and lA (large, ring[1]) after colored0 vertices; false when the
+// launch is refused.  ring: 10 words (zero except the frontier counts),
+// ring[6] receives the rounds; bar: kBarrierWords zeroed words; err: 1 too
+// many colors.  sA..lB: nv each.
 int jp_persistent_blocks();
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
-                          i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, bool count,
-                          i64 colored0, const u64 *prio, uint8_t *code, i64 max_front, int nblocks,
-                          hipStream_t s);
-// Jones-Plassmann rounds of a graph of at most kJpPushMaxNV vertices in one
-// persistent launch that pushes colours: a vertex coloured c sets bit c of
-// the forbidden-colour bitmap (forbid[u * kMaxColors/32 ..], zero on entry,
-// nv * kMaxColors/32 words) of every lower-priority neighbour u as it
-// releases it, so a vertex takes its colour from its own bitmap when its
-// count reaches zero and the entries of a heavy vertex can be shared out over
-// all workgroups.  Same colouring as the counted rounds; no hand-back to the
-// host.  ring: 10 zeroed words, [6] receives the rounds; false when the
-// launch is refused.
-constexpr i64 kJpPushMaxNV = i64(1) << 17;
-int jp_push_blocks();
-bool launch_jp_push(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA, i64 *lB,
-                    i64 *ring, unsigned *bar, int32_t *err, i64 tail_at, const u64 *prio, uint8_t *code,
-                    uint32_t *forbid, int nblocks, hipStream_t s);
+                          i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 colored0, const u64 *prio,
+                          uint8_t *code, int32_t *dep, i64 max_front, int nblocks, hipStream_t s);
 // frontiers below this many vertices finish in one persistent launch
 constexpr i64 kJpPersistFrontier = 16384;
 // a persistent launch hands back to full-GPU rounds once a frontier exceeds this
 constexpr i64 kJpPersistExit = 32768;
+// The core (jp_core.hip).  hist: kJpDegBuckets zeroed words, the vertex
+// count per degree bucket (degrees below 8 exact, then 8 buckets per octave;
+// jp_bucket_lo(b) is the smallest degree of bucket b).
+constexpr int kJpDegBuckets = 512;
+i64 jp_bucket_lo(int b);
+void launch_jp_degree_hist(const DevGraph &g, unsigned long long *hist, hipStream_t s);
+// hlist[0 .. K) = the vertices of degree >= dcore in increasing id, count[v] =
+// their index (flags, pos: nv scratch)
+void launch_jp_core_list(const DevGraph &g, u64 dcore, i64 *flags, i64 *pos, i64 *hlist, int32_t *cnt,
+                         hipStream_t s);
+// sizes[q] = core neighbours of core vertex q (q < K), sizes[K] = 0
+void launch_jp_core_sizes(const int32_t *core_cnt, i64 K, i64 *sizes, hipStream_t s);
+// ladj[loff[q] ..): the core indices of core vertex q's higher core
+// neighbours, then of its lower ones (codes of the count)
+void launch_jp_core_fill(const DevGraph &g, const uint8_t *code, const int32_t *cnt, const i64 *hlist, i64 K,
+                         const i64 *loff, const int32_t *core_cnt, uint16_t *ladj, hipStream_t s);
+// colours and depths of the K core vertices, one workgroup
+void launch_jp_core(const i64 *hlist, i64 K, const i64 *loff, const int32_t *core_cnt, const uint16_t *ladj,
+                    int32_t *colors, int32_t *dep, int32_t *err, hipStream_t s);
+// largest depth (maxd: one word, zero on entry)
+void launch_jp_depth_max(const int32_t *dep, i64 nv, int32_t *maxd, hipStream_t s);
+// hist[i] = vertices of depth top - i, i < kJpDepthWindow (zeroed words)
+constexpr int kJpDepthWindow = 4096;
+void launch_jp_depth_hist(const int32_t *dep, i64 nv, int32_t top, unsigned *hist, hipStream_t s);
+// vertices deeper than rounds become uncoloured (the tail)
+void launch_jp_depth_cut(const int32_t *dep, i64 nv, int32_t rounds, int32_t *colors, hipStream_t s);
 void launch_color_conflicts(const DevGraph &g, const int32_t *colors, i64 *count, hipStream_t s);
 
 // --- renumber / compose --------------------------------------------------

@@ -156,6 +156,11 @@ void giant_args(const PhaseSetup &P, int c, SweepArgs &A) {
 }
 
 // ----------------------------------------------------------------- coloring
+// Jones-Plassmann with degree-first priorities and a sequential tail (the
+// engine's own order; the reference's coloring, coloringDistanceOne.cpp:50-233,
+// is speculative and thread-dependent).  The core (the highest degrees) is
+// coloured first by one workgroup, the rest by frontier rounds; depths give
+// the tail (kernels.hpp, coloring).
 int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     const i64 nv = g.nv;
     if (nv == 0) return -1;
@@ -164,30 +169,94 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
                                          std::max<i64>(4, nv / 2048)});
     DBuf<i64> ring(10, s_);
     DBuf<unsigned> bar(kBarrierWords, s_);
-    DBuf<int32_t> cnt(nv, s_), err(1, s_), maxc(1, s_);
+    DBuf<int32_t> cnt(nv, s_), dep(nv, s_), err(1, s_), maxc(1, s_);
     DBuf<u64> prio(nv, s_);
     DBuf<uint8_t> code(std::max<i64>(g.nnz, 1), s_);
     launch_jp_prio(g, prio.p, s_);
     launch_fill_i32(err.p, 1, 0, s_);
     launch_fill_i32(maxc.p, 1, -1, s_);
-    int round = 0;
-    i64 colored = 0;
-    i64 hc[3] = {0, 0, 0};  // sizes of the current frontier's small, large and giant lists
     DBuf<i64> sA(nv, s_), sB(nv, s_), lA(nv, s_), lB(nv, s_), gA(nv, s_), gB(nv, s_), counts(8, s_);
     JpLists cur{sA.p, lA.p, gA.p, counts.p, prio.p, code.p}, next{sB.p, lB.p, gB.p, counts.p + 4, prio.p, code.p};
-    // Rounds in one persistent launch while the frontier does not fill the
-    // GPU.  count: start from scratch; else continue from the frontier in
-    // cur (hc, no giant list).  Returns 0 when the launch was refused, 1 when
-    // the coloring is complete, 2 when a large frontier handed it back (cur,
-    // hc, colored and round then describe the state).
-    auto persistent = [&](bool count) -> int {
-        i64 h[10] = {count ? 0 : hc[0], count ? 0 : hc[1], 0, 0, 0, 0, 0, 0, 0, 0};
+    cur.dep = next.dep = dep.p;
+
+    // the core: every vertex of a small graph; on a larger one the vertices
+    // of the largest degrees, at most kJpCoreMax, when they reach kJpCoreMinDeg
+    u64 dcore = kJpNoCore;
+    i64 K = 0;
+    if (nv <= kJpCoreMax) {
+        dcore = 0;
+        K = nv;
+    } else {
+        DBuf<unsigned long long> hist(kJpDegBuckets, s_);
+        launch_fill_i64((i64 *)hist.p, kJpDegBuckets, 0, s_);
+        launch_jp_degree_hist(g, hist.p, s_);
+        std::vector<unsigned long long> h(kJpDegBuckets);
+        Readback rb(s_);
+        rb.add(h.data(), hist.p, kJpDegBuckets * sizeof(unsigned long long));
+        rb.run();
+        i64 above = 0;
+        for (int b = kJpDegBuckets - 1; b > 0; b--) {
+            if (above + (i64)h[b] > kJpCoreMax) break;
+            above += (i64)h[b];
+            const i64 lo = jp_bucket_lo(b);
+            if (lo >= kJpCoreMinDeg && lo < (i64(1) << 23) && above > 0) {
+                dcore = (u64)lo;
+                K = above;
+            }
+        }
+    }
+    DBuf<i64> hlist, loff;
+    DBuf<int32_t> core_cnt;
+    if (K > 0) {
+        hlist.alloc(K, s_);
+        loff.alloc(K + 1, s_);
+        core_cnt.alloc(2 * K, s_);
+        launch_jp_core_list(g, dcore, gA.p, gB.p, hlist.p, cnt.p, s_);
+        cur.dcore = next.dcore = dcore;
+        cur.core_cnt = next.core_cnt = core_cnt.p;
+    }
+    // counts, codes and the first frontier of the non-core vertices; core
+    // neighbour counts of the core vertices
+    i64 hc[3] = {0, 0, 0};  // sizes of the current frontier's small, large and giant lists
+    launch_fill_i64(counts.p, 8, 0, s_);
+    Profiler::get().begin(K_COLOR, s_);
+    launch_jp_count(g, cnt.p, cur, next.small, cur.counts + 3, s_);  // heavy list in the next round's buffer
+    Profiler::get().end(K_COLOR, 0.0, s_);
+    i64 core_entries = 0;
+    if (K > 0) {
+        launch_jp_core_sizes(core_cnt.p, K, gA.p, s_);
+        scan_exclusive_i64(gA.p, loff.p, K + 1, s_);
+    }
+    {
+        Readback rb(s_);
+        rb.add(hc, cur.counts, sizeof(hc));
+        if (K > 0) rb.add(&core_entries, loff.p + K, sizeof(i64));
+        rb.run();
+    }
+    i64 colored = 0;
+    if (K > 0) {
+        DBuf<uint16_t> ladj(std::max<i64>(core_entries, 1), s_);
+        Profiler::get().begin(K_COLOR, s_);
+        launch_jp_core_fill(g, code.p, cnt.p, hlist.p, K, loff.p, core_cnt.p, ladj.p, s_);
+        launch_jp_core(hlist.p, K, loff.p, core_cnt.p, ladj.p, colors, dep.p, err.p, s_);
+        Profiler::get().end(K_COLOR, 0.0, s_);
+        colored = K;
+    }
+
+    // Rounds of the rest in one persistent launch while the frontier does not
+    // fill the GPU: continue from the frontier in cur (hc, no giant list).
+    // Returns 0 when the launch was refused, 1 when every vertex is coloured,
+    // 2 when a large frontier handed it back (cur, hc and colored then
+    // describe the state).
+    int prounds = 0;
+    auto persistent = [&]() -> int {
+        i64 h[10] = {hc[0], hc[1], 0, 0, 0, 0, 0, 0, 0, 0};
         GLV_HIP(hipMemcpyAsync(ring.p, h, sizeof(h), hipMemcpyHostToDevice, s_));
         launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
         const bool ok = pnb > 0 && launch_jp_persistent(g, colors, cnt.p, cur.small, next.small, cur.large,
-                                                        next.large, ring.p, bar.p, err.p, kJpTail, count,
-                                                        colored, prio.p, code.p, kJpPersistExit, pnb, s_);
+                                                        next.large, ring.p, bar.p, err.p, colored, prio.p, code.p,
+                                                        dep.p, kJpPersistExit, pnb, s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
         unsigned abort_flag = 0;
         Readback rb(s_);
@@ -197,7 +266,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         if (!ok) return 0;
         if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
         const int r = (int)h[6];
-        round += r;
+        prounds += r;
         colored = h[8];
         if (h[7] == 0) return 1;
         hc[0] = h[2 * (r % 3)];
@@ -206,56 +275,12 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         if (r % 2 == 1) std::swap(cur, next);  // the frontier sits in the second pair of lists
         return 2;
     };
-    bool done = false, have_frontier = false;
-    if (nv <= kJpPushMaxNV) {
-        // small graphs: every round in one launch with pushed colours, the
-        // entries of heavy vertices shared out over all workgroups
-        const int nb = (int)std::min<i64>({(i64)jp_push_blocks(), kPersistMaxBlocks,
-                                           std::max<i64>(8, g.nnz / 1048576)});
-        DBuf<uint32_t> forbid(nv * (kMaxColors / 32), s_);
-        launch_fill_i32((int32_t *)forbid.p, nv * (kMaxColors / 32), 0, s_);
-        launch_fill_i64(ring.p, 10, 0, s_);
-        launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
-        Profiler::get().begin(K_COLOR, s_);
-        const bool ok = nb > 0 && launch_jp_push(g, colors, cnt.p, sA.p, sB.p, lA.p, lB.p, ring.p, bar.p, err.p,
-                                                 kJpTail, prio.p, code.p, forbid.p, nb, s_);
-        Profiler::get().end(K_COLOR, 0.0, s_);
-        if (ok) {
-            i64 h[10];
-            unsigned abort_flag = 0;
-            Readback rb(s_);
-            rb.add(h, ring.p, sizeof(h));
-            rb.add(&abort_flag, bar.p, sizeof(unsigned));
-            rb.run();
-            if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the persistent coloring timed out");
-            round = (int)h[6];
-            colored = h[8];
-            done = true;
-        } else {
-            launch_fill_i32(colors, nv, -1, s_);
-        }
-    }
-    if (!done && nv <= kPersistMaxNV) {
-        const int st = persistent(true);
-        done = st == 1;
-        have_frontier = st == 2;
-        if (st == 0) launch_fill_i32(colors, nv, -1, s_);  // launch refused: host-driven rounds below
-    }
-    if (!done) {
+    {
         DBuf<uint32_t> gmark;
-        if (!have_frontier) {
-            launch_fill_i64(counts.p, 8, 0, s_);
-            Profiler::get().begin(K_COLOR, s_);
-            launch_jp_count(g, cnt.p, cur, next.small, cur.counts + 3, s_);  // heavy list in the next round's buffer
-            Profiler::get().end(K_COLOR, 0.0, s_);
-            Readback rb(s_);
-            rb.add(hc, cur.counts, sizeof(hc));
-            rb.run();
-        }
         while (hc[0] + hc[1] + hc[2] > 0) {
-            if (round >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
+            if (prounds >= (1 << 24)) fail(GLV_E_DEVICE, "coloring did not converge");
             const i64 won = hc[0] + hc[1] + hc[2];
-            if (won < kJpPersistFrontier && colored > 0) {
+            if (won < kJpPersistFrontier) {
                 // hand the following rounds to one persistent launch: giant
                 // frontier vertices join the large list
                 if (hc[2] > 0)
@@ -263,11 +288,12 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
                                            hipMemcpyDeviceToDevice, s_));
                 hc[1] += hc[2];
                 hc[2] = 0;
-                const int st = persistent(false);
+                const int st = persistent();
                 if (st == 1) break;
                 if (st == 2) continue;
             }
-            if (hc[2] > 0 && (i64)gmark.n < hc[2] * (kMaxColors / 32)) gmark.alloc(hc[2] * (kMaxColors / 32), s_);
+            const i64 words = hc[2] * (kMaxColors / 32 + 1);
+            if (hc[2] > 0 && (i64)gmark.n < words) gmark.alloc(words, s_);
             launch_fill_i64(next.counts, 4, 0, s_);
             Profiler::get().begin(K_COLOR, s_);
             launch_jp_frontier(g, colors, cnt.p, cur, hc, next, gmark.p, err.p, s_);
@@ -277,27 +303,59 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
             rb.run();
             colored += won;
             std::swap(cur, next);
-            round++;
-            const i64 rest = nv - colored;
-            if (rest > 0 && rest <= kJpTail) break;
+            prounds++;
         }
     }
-    if (colored < nv) {
-        // the last vertices one at a time (scratch: lists not holding a frontier)
-        const i64 rest = nv - colored;
-        DBuf<uint32_t> base(rest * (kMaxColors / 32), s_);
-        DBuf<u64> adj(rest * (kJpTail / 64), s_);
+    if (colored != nv) fail(GLV_E_DEVICE, "coloring left vertices unreached (is the adjacency symmetric?)");
+
+    // the tail: the smallest round r >= 1 after which at most kJpTail
+    // vertices remain, i.e. with at most kJpTail vertices deeper than r
+    int32_t maxd = 0;
+    {
+        DBuf<int32_t> md(1, s_);
+        launch_fill_i32(md.p, 1, 0, s_);
+        launch_jp_depth_max(dep.p, nv, md.p, s_);
+        maxd = fetch(md.p);
+    }
+    int32_t rounds = maxd;
+    i64 deeper = 0;  // vertices deeper than rounds
+    {
+        DBuf<unsigned> dh(kJpDepthWindow, s_);
+        std::vector<unsigned> h(kJpDepthWindow);
+        bool done = false;
+        for (int32_t top = maxd; top >= 1 && !done; top -= kJpDepthWindow) {
+            launch_fill_i32((int32_t *)dh.p, kJpDepthWindow, 0, s_);
+            launch_jp_depth_hist(dep.p, nv, top, dh.p, s_);
+            Readback rb(s_);
+            rb.add(h.data(), dh.p, kJpDepthWindow * sizeof(unsigned));
+            rb.run();
+            for (int i = 0; i < kJpDepthWindow && !done; i++) {
+                const int32_t d = top - i;  // moving the cut from d to d - 1 adds the vertices of depth d
+                if (d - 1 < 1 || deeper + (i64)h[i] > kJpTail) {
+                    done = true;
+                } else {
+                    deeper += h[i];
+                    rounds = d - 1;
+                }
+            }
+        }
+    }
+    if (deeper > 0) {
+        launch_jp_depth_cut(dep.p, nv, rounds, colors, s_);
+        DBuf<uint32_t> base(deeper * (kMaxColors / 32), s_);
+        DBuf<u64> adj(deeper * (kJpTail / 64), s_);
         Profiler::get().begin(K_COLOR_TAIL, s_);
-        launch_color_tail(g, colors, next.small, next.large, next.giant, rest, cnt.p, base.p, adj.p, err.p, s_);
+        launch_color_tail(g, colors, next.small, next.large, next.giant, deeper, cnt.p, base.p, adj.p, err.p, s_);
         Profiler::get().end(K_COLOR_TAIL, 0.0, s_);
     }
-    glv::trace("coloring nv %ld: %s, %d rounds, tail %ld", (long)nv,
-          nv <= kJpPushMaxNV && done && round >= 0 ? "rounds done" : "host rounds", round, (long)(nv - colored));
+    glv::trace("coloring nv %ld: core %ld (degree >= %ld), %d rounds after it, depth %d, %d rounds, tail %ld",
+               (long)nv, (long)K, K > 0 ? (long)dcore : -1L, prounds, maxd, rounds, (long)deeper);
     launch_jp_finish(colors, nv, maxc.p, s_);
     const int32_t e = fetch(err.p);
     if (e & 1) fail(GLV_E_TOO_MANY_COLORS, "distance-1 coloring needs more than 4096 colors");
+    if (e & 4) fail(GLV_E_DEVICE, "core coloring did not complete (is the adjacency symmetric?)");
     if (e) fail(GLV_E_DEVICE, "coloring tail overflow");
-    if (rounds_out) *rounds_out = round;
+    if (rounds_out) *rounds_out = rounds;
     return fetch(maxc.p);
 }
 

@@ -47,13 +47,14 @@ def test_coloring_valid_and_equal_to_restated_jp(name):
 
 @pytest.mark.parametrize("which", ["rmat19", "planted", "planted_200k", "planted_100k", "cliques"])
 def test_coloring_rounds_paths_equal_restated_jp(which):
-    """Above 2^18 vertices the rounds are launched from the host and handed to
-    one cooperative launch once the frontier is small; from 2^17 to 2^18 they
-    run inside one cooperative launch; up to 2^17 inside one launch that
-    pushes colours into per-vertex bitmaps (light, mid-degree and chunked
-    heavy vertices).  All equal the restated coloring, on a skewed graph (long
-    priority chains through hubs), planted ones and dense near-cliques with
-    hubs."""
+    """Graphs of more than 16384 vertices with hubs of degree 1024 or more
+    colour their hubs first in one workgroup (the core), the rest by frontier
+    rounds launched from the host and handed to one persistent launch once the
+    frontier is small; graphs without such hubs take the rounds alone; the
+    vertex depths decide the tail either way.  All equal the restated
+    coloring, on a skewed graph (long priority chains through hubs: a core of
+    hubs), planted ones (no core) and near-cliques with two hubs (a core of
+    two)."""
     from graphs import cliques
     from grappolo_amd.graph import gen_planted, gen_rmat
     g = {"rmat19": lambda: gen_rmat(19, seed=5),
