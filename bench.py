@@ -388,6 +388,8 @@ def main():
     ap.add_argument("--las-seeds", default="1,2,3,4,5", help="network seeds (one trained network each)")
     ap.add_argument("--las-warmup", type=int, default=1)
     ap.add_argument("--las-timed", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run with the default launch shapes instead of tuning them on the graph first")
     ap.add_argument("--no-learned", action="store_true",
                     help="leave the learned comparison out of the default line")
     ap.add_argument("--learned-budget", type=float, default=240.0,
@@ -425,6 +427,17 @@ def main():
     dg = engine.DeviceGraph(g, device="cuda:0")
     C = torch.empty(g.numVertices, dtype=torch.int64, device="cuda:0")
     params = dict(minGraphSize=1000, threshold=1e-9, C_threshold=1e-6)
+
+    # launch shapes: timed on this graph (coloring candidates, then whole
+    # runs) before the warm-up; the chosen shapes serve every run below
+    tuning = None
+    if not args.no_autotune:
+        t = time.perf_counter()
+        default = engine.get_tuning()
+        best, log = engine.autotune_dev(dg, runs=True, apply=True, reps=1, **params)
+        tuning = {"default": default, "chosen": best, "seconds": round(time.perf_counter() - t, 1),
+                  "trials": [{"shapes": sh, "timed_on": stage, "ms": round(ms, 2)} for sh, stage, ms in log]}
+        print(f"[bench] autotune: {best} in {tuning['seconds']} s", file=sys.stderr, flush=True)
 
     for i in range(args.warmup):
         st = engine.run_colored_dev(dg, C, **params)
@@ -488,7 +501,7 @@ def main():
                        "clusters": int(pp["phase_clusters"][i]),
                        "ms": round(float(pp["phase_seconds"][i]) * 1e3, 2)}
                       for i in range(len(pp["phase_nv"]))],
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "tuning": tuning,
         "kernels": {k: {"ms": round(v[0], 3), "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1)
                         if v[0] > 0 else None, "launches": v[2]} for k, v in kern.items() if v[2]},
     }

@@ -33,4 +33,12 @@ def host_lib():
 
 
 def hip_lib():
+    if HIP_LIB not in _cache:
+        # torch first: the HIP runtime it loads is the one this library binds
+        # to (loaded the other way round, a second runtime comes up that sees
+        # no device once torch has initialised its own)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     return _load(HIP_LIB, "libgrappolo_hip.so")

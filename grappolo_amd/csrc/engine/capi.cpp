@@ -1019,6 +1019,58 @@ int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, con
     });
 }
 
+int glv_get_tuning(glv_tuning *t) {
+    return guarded([&] {
+        if (!t) glv::fail(GLV_E_INVALID_ARG, "null tuning");
+        const glv::Tuning &c = glv::current_tuning();
+        *t = glv_tuning{c.jp_blocks, c.persist_blocks, c.wide_blocks, 0, c.core_min_deg};
+    });
+}
+
+int glv_set_tuning(const glv_tuning *t) {
+    return guarded([&] {
+        if (!t) glv::fail(GLV_E_INVALID_ARG, "null tuning");
+        glv::Tuning c;
+        c.jp_blocks = t->jp_blocks;
+        c.persist_blocks = t->persist_blocks;
+        c.wide_blocks = t->wide_blocks;
+        c.core_min_deg = t->core_min_deg;
+        if (!glv::valid_tuning(c)) glv::fail(GLV_E_INVALID_ARG, "launch shapes outside the kernels' limits");
+        glv::current_tuning() = c;
+    });
+}
+
+int glv_autotune_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                     const double *d_weights, int coloring, int64_t minGraphSize, double threshold,
+                     double C_threshold, int reps, int runs, int apply, glv_tuning *best,
+                     glv_tune_trial *trials, int cap, int *ntrials, void *stream) {
+    return guarded([&] {
+        if (!best || reps < 1 || cap < 0 || (cap > 0 && !trials)) glv::fail(GLV_E_INVALID_ARG, "bad output arguments");
+        hipStream_t s = engine_stream(stream);
+        check_dev_csr(nv, nnz, d_offsets, d_tails, s);
+        if (nnz == 0) glv::fail(GLV_E_INVALID_GRAPH, "graph has no edges");
+        DevGraph g = view(nv, nnz, d_offsets, d_tails, d_weights);
+        glv::RunParams p;
+        p.coloring = coloring;
+        p.minGraphSize = minGraphSize;
+        p.threshold = threshold;
+        p.C_threshold = C_threshold;
+        Engine E(s);
+        std::vector<glv::TuneTrial> log;
+        const glv::Tuning t = glv::autotune_launch(E, g, p, reps, runs != 0, &log);
+        GLV_HIP(hipStreamSynchronize(s));
+        *best = glv_tuning{t.jp_blocks, t.persist_blocks, t.wide_blocks, 0, t.core_min_deg};
+        const int n = (int)std::min<size_t>(log.size(), (size_t)cap);
+        for (int i = 0; i < n; i++) {
+            const glv::Tuning &c = log[i].t;
+            trials[i] = glv_tune_trial{{c.jp_blocks, c.persist_blocks, c.wide_blocks, 0, c.core_min_deg},
+                                       log[i].stage, 0, log[i].ms};
+        }
+        if (ntrials) *ntrials = (int)log.size();
+        if (apply) glv::current_tuning() = t;
+    });
+}
+
 int glv_algoColoringMultiHashMaxMin_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
                                         int32_t *d_colors, int nHash, int nItrs, int32_t *maxColor, void *stream) {
     return guarded([&] {

@@ -47,6 +47,11 @@ constexpr i64 kPersistMaxBlocks = 64;  // grid of the persistent sweep
 constexpr i64 kPersistMaxNV = 1 << 18;
 constexpr i64 kPersistMaxPerColor = 256;
 constexpr int kBarrierWords = 16 + 16 * 64;  // abort word + one 64-byte flag line per workgroup
+// The coloring of a graph of more than 16384 vertices takes a core (its
+// highest degrees, coloured by one workgroup) only when the core starts at
+// this degree or above (hubs, whose priority chains make the rounds deep);
+// the default of the tunable launch shapes.
+constexpr i64 kJpCoreMinDeg = 1024;
 // touched communities of one vertex, listed in LDS: at most min(degree, nv)
 constexpr int kDenseListMax = kDenseMaxNV;
 

@@ -18,9 +18,8 @@
 // order since the degree leads the priority -- is coloured here by one
 // workgroup: its vertices' core neighbours are listed (16-bit core indices,
 // higher ones first), a vertex's state (uncoloured: its count of uncoloured
-// higher core neighbours; coloured: depth and colour) lives in LDS, and each
-// wavefront takes the next vertex of a ready queue as soon as its last higher
-// neighbour is coloured -- no round barriers.  The frontier rounds then
+// higher core neighbours; coloured: depth and colour) lives in LDS, and the
+// rounds are separated by workgroup barriers only.  The frontier rounds then
 // colour the rest, taking the core's colours and depths as given.  Finally
 // the depths decide the tail: the smallest round r with at most kJpTail
 // vertices deeper than r, those vertices uncoloured and handed to the
@@ -36,10 +35,6 @@ constexpr int kCoreBlock = 1024;
 constexpr int kCoreWaves = kCoreBlock / kWave;
 constexpr int kColorWords = kMaxColors / 32;
 constexpr uint32_t kColoured = 0x80000000u;  // state word: coloured (depth << 12 | colour)
-constexpr uint32_t kQueueEmpty = 0xFFFFFFFFu;
-// polls of an unwritten queue slot before the core pass gives up (a DAG
-// whose counts do not match its lists, as an asymmetric adjacency gives)
-constexpr int kCoreSpinLimit = 1 << 22;
 
 static_assert(kMaxColors <= 4096, "colours take 12 bits of the core state");
 static_assert(kJpCoreMax < (1 << 16), "core indices are 16-bit");
@@ -133,102 +128,46 @@ __global__ __launch_bounds__(kFillBlock) void k_jp_core_fill(const i64 *off, con
     }
 }
 
-// The core pass, one workgroup.  st[q]: while q is uncoloured, its count of
-// uncoloured higher core neighbours; then kColoured | depth << 12 | colour.
-// A vertex enters the queue when its count reaches zero (its higher
-// neighbours are all coloured: their states are final); a wavefront takes the
-// next queue slot, waits until it is written, colours the vertex from its
-// higher neighbours' states and then counts its lower neighbours down,
-// queueing those that reach zero.  Every vertex enters the queue exactly
-// once, so K slots are taken and every wait ends (slots are taken in order
-// and a queued vertex's releases never wait on a later slot).
+// The core pass, one workgroup, in rounds: the vertices queued in round t
+// (their higher neighbours all coloured) are pairwise non-adjacent, so they
+// are coloured together and their releases queue round t + 1.  st[q]: while
+// q is uncoloured, its count of uncoloured higher core neighbours; then
+// kColoured | depth << 12 | colour.  A round of at least kCoreWaves vertices
+// gives each wavefront its share of them, one at a time; a smaller round
+// gives each vertex a group of G wavefronts (G = kCoreWaves / n rounded down
+// to a power of two), so a round along a chain of hubs spreads the hub's
+// list over the whole workgroup: one load of it, the colour choice and three
+// workgroup barriers.
+constexpr int kCoreRegs = 4;  // list entries per thread held across the two passes
 __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K, const i64 *loff,
                                                         const int32_t *core_cnt, const uint16_t *ladj,
                                                         int32_t *colors, int32_t *dep, int32_t *err) {
     __shared__ uint32_t st[kJpCoreMax];
-    __shared__ uint32_t queue[kJpCoreMax];
+    __shared__ uint16_t queue[kJpCoreMax];
     __shared__ uint32_t mk[kCoreWaves][kColorWords];
-    __shared__ int s_head, s_tail;
+    __shared__ int s_dm[kCoreWaves];
+    __shared__ int s_tail;
     const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
     const u64 below = (1ull << lane) - 1;
-    for (int q = tid; q < K; q += kCoreBlock) {
-        st[q] = (uint32_t)core_cnt[2 * q];
-        queue[q] = kQueueEmpty;
-    }
-    if (tid == 0) {
-        s_head = 0;
-        s_tail = 0;
-    }
+    for (int q = tid; q < K; q += kCoreBlock) st[q] = (uint32_t)core_cnt[2 * q];
+    if (tid == 0) s_tail = 0;
     __syncthreads();
+    // appends the ready lanes' vertices to the next round (whole wavefront)
     auto enqueue = [&](bool ready, int x) {
         const u64 bal = __ballot(ready);
         if (!bal) return;
         int base = 0;
         if (lane == 0) base = atomicAdd(&s_tail, __popcll(bal));
         base = __shfl(base, 0);
-        if (ready) __hip_atomic_store(&queue[base + __popcll(bal & below)], (uint32_t)x, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (ready) queue[base + __popcll(bal & below)] = (uint16_t)x;
     };
-    for (int q0 = 0; q0 < K; q0 += kCoreBlock) {
-        const int q = q0 + tid;
-        enqueue(q < K && core_cnt[2 * q] == 0, q);
-    }
-    __syncthreads();
-    uint32_t *m = mk[w];
-    while (true) {
-        int i = 0;
-        if (lane == 0) i = atomicAdd(&s_head, 1);
-        i = __shfl(i, 0);
-        if (i >= K) break;
-        uint32_t r = kQueueEmpty;
-        int spins = 0;
-        if (lane == 0) {
-            while ((r = __hip_atomic_load(&queue[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ==
-                       kQueueEmpty &&
-                   spins < kCoreSpinLimit) {
-                __builtin_amdgcn_s_sleep(1);
-                spins++;
-            }
-        }
-        r = __shfl(r, 0);
-        if (r == kQueueEmpty) {
-            if (lane == 0) atomicOr(err, 4);
-            break;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const i64 b = loff[r], len = loff[r + 1] - b;
-        const int nhi = core_cnt[2 * r];
-        m[lane] = 0;
-        m[lane + kWave] = 0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        int dm = 0;
-        for (i64 j0 = 0; j0 < nhi; j0 += 4 * kWave) {
-            int x[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const i64 j = j0 + k * kWave + lane;
-                x[k] = j < nhi ? (int)ladj[b + j] : -1;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (x[k] < 0) continue;
-                const uint32_t s = st[x[k]];
-                const int c = (int)(s & 0xFFFu), d = (int)((s >> 12) & 0x7FFFFu);
-                atomicOr(&m[c >> 5], 1u << (c & 31));
-                dm = d > dm ? d : dm;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
+    // first free colour of bitmap m (two words per lane) and the largest
+    // depth dm, over the wavefront; valid in every lane
+    auto first_free = [&](const uint32_t *m, int &dm) {
         int cand = kMaxColors;
-        {
-            const uint32_t m0 = m[lane], m1 = m[lane + kWave];
-            if (m0 != 0xFFFFFFFFu) cand = lane * 32 + __ffs((int)~m0) - 1;
-            else if (m1 != 0xFFFFFFFFu) cand = (lane + kWave) * 32 + __ffs((int)~m1) - 1;
-        }
-        // colours 0..2047 sit in the first words of the lanes: the smallest
-        // candidate over the wave is the first free colour
+        const uint32_t m0 = m[lane], m1 = m[lane + kWave];
+        if (m0 != 0xFFFFFFFFu) cand = lane * 32 + __ffs((int)~m0) - 1;
+        else if (m1 != 0xFFFFFFFFu) cand = (lane + kWave) * 32 + __ffs((int)~m1) - 1;
 #pragma unroll
         for (int o = kWave / 2; o > 0; o >>= 1) {
             const int oc = __shfl_xor(cand, o);
@@ -236,40 +175,149 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
             const int od = __shfl_xor(dm, o);
             dm = od > dm ? od : dm;
         }
-        if (lane == 0) {
-            int color = cand;
-            if (color >= kMaxColors) {
-                atomicOr(err, 1);
-                color = 0;
-            }
-            st[r] = kColoured | ((uint32_t)(dm + 1) << 12) | (uint32_t)color;
+        return cand;
+    };
+    auto state_of = [&](int color, int dm) {
+        if (color >= kMaxColors) {
+            atomicOr(err, 1);
+            color = 0;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        // the lower core neighbours: one count down each, queued at zero
-        for (i64 j0 = nhi; j0 < len; j0 += 4 * kWave) {
-            int x[4];
+        return kColoured | ((uint32_t)(dm + 1) << 12) | (uint32_t)color;
+    };
+    // colours and depth of the higher neighbours x (entry j of a list whose
+    // first nhi entries are the higher ones) into bitmap m and dm
+    auto mark = [&](int x, int j, int nhi, uint32_t *m, int &dm) {
+        if (x < 0 || j >= nhi) return;
+        const uint32_t sx = st[x];
+        const int c = (int)(sx & 0xFFFu), d = (int)((sx >> 12) & 0x7FFFFu);
+        atomicOr(&m[c >> 5], 1u << (c & 31));
+        dm = d > dm ? d : dm;
+    };
+    for (int q0 = 0; q0 < K; q0 += kCoreBlock) {
+        const int q = q0 + tid;
+        enqueue(q < K && core_cnt[2 * q] == 0, q);
+    }
+    int head = 0;
+    while (true) {
+        __syncthreads();  // the previous round is complete
+        const int tail = s_tail;
+        __syncthreads();  // and every thread knows where it ends before this round appends
+        const int n = tail - head;
+        if (n <= 0) break;
+        if (n >= kCoreWaves) {
+            // a wavefront per vertex
+            uint32_t *m = mk[w];
+            for (int i = head + w; i < tail; i += kCoreWaves) {
+                const int r = queue[i];
+                const i64 b = loff[r];
+                const int len = (int)(loff[r + 1] - b), nhi = core_cnt[2 * r];
+                m[lane] = 0;
+                m[lane + kWave] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                int dm = 0;
+                for (int j0 = 0; j0 < nhi; j0 += kCoreRegs * kWave) {
+                    int x[kCoreRegs];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const i64 j = j0 + k * kWave + lane;
+                    for (int k = 0; k < kCoreRegs; k++) {
+                        const int j = j0 + k * kWave + lane;
+                        x[k] = j < nhi ? (int)ladj[b + j] : -1;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kCoreRegs; k++) mark(x[k], j0 + k * kWave + lane, nhi, m, dm);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                const int color = first_free(m, dm);
+                if (lane == 0) st[r] = state_of(color, dm);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                for (int j0 = nhi; j0 < len; j0 += kCoreRegs * kWave) {
+                    int x[kCoreRegs];
+#pragma unroll
+                    for (int k = 0; k < kCoreRegs; k++) {
+                        const int j = j0 + k * kWave + lane;
+                        x[k] = j < len ? (int)ladj[b + j] : -1;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kCoreRegs; k++) enqueue(x[k] >= 0 && atomicSub(&st[x[k]], 1u) == 1u, x[k]);
+                }
+            }
+        } else {
+            // a group of G wavefronts per vertex, one vertex per group
+            const int G = n == 1 ? 16 : n == 2 ? 8 : n <= 4 ? 4 : n <= 8 ? 2 : 1;
+            const int g = w / G, gw = w % G, gt = gw * kWave + lane, gstride = G * kWave;
+            const bool act = g < n;
+            const int r = act ? queue[head + g] : 0;
+            const i64 b = act ? loff[r] : 0;
+            const int len = act ? (int)(loff[r + 1] - b) : 0, nhi = act ? core_cnt[2 * r] : 0;
+            uint32_t *m = mk[g * G];
+            if (act && gw == 0) {
+                m[lane] = 0;
+                m[lane + kWave] = 0;
+                if (lane == 0) s_dm[g] = 0;
+            }
+            // the first kCoreRegs entries per thread of the whole list, for both passes
+            int x[kCoreRegs];
+#pragma unroll
+            for (int k = 0; k < kCoreRegs; k++) {
+                const int j = k * gstride + gt;
                 x[k] = j < len ? (int)ladj[b + j] : -1;
             }
+            __syncthreads();
+            int dm = 0;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const bool ready = x[k] >= 0 && atomicSub(&st[x[k]], 1u) == 1u;
-                enqueue(ready, x[k]);
+            for (int k = 0; k < kCoreRegs; k++) mark(x[k], k * gstride + gt, nhi, m, dm);
+            for (int j0 = kCoreRegs * gstride; j0 < nhi; j0 += kCoreRegs * gstride) {
+                int y[kCoreRegs];
+#pragma unroll
+                for (int k = 0; k < kCoreRegs; k++) {
+                    const int j = j0 + k * gstride + gt;
+                    y[k] = j < nhi ? (int)ladj[b + j] : -1;
+                }
+#pragma unroll
+                for (int k = 0; k < kCoreRegs; k++) mark(y[k], j0 + k * gstride + gt, nhi, m, dm);
+            }
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const int od = __shfl_xor(dm, o);
+                dm = od > dm ? od : dm;
+            }
+            if (act && lane == 0 && dm > 0) atomicMax(&s_dm[g], dm);
+            __syncthreads();
+            if (act && gw == 0) {
+                int dmax = s_dm[g];
+                const int color = first_free(m, dmax);
+                if (lane == 0) st[r] = state_of(color, dmax);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kCoreRegs; k++) {
+                const int j = k * gstride + gt;
+                enqueue(x[k] >= 0 && j >= nhi && atomicSub(&st[x[k]], 1u) == 1u, x[k]);
+            }
+            for (int j0 = kCoreRegs * gstride > nhi ? kCoreRegs * gstride : nhi; j0 < len;
+                 j0 += kCoreRegs * gstride) {
+                int y[kCoreRegs];
+#pragma unroll
+                for (int k = 0; k < kCoreRegs; k++) {
+                    const int j = j0 + k * gstride + gt;
+                    y[k] = j < len ? (int)ladj[b + j] : -1;
+                }
+#pragma unroll
+                for (int k = 0; k < kCoreRegs; k++) enqueue(y[k] >= 0 && atomicSub(&st[y[k]], 1u) == 1u, y[k]);
             }
         }
+        head = tail;
     }
-    __syncthreads();
     for (int q = tid; q < K; q += kCoreBlock) {
-        const uint32_t s = st[q];
+        const uint32_t sq = st[q];
         const i64 v = hlist[q];
-        if (s & kColoured) {
-            colors[v] = (int32_t)(s & 0xFFFu);
-            dep[v] = (int32_t)((s >> 12) & 0x7FFFFu);
+        if (sq & kColoured) {
+            colors[v] = (int32_t)(sq & 0xFFFu);
+            dep[v] = (int32_t)((sq >> 12) & 0x7FFFFu);
         } else {
-            atomicOr(err, 4);
+            atomicOr(err, 4);  // never queued: the counts do not match the lists
         }
     }
 }

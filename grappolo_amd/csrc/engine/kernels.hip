@@ -1899,6 +1899,9 @@ __device__ inline void frontier_push(bool pred, i64 u, uint8_t code, const JpLis
     wave_append(giant, u, L.giant, &L.counts[2]);
 }
 constexpr int kStageCap = 2048;
+// vertices of at most this many entries are counted by one thread (a serial
+// loop of dependent loads); larger ones by a wavefront
+constexpr i64 kJpCountThreadDeg = 32;
 
 // One wavefront colours up to eight frontier entries front[lo .. lo+n): a
 // vertex of at most kJpLightDeg entries takes a group of kJpGroup lanes (its
@@ -2037,8 +2040,8 @@ __device__ inline void jp_wave8(const i64 *off, const i64 *tails, int32_t *color
     }
 }
 
-// count[v] of vertices up to 256 entries, one thread each; heavier ones are
-// listed in heavy[] (heavy_count) for k_jp_count_heavy.  A core vertex v
+// count[v] of vertices up to kJpCountThreadDeg entries, one thread each;
+// heavier ones are listed in heavy[] (heavy_count) for k_jp_count_heavy.  A core vertex v
 // (count[v] = its core index q on entry) gets its higher and lower core
 // neighbours counted into core_cnt[2q], [2q+1] instead, and never enters a
 // frontier: the core pass colours it.  A non-core vertex's higher core
@@ -2065,7 +2068,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_count(const i64 *off, const i64 *
             const i64 b = off[v], e = off[v + 1];
             const u64 pv = L.prio[v];
             core_v = (pv >> 40) >= L.dcore;
-            if (e - b > kBinMaxDeg[6]) {
+            if (e - b > kJpCountThreadDeg) {
                 heavy_v = true;
             } else {
                 int hi = 0, lo = 0;
@@ -2091,7 +2094,7 @@ __global__ __launch_bounds__(kBlock) void k_jp_count(const i64 *off, const i64 *
             }
         }
         wave_append(heavy_v, v, heavy, heavy_count);
-        stage_append(v < nv && !heavy_v && !core_v && c == 0, v, S, L.small, &L.counts[0]);  // at most 256 entries
+        stage_append(v < nv && !heavy_v && !core_v && c == 0, v, S, L.small, &L.counts[0]);  // a small-list vertex
     }
     stage_flush(S, L.small, &L.counts[0]);
 }
@@ -3120,7 +3123,7 @@ bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 colored0, const u64 *prio,
                           uint8_t *code, int32_t *dep, i64 max_front, int nblocks, hipStream_t s) {
     (void)prio;
-    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || g.nv <= 0) return false;
+    if (nblocks <= 0 || nblocks > kJpPersistMaxBlocks || g.nv <= 0) return false;
     const i64 *off = g.off, *tails = g.tails;
     i64 nv = g.nv, c0 = colored0, mf = max_front;
     uint8_t *cd = code;

@@ -165,9 +165,6 @@ constexpr uint8_t kJpRelease = 1, kJpLarge = 2, kJpGiant = 4, kJpSelf = 8, kJpHu
 constexpr i64 kJpMidDeg = 1024;
 // core vertices at most (one workgroup's LDS: state, counts and queue)
 constexpr i64 kJpCoreMax = 16384;
-// graphs of more than kJpCoreMax vertices take a core only when it starts at
-// this degree or above (hubs, whose priority chains make the rounds deep)
-constexpr i64 kJpCoreMinDeg = 1024;
 // Jones-Plassmann priorities: (min(degree, 2^23 - 1) << 40) | (splitmix64(v) >> 24),
 // equal keys broken by the larger id
 void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s);
@@ -195,12 +192,15 @@ constexpr i64 kJpTail = 1024;   // the tail: at most this many vertices
 // host-driven rounds), continuing from the frontier in sA (small, count
 // ring[0]) and lA (large, ring[1]) after colored0 vertices; false when the
 // launch is refused.  ring: 10 words (zero except the frontier counts),
-// ring[6] receives the rounds; bar: kBarrierWords zeroed words; err: 1 too
+// ring[6] receives the rounds; bar: kJpBarrierWords zeroed words; err: 1 too
 // many colors.  sA..lB: nv each.
 int jp_persistent_blocks();
 bool launch_jp_persistent(const DevGraph &g, int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB, i64 *lA,
                           i64 *lB, i64 *ring, unsigned *bar, int32_t *err, i64 colored0, const u64 *prio,
                           uint8_t *code, int32_t *dep, i64 max_front, int nblocks, hipStream_t s);
+// grid of the persistent rounds at most (bar: kJpBarrierWords words)
+constexpr i64 kJpPersistMaxBlocks = 256;
+constexpr int kJpBarrierWords = 16 + 16 * (int)kJpPersistMaxBlocks;
 // frontiers below this many vertices finish in one persistent launch
 constexpr i64 kJpPersistFrontier = 16384;
 // a persistent launch hands back to full-GPU rounds once a frontier exceeds this

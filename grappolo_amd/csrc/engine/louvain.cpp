@@ -20,6 +20,15 @@ BinConfig &current_bins() {
     return c;
 }
 
+Tuning &current_tuning() {
+    static Tuning t;
+    return t;
+}
+bool valid_tuning(const Tuning &t) {
+    return t.jp_blocks >= 1 && t.jp_blocks <= kJpPersistMaxBlocks && t.core_min_deg >= 1 &&
+           t.persist_blocks >= 1 && t.persist_blocks <= kPersistMaxBlocks && t.wide_blocks >= 0;
+}
+
 namespace {
 
 double now() {
@@ -165,10 +174,10 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     const i64 nv = g.nv;
     if (nv == 0) return -1;
     launch_fill_i32(colors, nv, -1, s_);
-    const int pnb = (int)std::min<i64>({(i64)jp_persistent_blocks(), kPersistMaxBlocks,
+    const int pnb = (int)std::min<i64>({(i64)jp_persistent_blocks(), (i64)tune_.jp_blocks,
                                          std::max<i64>(4, nv / 2048)});
     DBuf<i64> ring(10, s_);
-    DBuf<unsigned> bar(kBarrierWords, s_);
+    DBuf<unsigned> bar(kJpBarrierWords, s_);
     DBuf<int32_t> cnt(nv, s_), dep(nv, s_), err(1, s_), maxc(1, s_);
     DBuf<u64> prio(nv, s_);
     DBuf<uint8_t> code(std::max<i64>(g.nnz, 1), s_);
@@ -180,7 +189,8 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     cur.dep = next.dep = dep.p;
 
     // the core: every vertex of a small graph; on a larger one the vertices
-    // of the largest degrees, at most kJpCoreMax, when they reach kJpCoreMinDeg
+    // of the largest degrees, at most kJpCoreMax, when they reach the tuned
+    // core degree (kJpCoreMinDeg by default)
     u64 dcore = kJpNoCore;
     i64 K = 0;
     if (nv <= kJpCoreMax) {
@@ -199,7 +209,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
             if (above + (i64)h[b] > kJpCoreMax) break;
             above += (i64)h[b];
             const i64 lo = jp_bucket_lo(b);
-            if (lo >= kJpCoreMinDeg && lo < (i64(1) << 23) && above > 0) {
+            if (lo >= tune_.core_min_deg && lo < (i64(1) << 23) && above > 0) {
                 dcore = (u64)lo;
                 K = above;
             }
@@ -252,7 +262,7 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     auto persistent = [&]() -> int {
         i64 h[10] = {hc[0], hc[1], 0, 0, 0, 0, 0, 0, 0, 0};
         GLV_HIP(hipMemcpyAsync(ring.p, h, sizeof(h), hipMemcpyHostToDevice, s_));
-        launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
+        launch_fill_i32((int32_t *)bar.p, kJpBarrierWords, 0, s_);
         Profiler::get().begin(K_COLOR, s_);
         const bool ok = pnb > 0 && launch_jp_persistent(g, colors, cnt.p, cur.small, next.small, cur.large,
                                                         next.large, ring.p, bar.p, err.p, colored, prio.p, code.p,
@@ -429,9 +439,10 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
             // dense graphs: the grid's two halves take the color steps in turn
             // (an even count, twice the class size per half)
             const i64 want = nv <= kDenseMaxNV ? 4 * per_class : 2 * per_class;
-            if (pblocks > 0)
-                pblocks = (int)std::min<i64>({(i64)pblocks, kPersistMaxBlocks, std::max<i64>(4, want)});
-            if (nv <= kDenseMaxNV) pblocks &= ~1;
+            if (pblocks > 0) {
+                pblocks = (int)std::min<i64>({(i64)pblocks, (i64)tune_.persist_blocks, std::max<i64>(4, want)});
+                if (nv <= kDenseMaxNV) pblocks = std::max(2, pblocks & ~1);
+            }
         }
     }
     // large graphs: the whole-GPU persistent sweep (wide.hpp), plain form only
@@ -443,8 +454,10 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         // classes of more than kWideMaxPerColor vertices on average fill the GPU
         // with per-color launches of the binned kernels
         const bool small_classes = nonempty > 0 && nv / nonempty <= kWideMaxPerColor;
-        if (!las_mode && pblocks == 0 && nv > kPersistMaxNV && small_classes && wide_enabled())
+        if (!las_mode && pblocks == 0 && nv > kPersistMaxNV && small_classes && wide_enabled()) {
             wblocks = wide_blocks();
+            if (tune_.wide_blocks > 0) wblocks = std::min(wblocks, tune_.wide_blocks);
+        }
     }
     bool use_wide = wblocks > 0;
     DBuf<i64> gkeys;

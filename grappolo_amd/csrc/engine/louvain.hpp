@@ -107,11 +107,25 @@ void giant_args(const PhaseSetup &P, int c, SweepArgs &A);
 // The bin ranges every engine uses (process-wide; the autotuner's result).
 BinConfig &current_bins();
 
+// Launch shapes every engine uses (process-wide; the autotuner's result).
+// None of them changes a result: they choose grids and which vertices the
+// coloring's core pass takes.
+struct Tuning {
+    int jp_blocks = 64;                // grid of the persistent coloring rounds (<= kJpPersistMaxBlocks)
+    i64 core_min_deg = kJpCoreMinDeg;  // a coloring core of a large graph starts at this degree or above
+    int persist_blocks = 64;           // grid of the persistent sweep at most (<= kPersistMaxBlocks)
+    int wide_blocks = 0;               // grid of the whole-GPU sweep (0: one workgroup per CU)
+};
+Tuning &current_tuning();
+bool valid_tuning(const Tuning &t);
+
 class Engine {
 public:
-    explicit Engine(hipStream_t s) : s_(s), bins_(current_bins()) {}
+    explicit Engine(hipStream_t s) : s_(s), bins_(current_bins()), tune_(current_tuning()) {}
     hipStream_t stream() const { return s_; }
     void set_bins(const BinConfig &b) { bins_ = b; }
+    void set_tuning(const Tuning &t) { tune_ = t; }
+    const Tuning &tuning() const { return tune_; }
     // Times one plain sweep of g over every vertex (best of `reps`) with the
     // given bins, in ms.
     double time_sweep(const DevGraph &g, const BinConfig &b, int reps);
@@ -159,6 +173,7 @@ public:
 private:
     hipStream_t s_;
     BinConfig bins_;
+    Tuning tune_;
     i64 moves_ = 0;
     // small pinned-free scratch for scalars copied back to the host
     template <class T>
@@ -170,5 +185,19 @@ private:
 // Sweep bin ranges that time fastest on g (autotune.cpp); the default
 // ranges' and the result's sweep times in ms.
 BinConfig autotune_bins(Engine &E, const DevGraph &g, int reps, double *ms_default, double *ms_tuned);
+
+// One measured candidate of the launch-shape tuner.
+struct TuneTrial {
+    Tuning t;
+    int stage;  // 0: coloring, 1: whole colored run
+    double ms;
+};
+// Launch shapes that time fastest on g (autotune.cpp): the coloring's grid
+// and core degree by timing the coloring of g, then (runs > 0) the sweep
+// grids by timing whole colored runs of g with parameters p; candidates one
+// knob at a time, each from the best so far, a change kept when it saves at
+// least 2 %.  The trials go to log.
+Tuning autotune_launch(Engine &E, const DevGraph &g, const RunParams &p, int reps, bool runs,
+                       std::vector<TuneTrial> *log);
 
 }  // namespace glv

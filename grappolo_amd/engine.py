@@ -30,6 +30,22 @@ class GlvError(RuntimeError):
         self.code = code
 
 
+class Tuning(ctypes.Structure):
+    """glv_tuning: launch shapes of the coloring rounds, the sweeps and the coloring core."""
+    _fields_ = [("jp_blocks", ctypes.c_int32), ("persist_blocks", ctypes.c_int32),
+                ("wide_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("core_min_deg", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {"jp_blocks": self.jp_blocks, "persist_blocks": self.persist_blocks,
+                "wide_blocks": self.wide_blocks, "core_min_deg": self.core_min_deg}
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("t", Tuning), ("stage", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("ms", ctypes.c_double)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("phases", ctypes.c_int32), ("total_iterations", ctypes.c_int32),
                 ("final_clusters", ctypes.c_int64), ("final_modularity", ctypes.c_double),
@@ -108,6 +124,11 @@ _SIG = {
     "glv_profile_read": [ctypes.c_int, _F64P, _F64P, _I64P],
     "glv_runMultiPhaseBasic_batch_dev": [ctypes.c_int64, _I64P, ctypes.c_int64, ctypes.c_int64, _VP, _VP,
                                          _VP, _VP, ctypes.c_int, ctypes.c_double, _VP, ctypes.c_void_p],
+    "glv_get_tuning": [_VP],
+    "glv_set_tuning": [_VP],
+    "glv_autotune_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int64,
+                         ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP,
+                         _VP, ctypes.c_int, _VP, _VP],
     "glv_get_bins": [_I64P],
     "glv_set_bins": [_I64P],
     "glv_autotune_bins_dev": [ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, _I64P, _F64P,
@@ -608,6 +629,33 @@ def autotune_bins_dev(dg, reps=3):
     _call("glv_autotune_bins_dev", *dg.ptrs(), int(reps), _p(b, _I64P), ctypes.byref(t0),
           ctypes.byref(t1), _stream())
     return b, t0.value, t1.value
+
+
+def get_tuning():
+    """The launch shapes later calls use (dict)."""
+    t = Tuning()
+    _call("glv_get_tuning", ctypes.byref(t))
+    return t.as_dict()
+
+
+def set_tuning(jp_blocks=64, persist_blocks=64, wide_blocks=0, core_min_deg=1024):
+    t = Tuning(int(jp_blocks), int(persist_blocks), int(wide_blocks), 0, int(core_min_deg))
+    _call("glv_set_tuning", ctypes.byref(t))
+
+
+def autotune_dev(dg, runs=True, apply=True, reps=1, coloring=1, minGraphSize=1000, threshold=1e-9,
+                 C_threshold=1e-6, cap=64):
+    """Times launch-shape candidates on dg (glv_autotune_dev): returns
+    (best shapes, [(shapes, stage, ms), ...]); apply makes them current."""
+    best = Tuning()
+    trials = (TuneTrial * cap)()
+    n = ctypes.c_int(0)
+    _call("glv_autotune_dev", *dg.ptrs(), int(coloring), int(minGraphSize), float(threshold),
+          float(C_threshold), int(reps), int(bool(runs)), int(bool(apply)), ctypes.byref(best), trials,
+          int(cap), ctypes.byref(n), _stream())
+    log = [(trials[i].t.as_dict(), ("coloring", "run")[trials[i].stage], trials[i].ms)
+           for i in range(min(n.value, cap))]
+    return best.as_dict(), log
 
 
 _hip_rt = None

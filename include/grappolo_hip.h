@@ -342,6 +342,38 @@ int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, con
                           const double *d_weights, int reps, int64_t *max_deg, double *ms_default,
                           double *ms_tuned, void *stream);
 
+/* ---- launch shapes (no counterpart in the reference) ---------------------
+ * The grids of the persistent coloring rounds, the persistent sweep (coarse
+ * phases) and the whole-GPU sweep (large phases), and the degree at which
+ * the coloring's core starts on graphs above 16384 vertices.  None changes a
+ * partition on integer weights.  glv_set_tuning applies shapes to every
+ * later call in the process; glv_autotune_dev times candidates on the given
+ * graph -- the coloring for jp_blocks and core_min_deg, and with runs != 0
+ * whole colored runs (parameters as glv_runMultiPhaseColoring_dev) for
+ * persist_blocks and wide_blocks -- one knob at a time from the current
+ * shapes, keeping a change that saves at least 2 %, and writes the fastest
+ * to *best (made current when apply != 0).  trials (cap entries, may be
+ * null) receive every measured candidate; *ntrials their count. */
+typedef struct {
+    int32_t jp_blocks;      /* 1..256, default 64 */
+    int32_t persist_blocks; /* 1..64, default 64 */
+    int32_t wide_blocks;    /* 0: one workgroup per CU (default), else at most that many */
+    int32_t reserved;
+    int64_t core_min_deg;   /* >= 1, default 1024 */
+} glv_tuning;
+typedef struct {
+    glv_tuning t;
+    int32_t stage;          /* 0: timed on the coloring, 1: on a whole colored run */
+    int32_t reserved;
+    double ms;              /* best of reps */
+} glv_tune_trial;
+int glv_get_tuning(glv_tuning *t);
+int glv_set_tuning(const glv_tuning *t);
+int glv_autotune_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, const int64_t *d_tails,
+                     const double *d_weights, int coloring, int64_t minGraphSize, double threshold,
+                     double C_threshold, int reps, int runs, int apply, glv_tuning *best,
+                     glv_tune_trial *trials, int cap, int *ntrials, void *stream);
+
 /* ---- the other colorings of runMultiPhaseColoring ------------------------
  * algoColoringMultiHashMaxMin, DefineStructure/coloring.h:12 and
  * coloringMultiHashMaxMin.cpp:60-226 (coloring == 3): nItrs iterations of
