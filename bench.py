@@ -346,6 +346,7 @@ def run_batched(args):
                    "vertices": U.numVertices, "nnz": U.nnz, "method": "runMultiPhaseBasic",
                    "threshold": 1e-9},
         "batch_s": t_batch, "levels": res["levels"], "batched_iterations": res["iterations"],
+        "iteration_graph_launches": res["launches"],
         "per_graph_launches": {"graphs_per_s": 1.0 / per_graph, "sample": len(sample),
                                "mean_ms": per_graph * 1e3},
         "speedup": (len(graphs) / t_batch) * per_graph,

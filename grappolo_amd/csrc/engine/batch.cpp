@@ -26,7 +26,7 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 constexpr int kMaxPhaseIters = 100000;
-constexpr int kReplays = 2;  // iterations between two compactions of the sweep lists
+constexpr int kLevelUnroll = 4;  // iterations captured in one graph launch
 }  // namespace
 
 void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0, i64 *C_orig,
@@ -124,17 +124,26 @@ void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0,
         const i64 nkeys = kNumBins;
         DBuf<i64> flags(nv, s), fpos(nv, s), list(nv, s), start(nkeys + 1, s);
         std::vector<i64> hstart(nkeys + 1);
-        while (true) {
+        DBuf<int32_t> d_done(1, s);
+        DBuf<i64> d_live(1, s);
+        int32_t done = 0;
+        while (!done) {
             // sweep lists of the graphs still iterating
             launch_compact_graphs(P.perm.p, nv, gid.p, d_active.p, P.dptr.p, nkeys, flags.p, fpos.p, list.p,
                                   start.p, s);
             GLV_HIP(hipMemcpyAsync(hstart.data(), start.p, (nkeys + 1) * sizeof(i64), hipMemcpyDeviceToHost, s));
             GLV_HIP(hipStreamSynchronize(s));
-            if (hstart[nkeys] == 0) break;  // every graph's phase has ended
-            // One iteration (sweeps, per-graph modularity, decisions, commits) is
-            // captured once per list and replayed kReplays times: a graph whose
-            // phase ends meanwhile is still swept (its targets are not used),
-            // but nothing of it is committed.
+            const i64 listed = hstart[nkeys];
+            if (listed == 0) break;  // every graph's phase has ended
+            // kLevelUnroll iterations (sweeps, per-graph modularity, decisions,
+            // commits, the level's status) captured as one graph for these
+            // lists and launched until the level ends; once no graph iterates,
+            // the captured sweeps return at once and nothing is committed.  A
+            // graph whose phase ends meanwhile is still swept (its targets are
+            // not used) until the lists are compacted again, after the launch
+            // in which its phase ended (capturing and instantiating costs far
+            // less than the four sweeps of a launch).
+            launch_fill_i32(d_done.p, 1, 0, s);
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             GLV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -155,27 +164,35 @@ void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0,
             A.gid = gid.p;
             A.gconst = d_gconst.p;
             A.eii_v = eii.p;
-            for (int b = 1; b < kNumBins; b++) {
-                const i64 lo = hstart[b], cnt = hstart[b + 1] - lo;
-                if (cnt <= 0) continue;
-                A.verts = list.p + lo;
-                A.count = cnt;
-                A.eii_part = part.p;
-                if (b == 9) giant_args(P, 0, A);
-                launch_sweep(b, true, A, s);
+            A.skip = d_done.p;
+            for (int u = 0; u < kLevelUnroll; u++) {
+                for (int b = 1; b < kNumBins; b++) {
+                    const i64 lo = hstart[b], cnt = hstart[b + 1] - lo;
+                    if (cnt <= 0) continue;
+                    A.verts = list.p + lo;
+                    A.count = cnt;
+                    A.eii_part = part.p;
+                    if (b == 9) giant_args(P, 0, A);
+                    launch_sweep(b, true, A, s);
+                }
+                launch_isolated_targets(P.perm.p + P.bin_lo(0), P.bin_hi(0) - P.bin_lo(0), tgt, s);
+                launch_graph_mod(d_vb.p, ng, d_active.p, eii.p, P.a.p, d_exx.p, d_a2.p, s, d_done.p);
+                launch_batch_decide(ng, d_exx.p, d_a2.p, d_gconst.p, d_lower.p, p.threshold, kMaxPhaseIters,
+                                    d_prev.p, d_curr.p, d_itrs.p, d_active.p, d_apply.p, s);
+                launch_batch_apply(nv, gid.p, d_apply.p, tgt, curr, past, P.k.p, P.a.p, P.n.p, s, d_done.p);
+                launch_level_status(d_vb.p, ng, d_active.p, d_done.p, d_live.p, s);
             }
-            launch_isolated_targets(P.perm.p + P.bin_lo(0), P.bin_hi(0) - P.bin_lo(0), tgt, s);
-            launch_graph_mod(d_vb.p, ng, d_active.p, eii.p, P.a.p, d_exx.p, d_a2.p, s);
-            launch_batch_decide(ng, d_exx.p, d_a2.p, d_gconst.p, d_lower.p, p.threshold, kMaxPhaseIters,
-                                d_prev.p, d_curr.p, d_itrs.p, d_active.p, d_apply.p, s);
-            launch_batch_apply(nv, gid.p, d_apply.p, tgt, curr, past, P.k.p, P.a.p, P.n.p, s);
             GLV_HIP(hipStreamEndCapture(s, &graph));
             GLV_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            for (int rep = 0; rep < kReplays; rep++) {
+            while (true) {
                 GLV_HIP(hipGraphLaunch(exec, s));
-                r.iterations++;
+                r.launches++;
+                i64 live = 0;
+                GLV_HIP(hipMemcpyAsync(&done, d_done.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                GLV_HIP(hipMemcpyAsync(&live, d_live.p, sizeof(i64), hipMemcpyDeviceToHost, s));
+                GLV_HIP(hipStreamSynchronize(s));
+                if (done || live < listed) break;
             }
-            GLV_HIP(hipStreamSynchronize(s));
             (void)hipGraphExecDestroy(exec);
             (void)hipGraphDestroy(graph);
         }
@@ -195,6 +212,10 @@ void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0,
         std::vector<uint8_t> stop(ng, 0);
         std::vector<i64> base(ng, 0), kg(ng, 0);
         bool any_alive = false;
+        int level_itrs = 0;  // the level's iterations: those of its longest-running graph
+        for (i64 g = 0; g < ng; g++)
+            if (alive[g]) level_itrs = std::max(level_itrs, h_itrs[g]);
+        r.iterations += level_itrs;
         for (i64 g = 0; g < ng; g++) {
             kg[g] = h_hi[g] >= 0 ? h_hi[g] - h_lo[g] + 1 : 0;
             base[g] = h_hi[g] >= 0 ? h_lo[g] : 0;

@@ -45,7 +45,8 @@ __global__ __launch_bounds__(kBB) void k_graph_const(const i64 *vb, i64 ng, cons
 // per active graph: internal weight and sum of squared community degrees
 __global__ __launch_bounds__(kBB) void k_graph_mod(const i64 *vb, i64 ng, const uint8_t *active,
                                                    const double *eii, const double *a, double *exx,
-                                                   double *a2) {
+                                                   double *a2, const int32_t *skip) {
+    if (skip && *skip) return;
     for (i64 g = blockIdx.x; g < ng; g += gridDim.x) {
         if (!active[g]) continue;
         double e = 0, q = 0;
@@ -85,10 +86,30 @@ __global__ void k_batch_decide(i64 ng, const double *exx, const double *a2, cons
     }
 }
 
+// After an iteration: live = vertices of the graphs still iterating, done =
+// (none is); one workgroup, so a captured run of iterations can test it.
+__global__ __launch_bounds__(kBB) void k_level_status(const i64 *vb, i64 ng, const uint8_t *active,
+                                                      int32_t *done, i64 *live) {
+    __shared__ i64 s_part[kBB / kWave];
+    i64 t = 0;
+    for (i64 g = threadIdx.x; g < ng; g += kBB)
+        if (active[g]) t += vb[g + 1] - vb[g];
+    for (int o = kWave / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if (threadIdx.x % kWave == 0) s_part[threadIdx.x / kWave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        i64 sum = 0;
+        for (int i = 0; i < kBB / kWave; i++) sum += s_part[i];
+        *live = sum;
+        *done = sum == 0 ? 1 : 0;
+    }
+}
+
 // commit of one Jacobi iteration of the graphs with apply set
 // (parallelLouvainMethod.cpp:275-283): community updates, then past <- curr <- tgt
 __global__ void k_batch_apply(i64 nv, const int32_t *gid, const uint8_t *apply, const i64 *tgt, i64 *curr,
-                              i64 *past, const double *k, double *a, i64 *n) {
+                              i64 *past, const double *k, double *a, i64 *n, const int32_t *skip) {
+    if (skip && *skip) return;
     for (i64 v = blockIdx.x * (i64)blockDim.x + threadIdx.x; v < nv; v += (i64)gridDim.x * blockDim.x) {
         if (!apply[gid[v]]) continue;
         const i64 t = tgt[v], s = curr[v];
@@ -224,9 +245,9 @@ void launch_graph_const(const i64 *vb, i64 ng, const double *k, double *gconst, 
     GLV_LAUNCH_CHECK();
 }
 void launch_graph_mod(const i64 *vb, i64 ng, const uint8_t *active, const double *eii, const double *a,
-                      double *exx, double *a2, hipStream_t s) {
+                      double *exx, double *a2, hipStream_t s, const int32_t *skip) {
     if (ng <= 0) return;
-    k_graph_mod<<<graph_grid(ng), kBB, 0, s>>>(vb, ng, active, eii, a, exx, a2);
+    k_graph_mod<<<graph_grid(ng), kBB, 0, s>>>(vb, ng, active, eii, a, exx, a2, skip);
     GLV_LAUNCH_CHECK();
 }
 void launch_batch_decide(i64 ng, const double *exx, const double *a2, const double *gconst, const double *lower,
@@ -237,10 +258,14 @@ void launch_batch_decide(i64 ng, const double *exx, const double *a2, const doub
                                                           curr, itrs, active, apply);
     GLV_LAUNCH_CHECK();
 }
+void launch_level_status(const i64 *vb, i64 ng, const uint8_t *active, int32_t *done, i64 *live, hipStream_t s) {
+    k_level_status<<<1, kBB, 0, s>>>(vb, ng, active, done, live);
+    GLV_LAUNCH_CHECK();
+}
 void launch_batch_apply(i64 nv, const int32_t *gid, const uint8_t *apply, const i64 *tgt, i64 *curr, i64 *past,
-                        const double *k, double *a, i64 *n, hipStream_t s) {
+                        const double *k, double *a, i64 *n, hipStream_t s, const int32_t *skip) {
     if (nv <= 0) return;
-    k_batch_apply<<<grid_for(nv, kBB, 8192), kBB, 0, s>>>(nv, gid, apply, tgt, curr, past, k, a, n);
+    k_batch_apply<<<grid_for(nv, kBB, 8192), kBB, 0, s>>>(nv, gid, apply, tgt, curr, past, k, a, n, skip);
     GLV_LAUNCH_CHECK();
 }
 void launch_compact_graphs(const i64 *list, i64 n, const int32_t *gid, const uint8_t *active, const i64 *start,

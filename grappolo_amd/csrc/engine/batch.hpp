@@ -23,7 +23,8 @@ struct BatchResult {
     std::vector<int32_t> phases, total_iterations;
     std::vector<i64> clusters;
     int levels = 0;       // union levels (phases of the longest-running graph)
-    int iterations = 0;   // batched iterations over all levels
+    int iterations = 0;   // batched iterations over all levels (per level: its longest-running graph's)
+    int launches = 0;     // graph launches of the iteration loops (kLevelUnroll iterations each)
     double t_total = 0;
 };
 
@@ -35,13 +36,15 @@ void run_basic_batch(Engine &E, const DevGraph &g0, const std::vector<i64> &vb0,
 // --- kernels (batch.hip) ---------------------------------------------------
 void launch_fill_gid(const i64 *vb, i64 ng, int32_t *gid, hipStream_t s);
 void launch_graph_const(const i64 *vb, i64 ng, const double *k, double *gconst, hipStream_t s);
+// skip: nothing to do when *skip is set (after every graph's phase has ended)
 void launch_graph_mod(const i64 *vb, i64 ng, const uint8_t *active, const double *eii, const double *a,
-                      double *exx, double *a2, hipStream_t s);
+                      double *exx, double *a2, hipStream_t s, const int32_t *skip = nullptr);
 void launch_batch_decide(i64 ng, const double *exx, const double *a2, const double *gconst, const double *lower,
                          double thresh, int maxitr, double *prev, double *curr, int32_t *itrs, uint8_t *active,
                          uint8_t *apply, hipStream_t s);
+void launch_level_status(const i64 *vb, i64 ng, const uint8_t *active, int32_t *done, i64 *live, hipStream_t s);
 void launch_batch_apply(i64 nv, const int32_t *gid, const uint8_t *apply, const i64 *tgt, i64 *curr, i64 *past,
-                        const double *k, double *a, i64 *n, hipStream_t s);
+                        const double *k, double *a, i64 *n, hipStream_t s, const int32_t *skip = nullptr);
 void launch_compact_graphs(const i64 *list, i64 n, const int32_t *gid, const uint8_t *active, const i64 *start,
                            i64 nkeys, i64 *flags, i64 *pos, i64 *out, i64 *out_start, hipStream_t s);
 void launch_graph_label_range(const i64 *vb, i64 ng, const i64 *C, i64 *lo, i64 *hi, hipStream_t s);

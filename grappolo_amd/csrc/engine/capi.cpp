@@ -983,6 +983,7 @@ int glv_runMultiPhaseBasic_batch_dev(int64_t ngraphs, const int64_t *vbase, int6
             stats->levels = r.levels;
             stats->iterations = r.iterations;
             stats->time_total = r.t_total;
+            stats->launches = r.launches;
         }
     });
 }

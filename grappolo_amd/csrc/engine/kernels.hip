@@ -313,6 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *part, i64
 // community scores it; a butterfly over the group picks the best.
 template <int G, bool PLAIN>
 __global__ __launch_bounds__(kBlock) void k_sweep_group(SweepArgs A) {
+    if (A.skip && *A.skip) return;
     constexpr int VPB = kBlock / G;
     __shared__ i64 s_c[kBlock];
     __shared__ double s_w[kBlock];
@@ -452,6 +453,7 @@ __device__ inline void score_slots(const i64 *keys, const double *vals, i64 slot
 // One wavefront per vertex (degree <= 256), a private 512-slot LDS table.
 template <bool PLAIN>
 __global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
+    if (A.skip && *A.skip) return;
     constexpr int WPB = kBlock / kWave;
     constexpr int SLOTS = kWaveHashSlots;
     constexpr int SHIFT = 64 - 9;
@@ -526,6 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_wavehash(SweepArgs A) {
 // the degree in the block's private HBM region; blocks stride over vertices.
 template <bool PLAIN, bool GLOBAL>
 __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
+    if (A.skip && *A.skip) return;
     constexpr int LSLOTS = GLOBAL ? 1 : kBlockHashSlots;
     __shared__ i64 l_keys[LSLOTS];
     __shared__ double l_vals[LSLOTS];
@@ -603,6 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_blockhash(SweepArgs A) {
 // vertex's own entries plus nv/32 bitmap words, not nv slots.
 template <bool PLAIN>
 __global__ __launch_bounds__(kBlock) void k_sweep_dense(SweepArgs A) {
+    if (A.skip && *A.skip) return;
     __shared__ double acc[kDenseMaxNV];
     __shared__ uint32_t touched[kDenseMaxNV / 32];
     __shared__ uint16_t list[kDenseListMax];
@@ -682,6 +686,7 @@ constexpr int kGiantChunk = 2048;
 constexpr int kGiantLds = 2 * kGiantChunk;
 
 __global__ __launch_bounds__(kBlock) void k_giant_accum(SweepArgs A) {
+    if (A.skip && *A.skip) return;
     __shared__ i64 lk[kGiantLds];
     __shared__ double lv[kGiantLds];
     const i64 q = blockIdx.y;
@@ -728,6 +733,7 @@ __global__ __launch_bounds__(kBlock) void k_giant_accum(SweepArgs A) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_giant_score(SweepArgs A) {
+    if (A.skip && *A.skip) return;
     const i64 q = blockIdx.y;
     const i64 v = A.verts[q];
     const i64 s = A.C[v];
@@ -773,6 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_giant_score(SweepArgs A) {
 
 template <bool PLAIN>
 __global__ __launch_bounds__(kWave) void k_giant_decide(SweepArgs A) {
+    if (A.skip && *A.skip) return;
     const i64 q = blockIdx.x;
     const int lane = threadIdx.x;
     const i64 v = A.verts[q];

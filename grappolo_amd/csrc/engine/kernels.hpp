@@ -43,6 +43,9 @@ struct SweepArgs {
     const int32_t *gid = nullptr;
     const double *gconst = nullptr;
     double *eii_v = nullptr;
+    // batched graphs: the sweep returns at once when *skip is set (a captured
+    // iteration after every graph's phase has ended)
+    const int32_t *skip = nullptr;
     const i64 *gt_tab = nullptr;
     i64 *gt_keys = nullptr;
     double *gt_vals = nullptr;

@@ -492,11 +492,12 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     }
     DBuf<unsigned> bar(pblocks > 0 ? kBarrierWords : 0, s_);
     DBuf<double> ppart(pblocks > 0 ? 2 * pblocks : 0, s_);
+    if (use_wide && (nv >= (i64(1) << 31) || g.nnz >= (i64(1) << 40))) use_wide = false;  // 32-bit ids
     const bool dbl = pblocks > 0 || use_wide;
     DBuf<i64> cperm(dbl ? nv : 0, s_), cstart(dbl ? ncol + 1 : 0, s_);
-    // second state buffers of the persistent sweeps (one grid barrier per color step)
-    DBuf<i64> pC1(dbl ? nv : 0, s_), pn1(dbl ? nv : 0, s_);
-    DBuf<double> pa1(dbl ? nv : 0, s_);
+    // second state buffers of the persistent sweep (one grid barrier per color step)
+    DBuf<i64> pC1(pblocks > 0 ? nv : 0, s_), pn1(pblocks > 0 ? nv : 0, s_);
+    DBuf<double> pa1(pblocks > 0 ? nv : 0, s_);
     if (dbl) launch_color_major(P.perm.p, colors, nv, ncol, cperm.p, cstart.p, s_);
     bool persistent = pblocks > 0;
     WidePlan WP;
@@ -506,11 +507,20 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     DBuf<i64> wmoves(use_wide ? 1 : 0, s_);
     DBuf<CommInfo> wm0(use_wide ? nv : 0, s_), wm1(use_wide ? nv : 0, s_);
     if (use_wide) use_wide = build_wide_plan(g, P, wblocks, WP, s_);
+    // the wide sweep's 32-bit labels (two buffers) and neighbour ids; while
+    // it runs, the phase's labels live in wc0 (copied back to C when a full
+    // modularity pass or the end of the phase needs them)
+    DBuf<int32_t> wc0(use_wide ? nv : 0, s_), wc1(use_wide ? nv : 0, s_), wtails(use_wide ? g.nnz : 0, s_);
+    bool labels32 = false;
     if (use_wide) {
+        launch_narrow_ids(C, nv, wc0.p, s_);
+        launch_narrow_ids(g.tails, g.nnz, wtails.p, s_);
+        labels32 = true;
         launch_fill_i64(wmoves.p, 1, 0, s_);
         W.A = A;
-        W.C0 = C;
-        W.C1 = pC1.p;
+        W.C0 = wc0.p;
+        W.C1 = wc1.p;
+        W.tails = wtails.p;
         W.a = P.a.p;
         W.n = P.n.p;
         W.m0 = wm0.p;
@@ -662,6 +672,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                 if (abort_flag) fail(GLV_E_DEVICE, "grid barrier of the wide sweep timed out");
                 if (numItrs == 1 || !valid_colors) {
                     // a full pass: the first iteration, or classes that may hold an edge
+                    launch_widen_ids(wc0.p, nv, C, s_);
                     currMod = colored_modularity(g, P, C, part.p, res.p, &exx);
                 } else {
                     // moves of one color step are pairwise non-adjacent, so the
@@ -671,8 +682,11 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
                     currMod = exx * (double)c - h[1] * (double)c * (double)c;
                 }
                 done_iter = true;
+            } else {
+                // launch refused: this phase continues with per-color launches
+                launch_widen_ids(wc0.p, nv, C, s_);
+                labels32 = false;
             }
-            // launch refused: this phase continues with per-color launches
         }
         if (done_iter) {
         } else if (persistent) {
@@ -766,6 +780,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         if (numItrs >= kMaxPhaseIters) break;
         prevMod = currMod;
     }
+    if (labels32) launch_widen_ids(wc0.p, nv, C, s_);
     if (wmoves.n) moves_ += fetch(wmoves.p);
     if (mode) sync();
     if (las_off) {

@@ -207,9 +207,9 @@ __device__ inline void wg_best0(Cand &c, Red &R) {
 // The state of one color step: read the state after the previous step,
 // write the moves into the other buffer.
 struct Step {
-    const i64 *C;
+    const int32_t *C;
     const CommInfo *m;
-    i64 *Cq;
+    int32_t *Cq;
     CommInfo *mq;
 };
 
@@ -233,7 +233,7 @@ __device__ inline void decide(const WideArgs &W, const Step &S, i64 v, i64 s, do
     W.A.tgt[v] = target;
     if (target != s) {
         move_comm(target, s, kv, S.mq);
-        S.Cq[v] = target;
+        S.Cq[v] = (int32_t)target;
         dein += 2.0 * (c.e - eix);
         moved++;
     }
@@ -258,7 +258,7 @@ __device__ inline void wg_gather(const WideArgs &W, const Step &S, i64 v, i64 be
 #pragma unroll
         for (int u = 0; u < kWUnroll; u++) {
             const i64 j = j0 + (i64)u * kWBlock;
-            tt[u] = j < end ? W.A.tails[j] : -1;
+            tt[u] = j < end ? W.tails[j] : -1;
             wt[u] = j < end ? W.A.w[j] : 0.0;
         }
 #pragma unroll
@@ -536,7 +536,7 @@ __device__ void mid_vertex(const WideArgs &W, const Step &S, i64 v, uint32_t *ke
 #pragma unroll
         for (int u = 0; u < kWUnroll; u++) {
             const i64 j = j0 + (i64)u * kWave;
-            tt[u] = j < end ? W.A.tails[j] : -1;
+            tt[u] = j < end ? W.tails[j] : -1;
             wt[u] = j < end ? W.A.w[j] : 0.0;
         }
 #pragma unroll
@@ -587,7 +587,7 @@ __device__ void light_vertices(const WideArgs &W, const Step &S, i64 pos, int co
     double as = 0;
     i64 ns = 0;
     if (gl < deg) {
-        const i64 t = W.A.tails[beg + gl];
+        const i64 t = W.tails[beg + gl];
         ww = W.A.w[beg + gl];
         if (gl == 0) {
             const CommInfo ms = S.m[s];
@@ -696,7 +696,7 @@ __global__ __launch_bounds__(kWBlock) void k_wide_iteration(WideArgs W) {
             const i64 t = W.A.tgt[v], s = S.Cq[v];
             if (s >= 0 && t != s) {  // vertices without entries (label -1) never move
                 move_comm(t, s, W.A.k[v], S.mq);
-                S.Cq[v] = t;
+                S.Cq[v] = (int32_t)t;
             }
         }
         alive = grid_barrier(W.bar, nb, ++seq);
@@ -756,6 +756,25 @@ __global__ void k_wide_clear_giants(int32_t *gstate, double *gsc, double *gout, 
 }  // namespace
 
 #define GLV_LAUNCH_CHECK() GLV_HIP(hipGetLastError())
+
+__global__ void k_narrow_ids(const i64 *src, i64 n, int32_t *dst) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        dst[i] = (int32_t)src[i];
+}
+__global__ void k_widen_ids(const int32_t *src, i64 n, i64 *dst) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        dst[i] = (i64)src[i];
+}
+void launch_narrow_ids(const i64 *src, i64 n, int32_t *dst, hipStream_t s) {
+    if (n <= 0) return;
+    k_narrow_ids<<<grid_for(n, kBlock, 16384), kBlock, 0, s>>>(src, n, dst);
+    GLV_LAUNCH_CHECK();
+}
+void launch_widen_ids(const int32_t *src, i64 n, i64 *dst, hipStream_t s) {
+    if (n <= 0) return;
+    k_widen_ids<<<grid_for(n, kBlock, 16384), kBlock, 0, s>>>(src, n, dst);
+    GLV_LAUNCH_CHECK();
+}
 
 bool wide_enabled() {
     static const bool on = [] {

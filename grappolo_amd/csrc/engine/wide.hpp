@@ -14,6 +14,9 @@
 //   wavefront items: one mid vertex (65..256 entries, a wavefront's LDS table)
 //     or 64/G light vertices of degree bin b (G lanes each, the in-order sums
 //     of buildLocalMapCounter, Utility/utilityClusteringFunctions.cpp:172-194).
+// Labels and the adjacency's neighbour ids are held as 32-bit copies for the
+// phase (halving the bytes of the random label gathers and of the neighbour
+// stream), so the sweep serves graphs of fewer than 2^31 vertices.
 // Items are dealt to workgroups when the phase starts (longest first, each to
 // the least loaded workgroup; a workgroup's wavefronts take its wavefront
 // items in turn), so a run is reproducible.  Labels, community degrees and sizes are
@@ -85,7 +88,8 @@ struct CommInfo {
 
 struct WideArgs {
     SweepArgs A;                 // graph, k, constant, tgt
-    i64 *C0, *C1;                // labels, two buffers (C0: the phase's labels)
+    int32_t *C0, *C1;            // labels, two 32-bit buffers (C0: the phase's labels)
+    const int32_t *tails;        // the adjacency's neighbour ids, 32-bit
     double *a;                   // community degrees and sizes of the phase, read at the
     i64 *n;                      //   start of the iteration and written at its end
     CommInfo *m0, *m1;           // community degrees and sizes, two buffers
@@ -143,5 +147,8 @@ bool launch_wide_iteration(const WideArgs &W, int nblocks, hipStream_t s);
 // (vertex, degree) of perm[lo .. hi) into vd[2 * (hi - lo)]
 void launch_wide_degrees(const i64 *perm, const i64 *off, i64 lo, i64 hi, i64 *vd, hipStream_t s);
 void launch_wide_clear_giants(int32_t *gstate, double *gsc, double *gout, i64 ng, hipStream_t s);
+// 64-bit ids to 32-bit (all below 2^31) and back
+void launch_narrow_ids(const i64 *src, i64 n, int32_t *dst, hipStream_t s);
+void launch_widen_ids(const int32_t *src, i64 n, i64 *dst, hipStream_t s);
 
 }  // namespace glv

@@ -587,7 +587,8 @@ def gini_dev(sizes):
 class BatchStats(ctypes.Structure):
     _fields_ = [("final_modularity", _F64P), ("last_modularity", _F64P), ("phases", _I32P),
                 ("total_iterations", _I32P), ("final_clusters", _I64P), ("levels", ctypes.c_int32),
-                ("iterations", ctypes.c_int32), ("time_total", ctypes.c_double)]
+                ("iterations", ctypes.c_int32), ("time_total", ctypes.c_double),
+                ("launches", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def run_basic_batch_dev(dg, vbase, C_orig, basicOpt=0, threshold=1e-9):
@@ -604,7 +605,7 @@ def run_basic_batch_dev(dg, vbase, C_orig, basicOpt=0, threshold=1e-9):
                     _p(out["final_clusters"], _I64P), 0, 0, 0.0)
     _call("glv_runMultiPhaseBasic_batch_dev", ng, _p(vb, _I64P), *dg.ptrs(), _ptr(C_orig),
           int(basicOpt), float(threshold), _stream(), ctypes.byref(st))
-    out.update(levels=st.levels, iterations=st.iterations, time_total=st.time_total)
+    out.update(levels=st.levels, iterations=st.iterations, time_total=st.time_total, launches=st.launches)
     return out
 
 

@@ -321,8 +321,10 @@ typedef struct glv_batch_stats {
     int32_t *total_iterations;
     int64_t *final_clusters;
     int32_t levels;      /* phases of the longest-running graph */
-    int32_t iterations;  /* batched iterations over all levels */
+    int32_t iterations;  /* batched iterations over all levels (per level: its longest-running graph's) */
     double time_total;   /* seconds */
+    int32_t launches;    /* graph launches of the levels' iteration loops (4 iterations each) */
+    int32_t reserved;
 } glv_batch_stats;
 int glv_runMultiPhaseBasic_batch_dev(int64_t ngraphs, const int64_t *vbase, int64_t nv, int64_t nnz,
                                      const int64_t *d_offsets, const int64_t *d_tails,

@@ -45,6 +45,10 @@ def test_batch_equals_separate_runs(basic_opt):
     res = engine.run_basic_batch_dev(dg, vb, C, basicOpt=basic_opt)
     Cb = C.cpu().numpy()
     assert res["levels"] >= 2
+    # each level's iterations run as captured runs of 4 (one launch each;
+    # finished graphs leave the sweep lists between launches)
+    assert res["iterations"] >= max(res["total_iterations"])
+    assert 1 <= res["launches"] <= res["iterations"]
     for i, g in enumerate(gs):
         Ci, st = _standalone(g, basic_opt)
         np.testing.assert_array_equal(Cb[vb[i]:vb[i + 1]], Ci, err_msg=f"graph {i}")
