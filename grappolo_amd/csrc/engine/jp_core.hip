@@ -146,28 +146,20 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
     __shared__ uint16_t queue[kJpCoreMax];
     __shared__ uint32_t mk[kCoreWaves][kColorWords];
     __shared__ int s_dm[kCoreWaves];
-    __shared__ int s_cnt[3];  // vertices queued during rounds r = 0, 1, 2 (mod 3)
+    __shared__ int s_tail;
     const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
     const u64 below = (1ull << lane) - 1;
     for (int q = tid; q < K; q += kCoreBlock) st[q] = (uint32_t)core_cnt[2 * q];
-    for (int i = tid; i < kCoreWaves * kColorWords; i += kCoreBlock) (&mk[0][0])[i] = 0;
-    if (tid < kCoreWaves) s_dm[tid] = 0;
-    if (tid < 3) s_cnt[tid] = 0;
+    if (tid == 0) s_tail = 0;
     __syncthreads();
-    // Round r's queue is queue[head, tail); the vertices it releases are
-    // appended from tail on, counted in s_cnt[r % 3].  Every thread reads
-    // that count after the barrier that ends round r; s_cnt[(r + 2) % 3],
-    // last read before that barrier, is cleared during round r + 1 for the
-    // appends of round r + 2: one barrier per round boundary.
-    int round = 0, head = 0, tail = 0;
     // appends the ready lanes' vertices to the next round (whole wavefront)
     auto enqueue = [&](bool ready, int x) {
         const u64 bal = __ballot(ready);
         if (!bal) return;
         int base = 0;
-        if (lane == 0) base = atomicAdd(&s_cnt[round % 3], __popcll(bal));
+        if (lane == 0) base = atomicAdd(&s_tail, __popcll(bal));
         base = __shfl(base, 0);
-        if (ready) queue[tail + base + __popcll(bal & below)] = (uint16_t)x;
+        if (ready) queue[base + __popcll(bal & below)] = (uint16_t)x;
     };
     // first free colour of bitmap m (two words per lane) and the largest
     // depth dm, over the wavefront; valid in every lane
@@ -201,17 +193,15 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
         atomicOr(&m[c >> 5], 1u << (c & 31));
         dm = d > dm ? d : dm;
     };
-    // the sources (no higher core neighbour) as the queue of round 0's end
     for (int q0 = 0; q0 < K; q0 += kCoreBlock) {
         const int q = q0 + tid;
         enqueue(q < K && core_cnt[2 * q] == 0, q);
     }
+    int head = 0;
     while (true) {
-        __syncthreads();  // round `round` is complete: its appends, states and bitmap clears
-        head = tail;
-        tail += s_cnt[round % 3];
-        round++;
-        if (tid == 0) s_cnt[(round + 1) % 3] = 0;
+        __syncthreads();  // the previous round is complete
+        const int tail = s_tail;
+        __syncthreads();  // and every thread knows where it ends before this round appends
         const int n = tail - head;
         if (n <= 0) break;
         if (n >= kCoreWaves) {
@@ -253,8 +243,6 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
                     for (int k = 0; k < kCoreRegs; k++) enqueue(x[k] >= 0 && atomicSub(&st[x[k]], 1u) == 1u, x[k]);
                 }
             }
-            m[lane] = 0;  // left clear for a later round
-            m[lane + kWave] = 0;
         } else {
             // a group of G wavefronts per vertex, one vertex per group
             const int G = n == 1 ? 16 : n == 2 ? 8 : n <= 4 ? 4 : n <= 8 ? 2 : 1;
@@ -263,7 +251,12 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
             const int r = act ? queue[head + g] : 0;
             const i64 b = act ? loff[r] : 0;
             const int len = act ? (int)(loff[r + 1] - b) : 0, nhi = act ? core_cnt[2 * r] : 0;
-            uint32_t *m = mk[g * G];  // clear, as s_dm: every round leaves them so
+            uint32_t *m = mk[g * G];
+            if (act && gw == 0) {
+                m[lane] = 0;
+                m[lane + kWave] = 0;
+                if (lane == 0) s_dm[g] = 0;
+            }
             // the first kCoreRegs entries per thread of the whole list, for both passes
             int x[kCoreRegs];
 #pragma unroll
@@ -271,6 +264,7 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
                 const int j = k * gstride + gt;
                 x[k] = j < len ? (int)ladj[b + j] : -1;
             }
+            __syncthreads();
             int dm = 0;
 #pragma unroll
             for (int k = 0; k < kCoreRegs; k++) mark(x[k], k * gstride + gt, nhi, m, dm);
@@ -294,12 +288,7 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
             if (act && gw == 0) {
                 int dmax = s_dm[g];
                 const int color = first_free(m, dmax);
-                if (lane == 0) {
-                    st[r] = state_of(color, dmax);
-                    s_dm[g] = 0;
-                }
-                m[lane] = 0;  // clear for a later round (read above by this wavefront only)
-                m[lane + kWave] = 0;
+                if (lane == 0) st[r] = state_of(color, dmax);
             }
             __syncthreads();
 #pragma unroll
@@ -319,6 +308,7 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
                 for (int k = 0; k < kCoreRegs; k++) enqueue(y[k] >= 0 && atomicSub(&st[y[k]], 1u) == 1u, y[k]);
             }
         }
+        head = tail;
     }
     for (int q = tid; q < K; q += kCoreBlock) {
         const uint32_t sq = st[q];

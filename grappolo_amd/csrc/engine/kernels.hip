@@ -2105,16 +2105,22 @@ __global__ __launch_bounds__(kBlock) void k_jp_count(const i64 *off, const i64 *
     }
     stage_flush(S, L.small, &L.counts[0]);
 }
-// count[v] of the heavy vertices, one wavefront each
+// count[v] of the heavy vertices up to kJpCountWaveDeg entries, one
+// wavefront each (larger ones: k_jp_count_giant)
+constexpr i64 kJpCountWaveDeg = 4096;
 __global__ __launch_bounds__(kBlock) void k_jp_count_heavy(const i64 *off, const i64 *tails, int32_t *cnt,
                                                            const i64 *heavy, const i64 *heavy_count,
-                                                           JpLists L) {
+                                                           JpLists L, i64 *giant, i64 *giant_count) {
     constexpr int WPB = kBlock / kWave;
     const i64 nheavy = *heavy_count;
     const int lane = threadIdx.x % kWave;
     const i64 stride = (i64)gridDim.x * WPB;
     for (i64 q = (i64)blockIdx.x * WPB + threadIdx.x / kWave; q < nheavy; q += stride) {
         const i64 v = heavy[q];
+        if (off[v + 1] - off[v] > kJpCountWaveDeg) {  // wave-uniform
+            wave_append(lane == 0, v, giant, giant_count);
+            continue;
+        }
         const u64 pv = L.prio[v];
         const bool core_v = (pv >> 40) >= L.dcore;
         int c = 0, hi = 0, lo = 0;
@@ -2145,6 +2151,77 @@ __global__ __launch_bounds__(kBlock) void k_jp_count_heavy(const i64 *off, const
             if (lane == 0) cnt[v] = c;
             frontier_push(lane == 0 && c == 0, v, off, L);
         }
+    }
+}
+
+// count[v] of the heavy vertices above kJpCountWaveDeg entries (listed by
+// k_jp_count_heavy), one workgroup each (kJpUnroll passes of loads in flight)
+__global__ __launch_bounds__(kBlock) void k_jp_count_giant(const i64 *off, const i64 *tails, int32_t *cnt,
+                                                           const i64 *heavy, const i64 *heavy_count,
+                                                           JpLists L) {
+    __shared__ int s_c[3][kBlock / kWave];
+    const i64 nheavy = *heavy_count;
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    for (i64 q = blockIdx.x; q < nheavy; q += gridDim.x) {
+        const i64 v = heavy[q];
+        const i64 b = off[v], deg = off[v + 1] - b;
+        const u64 pv = L.prio[v];
+        const bool core_v = (pv >> 40) >= L.dcore;
+        int c = 0, hi = 0, lo = 0;
+        for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kBlock) {
+            i64 uu[kJpUnroll];
+            u64 pu[kJpUnroll];
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) {
+                const i64 j = j0 + k * kBlock + threadIdx.x;
+                uu[k] = j < deg ? tails[b + j] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) pu[k] = uu[k] >= 0 ? L.prio[uu[k]] : 0;
+#pragma unroll
+            for (int k = 0; k < kJpUnroll; k++) {
+                if (uu[k] < 0) continue;
+                const uint8_t cd = jp_code(pu[k], uu[k], pv, v, L.dcore);
+                L.code[b + j0 + k * kBlock + threadIdx.x] = cd;
+                if (cd & kJpSelf) continue;
+                if (cd & kJpCore) {
+                    if (cd & kJpRelease) lo++;
+                    else hi++;
+                } else if (!(cd & kJpRelease)) {
+                    c++;
+                }
+            }
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            c += __shfl_xor(c, o);
+            hi += __shfl_xor(hi, o);
+            lo += __shfl_xor(lo, o);
+        }
+        if (lane == 0) {
+            s_c[0][w] = c;
+            s_c[1][w] = hi;
+            s_c[2][w] = lo;
+        }
+        __syncthreads();
+        if (w == 0) {
+            c = hi = lo = 0;
+            for (int i = 0; i < kBlock / kWave; i++) {
+                c += s_c[0][i];
+                hi += s_c[1][i];
+                lo += s_c[2][i];
+            }
+            if (core_v) {
+                if (lane == 0) {
+                    const int cq = cnt[v];
+                    L.core_cnt[2 * cq] = hi;
+                    L.core_cnt[2 * cq + 1] = lo;
+                }
+            } else {
+                if (lane == 0) cnt[v] = c;
+                frontier_push(lane == 0 && c == 0, v, off, L);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -2425,6 +2502,7 @@ __global__ __launch_bounds__(kTailBlock) void k_tail_seq(const i64 *list, i64 n,
 // (mod 3), [6] receives the rounds run here, [7] receives 1 when the launch
 // ended on a large frontier, [8] the coloured vertices at the end.
 constexpr int kJpPBlock = 1024;
+constexpr int kJpLargeGroups = 4;  // large frontier vertices in flight per workgroup when they are many
 __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, const i64 *tails, i64 nv,
                                                              int32_t *colors, int32_t *cnt, i64 *sA, i64 *sB,
                                                              i64 *lA, i64 *lB, i64 *ring, unsigned *bar,
@@ -2432,7 +2510,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                                                              int32_t *dep, i64 max_front) {
     constexpr int WPB = kJpPBlock / kWave;
     __shared__ uint32_t mark[WPB][kMaxColors / 32];
-    __shared__ int s_color, s_dep;
+    __shared__ int s_color[kJpLargeGroups], s_dep[kJpLargeGroups];
     __shared__ Stage<kStageCap> S;  // appends to the next small frontier
     stage_init(S);
     __syncthreads();
@@ -2473,29 +2551,34 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 jp_wave8(off, tails, colors, dep, cnt, code, fs, lo, n, mk, err, push);
             }
         }
-        __syncthreads();  // mark[0] is the large loop's and wave 0's
-        // large frontier vertices: a workgroup each
-        for (i64 qq = blockIdx.x; qq < nlarge; qq += nb) {
-            const i64 v = fl[qq];
-            const i64 b = off[v], e = off[v + 1];
-            const i64 deg = e - b;
-            for (int i = threadIdx.x; i < kMaxColors / 32; i += kJpPBlock) mark[0][i] = 0;
-            if (threadIdx.x == 0) {
-                s_color = kMaxColors;
-                s_dep = 0;
+        __syncthreads();  // the marks are the large loop's from here
+        // large frontier vertices: a workgroup each, or a quarter of one each
+        // when there are more of them than workgroups (4 in flight per CU)
+        const int NG = nlarge > (i64)nb ? kJpLargeGroups : 1;
+        const int GT = kJpPBlock / NG, grp = threadIdx.x / GT, gtid = threadIdx.x % GT;
+        uint32_t *gm = mark[grp * (WPB / NG)];
+        for (i64 base = (i64)blockIdx.x * NG; base < nlarge; base += (i64)nb * NG) {
+            const i64 qq = base + grp;
+            const bool act = qq < nlarge;
+            const i64 v = act ? fl[qq] : 0;
+            const i64 b = act ? off[v] : 0, deg = act ? off[v + 1] - b : 0;
+            for (int i = gtid; i < kMaxColors / 32; i += GT) gm[i] = 0;
+            if (gtid == 0) {
+                s_color[grp] = kMaxColors;
+                s_dep[grp] = 0;
             }
             __syncthreads();
             // one pass: colours and depths of the higher-priority neighbours
             // (all coloured), release of the lower-priority ones (next round's
             // frontier)
             int dm = 0;
-            for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * kJpPBlock) {
+            for (i64 j0 = 0; j0 < deg; j0 += kJpUnroll * GT) {
                 i64 uu[kJpUnroll];
                 uint8_t cd[kJpUnroll];
                 bool rd[kJpUnroll];
 #pragma unroll
                 for (int k = 0; k < kJpUnroll; k++) {
-                    const i64 j = j0 + k * kJpPBlock + threadIdx.x;
+                    const i64 j = j0 + k * GT + gtid;
                     uu[k] = j < deg ? tails[b + j] : -1;
                     cd[k] = j < deg ? code[b + j] : kJpSelf;
                 }
@@ -2510,7 +2593,7 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                         const int c = colors[u];
                         const int du = dep[u];
                         dm = du > dm ? du : dm;
-                        if (c >= 0) atomicOr(&mark[0][c >> 5], 1u << (c & 31));
+                        if (c >= 0) atomicOr(&gm[c >> 5], 1u << (c & 31));
                     }
                 }
 #pragma unroll
@@ -2524,21 +2607,21 @@ __global__ __launch_bounds__(kJpPBlock) void k_jp_persistent(const i64 *off, con
                 const int x = __shfl_xor(dm, o);
                 dm = x > dm ? x : dm;
             }
-            if (lane == 0 && dm > 0) atomicMax(&s_dep, dm);
+            if (lane == 0 && dm > 0) atomicMax(&s_dep[grp], dm);
             __syncthreads();
-            if (threadIdx.x < kMaxColors / 32) {
-                const uint32_t m = mark[0][threadIdx.x];
-                if (m != 0xFFFFFFFFu) atomicMin(&s_color, (int)threadIdx.x * 32 + __ffs((int)~m) - 1);
+            if (gtid < kMaxColors / 32) {
+                const uint32_t m = gm[gtid];
+                if (m != 0xFFFFFFFFu) atomicMin(&s_color[grp], gtid * 32 + __ffs((int)~m) - 1);
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                int color = s_color;
+            if (act && gtid == 0) {
+                int color = s_color[grp];
                 if (color >= kMaxColors) {
                     atomicOr(err, 1);
                     color = 0;
                 }
                 colors[v] = color;
-                dep[v] = s_dep + 1;
+                dep[v] = s_dep[grp] + 1;
             }
             __syncthreads();
             if (S.n > kStageCap / 2) stage_flush(S, ns_, cs);  // S.n is settled here
@@ -2623,53 +2706,132 @@ __global__ void k_compose(i64 *C_orig, i64 n0, const i64 *C, bool first_phase) {
 // -------------------------------------------------------------------- coarsen
 // One wavefront per row: count / emit the entries i->j with C(i) >= C(j)
 // (buildNextPhase.cpp:157), keeping CSR order.
-__global__ __launch_bounds__(kBlock) void k_coarsen_count(const i64 *off, const i64 *tails, const i64 *C,
-                                                       i64 nv, i64 *cnt) {
-    const int lane = threadIdx.x % kWave;
-    for (i64 v = (i64)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < nv;
-         v += (i64)gridDim.x * (kBlock / kWave)) {
-        const i64 cv = C[v];
+// The contraction's entries: entry j of row v (a label cv >= 0) whose
+// neighbour's label ct is <= cv gives the pair (cv, ct) with weight w[j].
+// Entries are processed in chunks of kCoarsenChunk consecutive entries per
+// workgroup (so rows of any degree cost the same per entry): the rows a
+// chunk spans are found by binary search, their offsets and labels staged in
+// LDS (up to kCoarsenRows of them; more rows -- long runs of rows without
+// entries -- are searched in HBM), each entry finds its row in LDS.  One
+// pass counts each chunk's pairs; a second writes them, in entry order, at
+// the chunk's exclusive prefix.
+constexpr i64 kCoarsenChunk = 8192;
+constexpr int kCoarsenRows = 1024;
+struct ChunkRows {
+    i64 r0, nr;    // first row, rows spanned
+    bool staged;   // offsets and labels in LDS
+};
+// rows of entries [e0, e1) (nonempty): r0 = the row of e0, r1 = that of e1 - 1
+__device__ inline i64 row_of(const i64 *off, i64 lo, i64 hi, i64 j) {
+    // the largest r in [lo, hi] with off[r] <= j
+    while (lo < hi) {
+        const i64 mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ inline ChunkRows stage_rows(const i64 *off, const i64 *C, i64 nv, i64 e0, i64 e1, i64 *s_off,
+                                       i64 *s_c, i64 *s_r) {
+    if (threadIdx.x == 0) {
+        const i64 r0 = row_of(off, 0, nv - 1, e0);
+        s_r[0] = r0;
+        s_r[1] = row_of(off, r0, nv - 1, e1 - 1) - r0 + 1;
+    }
+    __syncthreads();
+    ChunkRows R{s_r[0], s_r[1], s_r[1] <= kCoarsenRows};
+    if (R.staged) {
+        for (i64 i = threadIdx.x; i <= R.nr; i += blockDim.x) s_off[i] = off[R.r0 + i];
+        for (i64 i = threadIdx.x; i < R.nr; i += blockDim.x) s_c[i] = C[R.r0 + i];
+    }
+    __syncthreads();
+    return R;
+}
+// the label of entry j's row
+__device__ inline i64 row_label(const ChunkRows &R, const i64 *s_off, const i64 *s_c, const i64 *off,
+                                const i64 *C, i64 j) {
+    if (R.staged) {
+        int lo = 0, hi = (int)R.nr - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= j) lo = mid;
+            else hi = mid - 1;
+        }
+        return s_c[lo];
+    }
+    return C[row_of(off, R.r0, R.r0 + R.nr - 1, j)];
+}
+__global__ __launch_bounds__(kBlock) void k_coarsen_count(const i64 *off, const i64 *tails, const i64 *C, i64 nv,
+                                                       i64 nnz, i64 *ccount) {
+    __shared__ i64 s_off[kCoarsenRows + 1], s_c[kCoarsenRows], s_r[2];
+    __shared__ i64 s_red[kBlock / kWave];
+    const i64 nchunks = (nnz + kCoarsenChunk - 1) / kCoarsenChunk;
+    for (i64 ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const i64 e0 = ch * kCoarsenChunk, e1 = e0 + kCoarsenChunk < nnz ? e0 + kCoarsenChunk : nnz;
+        const ChunkRows R = stage_rows(off, C, nv, e0, e1, s_off, s_c, s_r);
         i64 c = 0;
-        if (cv >= 0)
-            for (i64 j = off[v] + lane; j < off[v + 1]; j += kWave) c += (C[tails[j]] <= cv);
+        for (i64 j = e0 + threadIdx.x; j < e1; j += kBlock) {
+            const i64 cv = row_label(R, s_off, s_c, off, C, j);
+            if (cv >= 0) c += C[tails[j]] <= cv;
+        }
         for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor((long long)c, o);
-        if (lane == 0) cnt[v] = c;
+        if (threadIdx.x % kWave == 0) s_red[threadIdx.x / kWave] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            i64 t = 0;
+            for (int q = 0; q < kBlock / kWave; q++) t += s_red[q];
+            ccount[ch] = t;
+        }
+        __syncthreads();
     }
 }
 // nseeds: k (buildNextLevelGraphOpt: one zero self-loop per cluster) or 0
 // (buildNewGraphVF, vertexFollowing.cpp:127-131, which also sums weights
-// truncated to integers, :147-151: trunc).
-__global__ __launch_bounds__(kBlock) void k_coarsen_emit(const i64 *off, const i64 *tails,
-                                                      const double *w, const i64 *C, i64 nv,
-                                                      const i64 *pos, i64 k, int kb, u64 *keys,
-                                                      double *vals, i64 nseeds, bool trunc) {
-    const int lane = threadIdx.x % kWave;
+// truncated to integers, :147-151: trunc).  cbase: the chunks' exclusive
+// prefix of pair counts.
+__global__ __launch_bounds__(kBlock) void k_coarsen_emit(const i64 *off, const i64 *tails, const double *w,
+                                                      const i64 *C, i64 nv, i64 nnz, const i64 *cbase, int kb,
+                                                      u64 *keys, double *vals, i64 nseeds, bool trunc) {
+    __shared__ i64 s_off[kCoarsenRows + 1], s_c[kCoarsenRows], s_r[2];
+    __shared__ int s_wc[kBlock / kWave];
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+    const u64 lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
     // seeds: one zero self-loop per cluster (buildNextPhase.cpp:122)
     for (i64 c = blockIdx.x * (i64)kBlock + threadIdx.x; c < nseeds; c += (i64)gridDim.x * kBlock) {
         keys[c] = ((u64)c << kb) | (u64)c;
         vals[c] = 0.0;
     }
-    const u64 lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
-    for (i64 v = (i64)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < nv;
-         v += (i64)gridDim.x * (kBlock / kWave)) {
-        const i64 cv = C[v];
-        if (cv < 0) continue;
-        i64 base = nseeds + pos[v];
-        for (i64 j0 = off[v]; j0 < off[v + 1]; j0 += kWave) {
-            const i64 j = j0 + lane;
-            i64 ct = -1;
+    const i64 nchunks = (nnz + kCoarsenChunk - 1) / kCoarsenChunk;
+    for (i64 ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const i64 e0 = ch * kCoarsenChunk, e1 = e0 + kCoarsenChunk < nnz ? e0 + kCoarsenChunk : nnz;
+        const ChunkRows R = stage_rows(off, C, nv, e0, e1, s_off, s_c, s_r);
+        i64 base = nseeds + cbase[ch];
+        for (i64 j0 = e0; j0 < e1; j0 += kBlock) {
+            const i64 j = j0 + threadIdx.x;
+            i64 cv = -1, ct = 0;
             bool p = false;
-            if (j < off[v + 1]) {
-                ct = C[tails[j]];
-                p = ct <= cv;
+            if (j < e1) {
+                cv = row_label(R, s_off, s_c, off, C, j);
+                if (cv >= 0) {
+                    ct = C[tails[j]];
+                    p = ct <= cv;
+                }
             }
             const u64 m = __ballot(p);
+            if (lane == 0) s_wc[wv] = __popcll(m);
+            __syncthreads();
+            i64 before = 0, all = 0;
+            for (int q = 0; q < kBlock / kWave; q++) {
+                before += q < wv ? s_wc[q] : 0;
+                all += s_wc[q];
+            }
             if (p) {
-                const i64 at = base + __popcll(m & lt);
+                const i64 at = base + before + __popcll(m & lt);
                 keys[at] = ((u64)cv << kb) | (u64)ct;
                 vals[at] = trunc ? (double)(long)w[j] : w[j];
             }
-            base += __popcll(m);
+            base += all;
+            __syncthreads();  // s_wc is rewritten by the next pass
         }
     }
 }
@@ -3061,12 +3223,14 @@ void launch_mod_pass(int bin, const DevGraph &g, const i64 *C, const i64 *verts,
 }
 
 void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *heavy, i64 *heavy_count,
-                     hipStream_t s) {
+                     i64 *giant, i64 *giant_count, hipStream_t s) {
     if (g.nv <= 0) return;
     k_jp_count<<<grid_for(g.nv, kBlock, 2048), kBlock, 0, s>>>(g.off, g.tails, g.nv, cnt, L, heavy,
                                                                heavy_count);
     GLV_LAUNCH_CHECK();
-    k_jp_count_heavy<<<4096, kBlock, 0, s>>>(g.off, g.tails, cnt, heavy, heavy_count, L);
+    k_jp_count_heavy<<<4096, kBlock, 0, s>>>(g.off, g.tails, cnt, heavy, heavy_count, L, giant, giant_count);
+    GLV_LAUNCH_CHECK();
+    k_jp_count_giant<<<1024, kBlock, 0, s>>>(g.off, g.tails, cnt, giant, giant_count, L);
     GLV_LAUNCH_CHECK();
 }
 void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const JpLists &cur,
@@ -3176,19 +3340,20 @@ void launch_compose(i64 *C_orig, i64 n0, const i64 *C, bool first_phase, hipStre
     GLV_LAUNCH_CHECK();
 }
 
-void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *cnt, hipStream_t s) {
-    if (g.nv <= 0) return;
-    const i64 vpb = kBlock / kWave;
-    k_coarsen_count<<<(unsigned)std::min<i64>((g.nv + vpb - 1) / vpb, 16384), kBlock, 0, s>>>(
-        g.off, g.tails, C, g.nv, cnt);
+i64 coarsen_chunks(i64 nnz) { return (nnz + kCoarsenChunk - 1) / kCoarsenChunk; }
+void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *ccount, hipStream_t s) {
+    const i64 nch = coarsen_chunks(g.nnz);
+    if (g.nv <= 0 || nch <= 0) return;
+    k_coarsen_count<<<(unsigned)std::min<i64>(nch, 65536), kBlock, 0, s>>>(g.off, g.tails, C, g.nv, g.nnz,
+                                                                            ccount);
     GLV_LAUNCH_CHECK();
 }
-void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *pos, i64 k, int kb, u64 *keys,
+void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *cbase, i64 k, int kb, u64 *keys,
                          double *vals, hipStream_t s, bool vf) {
-    const i64 vpb = kBlock / kWave;
-    i64 blocks = std::max<i64>((g.nv + vpb - 1) / vpb, (k + kBlock - 1) / kBlock);
-    blocks = std::max<i64>(1, std::min<i64>(blocks, 16384));
-    k_coarsen_emit<<<(unsigned)blocks, kBlock, 0, s>>>(g.off, g.tails, g.w, C, g.nv, pos, k, kb, keys,
+    const i64 nch = coarsen_chunks(g.nnz);
+    i64 blocks = std::max<i64>(nch, vf ? 0 : (k + kBlock - 1) / kBlock);
+    blocks = std::max<i64>(1, std::min<i64>(blocks, 65536));
+    k_coarsen_emit<<<(unsigned)blocks, kBlock, 0, s>>>(g.off, g.tails, g.w, C, g.nv, g.nnz, cbase, kb, keys,
                                                        vals, vf ? 0 : k, vf);
     GLV_LAUNCH_CHECK();
 }

@@ -174,9 +174,9 @@ void launch_jp_prio(const DevGraph &g, u64 *prio, hipStream_t s);
 // count[v] = uncoloured non-core higher-priority neighbours of a non-core v;
 // count-0 vertices form the first frontier.  A core vertex v holds its core
 // index in count[v] on entry; its core neighbour counts go to L.core_cnt.
-// heavy: nv scratch, heavy_count: one zeroed word.
+// heavy, giant: nv scratch each, heavy_count, giant_count: one zeroed word each.
 void launch_jp_count(const DevGraph &g, int32_t *cnt, const JpLists &L, i64 *heavy, i64 *heavy_count,
-                     hipStream_t s);
+                     i64 *giant, i64 *giant_count, hipStream_t s);
 // color one frontier (n[0..2] vertices in cur's lists) and build the next
 // one into next (whose counts are zero); gmark: n[2] * (kMaxColors/32 + 1) words
 void launch_jp_frontier(const DevGraph &g, int32_t *colors, int32_t *cnt, const JpLists &cur,
@@ -243,9 +243,13 @@ void launch_relabel(i64 *C, i64 n, const i64 *first, const i64 *pos, hipStream_t
 void launch_compose(i64 *C_orig, i64 n0, const i64 *C, bool first_phase, hipStream_t s);
 
 // --- coarsen -------------------------------------------------------------
-void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *cnt, hipStream_t s);
+// The pairs (C[v], C[tail]) with C[tail] <= C[v] (C[v] >= 0) in entry order,
+// counted and written per chunk of consecutive entries: ccount[coarsen_chunks(nnz)]
+// the pairs of every chunk, cbase their exclusive prefix.
+i64 coarsen_chunks(i64 nnz);
+void launch_coarsen_count(const DevGraph &g, const i64 *C, i64 *ccount, hipStream_t s);
 // vf: no seeded self-loops and integer-truncated weights (buildNewGraphVF)
-void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *pos, i64 k, int kb,
+void launch_coarsen_emit(const DevGraph &g, const i64 *C, const i64 *cbase, i64 k, int kb,
                          u64 *keys, double *vals, hipStream_t s, bool vf = false);
 // nself (nullable, zeroed): += number of self-loop keys
 void launch_coarsen_mirror(const u64 *ukeys, const double *uvals, i64 nu, int kb, u64 *keys,

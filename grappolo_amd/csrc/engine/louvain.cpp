@@ -230,12 +230,14 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     i64 hc[3] = {0, 0, 0};  // sizes of the current frontier's small, large and giant lists
     launch_fill_i64(counts.p, 8, 0, s_);
     Profiler::get().begin(K_COLOR, s_);
-    launch_jp_count(g, cnt.p, cur, next.small, cur.counts + 3, s_);  // heavy list in the next round's buffer
+    // heavy and giant lists in the next round's buffers
+    launch_jp_count(g, cnt.p, cur, next.small, cur.counts + 3, next.large, next.counts + 3, s_);
     Profiler::get().end(K_COLOR, 0.0, s_);
     i64 core_entries = 0;
     if (K > 0) {
-        launch_jp_core_sizes(core_cnt.p, K, gA.p, s_);
-        scan_exclusive_i64(gA.p, loff.p, K + 1, s_);
+        DBuf<i64> sizes(K + 1, s_);  // (the giant lists hold the count's frontier)
+        launch_jp_core_sizes(core_cnt.p, K, sizes.p, s_);
+        scan_exclusive_i64(sizes.p, loff.p, K + 1, s_);
     }
     {
         Readback rb(s_);
@@ -989,15 +991,16 @@ void Engine::coarsen(const DevGraph &g, const i64 *C, i64 k, DevGraph &out, bool
     if (2 * kb > 64) fail(GLV_E_UNSUPPORTED, "more than 2^32 clusters");
     const i64 nv = g.nv;
     Profiler::get().begin(K_COARSEN, s_);
-    DBuf<i64> cnt(std::max<i64>(nv, 1), s_), pos(std::max<i64>(nv, 1), s_);
+    const i64 nch = coarsen_chunks(g.nnz);
+    DBuf<i64> cnt(std::max<i64>(nch, 1), s_), pos(std::max<i64>(nch, 1), s_);
     launch_coarsen_count(g, C, cnt.p, s_);
-    scan_exclusive_i64(cnt.p, pos.p, nv, s_);
+    scan_exclusive_i64(cnt.p, pos.p, nch, s_);
     i64 m = 0;
-    if (nv > 0) {
+    if (nch > 0) {
         i64 h[2];
         Readback rb(s_);
-        rb.add(&h[0], pos.p + nv - 1, sizeof(i64));
-        rb.add(&h[1], cnt.p + nv - 1, sizeof(i64));
+        rb.add(&h[0], pos.p + nch - 1, sizeof(i64));
+        rb.add(&h[1], cnt.p + nch - 1, sizeof(i64));
         rb.run();
         m = h[0] + h[1];
     }
