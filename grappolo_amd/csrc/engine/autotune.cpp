@@ -190,7 +190,7 @@ Tuning autotune_launch(Engine &E, const DevGraph &g, const RunParams &p, int rep
                 tb = t;
             }
         };
-        for (int pb : {16, 32, 64}) {
+        for (int pb : {16, 32, 64, 128, 256}) {
             Tuning c = best;
             if (pb == c.persist_blocks) continue;
             c.persist_blocks = pb;

@@ -40,13 +40,17 @@ constexpr int kBlockHashSlots = 4096;
 // 160 KB of LDS of a gfx950 CU.
 constexpr i64 kDenseMaxNV = 15360;
 constexpr int kDenseBlocks = 1024;
-constexpr i64 kPersistMaxBlocks = 64;  // grid of the persistent sweep
+// grid of the persistent sweep: at most one workgroup per CU of the dense
+// form, and at most kPersistMaxBlocksHash for the hash form, whose
+// workgroups each keep a dense row of the graph's communities in HBM
+constexpr i64 kPersistMaxBlocks = 256;
+constexpr i64 kPersistMaxBlocksHash = 64;
 // The persistent sweep serves graphs of at most this many vertices whose
 // color classes hold at most kPersistMaxPerColor vertices on average (coarse
 // phases: many small classes, where per-color launches would be latency-bound).
 constexpr i64 kPersistMaxNV = 1 << 18;
 constexpr i64 kPersistMaxPerColor = 256;
-constexpr int kBarrierWords = 16 + 16 * 64;  // abort word + one 64-byte flag line per workgroup
+constexpr int kBarrierWords = 16 + 16 * (int)kPersistMaxBlocks;  // abort word + one 64-byte flag line per workgroup
 // The coloring of a graph of more than 16384 vertices takes a core (its
 // highest degrees, coloured by one workgroup) only when the core starts at
 // this degree or above (hubs, whose priority chains make the rounds deep);

@@ -67,31 +67,29 @@ __device__ double block_sum(double v) {
     return r;
 }
 
-// Block-wide argmax, result valid in every thread.
+// Block-wide argmax, result valid in every thread (a block without a
+// candidate leaves each thread's gain and id as they were).
 __device__ void block_argmax(bool &cand, double &gain, i64 &id) {
-    __shared__ int sc[kBlock / kWave];
-    __shared__ double sg[kBlock / kWave];
+    __shared__ u64 sk[kBlock / kWave];
     __shared__ i64 si[kBlock / kWave];
-    group_argmax<kWave>(cand, gain, id);
+    u64 key = argmax_key(cand, gain);
+    i64 rid = id, none = 0;
+    group_argmax_key<kWave>(key, rid, none);
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
     if (l == 0) {
-        sc[w] = cand;
-        sg[w] = gain;
-        si[w] = id;
+        sk[w] = key;
+        si[w] = rid;
     }
     __syncthreads();
-    bool c = sc[0] != 0;
-    double g = sg[0];
-    i64 d = si[0];
-    for (int q = 1; q < kBlock / kWave; q++)
-        if (better(sc[q] != 0, sg[q], si[q], c, g, d)) {
-            c = true;
-            g = sg[q];
-            d = si[q];
-        }
-    cand = c;
-    gain = g;
-    id = d;
+    // every wavefront reduces the wavefronts' results (lane l takes wavefront l % kBlock/kWave)
+    key = sk[l % (kBlock / kWave)];
+    rid = si[l % (kBlock / kWave)];
+    group_argmax_key<kBlock / kWave>(key, rid, none);
+    cand = key != 0;
+    if (cand) {
+        gain = key_gain(key);
+        id = rid;
+    }
     __syncthreads();
 }
 
@@ -848,69 +846,54 @@ __device__ inline double block_sum_p(double v, double *scratch) {
     return r;
 }
 
-__device__ inline void block_argmax_p(bool &cand, double &gain, i64 &id, int *sc, double *sg, i64 *si) {
-    group_argmax<kWave>(cand, gain, id);
+// Block-wide argmax of the persistent sweeps (kPBlock threads; sk and si
+// hold the wavefronts' keys and ids), result valid in every thread.
+__device__ inline void block_argmax_p(bool &cand, double &gain, i64 &id, u64 *sk, i64 *si) {
+    u64 key = argmax_key(cand, gain);
+    i64 rid = id, none = 0;
+    group_argmax_key<kWave>(key, rid, none);
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
     __syncthreads();
     if (l == 0) {
-        sc[w] = cand;
-        sg[w] = gain;
-        si[w] = id;
+        sk[w] = key;
+        si[w] = rid;
     }
     __syncthreads();
-    bool c = sc[0] != 0;
-    double g = sg[0];
-    i64 d = si[0];
-    for (int q = 1; q < kPBlock / kWave; q++)
-        if (better(sc[q] != 0, sg[q], si[q], c, g, d)) {
-            c = true;
-            g = sg[q];
-            d = si[q];
-        }
-    cand = c;
-    gain = g;
-    id = d;
+    // every wavefront reduces the wavefronts' results (lane l takes wavefront l % kPBlock/kWave)
+    key = sk[l % (kPBlock / kWave)];
+    rid = si[l % (kPBlock / kWave)];
+    group_argmax_key<kPBlock / kWave>(key, rid, none);
+    cand = key != 0;
+    if (cand) {
+        gain = key_gain(key);
+        id = rid;
+    }
 }
 
 // block_argmax_p that also carries `nid` (the size of the candidate community)
-__device__ inline void block_argmax_pn(bool &cand, double &gain, i64 &id, i64 &nid, int *sc, double *sg, i64 *si,
-                                       i64 *sn) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const double og = __shfl_xor(gain, o);
-        const long long oid = __shfl_xor((long long)id, o);
-        const long long on = __shfl_xor((long long)nid, o);
-        const int ocd = __shfl_xor((int)cand, o);
-        if (better(ocd != 0, og, oid, cand, gain, id)) {
-            cand = true;
-            gain = og;
-            id = oid;
-            nid = on;
-        }
-    }
+__device__ inline void block_argmax_pn(bool &cand, double &gain, i64 &id, i64 &nid, u64 *sk, i64 *si, i64 *sn) {
+    u64 key = argmax_key(cand, gain);
+    i64 rid = id, rn = nid;
+    group_argmax_key<kWave, true>(key, rid, rn);
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
     __syncthreads();
     if (l == 0) {
-        sc[w] = cand;
-        sg[w] = gain;
-        si[w] = id;
-        sn[w] = nid;
+        sk[w] = key;
+        si[w] = rid;
+        sn[w] = rn;
     }
     __syncthreads();
-    bool c = sc[0] != 0;
-    double g = sg[0];
-    i64 d = si[0], nn = sn[0];
-    for (int q = 1; q < kPBlock / kWave; q++)
-        if (better(sc[q] != 0, sg[q], si[q], c, g, d)) {
-            c = true;
-            g = sg[q];
-            d = si[q];
-            nn = sn[q];
-        }
-    cand = c;
-    gain = g;
-    id = d;
-    nid = nn;
+    // every wavefront reduces the wavefronts' results (lane l takes wavefront l % kPBlock/kWave)
+    key = sk[l % (kPBlock / kWave)];
+    rid = si[l % (kPBlock / kWave)];
+    rn = sn[l % (kPBlock / kWave)];
+    group_argmax_key<kPBlock / kWave, true>(key, rid, rn);
+    cand = key != 0;
+    if (cand) {
+        gain = key_gain(key);
+        id = rid;
+        nid = rn;
+    }
 }
 
 // LDS of the persistent sweep: either the dense accumulator of graphs with at
@@ -946,8 +929,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
                                                                i64 *n1) {
     __shared__ __align__(16) unsigned char smem[kPSmemBytes];
     __shared__ double red[kPBlock / kWave];
-    __shared__ int sc[kPBlock / kWave];
-    __shared__ double sg[kPBlock / kWave];
+    __shared__ u64 sk[kPBlock / kWave];
     __shared__ i64 si[kPBlock / kWave];
     __shared__ int s_cnt;     // communities listed for the current vertex (dense view)
     __shared__ double s_sl;   // its self-loop weight
@@ -1213,7 +1195,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
             }
             const bool my_cand = cand;
             const i64 my_id = id;
-            block_argmax_p(cand, gain, id, sc, sg, si);
+            block_argmax_p(cand, gain, id, sk, si);
             if (A.feat) block_top2<kPBlock>(t1, t2);
             if (!HASH) {  // the size of the winner, from the lane that scored it
                 if (cand && my_cand && my_id == id) s_nid = nid;
@@ -1344,23 +1326,27 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration(SweepArgs A, i64 
 // labels again and adds them, so its accumulator holds exactly the entries a
 // gather after the barrier would have added (the sum order of LDS atomics is
 // unordered either way).  State, buffers and the modularity partials as in
-// k_colored_iteration.  colors: the class of every vertex.
+// k_colored_iteration.  Gathers read the labels from 16-bit copies (L16:
+// 2 nv, the copies of C and of C1; ids are below kDenseMaxNV), 30 KB that
+// stay in a CU's L1 instead of a 128-byte line of L2 per entry, and the
+// gathering half tests membership in step k's class with an LDS bitmap.
 constexpr int kSpecFix = 256;  // set-aside entries per vertex; more: gather after the barrier
+constexpr int kSpecUnroll = 4;  // entries in flight per thread when gathering
 __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A, i64 *C, double *a, i64 *n,
                                                                     const i64 *cperm, const i64 *cstart,
                                                                     int ncol, unsigned *bar, double *part,
                                                                     uint8_t *stale, i64 *C1, double *a1,
-                                                                    i64 *n1, const int32_t *colors) {
+                                                                    i64 *n1, uint16_t *L16) {
     __shared__ __align__(16) unsigned char smem[kPDenseBytes];
     __shared__ double red[kPBlock / kWave];
-    __shared__ int sc[kPBlock / kWave];
-    __shared__ double sg[kPBlock / kWave];
+    __shared__ u64 sk[kPBlock / kWave];
     __shared__ i64 si[kPBlock / kWave];
     __shared__ i64 sn[kPBlock / kWave];
     __shared__ int s_cnt;                // communities listed for the gathered vertex
     __shared__ double s_sl;              // its self-loop weight
     __shared__ int s_fn;                 // set-aside entries (above kSpecFix: too many)
     __shared__ uint16_t s_ft[kSpecFix];  // their neighbours
+    __shared__ uint32_t cbits[kDenseMaxNV / 32];  // gathering half: the vertices of step k's class
     __shared__ double s_fw[kSpecFix];    // and weights
     double *acc = reinterpret_cast<double *>(smem);
     uint32_t *touched = reinterpret_cast<uint32_t *>(smem + kDenseMaxNV * 8);
@@ -1378,11 +1364,15 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
         s_fn = 0;
     }
     for (int i = threadIdx.x; i < nv; i += kPBlock) acc[i] = 0.0;
-    for (int i = threadIdx.x; i < nwords; i += kPBlock) touched[i] = 0;
+    for (int i = threadIdx.x; i < nwords; i += kPBlock) {
+        touched[i] = 0;
+        cbits[i] = 0;
+    }
     for (i64 i = (i64)blockIdx.x * kPBlock + threadIdx.x; i < nv; i += (i64)nb * kPBlock) {
         C1[i] = C[i];
         a1[i] = a[i];
         n1[i] = n[i];
+        L16[i] = L16[nv + i] = (uint16_t)C[i];
     }
     // community cc gets weight ww; the first lane to touch it lists it (one LDS
     // counter update per wavefront); called by whole wavefronts
@@ -1433,24 +1423,152 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
         i64 *Cn = par ? C : C1;
         double *an = par ? a : a1;
         i64 *nn = par ? n : n1;
+        const uint16_t *Lr = L16 + (par ? nv : 0);  // the labels of A.C
+        uint16_t *Lw = L16 + (par ? 0 : nv);        // and of Cn
         const bool deciding = half == (unsigned)par;
-        if (!deciding) {
-            // this workgroup decided the previous step: its moves, into the other buffer
-            if (threadIdx.x == 0) {
-                if (mv_v >= 0 && mv_t != mv_s) {
-                    move_vertex(mv_t, mv_s, mv_k, an, nn);
-                    Cn[mv_v] = mv_t;
+        // decides vertex q of class c (into the other buffer); the first
+        // vertex of a deciding workgroup (q == wid) may be gathered ahead
+        auto decide = [&](i64 q, bool first) {
+            const bool sp = first && sp_c == c;
+            const i64 v = sp ? sp_v : cperm[lo + q];
+            const i64 beg = sp ? sp_beg : A.off[v], end = sp ? sp_end : A.off[v + 1];
+            const i64 s = sp ? sp_s : A.C[v];
+            if (s < 0) {  // a vertex without entries (label -1) stays where it is
+                if (threadIdx.x == 0) A.tgt[v] = s;
+                return;
+            }
+            const double as = A.a[s];
+            i64 ns = 0;
+            if (threadIdx.x == 0) ns = A.n[s];
+            if (sp) {
+                // the set-aside entries with the labels after the previous step
+                const int nf = s_fn;
+                for (int i0 = 0; i0 < nf; i0 += kPBlock) {
+                    const int i = i0 + threadIdx.x;
+                    const bool valid = i < nf;
+                    const i64 t = valid ? (i64)s_ft[i] : 0;
+                    const double ww = valid ? s_fw[i] : 0.0;
+                    const i64 cc = valid ? (i64)Lr[t] : 0;
+                    insert(valid, cc, ww);
                 }
-                for (i64 q = (i64)wid + nbh; q < ptot; q += nbh) {
-                    const i64 v = cperm[plo + q];
-                    const i64 t = A.tgt[v], s = Cn[v];
-                    if (t != s) {
-                        move_vertex(t, s, A.k[v], an, nn);
-                        Cn[v] = t;
+            } else {
+                for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
+                    i64 tt[kGatherUnroll], cl[kGatherUnroll];
+                    double wt[kGatherUnroll];
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const i64 j = j0 + (i64)u * kPBlock;
+                        tt[u] = j < end ? A.tails[j] : -1;
+                        wt[u] = j < end ? A.w[j] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) cl[u] = tt[u] >= 0 ? (i64)Lr[tt[u]] : 0;
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const bool valid = tt[u] >= 0;
+                        if (valid && tt[u] == v) atomicAdd(&s_sl, wt[u]);
+                        insert(valid, cl[u], wt[u]);
                     }
                 }
             }
+            __syncthreads();
+            const double sl = s_sl;
+            const double kv = sp ? sp_k : A.k[v];
+            const double ax = as - kv;
+            const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
+            const int ntouched = s_cnt;
+            const double e_s = acc[s];
+            const double eix = e_s - sl;
+            bool cand = false;
+            double gain = 0;
+            i64 id = INT64_MAX;
+            i64 nid = 0;
+            float t1 = -INFINITY, t2 = -INFINITY;
+            for (int k0 = threadIdx.x; k0 < ntouched; k0 += kGatherUnroll * kPBlock) {
+                i64 yy[kGatherUnroll], nyy[kGatherUnroll];
+                double ay[kGatherUnroll];
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    const int k = k0 + u * kPBlock;
+                    yy[u] = k < ntouched ? (i64)list[k] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    ay[u] = yy[u] >= 0 ? A.a[yy[u]] : 0.0;
+                    nyy[u] = yy[u] >= 0 ? A.n[yy[u]] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    const i64 y = yy[u];
+                    if (y < 0 || y == s) continue;
+                    const double g = gain_of(acc[y], eix, kv, ay[u], ax, A.constant);
+                    if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
+                    if (better(g > 0, g, y, cand, gain, id)) {
+                        cand = true;
+                        gain = g;
+                        id = y;
+                        nid = nyy[u];
+                    }
+                }
+            }
+            block_argmax_pn(cand, gain, id, nid, sk, si, sn);
+            if (A.feat) block_top2<kPBlock>(t1, t2);
+            if (threadIdx.x == 0) {
+                i64 target = s;
+                if (cand) {
+                    target = id;
+                    if (ns == 1 && target > s && nid == 1) target = s;
+                }
+                A.tgt[v] = target;
+                if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
+                if (target != s) {  // into the other buffer, read from the next step on
+                    move_vertex(target, s, kv, an, nn);
+                    Cn[v] = target;
+                    Lw[v] = (uint16_t)target;
+                    if (stale) stale[v] = 0;
+                }
+                if (first) {
+                    mv_v = v;
+                    mv_t = target;
+                    mv_s = s;
+                    mv_k = kv;
+                }
+            }
+            __syncthreads();  // every read of the accumulator is done
+            clear(ntouched);
+            __syncthreads();
+        };
+        if (!deciding) {
+            // this workgroup decided the previous step: its moves, into the other buffer
+            if (threadIdx.x == 0 && mv_v >= 0 && mv_t != mv_s) {
+                move_vertex(mv_t, mv_s, mv_k, an, nn);
+                Cn[mv_v] = mv_t;
+                Lw[mv_v] = (uint16_t)mv_t;
+            }
+            // and the previous class's other vertices (q >= nbh, whoever decided them), one per thread
+            for (i64 q = (i64)wid + nbh * (1 + (i64)threadIdx.x); q < ptot; q += nbh * (i64)kPBlock) {
+                const i64 v = cperm[plo + q];
+                const i64 t = A.tgt[v], s = Cn[v];
+                if (t != s) {
+                    move_vertex(t, s, A.k[v], an, nn);
+                    Cn[v] = t;
+                    Lw[v] = (uint16_t)t;
+                }
+            }
             mv_v = -1;
+        }
+        // this step's vertices: the deciding half takes q = wid (its first,
+        // maybe gathered ahead) and q = nbh + wid + k nb, the other half (in
+        // classes larger than a half) q = nb + wid + k nb
+        {
+            bool first = deciding;
+            for (i64 q = deciding ? (i64)wid : (i64)nb + wid; q < total;) {
+                decide(q, first);
+                q = first ? (i64)nbh + wid : q + nb;
+                first = false;
+            }
+        }
+        if (!deciding) {
             // gather ahead: this workgroup's first vertex of the next non-empty class
             int c2 = c + 1;
             i64 lo2 = hi, hi2 = hi;
@@ -1466,26 +1584,28 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
                 const i64 beg = A.off[v], end = A.off[v + 1];
                 const i64 s = A.C[v];  // changes only in v's own step
                 if (s >= 0) {
-                    for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
-                        i64 tt[kGatherUnroll], cl[kGatherUnroll];
-                        double wt[kGatherUnroll];
-                        int32_t co[kGatherUnroll];
+                    // this step's class, whose labels change, into the LDS bitmap
+                    for (i64 q = threadIdx.x; q < total; q += kPBlock) {
+                        const i64 u = cperm[lo + q];
+                        atomicOr(&cbits[u >> 5], 1u << (u & 31));
+                    }
+                    __syncthreads();
+                    for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kSpecUnroll * kPBlock) {
+                        int32_t tt[kSpecUnroll], cl[kSpecUnroll];
+                        double wt[kSpecUnroll];
 #pragma unroll
-                        for (int u = 0; u < kGatherUnroll; u++) {
+                        for (int u = 0; u < kSpecUnroll; u++) {
                             const i64 j = j0 + (i64)u * kPBlock;
-                            tt[u] = j < end ? A.tails[j] : -1;
+                            tt[u] = j < end ? (int32_t)A.tails[j] : -1;
                             wt[u] = j < end ? A.w[j] : 0.0;
                         }
 #pragma unroll
-                        for (int u = 0; u < kGatherUnroll; u++) {
-                            cl[u] = tt[u] >= 0 ? A.C[tt[u]] : 0;
-                            co[u] = tt[u] >= 0 ? colors[tt[u]] : -1;
-                        }
+                        for (int u = 0; u < kSpecUnroll; u++) cl[u] = tt[u] >= 0 ? (int32_t)Lr[tt[u]] : 0;
 #pragma unroll
-                        for (int u = 0; u < kGatherUnroll; u++) {
+                        for (int u = 0; u < kSpecUnroll; u++) {
                             const bool valid = tt[u] >= 0;
                             if (valid && tt[u] == v) atomicAdd(&s_sl, wt[u]);
-                            const bool aside = valid && co[u] == c && tt[u] != v;
+                            const bool aside = valid && tt[u] != v && ((cbits[tt[u] >> 5] >> (tt[u] & 31)) & 1u);
                             if (aside) {
                                 const int at = atomicAdd(&s_fn, 1);
                                 if (at < kSpecFix) {
@@ -1499,6 +1619,7 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
                     __syncthreads();
                     // every wavefront reads the counts before thread 0 may reset them in clear()
                     const int fn = s_fn, cn = s_cnt;
+                    for (int i = threadIdx.x; i < nwords; i += kPBlock) cbits[i] = 0;
                     __syncthreads();
                     if (fn > kSpecFix) {  // too many set aside: gather again after the barrier
                         clear(cn);
@@ -1514,116 +1635,6 @@ __global__ __launch_bounds__(kPBlock) void k_colored_iteration_spec(SweepArgs A,
                 }
             }
         } else {
-            for (i64 q = wid; q < total; q += nbh) {
-                const bool first = q == (i64)wid;
-                const bool sp = first && sp_c == c;
-                const i64 v = sp ? sp_v : cperm[lo + q];
-                const i64 beg = sp ? sp_beg : A.off[v], end = sp ? sp_end : A.off[v + 1];
-                const i64 s = sp ? sp_s : A.C[v];
-                if (s < 0) {  // a vertex without entries (label -1) stays where it is
-                    if (threadIdx.x == 0) A.tgt[v] = s;
-                    continue;
-                }
-                const double as = A.a[s];
-                i64 ns = 0;
-                if (threadIdx.x == 0) ns = A.n[s];
-                if (sp) {
-                    // the set-aside entries with the labels after the previous step
-                    const int nf = s_fn;
-                    for (int i0 = 0; i0 < nf; i0 += kPBlock) {
-                        const int i = i0 + threadIdx.x;
-                        const bool valid = i < nf;
-                        const i64 t = valid ? (i64)s_ft[i] : 0;
-                        const double ww = valid ? s_fw[i] : 0.0;
-                        const i64 cc = valid ? A.C[t] : 0;
-                        insert(valid, cc, ww);
-                    }
-                } else {
-                    for (i64 j0 = beg + threadIdx.x; j0 < end; j0 += kGatherUnroll * kPBlock) {
-                        i64 tt[kGatherUnroll], cl[kGatherUnroll];
-                        double wt[kGatherUnroll];
-#pragma unroll
-                        for (int u = 0; u < kGatherUnroll; u++) {
-                            const i64 j = j0 + (i64)u * kPBlock;
-                            tt[u] = j < end ? A.tails[j] : -1;
-                            wt[u] = j < end ? A.w[j] : 0.0;
-                        }
-#pragma unroll
-                        for (int u = 0; u < kGatherUnroll; u++) cl[u] = tt[u] >= 0 ? A.C[tt[u]] : 0;
-#pragma unroll
-                        for (int u = 0; u < kGatherUnroll; u++) {
-                            const bool valid = tt[u] >= 0;
-                            if (valid && tt[u] == v) atomicAdd(&s_sl, wt[u]);
-                            insert(valid, cl[u], wt[u]);
-                        }
-                    }
-                }
-                __syncthreads();
-                const double sl = s_sl;
-                const double kv = sp ? sp_k : A.k[v];
-                const double ax = as - kv;
-                const double inv2k = A.feat ? 1.0 / (2.0 * kv) : 0.0;
-                const int ntouched = s_cnt;
-                const double e_s = acc[s];
-                const double eix = e_s - sl;
-                bool cand = false;
-                double gain = 0;
-                i64 id = INT64_MAX;
-                i64 nid = 0;
-                float t1 = -INFINITY, t2 = -INFINITY;
-                for (int k0 = threadIdx.x; k0 < ntouched; k0 += kGatherUnroll * kPBlock) {
-                    i64 yy[kGatherUnroll], nyy[kGatherUnroll];
-                    double ay[kGatherUnroll];
-#pragma unroll
-                    for (int u = 0; u < kGatherUnroll; u++) {
-                        const int k = k0 + u * kPBlock;
-                        yy[u] = k < ntouched ? (i64)list[k] : -1;
-                    }
-#pragma unroll
-                    for (int u = 0; u < kGatherUnroll; u++) {
-                        ay[u] = yy[u] >= 0 ? A.a[yy[u]] : 0.0;
-                        nyy[u] = yy[u] >= 0 ? A.n[yy[u]] : 0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < kGatherUnroll; u++) {
-                        const i64 y = yy[u];
-                        if (y < 0 || y == s) continue;
-                        const double g = gain_of(acc[y], eix, kv, ay[u], ax, A.constant);
-                        if (A.feat) top2_push(t1, t2, (float)(g * inv2k));
-                        if (better(g > 0, g, y, cand, gain, id)) {
-                            cand = true;
-                            gain = g;
-                            id = y;
-                            nid = nyy[u];
-                        }
-                    }
-                }
-                block_argmax_pn(cand, gain, id, nid, sc, sg, si, sn);
-                if (A.feat) block_top2<kPBlock>(t1, t2);
-                if (threadIdx.x == 0) {
-                    i64 target = s;
-                    if (cand) {
-                        target = id;
-                        if (ns == 1 && target > s && nid == 1) target = s;
-                    }
-                    A.tgt[v] = target;
-                    if (A.feat) A.feat[v] = las_record(e_s, kv, t1, t2);
-                    if (target != s) {  // into the other buffer, read from the next step on
-                        move_vertex(target, s, kv, an, nn);
-                        Cn[v] = target;
-                        if (stale) stale[v] = 0;
-                    }
-                    if (first) {
-                        mv_v = v;
-                        mv_t = target;
-                        mv_s = s;
-                        mv_k = kv;
-                    }
-                }
-                __syncthreads();  // every read of the accumulator is done
-                clear(ntouched);
-                __syncthreads();
-            }
             sp_c = -1;
         }
         alive = grid_barrier(bar, nb, ++seq);
@@ -3130,9 +3141,10 @@ int persistent_blocks() {
 }
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
                               unsigned *bar, double *part, int nblocks, hipStream_t s, uint8_t *stale,
-                              i64 *C1, double *a1, i64 *n1, const int32_t *colors) {
-    if (nblocks <= 0 || nblocks > kPersistMaxBlocks || A.nv > kPersistMaxNV) return false;
+                              i64 *C1, double *a1, i64 *n1, uint16_t *L16) {
     const bool hash = A.nv > kDenseMaxNV;
+    if (nblocks <= 0 || nblocks > (hash ? kPersistMaxBlocksHash : kPersistMaxBlocks) || A.nv > kPersistMaxNV)
+        return false;
     // degrees above the LDS table's reach need the workgroup's dense HBM row
     if (hash && A.dacc == nullptr) return false;
     SweepArgs aa = A;
@@ -3148,9 +3160,9 @@ bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *st
     i64 *c1 = C1;
     double *ca1 = a1;
     i64 *cn1 = n1;
-    const int32_t *co = colors;
-    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt, &sf, &c1, &ca1, &cn1, &co};
-    const bool spec = !hash && colors && nblocks >= 2 && nblocks % 2 == 0;
+    uint16_t *l16 = L16;
+    void *args[] = {&aa, &C, &ca, &cn, &p, &st, &nc, &b, &pt, &sf, &c1, &ca1, &cn1, &l16};
+    const bool spec = !hash && L16 && nblocks >= 2 && nblocks % 2 == 0;
     const void *fn = spec ? (const void *)k_colored_iteration_spec
                           : hash ? (const void *)k_colored_iteration<true> : (const void *)k_colored_iteration<false>;
     return launch_resident(fn, nblocks, kPBlock, args, persistent_blocks(), s, "persistent colored iteration");

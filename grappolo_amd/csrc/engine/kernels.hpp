@@ -99,11 +99,12 @@ int persistent_blocks();
 void launch_color_major(const i64 *perm, const int32_t *colors, i64 n, int ncol, i64 *cperm,
                         i64 *cstart, hipStream_t s);
 // C1/a1/n1: second buffers of the labels, community degrees and sizes (nv each);
-// colors: the class of every vertex (graphs of at most kDenseMaxNV vertices
-// on an even grid take the variant whose halves alternate)
+// L16 (nullable; 2 nv): scratch of 16-bit label copies, given for graphs of
+// at most kDenseMaxNV vertices on an even grid, which take the variant whose
+// halves alternate
 bool launch_colored_iteration(const SweepArgs &A, const i64 *perm, const i64 *start, int ncol,
                               unsigned *bar, double *part, int nblocks, hipStream_t s, uint8_t *stale,
-                              i64 *C1, double *a1, i64 *n1, const int32_t *colors);
+                              i64 *C1, double *a1, i64 *n1, uint16_t *L16);
 void launch_isolated_targets(const i64 *verts, i64 count, i64 *tgt, hipStream_t s);
 // stale (nullable): reset to 0 for every mover; dsum (nullable): += sum of
 // dein over the movers, in a fixed order (dpart: kApplyMaxBlocks scratch)

@@ -425,9 +425,13 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     DBuf<double> part(mod_part_count(P), s_), res(2, s_);
 
     std::vector<char> color_nonempty(ncol, 0);
-    for (int c = 0; c < ncol; c++)
-        for (int b = 1; b < kNumBins; b++)
-            if (P.hi(b, c) > P.lo(b, c)) color_nonempty[c] = 1;
+    i64 max_class = 0;
+    for (int c = 0; c < ncol; c++) {
+        i64 size = 0;
+        for (int b = 1; b < kNumBins; b++) size += P.hi(b, c) - P.lo(b, c);
+        color_nonempty[c] = size > 0;
+        max_class = std::max(max_class, size);
+    }
     // coarse graphs (small, or many small color classes): one persistent
     // launch per iteration (all colors + modularity), grid sized to the color
     // classes: every extra workgroup only lengthens the barriers
@@ -439,11 +443,15 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
         if (nv <= kDenseMaxNV || per_class <= kPersistMaxPerColor) {
             pblocks = persistent_blocks();
             // dense graphs: the grid's two halves take the color steps in turn
-            // (an even count, twice the class size per half)
-            const i64 want = nv <= kDenseMaxNV ? 4 * per_class : 2 * per_class;
+            // (an even count; twice the mean class size per half, or a
+            // workgroup per vertex of the largest class, whose step is a
+            // chain of ceil(size / half) vertices otherwise)
+            const bool dense = nv <= kDenseMaxNV;
+            const i64 want = dense ? std::max(4 * per_class, 2 * max_class) : 2 * per_class;
             if (pblocks > 0) {
-                pblocks = (int)std::min<i64>({(i64)pblocks, (i64)tune_.persist_blocks, std::max<i64>(4, want)});
-                if (nv <= kDenseMaxNV) pblocks = std::max(2, pblocks & ~1);
+                pblocks = (int)std::min<i64>({(i64)pblocks, (i64)tune_.persist_blocks, std::max<i64>(4, want),
+                                              dense ? kPersistMaxBlocks : kPersistMaxBlocksHash});
+                if (dense) pblocks = std::max(2, pblocks & ~1);
             }
         }
     }
@@ -500,6 +508,7 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
     // second state buffers of the persistent sweep (one grid barrier per color step)
     DBuf<i64> pC1(pblocks > 0 ? nv : 0, s_), pn1(pblocks > 0 ? nv : 0, s_);
     DBuf<double> pa1(pblocks > 0 ? nv : 0, s_);
+    DBuf<uint16_t> pl16(pblocks >= 2 && nv <= kDenseMaxNV ? 2 * nv : 0, s_);  // the dense form's 16-bit labels
     if (dbl) launch_color_major(P.perm.p, colors, nv, ncol, cperm.p, cstart.p, s_);
     bool persistent = pblocks > 0;
     WidePlan WP;
@@ -695,7 +704,8 @@ double Engine::colored_phase(const DevGraph &g, i64 *C, const int32_t *colors, i
             launch_fill_i32((int32_t *)bar.p, kBarrierWords, 0, s_);
             Profiler::get().begin(K_PERSIST, s_);
             persistent = launch_colored_iteration(A, plist, pstart, ncol, bar.p, ppart.p, pblocks, s_,
-                                                  mode ? stale.p : nullptr, pC1.p, pa1.p, pn1.p, colors);
+                                                  mode ? stale.p : nullptr, pC1.p, pa1.p, pn1.p,
+                                                  pl16.n ? pl16.p : nullptr);
             Profiler::get().end(K_PERSIST, 24.0 * g.nnz + 32.0 * nv, s_);
             if (persistent) {
                 launch_sum_partials(ppart.p, pblocks, res.p, s_);

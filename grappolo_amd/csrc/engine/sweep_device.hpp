@@ -39,18 +39,77 @@ __device__ inline bool better(bool oc, double og, i64 oid, bool c, double g, i64
     return oc && (!c || og > g || (og == g && oid < id));
 }
 
+// A candidate (cand, gain) as one unsigned key whose order is better()'s
+// order on gains: 0 for no candidate, otherwise the gain's bits mapped so
+// that unsigned order is the doubles' order (-0 taken as +0; never 0 for a
+// non-NaN gain).
+__device__ inline u64 argmax_key(bool cand, double gain) {
+    const u64 b = (u64)__double_as_longlong(gain + 0.0);
+    return cand ? ((b >> 63) ? ~b : (b | (1ULL << 63))) : 0;
+}
+__device__ inline double key_gain(u64 k) {
+    return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ULL << 63)) : ~k));
+}
+
+// 64-bit value of another lane in the same row of 16 (DPP control CTRL)
+template <int CTRL>
+__device__ inline u64 dpp64(u64 x) {
+    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)x, CTRL, 0xf, 0xf, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(x >> 32), CTRL, 0xf, 0xf, false);
+    return ((u64)hi << 32) | lo;
+}
+template <int CTRL, bool N>
+__device__ inline void argmax_dpp(u64 &key, i64 &id, i64 &nid) {
+    const u64 ok = dpp64<CTRL>(key);
+    const i64 oid = (i64)dpp64<CTRL>((u64)id);
+    const bool t = ok > key || (ok == key && oid < id);
+    key = t ? ok : key;
+    id = t ? oid : id;
+    if (N) {
+        const i64 on = (i64)dpp64<CTRL>((u64)nid);
+        nid = t ? on : nid;
+    }
+}
+template <bool N>
+__device__ inline void argmax_xor(u64 &key, i64 &id, i64 &nid, int o) {
+    const u64 ok = (u64)__shfl_xor((long long)key, o);
+    const i64 oid = __shfl_xor((long long)id, o);
+    const bool t = ok > key || (ok == key && oid < id);
+    key = t ? ok : key;
+    id = t ? oid : id;
+    if (N) {
+        const i64 on = __shfl_xor((long long)nid, o);
+        nid = t ? on : nid;
+    }
+}
+
+// Maximum of (key, id) over aligned groups of W lanes (W a power of two up
+// to 64), with `nid` following the winner when N; the result in every lane
+// of the group.  Within a row of 16 lanes by DPP (lane pairs i^1 and i^2 by
+// quad permutes, then the half-row and row mirrors pair each lane with the
+// other half of its group of 8 and of 16), across rows by ds_bpermute.  It is
+// the maximum of a total order, so the pairing does not change the result.
+template <int W, bool N = false>
+__device__ inline void group_argmax_key(u64 &key, i64 &id, i64 &nid) {
+    if (W >= 2) argmax_dpp<0xB1, N>(key, id, nid);   // quad_perm [1,0,3,2]
+    if (W >= 4) argmax_dpp<0x4E, N>(key, id, nid);   // quad_perm [2,3,0,1]
+    if (W >= 8) argmax_dpp<0x141, N>(key, id, nid);  // row_half_mirror
+    if (W >= 16) argmax_dpp<0x140, N>(key, id, nid); // row_mirror
+    if (W >= 32) argmax_xor<N>(key, id, nid, 16);
+    if (W >= 64) argmax_xor<N>(key, id, nid, 32);
+}
+
+// better()'s maximum over aligned groups of W lanes, in every lane of the
+// group; a group without a candidate leaves each lane's (gain, id) as it was.
 template <int W>
 __device__ inline void group_argmax(bool &cand, double &gain, i64 &id) {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) {
-        double og = __shfl_xor(gain, o, W);
-        long long oid = __shfl_xor((long long)id, o, W);
-        int ocd = __shfl_xor((int)cand, o, W);
-        if (better(ocd != 0, og, oid, cand, gain, id)) {
-            cand = true;
-            gain = og;
-            id = oid;
-        }
+    u64 key = argmax_key(cand, gain);
+    i64 rid = id, none = 0;
+    group_argmax_key<W>(key, rid, none);
+    cand = key != 0;
+    if (cand) {
+        gain = key_gain(key);
+        id = rid;
     }
 }
 

@@ -358,7 +358,7 @@ int glv_autotune_bins_dev(int64_t nv, int64_t nnz, const int64_t *d_offsets, con
  * null) receive every measured candidate; *ntrials their count. */
 typedef struct {
     int32_t jp_blocks;      /* 1..256, default 64 */
-    int32_t persist_blocks; /* 1..64, default 64 */
+    int32_t persist_blocks; /* 1..256, default 64 (graphs above 15360 vertices: at most 64 used) */
     int32_t wide_blocks;    /* 0: one workgroup per CU (default), else at most that many */
     int32_t reserved;
     int64_t core_min_deg;   /* >= 1, default 1024 */

@@ -30,7 +30,7 @@ def test_tuning_limits_and_round_trip():
     from grappolo_amd import engine
     assert engine.get_tuning() == DEFAULT
     try:
-        for bad in ({"jp_blocks": 0}, {"jp_blocks": 257}, {"persist_blocks": 65}, {"persist_blocks": 0},
+        for bad in ({"jp_blocks": 0}, {"jp_blocks": 257}, {"persist_blocks": 257}, {"persist_blocks": 0},
                     {"wide_blocks": -1}, {"core_min_deg": 0}):
             with pytest.raises(engine.GlvError):
                 engine.set_tuning(**{**DEFAULT, **bad})
