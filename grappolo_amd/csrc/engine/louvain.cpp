@@ -232,7 +232,12 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
     Profiler::get().begin(K_COLOR, s_);
     // heavy and giant lists in the next round's buffers
     launch_jp_count(g, cnt.p, cur, next.small, cur.counts + 3, next.large, next.counts + 3, s_);
-    Profiler::get().end(K_COLOR, 0.0, s_);
+    // algorithmic bytes of the whole coloring, credited here once: per directed
+    // entry 17 in the count (tail, the neighbour's priority, the entry's code)
+    // and 13 when its vertex is coloured (tail, code, the neighbour's colour or
+    // count), per vertex 20 (offsets, priority, count); the tail's first-fit
+    // over at most kJpTail vertices is left out
+    Profiler::get().end(K_COLOR, 30.0 * g.nnz + 20.0 * nv, s_);
     i64 core_entries = 0;
     if (K > 0) {
         DBuf<i64> sizes(K + 1, s_);  // (the giant lists hold the count's frontier)
