@@ -158,28 +158,61 @@ __device__ inline void offer(Cand &c, double g, i64 y, double e, i64 n) {
     }
 }
 
-// best over a group of W lanes (every lane gets it)
+// One pairing step of group_best over DPP control CTRL (or lane xor o when
+// CTRL is 0): the other lane's candidate wins on a larger key (argmax_key)
+// or an equal key and a smaller id; e and n follow the winner.
+template <int CTRL>
+__device__ inline void best_step(u64 &key, i64 &id, double &e, i64 &n, int o) {
+    u64 ok, oe;
+    i64 oid, on;
+    if (CTRL) {
+        ok = dpp64<CTRL>(key);
+        oid = (i64)dpp64<CTRL>((u64)id);
+        oe = dpp64<CTRL>((u64)__double_as_longlong(e));
+        on = (i64)dpp64<CTRL>((u64)n);
+    } else {
+        ok = (u64)__shfl_xor((long long)key, o);
+        oid = __shfl_xor((long long)id, o);
+        oe = (u64)__shfl_xor(__double_as_longlong(e), o);
+        on = __shfl_xor((long long)n, o);
+    }
+    const bool t = ok > key || (ok == key && oid < id);
+    key = t ? ok : key;
+    id = t ? oid : id;
+    e = t ? __longlong_as_double((long long)oe) : e;
+    n = t ? on : n;
+}
+
+// The best candidate over aligned groups of W lanes (W a power of two up to
+// 64): every lane gets it, or keeps its own when the group has none.  The
+// pairing is that of group_argmax_key (DPP within a row, bpermute across).
+template <int W>
+__device__ inline void group_best_key(u64 &key, i64 &id, double &e, i64 &n) {
+    if (W >= 2) best_step<0xB1>(key, id, e, n, 0);   // quad_perm [1,0,3,2]
+    if (W >= 4) best_step<0x4E>(key, id, e, n, 0);   // quad_perm [2,3,0,1]
+    if (W >= 8) best_step<0x141>(key, id, e, n, 0);  // row_half_mirror
+    if (W >= 16) best_step<0x140>(key, id, e, n, 0); // row_mirror
+    if (W >= 32) best_step<0>(key, id, e, n, 16);
+    if (W >= 64) best_step<0>(key, id, e, n, 32);
+}
 template <int W>
 __device__ inline void group_best(Cand &c) {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) {
-        const double og = __shfl_xor(c.gain, o, W);
-        const long long oid = __shfl_xor((long long)c.id, o, W);
-        const double oe = __shfl_xor(c.e, o, W);
-        const long long on = __shfl_xor((long long)c.n, o, W);
-        const int ok = __shfl_xor((int)c.ok, o, W);
-        if (better(ok != 0, og, oid, c.ok, c.gain, c.id)) {
-            c.ok = true;
-            c.gain = og;
-            c.id = oid;
-            c.e = oe;
-            c.n = on;
-        }
+    u64 key = argmax_key(c.ok, c.gain);
+    i64 id = c.id, n = c.n;
+    double e = c.e;
+    group_best_key<W>(key, id, e, n);
+    if (key != 0) {
+        c.ok = true;
+        c.gain = key_gain(key);
+        c.id = id;
+        c.e = e;
+        c.n = n;
     }
 }
 
 // best over the workgroup, in thread 0 only (one barrier; R's slots are
-// rewritten only after the caller's next barrier)
+// rewritten only after the caller's next barrier): the wavefronts' bests,
+// then the first wavefront reduces the kWWaves of them
 __device__ inline void wg_best0(Cand &c, Red &R) {
     group_best<kWave>(c);
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
@@ -191,16 +224,21 @@ __device__ inline void wg_best0(Cand &c, Red &R) {
         R.n[w] = c.n;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (w != 0) return;
+    const int q = l % kWWaves;
+    u64 key = argmax_key(R.c[q] != 0, R.d[q]);
+    i64 id = R.i[q], n = R.n[q];
+    double e = R.e[q];
+    group_best_key<kWWaves>(key, id, e, n);
+    if (l != 0) return;
     Cand b;
-    for (int q = 0; q < kWWaves; q++)
-        if (better(R.c[q] != 0, R.d[q], R.i[q], b.ok, b.gain, b.id)) {
-            b.ok = true;
-            b.gain = R.d[q];
-            b.id = R.i[q];
-            b.e = R.e[q];
-            b.n = R.n[q];
-        }
+    if (key != 0) {
+        b.ok = true;
+        b.gain = key_gain(key);
+        b.id = id;
+        b.e = e;
+        b.n = n;
+    }
     c = b;
 }
 
