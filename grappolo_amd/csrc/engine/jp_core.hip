@@ -137,11 +137,14 @@ __global__ __launch_bounds__(kFillBlock) void k_jp_core_fill(const i64 *off, con
 // gives each vertex a group of G wavefronts (G = kCoreWaves / n rounded down
 // to a power of two), so a round along a chain of hubs spreads the hub's
 // list over the whole workgroup: one load of it, the colour choice and three
-// workgroup barriers.
+// workgroup barriers.  With stop >= 0 (the core is the whole graph) the
+// rounds end after the first one that leaves at most stop vertices
+// uncoloured -- the tail's, first-fit afterwards -- which get colour -1 and
+// depth rounds + 1, so the depth cut finds the same round.
 constexpr int kCoreRegs = 4;  // list entries per thread held across the two passes
 __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K, const i64 *loff,
                                                         const int32_t *core_cnt, const uint16_t *ladj,
-                                                        int32_t *colors, int32_t *dep, int32_t *err) {
+                                                        int32_t *colors, int32_t *dep, int32_t *err, int stop) {
     __shared__ uint32_t st[kJpCoreMax];
     __shared__ uint16_t queue[kJpCoreMax];
     __shared__ uint32_t mk[kCoreWaves][kColorWords];
@@ -197,13 +200,20 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
         const int q = q0 + tid;
         enqueue(q < K && core_cnt[2 * q] == 0, q);
     }
-    int head = 0;
+    int head = 0, rounds = 0;
+    bool stopped = false;
     while (true) {
         __syncthreads();  // the previous round is complete
         const int tail = s_tail;
         __syncthreads();  // and every thread knows where it ends before this round appends
         const int n = tail - head;
         if (n <= 0) break;
+        // after round 1 or later, at most `stop` vertices left: they are the tail's
+        if (stop >= 0 && rounds > 0 && K - head <= stop) {
+            stopped = true;
+            break;
+        }
+        rounds++;
         if (n >= kCoreWaves) {
             // a wavefront per vertex
             uint32_t *m = mk[w];
@@ -316,6 +326,9 @@ __global__ __launch_bounds__(kCoreBlock) void k_jp_core(const i64 *hlist, int K,
         if (sq & kColoured) {
             colors[v] = (int32_t)(sq & 0xFFFu);
             dep[v] = (int32_t)((sq >> 12) & 0x7FFFFu);
+        } else if (stopped) {
+            colors[v] = -1;  // left for the tail
+            dep[v] = rounds + 1;
         } else {
             atomicOr(err, 4);  // never queued: the counts do not match the lists
         }
@@ -384,13 +397,13 @@ void launch_jp_core_fill(const DevGraph &g, const uint8_t *code, const int32_t *
     GLV_LAUNCH_CHECK();
 }
 void launch_jp_core(const i64 *hlist, i64 K, const i64 *loff, const int32_t *core_cnt, const uint16_t *ladj,
-                    int32_t *colors, int32_t *dep, int32_t *err, hipStream_t s) {
+                    int32_t *colors, int32_t *dep, int32_t *err, int stop, hipStream_t s) {
     if (K <= 0) return;
     if (K > kJpCoreMax) {  // callers pass at most kJpCoreMax vertices
         launch_fill_i32(err, 1, 4, s);
         return;
     }
-    k_jp_core<<<1, kCoreBlock, 0, s>>>(hlist, (int)K, loff, core_cnt, ladj, colors, dep, err);
+    k_jp_core<<<1, kCoreBlock, 0, s>>>(hlist, (int)K, loff, core_cnt, ladj, colors, dep, err, stop);
     GLV_LAUNCH_CHECK();
 }
 void launch_jp_depth_max(const int32_t *dep, i64 nv, int32_t *maxd, hipStream_t s) {

@@ -225,9 +225,12 @@ void launch_jp_core_sizes(const int32_t *core_cnt, i64 K, i64 *sizes, hipStream_
 // neighbours, then of its lower ones (codes of the count)
 void launch_jp_core_fill(const DevGraph &g, const uint8_t *code, const int32_t *cnt, const i64 *hlist, i64 K,
                          const i64 *loff, const int32_t *core_cnt, uint16_t *ladj, hipStream_t s);
-// colours and depths of the K core vertices, one workgroup
+// colours and depths of the K core vertices, one workgroup; stop >= 0 (the
+// core is the whole graph): the rounds end after the first round that leaves
+// at most stop vertices uncoloured, which get depth rounds + 1 (deeper than
+// every coloured vertex) for the tail
 void launch_jp_core(const i64 *hlist, i64 K, const i64 *loff, const int32_t *core_cnt, const uint16_t *ladj,
-                    int32_t *colors, int32_t *dep, int32_t *err, hipStream_t s);
+                    int32_t *colors, int32_t *dep, int32_t *err, int stop, hipStream_t s);
 // largest depth (maxd: one word, zero on entry)
 void launch_jp_depth_max(const int32_t *dep, i64 nv, int32_t *maxd, hipStream_t s);
 // hist[i] = vertices of depth top - i, i < kJpDepthWindow (zeroed words)

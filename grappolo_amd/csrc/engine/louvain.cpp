@@ -255,7 +255,9 @@ int Engine::color(const DevGraph &g, int32_t *colors, int *rounds_out) {
         DBuf<uint16_t> ladj(std::max<i64>(core_entries, 1), s_);
         Profiler::get().begin(K_COLOR, s_);
         launch_jp_core_fill(g, code.p, cnt.p, hlist.p, K, loff.p, core_cnt.p, ladj.p, s_);
-        launch_jp_core(hlist.p, K, loff.p, core_cnt.p, ladj.p, colors, dep.p, err.p, s_);
+        // a core that is the whole graph stops where the tail starts
+        launch_jp_core(hlist.p, K, loff.p, core_cnt.p, ladj.p, colors, dep.p, err.p, K == nv ? (int)kJpTail : -1,
+                       s_);
         Profiler::get().end(K_COLOR, 0.0, s_);
         colored = K;
     }
