@@ -45,7 +45,7 @@ constexpr int kWMidSlots = 512;          // table of a wavefront (mid vertices: 
 constexpr i64 kWideMaxPerColor = 100000;
 
 // Relative costs of workgroup items (microseconds, measured order of
-// magnitude): a heavy vertex kWCostHeavy plus one per 1024 entries.
+// magnitude): a heavy vertex kWCostHeavy plus three per 1024 entries.
 constexpr double kWCostHeavy = 10.0, kWCostChunk = 10.0, kWCostPart = 10.0;
 // wavefront items: a mid vertex, a batch of light vertices (a workgroup runs
 // kWWavesPerGroup of them at a time)

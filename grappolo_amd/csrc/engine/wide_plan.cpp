@@ -72,7 +72,7 @@ static void plan_class(const PhaseSetup &P, const std::vector<i64> &vd, i64 hlo,
     std::vector<size_t> order(cls.size());
     for (size_t i = 0; i < cls.size(); i++) order[i] = i;
     auto item_cost = [](const WideItem &it) {
-        return it.kind == kWHeavy ? kWCostHeavy + it.y / 1024.0 : it.kind == kWGiantChunk ? kWCostChunk : kWCostPart;
+        return it.kind == kWHeavy ? kWCostHeavy + 3.0 * it.y / 1024.0 : it.kind == kWGiantChunk ? kWCostChunk : kWCostPart;
     };
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return item_cost(cls[a]) > item_cost(cls[b]); });
