@@ -38,6 +38,9 @@ def test_tuning_limits_and_round_trip():
         engine.set_tuning(jp_blocks=256, persist_blocks=16, wide_blocks=128, core_min_deg=256)
         assert engine.get_tuning() == {"jp_blocks": 256, "persist_blocks": 16, "wide_blocks": 128,
                                        "core_min_deg": 256}
+        # the dense persistent sweep takes up to one workgroup per CU
+        engine.set_tuning(**{**DEFAULT, "persist_blocks": 256})
+        assert engine.get_tuning()["persist_blocks"] == 256
     finally:
         engine.set_tuning(**DEFAULT)
 
